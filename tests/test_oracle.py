@@ -1,0 +1,232 @@
+"""Pin the CPU oracle: reference golden vectors, closed forms, brute force.
+
+These run on the CPU only.  A planted-mistake test at the end checks that the
+checks are sharp enough to catch a plausible error in the oracle itself.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from kilosort_amd import synth
+from kilosort_amd.batches import ArrayBatches
+from oracle import brute
+from oracle import tm_oracle as O
+
+CPU = torch.device("cpu")
+
+
+def _ops(nt=61, Th=8, max_peels=100, W=None, NT=3000, nt0min=20, nC=10,
+         position_limit=100.0, xc=None, yc=None):
+    return dict(nt=nt, Th_learned=Th, max_peels=max_peels, wPCA=W, nt0min=nt0min,
+                batch_size=NT, xc=xc, yc=yc,
+                settings=dict(nearest_chans=nC, position_limit=position_limit,
+                              n_pcs=6))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _one_thread():
+    n = torch.get_num_threads()
+    torch.set_num_threads(1)
+    yield
+    torch.set_num_threads(n)
+
+
+def test_run_matching_matches_reference_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
+    W = torch.from_numpy(g["wPCA"])
+    U = torch.from_numpy(g["U"])
+    ops = _ops(W=W)
+    ctc = O.prepare_matching(ops, U)
+    assert torch.equal(ctc, torch.from_numpy(g["ctc"]))
+    st, amps, th, Xres = O.run_matching(ops, torch.from_numpy(g["X"]), U, ctc)
+    assert len(st) > 10
+    np.testing.assert_array_equal(st.numpy(), g["st"])
+    np.testing.assert_array_equal(amps.numpy(), g["amps"])
+    np.testing.assert_array_equal(th.numpy(), g["th_amps"])
+    np.testing.assert_array_equal(Xres.numpy(), g["Xres"])
+
+
+def test_run_matching_single_peel_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
+    g1 = np.load(os.path.join(golden_dir, "run_matching_c16_k8_peel1.npz"))
+    ops = _ops(W=torch.from_numpy(g["wPCA"]), max_peels=1)
+    U = torch.from_numpy(g["U"])
+    st, amps, th, _ = O.run_matching(ops, torch.from_numpy(g["X"]), U,
+                                     O.prepare_matching(ops, U))
+    np.testing.assert_array_equal(st.numpy(), g1["st"])
+    np.testing.assert_array_equal(amps.numpy(), g1["amps"])
+    np.testing.assert_array_equal(th.numpy(), g1["th_amps"])
+    # one iteration accepts window maxima only: spikes are > nt apart
+    assert np.all(np.diff(g1["st"][:, 0]) > 61)
+
+
+def test_extract_matches_reference_golden(golden_dir):
+    g = np.load(os.path.join(golden_dir, "extract_c24_k12.npz"))
+    NT = int(g["NT"])
+    ops = _ops(W=torch.from_numpy(g["wPCA"]), NT=NT, nC=int(g["nC"]),
+               position_limit=float(g["position_limit"]), xc=g["xc"], yc=g["yc"])
+    bf = ArrayBatches(g["data"], NT=NT, nt=61, device=CPU)
+    assert bf.n_batches == int(g["n_batches"]) == 3
+    st, tF, ops = O.extract(ops, bf, torch.from_numpy(g["U"]))
+    np.testing.assert_array_equal(st, g["st"])
+    np.testing.assert_array_equal(tF.numpy(), g["tF"])
+    np.testing.assert_array_equal(ops["iCC"].numpy(), g["iCC"])
+    np.testing.assert_array_equal(ops["iCC_mask"].numpy(), g["iCC_mask"])
+    np.testing.assert_array_equal(ops["iU"].numpy(), g["iU"])
+
+
+@pytest.mark.parametrize("C", [64, 383])
+def test_prepare_extract_golden(golden_dir, C):
+    g = np.load(os.path.join(golden_dir, f"prepare_extract_c{C}.npz"))
+    xc, yc = synth.probe_2col(C)
+    U, _ = synth.make_templates(np.random.default_rng(int(g["seed"])), int(g["K"]),
+                                xc, yc)
+    iCC, mask, iU, Ucc = O.prepare_extract(xc, yc, torch.from_numpy(U), 10, 100.0)
+    np.testing.assert_array_equal(iCC.numpy(), g["iCC"])
+    np.testing.assert_array_equal(mask.numpy(), g["iCC_mask"])
+    np.testing.assert_array_equal(iU.numpy(), g["iU"])
+    np.testing.assert_array_equal(Ucc.numpy(), g["Ucc"])
+    # closed form: same sets of nearest channels where distances do not tie
+    ref_sets, d2 = brute.nearest_channels(xc, yc, 10)
+    for c in range(C):
+        strict = d2[:, c] < d2[-1, c]
+        assert set(g["iCC"][strict, c]) == set(ref_sets[strict, c])
+
+
+def test_ctc_closed_form_and_symmetry():
+    rec = synth.make_recording(7, 12, 6, 200, with_data=False)
+    W = torch.from_numpy(rec.wPCA).double()
+    U = torch.from_numpy(rec.U).double()
+    ctc = O.prepare_matching(_ops(W=W), U).numpy()
+    direct = brute.ctc_direct(rec.U, rec.wPCA)
+    np.testing.assert_allclose(ctc, direct, atol=1e-12)
+    # end lags only see zero padding
+    assert np.all(direct[:, :, 0] == 0) and np.all(direct[:, :, -1] == 0)
+    # ctc[i, j, l] = ctc[j, i, 2nt - l]
+    np.testing.assert_allclose(direct, direct.transpose(1, 0, 2)[:, :, ::-1],
+                               atol=1e-12)
+    # zero lag of the diagonal = ||T_i||^2 = nm_i for an orthonormal basis
+    nm = (rec.U.astype(np.float64) ** 2).sum((1, 2))
+    np.testing.assert_allclose(np.diagonal(direct[:, :, 61]), nm, rtol=1e-6)
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_oracle_agrees_with_brute_force(seed):
+    rec = synth.make_recording(seed, 16, 8, 2400, rate_range=(20.0, 60.0),
+                               amp_range=(10.0, 30.0))
+    X = np.ascontiguousarray(rec.data.T).astype(np.float64)
+    W = torch.from_numpy(rec.wPCA).double()
+    U = torch.from_numpy(rec.U).double()
+    ops = _ops(W=W)
+    st, amps, th, Xres = O.run_matching(ops, torch.from_numpy(X), U,
+                                        O.prepare_matching(ops, U))
+    bt, bi, ba, bth, bX = brute.brute_matching(X, rec.U, rec.wPCA, 61, 8, 100)
+    assert len(bt) > 5
+    np.testing.assert_array_equal(st[:, 0].numpy(), bt)
+    np.testing.assert_array_equal(st[:, 1].numpy(), bi)
+    np.testing.assert_allclose(amps[:, 0].numpy(), ba, rtol=1e-9)
+    np.testing.assert_allclose(th[:, 0].numpy(), bth, rtol=1e-9)
+    np.testing.assert_allclose(Xres.numpy(), bX, atol=1e-9)
+
+
+def test_isolated_spike_known_answer():
+    """One noise-free spike: exact time/id, amps = amplitude, Xres ~ 0."""
+    rec = synth.make_recording(21, 16, 5, 1000, with_data=False)
+    nt, nt0min = 61, 20
+    Tn = 1000
+    X = np.zeros((16, Tn))
+    j, a, x = 3, 17.5, 400
+    X[:, x - 30:x + 31] += a * brute.full_templates(rec.U, rec.wPCA)[j]
+    W = torch.from_numpy(rec.wPCA).double()
+    U = torch.from_numpy(rec.U).double()
+    ops = _ops(W=W)
+    st, amps, th, Xres = O.run_matching(ops, torch.from_numpy(X), U,
+                                        O.prepare_matching(ops, U))
+    assert st.tolist() == [[x, j]]
+    nm = float((U[j] ** 2).sum())
+    assert amps[0, 0].item() == pytest.approx(a, rel=1e-6)
+    assert th[0, 0].item() == pytest.approx(a * np.sqrt(nm), rel=1e-6)
+    assert np.abs(Xres.numpy()).max() < 1e-5 * a
+    # the reported absolute time is the trough: window start + nt0min
+    assert brute.reference_time(x, 0, 3000, nt, nt0min) == x - 30 - nt + nt0min
+
+
+def test_edges_and_subthreshold():
+    rec = synth.make_recording(22, 16, 5, 1000, with_data=False)
+    T = brute.full_templates(rec.U, rec.wPCA)
+    W = torch.from_numpy(rec.wPCA).double()
+    U = torch.from_numpy(rec.U).double()
+    ops = _ops(W=W)
+    ctc = O.prepare_matching(ops, U)
+    # a spike centred inside the first nt columns is never accepted
+    X = np.zeros((16, 1000))
+    X[:, 10:71] += 30 * T[1]
+    st, *_ = O.run_matching(ops, torch.from_numpy(X), U, ctc)
+    assert len(st) == 0
+    # sub-threshold amplitude: Cf = a^2 nm < 64
+    X = np.zeros((16, 1000))
+    X[:, 470:531] += 7.9 * T[1]
+    st, *_ = O.run_matching(ops, torch.from_numpy(X), U, ctc)
+    assert len(st) == 0
+
+
+def test_overlapping_spikes_need_two_iterations():
+    rec = synth.make_recording(23, 16, 5, 1000, with_data=False)
+    T = brute.full_templates(rec.U, rec.wPCA)
+    W = torch.from_numpy(rec.wPCA).double()
+    U = torch.from_numpy(rec.U).double()
+    ops = _ops(W=W)
+    ctc = O.prepare_matching(ops, U)
+    X = np.zeros((16, 1000))
+    X[:, 370:431] += 25 * T[2]
+    X[:, 390:451] += 15 * T[2]
+    rec_it = []
+    st, amps, *_ = O.run_matching(ops, torch.from_numpy(X), U, ctc,
+                                  record=rec_it)
+    assert sorted(st[:, 0].tolist()) == [400, 420]
+    assert [r["n"] for r in rec_it][:2] == [1, 1]
+    ops1 = dict(ops, max_peels=1)
+    st1, *_ = O.run_matching(ops1, torch.from_numpy(X), U, ctc)
+    assert len(st1) == 1
+
+
+def test_batch_edges_known_values():
+    """Padded batch edges, after the pattern of the reference's IO tests."""
+    N, C, NT, nt = 1000, 4, 300, 61
+    data = np.repeat(np.arange(N, dtype=np.float32)[:, None], C, 1)
+    bf = ArrayBatches(data, NT=NT, nt=nt, device=CPU)
+    assert bf.n_batches == 4
+    X0 = bf.padded_batch_to_torch(0)
+    assert X0.shape == (C, NT + 2 * nt)
+    assert X0.min() == 0 and X0.max() == NT + nt - 1
+    assert torch.all(X0[:, :nt + 1] == 0)
+    X1 = bf.padded_batch_to_torch(1)
+    assert X1.min() == NT - nt and X1.max() == 2 * NT + nt - 1
+    X3 = bf.padded_batch_to_torch(3)
+    assert X3.min() == 3 * NT - nt and X3.max() == N - 1
+    assert torch.all(X3[:, N - (3 * NT - nt):] == N - 1)
+    # a last batch shorter than nt samples is dropped
+    bf2 = ArrayBatches(np.zeros((2 * NT + 30, C), np.float32), NT=NT, nt=nt)
+    assert bf2.n_batches == 2
+
+
+def test_planted_lag_flip_is_caught():
+    """A flipped ctc lag axis (a plausible oracle mistake) breaks agreement."""
+    rec = synth.make_recording(12, 16, 8, 2400, rate_range=(20.0, 60.0),
+                               amp_range=(10.0, 30.0))
+    X = torch.from_numpy(np.ascontiguousarray(rec.data.T)).double()
+    W = torch.from_numpy(rec.wPCA).double()
+    U = torch.from_numpy(rec.U).double()
+    ops = _ops(W=W)
+    ctc = O.prepare_matching(ops, U)
+    good = O.run_matching(ops, X, U, ctc)
+    bad = O.run_matching(ops, X, U, torch.flip(ctc, [2]).transpose(0, 1))
+    bt, bi, *_ = brute.brute_matching(X.numpy(), rec.U, rec.wPCA, 61, 8, 100)
+    assert np.array_equal(good[0][:, 0].numpy(), bt)
+    # the transposed-and-flipped Gram equals the original (symmetry), so a
+    # plain flip is the planted mistake
+    bad = O.run_matching(ops, X, U, torch.flip(ctc, [2]))
+    assert not (len(bad[0]) == len(bt) and np.array_equal(bad[0][:, 0].numpy(), bt)
+                and np.allclose(bad[1][:, 0].numpy(), good[1][:, 0].numpy()))
