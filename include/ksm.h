@@ -1,0 +1,169 @@
+/*
+ * ksm.h -- C ABI of the MI355X (gfx950) learned-template matching library
+ * (libksm.so).
+ *
+ * The library replaces the tensor operations of Kilosort4's matching-pursuit
+ * spike extraction, kilosort/template_matching.py:14-244, with hand-written
+ * HIP kernels.  Every entry point takes plain device pointers, sizes and a
+ * stream (a hipStream_t passed as void*; NULL = the default stream), returns
+ * a ksm_status code, and is asynchronous with respect to the host unless
+ * stated otherwise.  No entry point allocates device memory: scratch space is
+ * passed in as a workspace whose size the matching *_workspace query returns.
+ *
+ * Conventions (reference symbols in brackets):
+ *   K  templates [Nfilt]      P  temporal PCs [n_pcs]      C  channels [Nchan]
+ *   nt waveform length (odd)  T  padded batch length [NT + 2*nt]
+ *   U    float32 [K][P][C]   spatial components, the layout run_matching takes
+ *   W    float32 [P][nt]     temporal basis ops['wPCA']
+ *   X    float32 [C][ldx]    one padded, whitened batch (T valid columns)
+ *   ctc  float32 [K][K][ldl] template Gram, stored firing-template major:
+ *        ctc[(j*K + i)*ldl + l] = reference ctc[i, j, l], l = 0..2*nt
+ *        (the reference tensor is the permuted view (1, 0, 2) of this array)
+ */
+#ifndef KSM_H
+#define KSM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ksm_status {
+    KSM_OK = 0,
+    KSM_ERR_INVALID = 1,        /* bad sizes, pointers or parameters        */
+    KSM_ERR_CAPACITY = 2,       /* more spikes than the st capacity         */
+    KSM_ERR_HIP = 3,            /* a HIP runtime call or launch failed      */
+    KSM_ERR_WORKSPACE = 4,      /* workspace smaller than the query result  */
+    KSM_ERR_TIES = 5            /* exact score ties beyond slot capacity    */
+} ksm_status;
+
+typedef enum ksm_precision {
+    KSM_PREC_FP32 = 0,          /* FP32 conv + FP32 MFMA projection          */
+    KSM_PREC_BF16 = 1           /* BF16 MFMA projection, FP32 accumulation   */
+} ksm_precision;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int ksm_version(void);
+
+/* Static text for a status code. */
+const char *ksm_status_string(int status);
+
+/* Detail of the last error raised on the calling host thread ("" if none). */
+const char *ksm_last_error(void);
+
+/*
+ * Peak channel and peak-channel PC features of every template.
+ * Replaces template_matching.py:50-51 (prepare_extract):
+ *   iU[i]       = argmax_c sum_p U[i,p,c]^2   (first index on ties)
+ *   Ucc[n,i,p]  = U[i, p, iCC[n, iU[i]]]
+ * iCC: int64 [nC][C] nearest-channel table (computed on the host with the
+ * reference's own numpy argsort, template_matching.py:44-46).
+ * Outputs iU int64 [K], Ucc float32 [nC][K][P].
+ */
+int ksm_prepare_extract(const float *U, int K, int P, int C,
+                        const int64_t *iCC, int nC,
+                        int64_t *iU, float *Ucc, void *stream);
+
+/*
+ * Template norms nm[i] = sum_p sum_c U[i,p,c]^2, accumulated in float64 and
+ * rounded once.  Replaces template_matching.py:176 (run_matching).
+ */
+int ksm_template_norms(const float *U, int K, int P, int C, float *nm,
+                       void *stream);
+
+/* Workspace bytes of ksm_prepare_matching. */
+size_t ksm_prepare_matching_workspace(int K, int P, int C, int nt);
+
+/*
+ * Template Gram ctc, formed in float64 and rounded once to float32.
+ * Replaces template_matching.py:155-167 (prepare_matching):
+ *   ctc[i,j,d+nt] = sum_c sum_k T_i[c,k] T_j[c,k+d],  T_i = U_i^T W,
+ * written firing-template major (see the header comment), ldl >= 2*nt+1;
+ * padding lags l in [2*nt+1, ldl) are written as 0.
+ */
+int ksm_prepare_matching(const float *U, const float *W, int K, int P, int C,
+                         int nt, float *ctc, int ldl, void *workspace,
+                         size_t ws_bytes, void *stream);
+
+/*
+ * Nonzero-row lists of the Gram: for each firing template j, the increasing
+ * list rows[j*K + q], q < nrows[j], of templates i whose ctc[i, j, :] has a
+ * nonzero entry.  Rows left out are exactly zero, so skipping them in the
+ * peel update B -= amp*ctc changes nothing (template_matching.py:238).
+ */
+int ksm_ctc_rows(const float *ctc, int K, int L, int ldl, int32_t *rows,
+                 int32_t *nrows, void *stream);
+
+/* Workspace bytes of ksm_project for the given sizes and precision. */
+size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision);
+
+/*
+ * Scores of every template at every time: the temporal-PC correlation of
+ * each channel followed by the projection on the spatial components.
+ * Replaces template_matching.py:180-181 (run_matching):
+ *   B[i,t] = sum_p sum_c U[i,p,c] * sum_k W[p,k] * X[c, t + k - nt/2]
+ * (zero padding outside [0, T)).  B is float32 [rows >= K][ldb], ldb >= T
+ * rounded up to 128; columns >= T are written with values that the peel
+ * never reads.
+ */
+int ksm_project(const float *X, int ldx, const float *W, const float *U,
+                int C, int T, int K, int P, int nt, float *B, int ldb,
+                int precision, void *workspace, size_t ws_bytes, void *stream);
+
+/* Workspace bytes of ksm_peel. */
+size_t ksm_peel_workspace(int T, int max_peels);
+
+/*
+ * Matching pursuit (the peel).  Replaces template_matching.py:183-244:
+ * up to max_peels iterations of
+ *   Cf = relu(B)^2 / nm, first and last nt columns zeroed;
+ *   accept columns x where max over +-nt of max_i Cf[i, .] exceeds Th^2 and
+ *   equals max_i Cf[i, x]; record (x, argmax_i), amps = B/nm,
+ *   th_amps = sqrt(Cmax); subtract amps * (U^T W) from Xres and
+ *   amps * ctc[:, id, :] from B in two interleaved phases (even-ranked
+ *   spikes first), stopping early when nothing is accepted.
+ * B (from ksm_project) and Xres (a copy of X) are updated in place.
+ * Outputs: st int64 [capacity][2] (time column, template id), amps and
+ * th_amps float32 [capacity], and info int32 [4] on the device:
+ *   info[0] = spike count k, info[1] = ksm_status of the device run,
+ *   info[2] = iterations that accepted spikes.
+ * Spikes are ordered as the reference orders them: by iteration, then time.
+ */
+int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U,
+             const float *W, const float *nm, const float *ctc, int ldl,
+             const int32_t *rows, const int32_t *nrows, int K, int P, int C,
+             int T, int nt, float Th, int max_peels, int64_t *st, float *amps,
+             float *th_amps, int capacity, int32_t *info, void *workspace,
+             size_t ws_bytes, void *stream);
+
+/*
+ * PC features of each spike on the nC channels nearest its template's peak
+ * channel: the final residual projected on W plus the spike's own template.
+ * Replaces template_matching.py:85-86 (extract):
+ *   tF[s,n,p] = sum_k Xres[iCC[n, iU[id_s]], t_s + k - nt/2] W[p,k]
+ *             + amps[s] * Ucc[n, id_s, p]
+ * st int64 [nsp][2] as produced by ksm_peel; output tF float32 [nsp][nC][P].
+ */
+int ksm_features(const float *Xres, int ldx, const float *W,
+                 const int64_t *st, const float *amps, int nsp,
+                 const int64_t *iCC, const int64_t *iU, const float *Ucc,
+                 int nC, int K, int P, int C, int nt, float *tF, void *stream);
+
+/*
+ * Per-kernel timing (benchmark and test use; no reference counterpart).
+ * When enabled, ksm_project and ksm_peel bracket their main kernels with HIP
+ * event pairs on the launch stream.  ksm_profile_query synchronises those
+ * events and returns the summed milliseconds and launch count of one kernel
+ * ("conv_pc", "gemm_f32", "peel_select", "peel_update", "colmax_all").
+ */
+int ksm_profile_enable(int on);
+int ksm_profile_reset(void);
+int ksm_profile_query(const char *name, double *total_ms, int *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KSM_H */

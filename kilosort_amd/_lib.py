@@ -1,0 +1,118 @@
+"""ctypes binding of libksm.so (the C ABI declared in include/ksm.h).
+
+The library is built in-tree by ``python -m kilosort_amd.build``.  There is no
+fallback: if the library is missing or no ROCm GPU is present, every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libksm.so")
+
+KSM_OK = 0
+KSM_ERR_INVALID = 1
+KSM_ERR_CAPACITY = 2
+KSM_ERR_HIP = 3
+KSM_ERR_WORKSPACE = 4
+KSM_ERR_TIES = 5
+
+PREC_FP32 = 0
+PREC_BF16 = 1
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_sz = ctypes.c_size_t
+
+# name -> (restype, argtypes); must match include/ksm.h
+SIGNATURES = {
+    "ksm_version": (_i, []),
+    "ksm_status_string": (ctypes.c_char_p, [_i]),
+    "ksm_last_error": (ctypes.c_char_p, []),
+    "ksm_prepare_extract": (_i, [_p, _i, _i, _i, _p, _i, _p, _p, _p]),
+    "ksm_template_norms": (_i, [_p, _i, _i, _i, _p, _p]),
+    "ksm_prepare_matching_workspace": (_sz, [_i, _i, _i, _i]),
+    "ksm_prepare_matching": (_i, [_p, _p, _i, _i, _i, _i, _p, _i, _p, _sz, _p]),
+    "ksm_ctc_rows": (_i, [_p, _i, _i, _i, _p, _p, _p]),
+    "ksm_project_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
+    "ksm_project": (_i, [_p, _i, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
+    "ksm_peel_workspace": (_sz, [_i, _i]),
+    "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _f,
+                      _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
+    "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
+    "ksm_profile_enable": (_i, [_i]),
+    "ksm_profile_reset": (_i, []),
+    "ksm_profile_query": (_i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                               ctypes.POINTER(ctypes.c_int)]),
+}
+
+_lib = None
+
+
+class KsmError(RuntimeError):
+    """A libksm call returned a non-zero status."""
+
+    def __init__(self, status, message):
+        super().__init__(message)
+        self.status = status
+
+
+def load():
+    """Load libksm.so (once) and declare its signatures."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m kilosort_amd.build` "
+                "(there is no CPU fallback)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(status: int, what: str):
+    if status == KSM_OK:
+        return
+    lib = load()
+    detail = lib.ksm_last_error().decode(errors="replace")
+    text = lib.ksm_status_string(status).decode(errors="replace")
+    msg = f"{what}: {text}" + (f" ({detail})" if detail else "")
+    if status == KSM_ERR_INVALID:
+        raise ValueError(msg)
+    raise KsmError(status, msg)
+
+
+def require_gpu(device) -> torch.device:
+    """The matching path runs on a ROCm GPU only."""
+    device = torch.device(device) if device is not None else torch.device("cuda")
+    if device.type != "cuda":
+        raise RuntimeError(f"kilosort_amd runs on a ROCm GPU only (got device {device})")
+    if not torch.cuda.is_available():
+        raise RuntimeError("kilosort_amd needs a ROCm GPU; torch.cuda.is_available() is False")
+    return device
+
+
+def stream_of(device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t) -> int:
+    return 0 if t is None else t.data_ptr()
+
+
+def profile_query(name: str):
+    """(total_ms, launches) of one kernel recorded while profiling was on."""
+    lib = load()
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int(0)
+    check(lib.ksm_profile_query(name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+          "ksm_profile_query")
+    return ms.value, n.value

@@ -1,0 +1,100 @@
+// Optional per-kernel timing with HIP events on the launch stream.
+//
+// When enabled, the entry points bracket their main kernels with event
+// pairs; ksm_profile_query synchronises the recorded stop events and returns
+// the summed elapsed time of one kernel name.  Disabled by default, so the
+// normal path records nothing.
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ksm_common.h"
+#include "ksm_profile.h"
+
+namespace {
+
+struct Record {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Record> g_records;
+std::vector<hipEvent_t> g_free;
+
+hipEvent_t take_event() {
+    if (!g_free.empty()) {
+        hipEvent_t e = g_free.back();
+        g_free.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+}  // namespace
+
+namespace ksm {
+
+ProfileScope::ProfileScope(const char *name, hipStream_t s) : name_(name), stream_(s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_on) return;
+    start_ = take_event();
+    if (start_) hipEventRecord(start_, s);
+}
+
+ProfileScope::~ProfileScope() {
+    if (!start_) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipEvent_t stop = take_event();
+    if (!stop) return;
+    hipEventRecord(stop, stream_);
+    g_records.push_back(Record{name_, start_, stop});
+}
+
+}  // namespace ksm
+
+extern "C" {
+
+int ksm_profile_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_on = on != 0;
+    return KSM_OK;
+}
+
+int ksm_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto &r : g_records) {
+        hipEventSynchronize(r.stop);
+        g_free.push_back(r.start);
+        g_free.push_back(r.stop);
+    }
+    g_records.clear();
+    return KSM_OK;
+}
+
+int ksm_profile_query(const char *name, double *total_ms, int *launches) {
+    if (!name || !total_ms || !launches)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_profile_query: null argument");
+    std::lock_guard<std::mutex> lk(g_mu);
+    double sum = 0.0;
+    int n = 0;
+    for (auto &r : g_records) {
+        if (r.name != name) continue;
+        hipError_t e = hipEventSynchronize(r.stop);
+        if (e != hipSuccess) return ksm::hip_fail(e, "ksm_profile_query");
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, r.start, r.stop);
+        if (e != hipSuccess) return ksm::hip_fail(e, "ksm_profile_query");
+        sum += ms;
+        ++n;
+    }
+    *total_ms = sum;
+    *launches = n;
+    return KSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,293 @@
+// Template scores B = U * (W (*) X): the temporal-PC correlation of every
+// channel (VALU, register-blocked) followed by the channel projection as a
+// GEMM on the matrix cores.
+//
+// Reference: kilosort/template_matching.py:180-181 (run_matching):
+//   B = conv1d(X[:, None], W[:, None], padding=nt//2)     -> Bc [C, P, T]
+//   B = einsum('ijk, kjl -> il', U, B)                     -> B  [K, T]
+// Here Bc is stored [P*C (+pad)][ldc] with row r = p*C + c, so that U viewed
+// as [K][P*C] is the left GEMM operand without reshuffling.
+#include "ksm_common.h"
+#include "ksm_profile.h"
+
+namespace {
+
+constexpr int kConvThreads = 256;
+constexpr int kConvTT = 8;                          // outputs per thread
+constexpr int kConvSpan = kConvThreads * kConvTT;   // 2048 columns per block
+
+// Skewed LDS index: one pad word every 8 words so that lanes reading 8-word
+// strided windows hit distinct banks.
+__host__ __device__ constexpr int skew(int e) { return e + (e >> 3); }
+
+// Bc[(p*C + c)*ldc + t] = sum_k W[p,k] * X[c, t + k - NTAP/2], t < ldc.
+// Columns t >= T are written as 0.
+template <int P, int NTAP>
+__global__ __launch_bounds__(kConvThreads) void conv_pc_kernel(
+    const float *__restrict__ X, int ldx, const float *__restrict__ W, int C, int T,
+    int ldc, float *__restrict__ Bc) {
+    constexpr int H = NTAP / 2;
+    constexpr int NWIN = kConvTT + NTAP - 1;
+    constexpr int NLOAD = kConvSpan + NTAP - 1;
+    __shared__ float xs[skew(NLOAD) + 8];
+    const int c = blockIdx.y;
+    const int t0 = blockIdx.x * kConvSpan;
+    const float *xr = X + (size_t)c * ldx;
+    for (int e = threadIdx.x; e < NLOAD; e += kConvThreads) {
+        const int t = t0 - H + e;
+        xs[skew(e)] = (t >= 0 && t < T) ? xr[t] : 0.f;
+    }
+    __syncthreads();
+    const int base = threadIdx.x * kConvTT;
+    float win[NWIN];
+#pragma unroll
+    for (int e = 0; e < NWIN; ++e) win[e] = xs[skew(base + e)];
+    float acc[kConvTT][P];
+#pragma unroll
+    for (int u = 0; u < kConvTT; ++u)
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[u][p] = 0.f;
+#pragma unroll
+    for (int k = 0; k < NTAP; ++k) {
+        float w[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) w[p] = W[p * NTAP + k];
+#pragma unroll
+        for (int u = 0; u < kConvTT; ++u)
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[u][p] = __builtin_fmaf(win[u + k], w[p], acc[u][p]);
+    }
+    const int tb = t0 + base;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        float *dst = Bc + ((size_t)p * C + c) * ldc + tb;
+        if (tb + kConvTT <= ldc) {
+            float4 v0, v1;
+            v0.x = tb + 0 < T ? acc[0][p] : 0.f;
+            v0.y = tb + 1 < T ? acc[1][p] : 0.f;
+            v0.z = tb + 2 < T ? acc[2][p] : 0.f;
+            v0.w = tb + 3 < T ? acc[3][p] : 0.f;
+            v1.x = tb + 4 < T ? acc[4][p] : 0.f;
+            v1.y = tb + 5 < T ? acc[5][p] : 0.f;
+            v1.z = tb + 6 < T ? acc[6][p] : 0.f;
+            v1.w = tb + 7 < T ? acc[7][p] : 0.f;
+            reinterpret_cast<float4 *>(dst)[0] = v0;
+            reinterpret_cast<float4 *>(dst)[1] = v1;
+        } else {
+#pragma unroll
+            for (int u = 0; u < kConvTT; ++u)
+                if (tb + u < ldc) dst[u] = tb + u < T ? acc[u][p] : 0.f;
+        }
+    }
+}
+
+// Generic fallback: one output per thread, runtime P and nt.
+__global__ __launch_bounds__(256) void conv_pc_generic(const float *__restrict__ X, int ldx,
+                                                       const float *__restrict__ W, int P,
+                                                       int nt, int C, int T, int ldc,
+                                                       float *__restrict__ Bc) {
+    const int c = blockIdx.y;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ldc) return;
+    const int H = nt / 2;
+    const float *xr = X + (size_t)c * ldx;
+    for (int p = 0; p < P; ++p) {
+        float acc = 0.f;
+        if (t < T)
+            for (int k = 0; k < nt; ++k) {
+                const int s = t + k - H;
+                const float x = (s >= 0 && s < T) ? xr[s] : 0.f;
+                acc = __builtin_fmaf(x, W[p * nt + k], acc);
+            }
+        Bc[((size_t)p * C + c) * ldc + t] = acc;
+    }
+}
+
+// At[r][Kp] = U[k][r] (k < K), 0 for K <= k < Kp or r >= R.
+__global__ void transpose_pad_kernel(const float *__restrict__ U, int K, int R, int Kp, int Rp,
+                                     float *__restrict__ At) {
+    const size_t n = (size_t)Rp * Kp;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int k = (int)(e % Kp);
+        const int r = (int)(e / Kp);
+        At[e] = (k < K && r < R) ? U[(size_t)k * R + r] : 0.f;
+    }
+}
+
+__global__ void zero_rows_kernel(float *__restrict__ p, size_t n) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        p[e] = 0.f;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 GEMM on v_mfma_f32_32x32x2f32: Bout[m][n] = sum_r At[r][m] * Bc[r][n].
+// Block tile 128x128, depth step 16, 4 waves each owning a 64x64 quadrant as
+// 2x2 MFMA tiles.  Operand tiles are double buffered in LDS with the next
+// step's global loads in flight during the MFMAs.
+
+constexpr int GBM = 128, GBN = 128, GBK = 16;
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Maps the hardware block id to a logical tile so that each XCD (blocks are
+// dispatched round-robin over 8 XCDs) works on a contiguous run of tiles;
+// tiles are ordered template-tile fastest, so the blocks that share one Bc
+// column panel run on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    const int xcd = bid & 7, pos = bid >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+}
+
+__global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
+    const float *__restrict__ At, int Kp, const float *__restrict__ Bc, int ldc, int Rp,
+    float *__restrict__ Bout, int ldb, int K, int nTm, int nTn) {
+    __shared__ float As[2][GBK][GBM];
+    __shared__ float Bs[2][GBK][GBN];
+    const int logical = xcd_remap(blockIdx.x, nTm * nTn);
+    const int tm = logical % nTm, tn = logical / nTm;
+    const int m0 = tm * GBM, n0 = tn * GBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // global->LDS mapping: 2 float4 per operand per thread (16 rows x 32 float4)
+    const int lr = tid >> 5, lc = (tid & 31) * 4;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    const int nk = Rp / GBK;
+    const float *ga = At + (size_t)lr * Kp + m0 + lc;
+    const float *gb = Bc + (size_t)lr * ldc + n0 + lc;
+    const size_t sa = (size_t)8 * Kp, sb = (size_t)8 * ldc;
+    float4 ra0 = *reinterpret_cast<const float4 *>(ga);
+    float4 ra1 = *reinterpret_cast<const float4 *>(ga + sa);
+    float4 rb0 = *reinterpret_cast<const float4 *>(gb);
+    float4 rb1 = *reinterpret_cast<const float4 *>(gb + sb);
+    *reinterpret_cast<float4 *>(&As[0][lr][lc]) = ra0;
+    *reinterpret_cast<float4 *>(&As[0][lr + 8][lc]) = ra1;
+    *reinterpret_cast<float4 *>(&Bs[0][lr][lc]) = rb0;
+    *reinterpret_cast<float4 *>(&Bs[0][lr + 8][lc]) = rb1;
+    __syncthreads();
+    const int kr = lane >> 5, kc = lane & 31;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            ga += (size_t)GBK * Kp;
+            gb += (size_t)GBK * ldc;
+            ra0 = *reinterpret_cast<const float4 *>(ga);
+            ra1 = *reinterpret_cast<const float4 *>(ga + sa);
+            rb0 = *reinterpret_cast<const float4 *>(gb);
+            rb1 = *reinterpret_cast<const float4 *>(gb + sb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 2) {
+            const float a0 = As[buf][kk + kr][wm * 64 + kc];
+            const float a1 = As[buf][kk + kr][wm * 64 + 32 + kc];
+            const float b0 = Bs[buf][kk + kr][wn * 64 + kc];
+            const float b1 = Bs[buf][kk + kr][wn * 64 + 32 + kc];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (more) {
+            const int nb = buf ^ 1;
+            *reinterpret_cast<float4 *>(&As[nb][lr][lc]) = ra0;
+            *reinterpret_cast<float4 *>(&As[nb][lr + 8][lc]) = ra1;
+            *reinterpret_cast<float4 *>(&Bs[nb][lr][lc]) = rb0;
+            *reinterpret_cast<float4 *>(&Bs[nb][lr + 8][lc]) = rb1;
+        }
+        __syncthreads();
+    }
+    // D layout of 32x32 MFMA: lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4,
+    // column l%32.
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+                if (m < K) Bout[(size_t)m * ldb + n] = acc[i][j][r];
+            }
+}
+
+struct ProjectLayout {
+    int R, Rp, Kp, ldc;
+    size_t bc_bytes, at_bytes;
+};
+
+ProjectLayout project_layout(int C, int T, int K, int P) {
+    ProjectLayout L;
+    L.R = P * C;
+    L.Rp = (int)ksm::align_up(L.R, GBK);
+    L.Kp = (int)ksm::align_up(K, GBM);
+    L.ldc = (int)ksm::align_up(T, GBN);
+    L.bc_bytes = (size_t)L.Rp * L.ldc * sizeof(float);
+    L.at_bytes = (size_t)L.Rp * L.Kp * sizeof(float);
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) {
+    (void)nt;
+    (void)precision;
+    if (C <= 0 || T <= 0 || K <= 0 || P <= 0) return 0;
+    const ProjectLayout L = project_layout(C, T, K, P);
+    return ksm::align_up(L.bc_bytes, 256) + ksm::align_up(L.at_bytes, 256) + 512;
+}
+
+int ksm_project(const float *X, int ldx, const float *W, const float *U, int C, int T, int K,
+                int P, int nt, float *B, int ldb, int precision, void *workspace,
+                size_t ws_bytes, void *stream) {
+    if (!X || !W || !U || !B || C <= 0 || T <= 0 || K <= 0 || P <= 0 || nt <= 0 ||
+        (nt % 2) == 0 || ldx < T)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_project: bad arguments");
+    if (precision != KSM_PREC_FP32)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_project: precision %d not built", precision);
+    const ProjectLayout L = project_layout(C, T, K, P);
+    if (ldb < L.ldc)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_project: ldb %d < %d", ldb, L.ldc);
+    if (!workspace || ws_bytes < ksm_project_workspace(C, T, K, P, nt, precision))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_project: workspace too small");
+    hipStream_t s = ksm::as_stream(stream);
+    ksm::Carver cv(workspace);
+    float *Bc = cv.take<float>((size_t)L.Rp * L.ldc);
+    float *At = cv.take<float>((size_t)L.Rp * L.Kp);
+
+    if (P == 6 && nt == 61) {
+        ksm::ProfileScope prof("conv_pc", s);
+        dim3 grid((L.ldc + kConvSpan - 1) / kConvSpan, C);
+        conv_pc_kernel<6, 61><<<grid, kConvThreads, 0, s>>>(X, ldx, W, C, T, L.ldc, Bc);
+    } else {
+        ksm::ProfileScope prof("conv_pc", s);
+        dim3 grid((L.ldc + 255) / 256, C);
+        conv_pc_generic<<<grid, 256, 0, s>>>(X, ldx, W, P, nt, C, T, L.ldc, Bc);
+    }
+    KSM_LAUNCH_CHECK("conv_pc_kernel");
+    if (L.Rp > L.R) {
+        zero_rows_kernel<<<64, 256, 0, s>>>(Bc + (size_t)L.R * L.ldc,
+                                            (size_t)(L.Rp - L.R) * L.ldc);
+        KSM_LAUNCH_CHECK("zero_rows_kernel");
+    }
+    transpose_pad_kernel<<<512, 256, 0, s>>>(U, K, L.R, L.Kp, L.Rp, At);
+    KSM_LAUNCH_CHECK("transpose_pad_kernel");
+    const int nTm = L.Kp / GBM, nTn = L.ldc / GBN;
+    ksm::ProfileScope prof("gemm_f32", s);
+    gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, B, ldb, K, nTm, nTn);
+    KSM_LAUNCH_CHECK("gemm_f32_mfma");
+    return KSM_OK;
+}
+
+}  // extern "C"

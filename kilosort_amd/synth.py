@@ -131,20 +131,21 @@ def make_spikes(rng, K, n_samples, fs, rate_range, amp_range, nt, nt0min,
         # move a fraction of the spikes next to a spike of a nearby template
         d = np.sqrt(((centres[:, None, :] - centres[None, :, :]) ** 2).sum(-1))
         lag = int(round(overlap_ms * fs / 1000.0))
-        pick = rng.random(len(times)) < overlap_frac
-        for s in np.nonzero(pick)[0]:
+        pick = np.nonzero(rng.random(len(times)) < overlap_frac)[0]
+        by_id = [times[ids == j].copy() for j in range(K)]
+        for s in pick:
             j = ids[s]
             nb = np.nonzero((d[j] < overlap_um) & (np.arange(K) != j))[0]
             if len(nb) == 0:
                 continue
-            partner = rng.choice(nb)
-            cand = np.nonzero(ids == partner)[0]
+            partner = nb[rng.integers(len(nb))]
+            cand = by_id[partner]
             if len(cand) == 0:
                 continue
             shift = 0
             while shift == 0:
                 shift = int(rng.integers(-lag, lag + 1))
-            times[s] = times[rng.choice(cand)] + shift
+            times[s] = cand[rng.integers(len(cand))] + shift
         # restore the refractory period within each train
         keep = np.ones(len(times), bool)
         order = np.lexsort((times, ids))
@@ -214,3 +215,26 @@ def config2_recording(seed: int = 2, n_batches: int = 1, NT: int = 60000,
     return make_recording(seed, 383, 700, n_batches * NT, rate_range=(1.0, 20.0),
                           amp_range=(8.0, 40.0), overlap_frac=0.2,
                           with_data=with_data)
+
+
+def synthesize_torch(n_samples, C, times, ids, amps, U, wPCA, nt0min, device,
+                     seed=0, noise_std=1.0, chunk=4096):
+    """Device-side version of ``synthesize`` for large workloads: returns a
+    [n_samples, C] float32 tensor on ``device`` (noise from torch's generator,
+    so values differ from the numpy version for the same seed)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    nt = wPCA.shape[1]
+    X = noise_std * torch.randn((C, n_samples), generator=g, device=device)
+    T = torch.from_numpy(template_waveforms(U, wPCA).astype(np.float32)).to(device)
+    starts = torch.from_numpy(np.asarray(times) - nt0min).to(device)
+    idt = torch.from_numpy(np.asarray(ids)).to(device)
+    amt = torch.from_numpy(np.asarray(amps, np.float32)).to(device)
+    lag = torch.arange(nt, device=device)
+    for a in range(0, len(starts), chunk):
+        s0 = starts[a:a + chunk]
+        vals = amt[a:a + chunk, None, None] * T[idt[a:a + chunk]]        # [S, C, nt]
+        idx = (s0[:, None] + lag[None, :]).reshape(-1)                   # [S*nt]
+        X.index_add_(1, idx, vals.permute(1, 0, 2).reshape(C, -1))
+    return X.T.contiguous()

@@ -1,0 +1,322 @@
+"""Drop-in replacement for Kilosort4's learned-template matching path.
+
+Same Python surface as ``kilosort/template_matching.py``:
+
+* ``prepare_extract(xc, yc, U, nC, position_limit, device)`` (ref :14-53)
+* ``extract(ops, bfile, U, device, progress_bar)``             (ref :56-126)
+* ``prepare_matching(ops, U)``                                  (ref :155-167)
+* ``run_matching(ops, X, U, ctc, device)``                      (ref :170-244)
+
+with the same ``ops`` read set (``nt``, ``Th_learned``, ``max_peels``,
+``wPCA``, ``nt0min``, ``batch_size``, ``xc``, ``yc``,
+``settings[nearest_chans, position_limit, n_pcs]``), the same ``ops`` writes
+(``iCC``, ``iCC_mask``, ``iU``) and the same outputs: ``st`` float64 numpy
+[N, 3] (time, template, th_amp), ``tF`` float32 CPU tensor [N, nC, P].
+
+All arithmetic runs in the HIP kernels of ``libksm.so`` on a ROCm GPU; torch
+provides device memory and the stream.  The one host computation is the
+nearest-channel table, which the reference also computes on the host with
+numpy (its unstable ``argsort`` tie order is reproduced by making the same
+call).  There is no CPU path: a CPU ``device`` or a missing library raises.
+
+Optional key ``ops['matching_precision']``: ``'fp32'`` (default).
+"""
+from __future__ import annotations
+
+import logging
+
+import numpy as np
+import torch
+
+from . import _lib
+
+logger = logging.getLogger(__name__)
+
+ST_CAPACITY = 100000          # per-batch spike buffer of run_matching (ref :186)
+
+
+def _lags(nt: int) -> int:
+    return 2 * nt + 1
+
+
+def _ldl(nt: int) -> int:
+    return (_lags(nt) + 3) // 4 * 4
+
+
+def _f32(t, device, name):
+    if not torch.is_tensor(t):
+        t = torch.as_tensor(np.asarray(t))
+    if t.device != device:
+        t = t.to(device)
+    if t.dtype != torch.float32:
+        raise TypeError(f"{name} must be float32 (got {t.dtype})")
+    return t.contiguous()
+
+
+def _check_U(U, device):
+    U = _f32(U, device, "U")
+    if U.ndim != 3:
+        raise ValueError(f"U must be [K, n_pcs, C] (got shape {tuple(U.shape)})")
+    return U
+
+
+class _Workspace:
+    """Grow-only device scratch buffers, one set per device."""
+
+    def __init__(self):
+        self._bufs = {}
+
+    def get(self, device, name, nbytes):
+        key = (str(device), name)
+        buf = self._bufs.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
+            self._bufs[key] = buf
+        return buf
+
+    def clear(self):
+        self._bufs.clear()
+
+
+_ws = _Workspace()
+
+
+class _GramCache:
+    """Per-Gram derived data: nonzero-row lists (and a native-layout copy when
+    the caller passes the reference's contiguous layout)."""
+
+    def __init__(self):
+        self.key = None
+        self.val = None
+
+    def get(self, ctc, K, nt, device):
+        key = (ctc.data_ptr(), ctc._version, tuple(ctc.shape), tuple(ctc.stride()))
+        if self.key == key:
+            return self.val
+        L = _lags(nt)
+        if tuple(ctc.shape) != (K, K, L):
+            raise ValueError(f"ctc must be [{K}, {K}, {L}] (got {tuple(ctc.shape)})")
+        s = ctc.stride()
+        if ctc.dtype == torch.float32 and s[2] == 1 and s[1] == K * s[0] and s[0] >= L \
+                and s[0] % 4 == 0:
+            base, ldl = ctc, s[0]            # firing-template-major storage
+        else:
+            ldl = _ldl(nt)
+            store = torch.zeros((K, K, ldl), dtype=torch.float32, device=device)
+            store[:, :, :L] = ctc.to(device, torch.float32).permute(1, 0, 2)
+            base = store[:, :, :L].permute(1, 0, 2)
+        rows = torch.empty((K, K), dtype=torch.int32, device=device)
+        nrows = torch.empty((K,), dtype=torch.int32, device=device)
+        lib = _lib.load()
+        _lib.check(lib.ksm_ctc_rows(base.data_ptr(), K, L, ldl, rows.data_ptr(),
+                                    nrows.data_ptr(), _lib.stream_of(device)),
+                   "ksm_ctc_rows")
+        self.key = key
+        self.val = (base, ldl, rows, nrows)
+        return self.val
+
+
+_grams = _GramCache()
+
+
+def prepare_extract(xc, yc, U, nC, position_limit, device=torch.device('cuda')):
+    """Nearest channels, position mask, peak channel and peak-channel PCs.
+
+    Reference ``template_matching.py:14-53``.  ``iCC``/``iCC_mask`` come from
+    the same numpy calls on the host (for the reference's argsort tie order);
+    ``iU`` and ``Ucc`` from ``ksm_prepare_extract``.
+    """
+    device = _lib.require_gpu(device)
+    xc = np.asarray(xc)
+    yc = np.asarray(yc)
+    ds = (xc - xc[:, np.newaxis]) ** 2 + (yc - yc[:, np.newaxis]) ** 2
+    iCC = torch.from_numpy(np.argsort(ds, 0)[:nC]).to(device)
+    iCC_mask = torch.from_numpy(np.sort(ds, 0)[:nC] < position_limit ** 2).to(device)
+    U = _check_U(U, device)
+    K, P, C = U.shape
+    if iCC.shape[1] != C:
+        raise ValueError(f"probe has {iCC.shape[1]} channels but U has {C}")
+    iCC = iCC.to(torch.int64).contiguous()
+    nC = iCC.shape[0]
+    iU = torch.empty((K,), dtype=torch.int64, device=device)
+    Ucc = torch.empty((nC, K, P), dtype=torch.float32, device=device)
+    lib = _lib.load()
+    _lib.check(lib.ksm_prepare_extract(U.data_ptr(), K, P, C, iCC.data_ptr(), nC,
+                                       iU.data_ptr(), Ucc.data_ptr(),
+                                       _lib.stream_of(device)),
+               "ksm_prepare_extract")
+    return iCC, iCC_mask, iU, Ucc
+
+
+def prepare_matching(ops, U):
+    """Template Gram ctc [K, K, 2*nt+1] (reference ``template_matching.py:155-167``).
+
+    Formed in float64 by ``ksm_prepare_matching`` and rounded to float32.  The
+    returned tensor indexes like the reference's (``ctc[i, j, l]``) and is a
+    permuted view of firing-template-major storage, the layout the peel reads.
+    """
+    W = ops['wPCA']
+    device = _lib.require_gpu(W.device if torch.is_tensor(W) else 'cuda')
+    W = _f32(W, device, "wPCA")
+    U = _check_U(U, device)
+    K, P, C = U.shape
+    nt = int(ops['nt'])
+    if tuple(W.shape) != (P, nt):
+        raise ValueError(f"wPCA must be [{P}, {nt}] (got {tuple(W.shape)})")
+    L, ldl = _lags(nt), _ldl(nt)
+    store = torch.empty((K, K, ldl), dtype=torch.float32, device=device)
+    lib = _lib.load()
+    nbytes = lib.ksm_prepare_matching_workspace(K, P, C, nt)
+    ws = _ws.get(device, "prepare", nbytes)
+    _lib.check(lib.ksm_prepare_matching(U.data_ptr(), W.data_ptr(), K, P, C, nt,
+                                        store.data_ptr(), ldl, ws.data_ptr(), nbytes,
+                                        _lib.stream_of(device)),
+               "ksm_prepare_matching")
+    return store[:, :, :L].permute(1, 0, 2)
+
+
+def _precision(ops):
+    name = str(ops.get('matching_precision', 'fp32')).lower()
+    if name == 'fp32':
+        return _lib.PREC_FP32
+    raise ValueError(f"unknown matching_precision {name!r}")
+
+
+def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
+    """Matching pursuit over one padded batch (reference ``template_matching.py:170-244``).
+
+    Returns ``st`` int64 [k, 2] (column, template), ``amps`` and ``th_amps``
+    float32 [k, 1], and the residual ``Xres`` [C, T], all on the device, in the
+    reference's order (by peel iteration, then by time).
+    """
+    device = _lib.require_gpu(device)
+    Th = float(ops['Th_learned'])
+    nt = int(ops['nt'])
+    max_peels = int(ops['max_peels'])
+    W = _f32(ops['wPCA'], device, "wPCA")
+    U = _check_U(U, device)
+    X = _f32(X, device, "X")
+    K, P, C = U.shape
+    if X.ndim != 2 or X.shape[0] != C:
+        raise ValueError(f"X must be [{C}, T] (got {tuple(X.shape)})")
+    T = X.shape[1]
+    prec = _precision(ops)
+    lib = _lib.load()
+    stream = _lib.stream_of(device)
+
+    base, ldl, rows, nrows = _grams.get(ctc, K, nt, device)
+    nm = torch.empty((K,), dtype=torch.float32, device=device)
+    _lib.check(lib.ksm_template_norms(U.data_ptr(), K, P, C, nm.data_ptr(), stream),
+               "ksm_template_norms")
+
+    ldb = (T + 127) // 128 * 128
+    B = _ws.get(device, "B", K * ldb * 4)
+    pbytes = lib.ksm_project_workspace(C, T, K, P, nt, prec)
+    pws = _ws.get(device, "project", pbytes)
+    _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), C, T, K, P, nt,
+                               B.data_ptr(), ldb, prec, pws.data_ptr(), pbytes, stream),
+               "ksm_project")
+
+    Xres = X.clone()
+    st = torch.empty((ST_CAPACITY, 2), dtype=torch.int64, device=device)
+    amps = torch.empty((ST_CAPACITY, 1), dtype=torch.float32, device=device)
+    th_amps = torch.empty((ST_CAPACITY, 1), dtype=torch.float32, device=device)
+    info = torch.zeros((4,), dtype=torch.int32, device=device)
+    kbytes = lib.ksm_peel_workspace(T, max_peels)
+    kws = _ws.get(device, "peel", kbytes)
+    _lib.check(lib.ksm_peel(B.data_ptr(), ldb, Xres.data_ptr(), T, U.data_ptr(), W.data_ptr(),
+                            nm.data_ptr(), base.data_ptr(), ldl, rows.data_ptr(),
+                            nrows.data_ptr(), K, P, C, T, nt, Th, max_peels, st.data_ptr(),
+                            amps.data_ptr(), th_amps.data_ptr(), ST_CAPACITY, info.data_ptr(),
+                            kws.data_ptr(), kbytes, stream),
+               "ksm_peel")
+    k, status, iters, _ = (int(v) for v in info.cpu())
+    if status == _lib.KSM_ERR_CAPACITY:
+        raise RuntimeError(f"more than {ST_CAPACITY} spikes in one batch "
+                           "(the reference's per-batch buffer overflows too)")
+    _lib.check(status, "ksm_peel (device)")
+    logger.debug("run_matching: %d spikes in %d peel iterations", k, iters)
+    return st[:k], amps[:k], th_amps[:k], Xres
+
+
+def _features(ops, Xres, st, amps, iCC, iU, Ucc, device):
+    nsp = st.shape[0]
+    nC, K, P = Ucc.shape
+    C, T = Xres.shape
+    tF = torch.empty((nsp, nC, P), dtype=torch.float32, device=device)
+    if nsp == 0:
+        return tF
+    W = _f32(ops['wPCA'], device, "wPCA")
+    lib = _lib.load()
+    _lib.check(lib.ksm_features(Xres.data_ptr(), T, W.data_ptr(), st.contiguous().data_ptr(),
+                                amps.contiguous().data_ptr(), nsp, iCC.data_ptr(),
+                                iU.data_ptr(), Ucc.data_ptr(), nC, K, P, C, int(ops['nt']),
+                                tF.data_ptr(), _lib.stream_of(device)),
+               "ksm_features")
+    return tF
+
+
+def _extract_batch(ops, X, U, ctc, iCC, iU, Ucc, device):
+    """run_matching + features of one padded batch; host copies of
+    (st int64 [k, 2], th_amps float32 [k, 1], tF float32 [k, nC, P])."""
+    stt, amps, th_amps, Xres = run_matching(ops, X, U, ctc, device=device)
+    xfeat = _features(ops, Xres, stt, amps, iCC, iU, Ucc, device)
+    return stt.cpu(), th_amps.cpu(), xfeat.cpu()
+
+
+def extract(ops, bfile, U, device=torch.device('cuda'), progress_bar=None):
+    """Spike extraction over every batch of ``bfile`` (reference ``template_matching.py:56-126``).
+
+    Returns ``(st, tF, ops)``: ``st`` float64 [N, 3] = (time in samples,
+    template id, th_amp) sorted by time with numpy's default argsort, ``tF``
+    float32 [N, nC, n_pcs] on the CPU; ``ops`` gains ``iCC``, ``iCC_mask``,
+    ``iU``.
+    """
+    device = _lib.require_gpu(device)
+    nC = ops['settings']['nearest_chans']
+    position_limit = ops['settings']['position_limit']
+    U = _check_U(U, device)
+    iCC, iCC_mask, iU, Ucc = prepare_extract(ops['xc'], ops['yc'], U, nC, position_limit,
+                                             device=device)
+    ops['iCC'] = iCC
+    ops['iCC_mask'] = iCC_mask
+    ops['iU'] = iU
+    nt = ops['nt']
+    ctc = prepare_matching(ops, U)
+    st = np.zeros((10 ** 6, 3), 'float64')
+    tF = torch.zeros((10 ** 6, iCC.shape[0], ops['settings']['n_pcs']))
+    k = 0
+    n_batches = int(bfile.n_batches)
+    ibatch, X, stt_h = -1, None, None
+    try:
+        for ibatch in range(n_batches):
+            if ibatch % 100 == 0:
+                logger.debug("extract: batch %d of %d", ibatch, n_batches)
+            X = bfile.padded_batch_to_torch(ibatch, ops)
+            stt_h, th_h, xfeat_h = _extract_batch(ops, X, U, ctc, iCC, iU, Ucc, device)
+            if ibatch == 0:
+                # times before the recording start (trough alignment) are dropped
+                neg = (stt_h[:, 0] - nt - nt // 2 + ops['nt0min']) < 0
+                stt_h, th_h, xfeat_h = stt_h[~neg], th_h[~neg], xfeat_h[~neg]
+            nsp = len(stt_h)
+            if k + nsp > st.shape[0]:
+                st = np.concatenate((st, np.zeros_like(st)), 0)
+                tF = torch.cat((tF, torch.zeros_like(tF)), 0)
+            t_shift = ibatch * bfile.batch_downsampling * (ops['batch_size'])
+            sd = stt_h.double()
+            st[k:k + nsp, 0] = ((sd[:, 0] - nt) + t_shift).numpy() - nt // 2 + ops['nt0min']
+            st[k:k + nsp, 1] = sd[:, 1].numpy()
+            st[k:k + nsp, 2] = th_h.double().numpy().reshape(-1)
+            tF[k:k + nsp] = xfeat_h
+            k += nsp
+            if progress_bar is not None:
+                progress_bar.emit(int((ibatch + 1) / n_batches * 100))
+    except Exception:
+        logger.exception(f'Error in template_matching.extract on batch {ibatch}')
+        logger.debug(f'X shape: {None if X is None else tuple(X.shape)}')
+        logger.debug(f'stt shape: {None if stt_h is None else tuple(stt_h.shape)}')
+        raise
+    isort = np.argsort(st[:k, 0])
+    st = st[isort]
+    tF = tF[isort]
+    return st, tF, ops

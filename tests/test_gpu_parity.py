@@ -1,0 +1,290 @@
+"""Parity of the HIP matching path with the CPU oracle and the golden vectors.
+
+Bars (from the path's specification):
+* spike times and template ids: identical (bit-exact integer outputs);
+* amps, th_amps: within 1e-5 relative;
+* PC features tF: within 1e-5 of each spike's largest feature magnitude
+  (norm-wise relative; features near zero carry only absolute error);
+* residual Xres: within 1e-5 of the batch's largest |X|;
+* the Gram ctc: within 1e-6 of its largest entry vs the float64 closed form.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from kilosort_amd import _lib, synth
+from kilosort_amd import template_matching as tm
+from kilosort_amd.batches import ArrayBatches
+from oracle import brute
+from oracle import tm_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda")
+CPU = torch.device("cpu")
+
+
+def say(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def make_ops(rec, NT, Th=8, max_peels=100, nC=10, position_limit=100.0, device=DEV):
+    return dict(nt=rec.nt, nt0min=rec.nt0min, Th_learned=Th, max_peels=max_peels,
+                batch_size=NT, wPCA=torch.from_numpy(rec.wPCA).to(device),
+                xc=rec.xc, yc=rec.yc,
+                settings=dict(nearest_chans=nC, position_limit=position_limit, n_pcs=6))
+
+
+def cpu_ops(ops):
+    out = dict(ops)
+    out["wPCA"] = ops["wPCA"].cpu()
+    return out
+
+
+def assert_spikes_equal(st_gpu, st_ref):
+    st_gpu = np.asarray(st_gpu)
+    st_ref = np.asarray(st_ref)
+    assert st_gpu.shape == st_ref.shape, (st_gpu.shape, st_ref.shape)
+    np.testing.assert_array_equal(st_gpu, st_ref)
+
+
+def rel_close(a, b, rtol):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    err = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+    assert err.size == 0 or err.max() <= rtol, err.max()
+
+
+def features_close(tf, tf_ref, rtol=1e-5):
+    tf = np.asarray(tf, np.float64)
+    tf_ref = np.asarray(tf_ref, np.float64)
+    assert tf.shape == tf_ref.shape
+    if tf.size == 0:
+        return
+    scale = np.abs(tf_ref).reshape(len(tf_ref), -1).max(1)
+    err = np.abs(tf - tf_ref).reshape(len(tf_ref), -1).max(1)
+    assert np.all(err <= rtol * np.maximum(scale, 1e-30)), (err / scale).max()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    _lib.load()
+    yield
+
+
+@pytest.mark.parametrize("C", [64, 383])
+def test_prepare_extract_matches_golden(golden_dir, C):
+    say(f"prepare_extract golden C={C}")
+    g = np.load(os.path.join(golden_dir, f"prepare_extract_c{C}.npz"))
+    xc, yc = synth.probe_2col(C)
+    U, _ = synth.make_templates(np.random.default_rng(int(g["seed"])), int(g["K"]), xc, yc)
+    iCC, mask, iU, Ucc = tm.prepare_extract(xc, yc, torch.from_numpy(U).to(DEV), 10, 100.0,
+                                            device=DEV)
+    np.testing.assert_array_equal(iCC.cpu().numpy(), g["iCC"])
+    np.testing.assert_array_equal(mask.cpu().numpy(), g["iCC_mask"])
+    np.testing.assert_array_equal(iU.cpu().numpy(), g["iU"])
+    np.testing.assert_array_equal(Ucc.cpu().numpy(), g["Ucc"])
+
+
+def test_prepare_matching_matches_closed_form():
+    say("prepare_matching closed form C=24 K=12")
+    rec = synth.make_recording(31, 24, 12, 300, with_data=False)
+    ops = make_ops(rec, 300)
+    ctc = tm.prepare_matching(ops, torch.from_numpy(rec.U).to(DEV))
+    assert tuple(ctc.shape) == (12, 12, 123)
+    got = ctc.cpu().numpy().astype(np.float64)
+    ref = brute.ctc_direct(rec.U, rec.wPCA)
+    assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max()
+    assert np.all(got[:, :, 0] == 0) and np.all(got[:, :, -1] == 0)
+    # indexing follows the reference: ctc[:, j, :] is the response to template j
+    o32 = O.prepare_matching(cpu_ops(ops), torch.from_numpy(rec.U)).numpy()
+    assert np.abs(got - o32).max() <= 1e-5 * np.abs(o32).max()
+
+
+def test_prepare_matching_sparse_rows_are_exact():
+    say("ctc row lists on a long probe")
+    rec = synth.make_recording(32, 383, 64, 300, with_data=False)
+    ops = make_ops(rec, 300)
+    U = torch.from_numpy(rec.U).to(DEV)
+    ctc = tm.prepare_matching(ops, U)
+    base, ldl, rows, nrows = tm._grams.get(ctc, 64, 61, DEV)
+    got = ctc.cpu().numpy()
+    nz = np.abs(got).sum(2) != 0                      # [i, j]
+    nrows = nrows.cpu().numpy()
+    rows = rows.cpu().numpy()
+    for j in range(64):
+        np.testing.assert_array_equal(rows[j, :nrows[j]], np.nonzero(nz[:, j])[0])
+    assert nrows.min() < 64                            # sparsity exists on 3.8 mm
+
+
+def test_projection_matches_float64():
+    say("projection C=64 K=64 T=60122 vs float64")
+    rec = synth.config1_recording(seed=41, n_batches=1)
+    NT = 60000
+    ops = make_ops(rec, NT)
+    bf = ArrayBatches(rec.data, NT=NT, nt=61, device=CPU)
+    X = bf.padded_batch_to_torch(0)
+    C, T = X.shape
+    K = rec.U.shape[0]
+    lib = _lib.load()
+    Xd = X.to(DEV).contiguous()
+    Ud = torch.from_numpy(rec.U).to(DEV)
+    Wd = ops["wPCA"]
+    ldb = (T + 127) // 128 * 128
+    B = torch.empty((K, ldb), device=DEV)
+    nb = lib.ksm_project_workspace(C, T, K, 6, 61, 0)
+    ws = torch.empty(nb, dtype=torch.uint8, device=DEV)
+    _lib.check(lib.ksm_project(Xd.data_ptr(), T, Wd.data_ptr(), Ud.data_ptr(), C, T, K, 6, 61,
+                               B.data_ptr(), ldb, 0, ws.data_ptr(), nb,
+                               _lib.stream_of(DEV)), "ksm_project")
+    got = B[:, :T].cpu().double().numpy()
+    ops64 = cpu_ops(ops)
+    ops64["wPCA"] = ops64["wPCA"].double()
+    ref = O.project(ops64, X.double(), torch.from_numpy(rec.U).double()).numpy()
+    assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_run_matching_matches_reference_golden(golden_dir):
+    say("run_matching golden C=16 K=8")
+    g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
+    ops = dict(nt=61, Th_learned=8, max_peels=100, wPCA=torch.from_numpy(g["wPCA"]).to(DEV))
+    U = torch.from_numpy(g["U"]).to(DEV)
+    X = torch.from_numpy(g["X"]).to(DEV)
+    ctc = tm.prepare_matching(ops, U)
+    st, amps, th, Xres = tm.run_matching(ops, X, U, ctc, device=DEV)
+    assert_spikes_equal(st.cpu().numpy(), g["st"])
+    rel_close(amps.cpu().numpy(), g["amps"], 1e-5)
+    rel_close(th.cpu().numpy(), g["th_amps"], 1e-5)
+    scale = np.abs(g["X"]).max()
+    assert np.abs(Xres.cpu().numpy() - g["Xres"]).max() <= 1e-5 * scale
+    # the input batch is not modified
+    np.testing.assert_array_equal(X.cpu().numpy(), g["X"])
+
+
+def test_run_matching_single_peel_golden(golden_dir):
+    say("run_matching golden max_peels=1")
+    g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
+    g1 = np.load(os.path.join(golden_dir, "run_matching_c16_k8_peel1.npz"))
+    ops = dict(nt=61, Th_learned=8, max_peels=1, wPCA=torch.from_numpy(g["wPCA"]).to(DEV))
+    U = torch.from_numpy(g["U"]).to(DEV)
+    ctc = tm.prepare_matching(ops, U)
+    st, amps, th, _ = tm.run_matching(ops, torch.from_numpy(g["X"]).to(DEV), U, ctc, device=DEV)
+    assert_spikes_equal(st.cpu().numpy(), g1["st"])
+    rel_close(amps.cpu().numpy(), g1["amps"], 1e-5)
+    rel_close(th.cpu().numpy(), g1["th_amps"], 1e-5)
+
+
+def test_run_matching_accepts_reference_layout_ctc(golden_dir):
+    say("run_matching with a contiguous reference-layout ctc")
+    g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
+    ops = dict(nt=61, Th_learned=8, max_peels=100, wPCA=torch.from_numpy(g["wPCA"]).to(DEV))
+    U = torch.from_numpy(g["U"]).to(DEV)
+    ctc = torch.from_numpy(g["ctc"]).to(DEV)          # the reference's own Gram
+    st, amps, th, _ = tm.run_matching(ops, torch.from_numpy(g["X"]).to(DEV), U, ctc, device=DEV)
+    assert_spikes_equal(st.cpu().numpy(), g["st"])
+    rel_close(amps.cpu().numpy(), g["amps"], 1e-5)
+
+
+def test_extract_matches_reference_golden(golden_dir):
+    say("extract golden C=24 K=12 3 batches")
+    g = np.load(os.path.join(golden_dir, "extract_c24_k12.npz"))
+    NT = int(g["NT"])
+    ops = dict(nt=61, nt0min=int(g["nt0min"]), Th_learned=8, max_peels=100, batch_size=NT,
+               wPCA=torch.from_numpy(g["wPCA"]).to(DEV), xc=g["xc"], yc=g["yc"],
+               settings=dict(nearest_chans=int(g["nC"]),
+                             position_limit=float(g["position_limit"]), n_pcs=6))
+    bf = ArrayBatches(g["data"], NT=NT, nt=61, device=DEV)
+    st, tF, ops = tm.extract(ops, bf, torch.from_numpy(g["U"]).to(DEV), device=DEV)
+    assert st.dtype == np.float64 and tF.dtype == torch.float32 and tF.device.type == "cpu"
+    assert_spikes_equal(st[:, :2], g["st"][:, :2])
+    rel_close(st[:, 2], g["st"][:, 2], 1e-5)
+    features_close(tF.numpy(), g["tF"])
+    np.testing.assert_array_equal(ops["iCC"].cpu().numpy(), g["iCC"])
+    np.testing.assert_array_equal(ops["iCC_mask"].cpu().numpy(), g["iCC_mask"])
+    np.testing.assert_array_equal(ops["iU"].cpu().numpy(), g["iU"])
+
+
+def test_extract_config1_matches_oracle():
+    """64 channels, 64 templates, 2 batches of 60000 (the CPU-runnable case)."""
+    say("extract config1 C=64 K=64 2x60000 vs oracle")
+    rec = synth.config1_recording(seed=1, n_batches=2)
+    NT = 60000
+    ops = make_ops(rec, NT)
+    U = torch.from_numpy(rec.U)
+    bf_cpu = ArrayBatches(rec.data, NT=NT, nt=61, device=CPU)
+    per_batch = []
+    st_ref, tF_ref, _ = O.extract(cpu_ops(ops), bf_cpu, U, per_batch=per_batch)
+    bf = ArrayBatches(rec.data, NT=NT, nt=61, device=DEV)
+    st, tF, _ = tm.extract(dict(ops), bf, U.to(DEV), device=DEV)
+    say(f"config1: {len(st)} spikes (oracle {len(st_ref)}, truth {len(rec.times)})")
+    assert len(st_ref) > 1000
+    assert_spikes_equal(st[:, :2], st_ref[:, :2])
+    rel_close(st[:, 2], st_ref[:, 2], 1e-5)
+    features_close(tF.numpy(), tF_ref.numpy())
+    # sanity against ground truth: most injected spikes are found at their time
+    found = set(map(tuple, st[:, :2].astype(np.int64)))
+    hit = sum((int(t), int(i)) in found for t, i in zip(rec.times, rec.ids))
+    assert hit / len(rec.times) > 0.8
+
+
+def test_no_spikes_and_untouched_residual():
+    say("zero batch")
+    rec = synth.make_recording(51, 32, 16, 300, with_data=False)
+    ops = make_ops(rec, 3000)
+    U = torch.from_numpy(rec.U).to(DEV)
+    X = torch.zeros((32, 3122), device=DEV)
+    st, amps, th, Xres = tm.run_matching(ops, X, U, tm.prepare_matching(ops, U), device=DEV)
+    assert st.shape == (0, 2) and amps.shape == (0, 1)
+    assert torch.equal(Xres, X)
+
+
+def test_capacity_overflow_raises(monkeypatch, golden_dir):
+    say("capacity overflow")
+    g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
+    ops = dict(nt=61, Th_learned=8, max_peels=100, wPCA=torch.from_numpy(g["wPCA"]).to(DEV))
+    U = torch.from_numpy(g["U"]).to(DEV)
+    monkeypatch.setattr(tm, "ST_CAPACITY", 5)
+    with pytest.raises(RuntimeError, match="spikes in one batch"):
+        tm.run_matching(ops, torch.from_numpy(g["X"]).to(DEV), U, tm.prepare_matching(ops, U),
+                        device=DEV)
+
+
+@pytest.mark.parametrize("max_peels", [1, 3])
+def test_peel_limit_matches_oracle(max_peels):
+    say(f"max_peels={max_peels}")
+    rec = synth.make_recording(52, 48, 40, 20000, rate_range=(20.0, 60.0),
+                               amp_range=(10.0, 30.0))
+    ops = make_ops(rec, 20000, max_peels=max_peels)
+    bf = ArrayBatches(rec.data, NT=20000, nt=61, device=CPU)
+    X = bf.padded_batch_to_torch(0)
+    U = torch.from_numpy(rec.U)
+    ref = O.run_matching(cpu_ops(ops), X, U, O.prepare_matching(cpu_ops(ops), U))
+    got = tm.run_matching(ops, X.to(DEV), U.to(DEV), tm.prepare_matching(ops, U.to(DEV)),
+                          device=DEV)
+    assert_spikes_equal(got[0].cpu().numpy(), ref[0].numpy())
+    rel_close(got[1].cpu().numpy(), ref[1].numpy(), 1e-5)
+
+
+def test_config2_batch_matches_oracle():
+    """One full-size batch: 383 channels, 700 templates, NT=60000."""
+    say("config2 one batch C=383 K=700 vs oracle (CPU oracle takes a while)")
+    rec = synth.config2_recording(seed=2, n_batches=1)
+    NT = 60000
+    ops = make_ops(rec, NT)
+    bf = ArrayBatches(rec.data, NT=NT, nt=61, device=CPU)
+    X = bf.padded_batch_to_torch(0)
+    U = torch.from_numpy(rec.U)
+    got = tm.run_matching(ops, X.to(DEV), U.to(DEV), tm.prepare_matching(ops, U.to(DEV)),
+                          device=DEV)
+    say(f"config2 GPU: {got[0].shape[0]} spikes")
+    ref = O.run_matching(cpu_ops(ops), X, U, O.prepare_matching(cpu_ops(ops), U))
+    say(f"config2 oracle: {ref[0].shape[0]} spikes")
+    assert_spikes_equal(got[0].cpu().numpy(), ref[0].numpy())
+    rel_close(got[1].cpu().numpy(), ref[1].numpy(), 1e-5)
+    rel_close(got[2].cpu().numpy(), ref[2].numpy(), 1e-5)
