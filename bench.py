@@ -117,6 +117,7 @@ def main():
     lib.ksm_profile_reset()
     lib.ksm_profile_enable(1)
     nsp = 0
+    iters = 0
     log(f"timing {args.steps} steps")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -124,12 +125,14 @@ def main():
         st, th, tf = tm._extract_batch(ops, batches[(args.warmup + i) % n_distinct], U, ctc,
                                        iCC, iU, Ucc, device)
         nsp += len(st)
+        iters += tm.last_peel_iterations
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     lib.ksm_profile_enable(0)
 
     kern = {}
-    for name in ("conv_pc", "gemm_f32", "colmax_all", "peel_select", "peel_update"):
+    for name in ("conv_pc", "gemm_f32", "colmax_all", "peel_select", "peel_update",
+                 "peel_xres"):
         ms, n = _lib.profile_query(name)
         kern[name] = (ms, n)
     lib.ksm_profile_reset()
@@ -176,6 +179,7 @@ def main():
                      "flops_per_launch": flops_gemm, "avg_launch_ms": gemm_avg * 1e3},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
         "spikes_per_batch": nsp / args.steps,
+        "peel_iterations_per_batch": iters / args.steps,
         "cpu_baseline": cpu,
         "agreement": agreement,
     }

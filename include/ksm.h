@@ -156,7 +156,8 @@ int ksm_features(const float *Xres, int ldx, const float *W,
  * When enabled, ksm_project and ksm_peel bracket their main kernels with HIP
  * event pairs on the launch stream.  ksm_profile_query synchronises those
  * events and returns the summed milliseconds and launch count of one kernel
- * ("conv_pc", "gemm_f32", "peel_select", "peel_update", "colmax_all").
+ * ("conv_pc", "gemm_f32", "colmax_all", "peel_select", "peel_update",
+ * "peel_xres").
  */
 int ksm_profile_enable(int on);
 int ksm_profile_reset(void);

@@ -5,30 +5,41 @@
 //
 // Device state per batch (carved from the caller's workspace):
 //   cfmax[T], imax[T]   best template score and id of every column
-//                        (Cf = relu(B)^2/nm, columns < nt and >= T-nt are 0)
+//                        (Cf = relu(B)^2/nm; columns < nt and >= T-nt are 0)
 //   slots               accepted spikes of the current iteration, per chunk
 //                        of kChunk columns, in increasing time
 //   state               done flag, status, iteration count, and kbase[it]
 //                        (spikes recorded before iteration it)
 //
-// One iteration = peel_select (window max, threshold, ordered compaction)
-// + peel_update (each block owns a slab of kSlab columns of B for all
-// templates: it applies the ctc updates of the spikes whose +-nt window
-// touches the slab -- even-ranked spikes first, as the reference's two
-// interleaved phases do -- then recomputes cfmax/imax of its columns, and
-// subtracts the templates of the spikes it owns from Xres).  Once an
-// iteration accepts nothing the done flag makes every later launch return
-// at once, so the host never waits inside the loop.
+// One iteration is three launches on the caller's stream:
+//   peel_select  window max, threshold and ordered compaction into slots;
+//   peel_update  each block owns a slab of kSlab columns of B for all
+//                templates.  In a single pass over the rows it subtracts
+//                amp * ctc of the (at most two) spikes whose +-nt window
+//                covers each column -- the even-ranked spike first, as the
+//                reference's two interleaved phases do -- and recomputes
+//                cfmax/imax of its columns; it also records the spikes whose
+//                column lies in the slab into st/amps/th_amps;
+//   peel_xres    subtracts amp * (U^T W) of the iteration's spikes from the
+//                residual, spike x channel-group parallel (the peel itself
+//                never reads the residual).
+// Once an iteration accepts nothing the done flag makes every later launch
+// return at once, so the host never waits inside the loop.
 #include "ksm_common.h"
 #include "ksm_profile.h"
 
 namespace {
 
-constexpr int kChunk = 1024;        // columns per select block (4 per thread)
-constexpr int kSlotCap = 32;        // accepted spikes per chunk (>= 1024/62)
+constexpr int kChunk = 256;         // columns per select block (one per thread)
+constexpr int kSlotCap = 16;        // accepted spikes per chunk (>= 256/62 + ties)
 constexpr int kSlab = 64;           // columns per update block (one per lane)
+constexpr int kUpdWaves = 8;        // waves per update block (rows split 8 ways)
+constexpr int kUpdThreads = kUpdWaves * 64;
+constexpr int kUnroll = 8;          // rows in flight per wave
 constexpr int kMaxTouch = 16;       // spikes touching one slab (<= 3 without ties)
 constexpr int kMaxNt = 255;
+constexpr int kMaxK = 8192;
+constexpr int kXresGrid = 1024;
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
 
@@ -56,44 +67,83 @@ __device__ __forceinline__ bool beats(float v, int i, float bv, int bi) {
     return v > bv || (v == bv && i < bi);
 }
 
-// cfmax/imax of columns [s0, s0 + kSlab) from all K rows of B.
-// 256 threads: lane = column, wave w scans rows w, w+4, ...
-__device__ void slab_colmax(const float *__restrict__ B, int ldb, const float *__restrict__ nm,
-                            int K, int s0, int T, int nt, float *__restrict__ cfmax,
-                            int *__restrict__ imax, float *s_val, int *s_idx) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+// Cf = relu(b)^2 / nm, rounded as torch rounds relu(B)**2 / nm.
+__device__ __forceinline__ float score(float b, float n) {
+    const float r = b > 0.f ? b : 0.f;
+    return (r * r) / n;
+}
+
+// Per-lane description of the (at most two) spikes whose ctc update covers
+// the lane's column, in application order.
+struct LaneSpikes {
+    const float *ca = nullptr, *cb = nullptr;        // ctc row base at this column
+    const uint32_t *ma = nullptr, *mb = nullptr;     // nonzero-row bitmaps
+    float aa = 0.f, ab = 0.f;
+};
+
+// One pass over rows wave, wave+kUpdWaves, ... of the lane's column: apply
+// the covering spikes' ctc updates (products rounded before the subtract),
+// store changed values, and track the best score (first index on ties).
+// Writes cfmax/imax of the slab's columns where `need` is set.
+__device__ __forceinline__ void column_pass(float *__restrict__ B, int ldb,
+                                            const float *__restrict__ nm, int K, int ldl, int s0,
+                                            int T, int nt, bool need, LaneSpikes ls,
+                            float *__restrict__ cfmax, int *__restrict__ imax, float *s_val,
+                            int *s_idx) {
+    const int lane = threadIdx.x & 63;
+    // wave index as a scalar: row indices, nm[i] and the row base addresses
+    // of B then live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = s0 + lane;
     const bool live = col >= nt && col < T - nt;
     float best = 0.f;
     int bi = 0x7fffffff;
-    if (live) {
-        const float *bp = B + col;
-        int i = wave;
-        if (i < K) {
-            const float b = bp[(size_t)i * ldb];
-            const float r = b > 0.f ? b : 0.f;
-            best = (r * r) / nm[i];
-            bi = i;
-            i += 4;
-        }
-        for (; i < K; i += 4) {
-            const float b = bp[(size_t)i * ldb];
-            const float r = b > 0.f ? b : 0.f;
-            const float cf = (r * r) / nm[i];
-            if (cf > best || (cf != cf && best == best)) {
-                best = cf;
-                bi = i;
+    if (need && col < T) {
+        float *bc = B + col;
+        for (int i0 = wave; i0 < K; i0 += kUpdWaves * kUnroll) {
+            float b[kUnroll], va[kUnroll], vb[kUnroll];
+            bool ha[kUnroll], hb[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int i = i0 + kUpdWaves * u;
+                const bool in = i < K;
+                ha[u] = in && ls.ma && ((ls.ma[i >> 5] >> (i & 31)) & 1u);
+                hb[u] = in && ls.mb && ((ls.mb[i >> 5] >> (i & 31)) & 1u);
+                b[u] = in ? bc[(size_t)i * ldb] : 0.f;
+                va[u] = ha[u] ? ls.ca[(size_t)i * ldl] : 0.f;
+                vb[u] = hb[u] ? ls.cb[(size_t)i * ldl] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; ++u) {
+                const int i = i0 + kUpdWaves * u;
+                if (i >= K) continue;
+                if (ha[u]) {
+                    const float prod = ls.aa * va[u];
+                    b[u] = b[u] - prod;
+                }
+                if (hb[u]) {
+                    const float prod = ls.ab * vb[u];
+                    b[u] = b[u] - prod;
+                }
+                if (ha[u] || hb[u]) bc[(size_t)i * ldb] = b[u];
+                if (live) {
+                    const float cf = score(b[u], nm[i]);
+                    if (bi == 0x7fffffff || cf > best || (cf != cf && best == best)) {
+                        best = cf;
+                        bi = i;
+                    }
+                }
             }
         }
     }
     s_val[threadIdx.x] = best;
     s_idx[threadIdx.x] = bi;
     __syncthreads();
-    if (wave == 0 && col < T) {
+    if (wave == 0 && need && col < T) {
         float v = s_val[lane];
         int vi = s_idx[lane];
 #pragma unroll
-        for (int w = 1; w < 4; ++w) {
+        for (int w = 1; w < kUpdWaves; ++w) {
             const float o = s_val[w * 64 + lane];
             const int oi = s_idx[w * 64 + lane];
             if (beats(o, oi, v, vi)) {
@@ -114,43 +164,56 @@ __global__ void peel_init_kernel(int *state, int n_state) {
     for (int e = threadIdx.x; e < n_state; e += blockDim.x) state[e] = 0;
 }
 
-__global__ __launch_bounds__(256) void colmax_all_kernel(const float *__restrict__ B, int ldb,
-                                                         const float *__restrict__ nm, int K,
-                                                         int T, int nt,
-                                                         float *__restrict__ cfmax,
-                                                         int *__restrict__ imax) {
-    __shared__ float s_val[256];
-    __shared__ int s_idx[256];
-    slab_colmax(B, ldb, nm, K, blockIdx.x * kSlab, T, nt, cfmax, imax, s_val, s_idx);
+__global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
+    float *__restrict__ B, int ldb, const float *__restrict__ nm, int K, int ldl, int T, int nt,
+    float *__restrict__ cfmax, int *__restrict__ imax) {
+    __shared__ float s_val[kUpdThreads];
+    __shared__ int s_idx[kUpdThreads];
+    LaneSpikes none;
+    column_pass(B, ldb, nm, K, ldl, blockIdx.x * kSlab, T, nt, true, none, cfmax, imax, s_val,
+                s_idx);
 }
 
 // Accepts columns whose score is the maximum of its +-nt window and exceeds
 // Th^2: (Cmax > Th^2) & (|Cmax - Cfmax| < 1e-9) with Cmax the window max
 // (template_matching.py:205-212), and records them in time order.
-__global__ __launch_bounds__(256) void peel_select_kernel(
+__global__ __launch_bounds__(kChunk) void peel_select_kernel(
     const float *__restrict__ cfmax, const int *__restrict__ imax, const float *__restrict__ B,
-    int ldb, const float *__restrict__ nm, int T, int nt, float th2, float th2_lo,
+    int ldb, const float *__restrict__ nm, int T, int nt, float th2, float th2_lo, int early_exit,
     int *__restrict__ state, int *__restrict__ chunk_count, Slot *__restrict__ slots) {
-    if (__builtin_nontemporal_load(&state[ST_DONE])) return;
+    if (state[ST_DONE]) return;
     __shared__ float win[kChunk + 2 * kMaxNt];
-    __shared__ int s_wc[4];
+    __shared__ int s_wc[kChunk / 64];
     const int t0 = blockIdx.x * kChunk;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < kChunk + 2 * nt; e += 256) {
+    for (int e = tid; e < kChunk + 2 * nt; e += kChunk) {
         const int x = t0 - nt + e;
         win[e] = (x >= 0 && x < T) ? cfmax[x] : -INFINITY;
     }
     __syncthreads();
-    int count = 0;
-    Slot *out = slots + (size_t)blockIdx.x * kSlotCap;
-    for (int u = 0; u < 4; ++u) {
-        const int x = t0 + u * 256 + tid;
-        bool acc = false;
-        float cmax = 0.f;
-        if (x >= nt && x < T - nt) {
-            const int e0 = x - t0 + nt;
-            const float v = win[e0];
-            if (v >= th2_lo) {
+    const int x = t0 + tid;
+    bool acc = false;
+    float cmax = 0.f;
+    if (x >= nt && x < T - nt) {
+        const int e0 = tid + nt;
+        const float v = win[e0];
+        if (v >= th2_lo) {
+            if (early_exit) {
+                // With v >= 2^-6 any larger neighbour differs from v by at
+                // least one ulp of v (> 1e-9), so the column is rejected as
+                // soon as one is found; scanning outwards finds it within a
+                // few samples for columns that are not a peak.
+                bool ok = true;
+                for (int d = 1; d <= nt; ++d) {
+                    const float a = win[e0 - d], b = win[e0 + d];
+                    if (a > v || b > v || a != a || b != b) {
+                        ok = false;
+                        break;
+                    }
+                }
+                cmax = v;
+                acc = ok && (v > th2);
+            } else {
                 cmax = v;
                 for (int d = -nt; d <= nt; ++d) {
                     const float o = win[e0 + d];
@@ -160,26 +223,27 @@ __global__ __launch_bounds__(256) void peel_select_kernel(
                 acc = (cmax > th2) && (fabsf(cmax - v) < 1e-9f);
             }
         }
-        const unsigned long long m = __ballot(acc);
-        if (lane == 0) s_wc[wave] = __popcll(m);
-        __syncthreads();
-        int off = count;
-        for (int w = 0; w < wave; ++w) off += s_wc[w];
-        const int total = s_wc[0] + s_wc[1] + s_wc[2] + s_wc[3];
-        if (acc) {
-            const int pos = off + __popcll(m & lanemask_lt(lane));
-            if (pos < kSlotCap) {
-                const int iy = imax[x];
-                Slot sl;
-                sl.x = x;
-                sl.iy = iy;
-                sl.amp = B[(size_t)iy * ldb + x] / nm[iy];
-                sl.tha = sqrtf(cmax);
-                out[pos] = sl;
-            }
+    }
+    const unsigned long long m = __ballot(acc);
+    if (lane == 0) s_wc[wave] = __popcll(m);
+    __syncthreads();
+    int off = 0, count = 0;
+#pragma unroll
+    for (int w = 0; w < kChunk / 64; ++w) {
+        if (w < wave) off += s_wc[w];
+        count += s_wc[w];
+    }
+    if (acc) {
+        const int pos = off + __popcll(m & lanemask_lt(lane));
+        if (pos < kSlotCap) {
+            const int iy = imax[x];
+            Slot sl;
+            sl.x = x;
+            sl.iy = iy;
+            sl.amp = B[(size_t)iy * ldb + x] / nm[iy];
+            sl.tha = sqrtf(cmax);
+            slots[(size_t)blockIdx.x * kSlotCap + pos] = sl;
         }
-        count += total;
-        __syncthreads();
     }
     if (tid == 0) {
         if (count > kSlotCap) {
@@ -191,20 +255,20 @@ __global__ __launch_bounds__(256) void peel_select_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void peel_update_kernel(
-    int it, float *__restrict__ B, int ldb, float *__restrict__ Xres, int ldx,
-    const float *__restrict__ U, const float *__restrict__ W, const float *__restrict__ nm,
+__global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
+    int it, float *__restrict__ B, int ldb, const float *__restrict__ nm,
     const float *__restrict__ ctc, int ldl, const int32_t *__restrict__ rows,
-    const int32_t *__restrict__ nrows, int K, int P, int C, int T, int nt, int nchunks,
+    const int32_t *__restrict__ nrows, int K, int T, int nt, int nchunks,
     const int *__restrict__ chunk_count, const Slot *__restrict__ slots, int *__restrict__ state,
     int64_t *__restrict__ st, float *__restrict__ amps, float *__restrict__ th_amps, int capacity,
     float *__restrict__ cfmax, int *__restrict__ imax) {
-    if (__builtin_nontemporal_load(&state[ST_DONE])) return;
+    if (state[ST_DONE]) return;
+    extern __shared__ uint32_t s_bits[];            // [n][nw] nonzero-row bitmaps
     __shared__ Slot s_sp[kMaxTouch];
     __shared__ int s_rank[kMaxTouch];
-    __shared__ int s_total, s_n;
-    __shared__ float s_val[256];
-    __shared__ int s_idx[256];
+    __shared__ int s_total, s_n, s_bad;
+    __shared__ float s_val[kUpdThreads];
+    __shared__ int s_idx[kUpdThreads];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int s0 = blockIdx.x * kSlab;
     const int s1 = min(s0 + kSlab, T);
@@ -223,27 +287,25 @@ __global__ __launch_bounds__(256) void peel_update_kernel(
         int n = 0;
         for (int c = c_lo; c <= c_hi; ++c) {
             const int cnt = chunk_count[c];
-            for (int b = 0; b < cnt; b += 64) {
-                const int q = b + lane;
-                Slot sl;
-                bool in = false;
-                if (q < cnt) {
-                    sl = slots[(size_t)c * kSlotCap + q];
-                    in = sl.x >= lo_x && sl.x <= hi_x;
-                }
-                const unsigned long long m = __ballot(in);
-                const int pos = n + __popcll(m & lanemask_lt(lane));
-                if (in && pos < kMaxTouch) {
-                    s_sp[pos] = sl;
-                    s_rank[pos] = pre + q;
-                }
-                n += __popcll(m);
+            Slot sl = {0, 0, 0.f, 0.f};
+            bool in = false;
+            if (lane < cnt) {
+                sl = slots[(size_t)c * kSlotCap + lane];
+                in = sl.x >= lo_x && sl.x <= hi_x;
             }
+            const unsigned long long m = __ballot(in);
+            const int pos = n + __popcll(m & lanemask_lt(lane));
+            if (in && pos < kMaxTouch) {
+                s_sp[pos] = sl;
+                s_rank[pos] = pre + lane;
+            }
+            n += __popcll(m);
             pre += cnt;
         }
         if (lane == 0) {
             s_total = tot;
             s_n = n;
+            s_bad = 0;
         }
     }
     __syncthreads();
@@ -284,44 +346,117 @@ __global__ __launch_bounds__(256) void peel_update_kernel(
             th_amps[pos] = sl.tha;
         }
     }
-    // B[:, x-nt .. x+nt] -= amp * ctc[:, id, :]: phase 0 (even rank), phase 1
+    // nonzero-row bitmaps of the touching spikes' templates
+    const int nw = (K + 31) >> 5;
+    for (int e = tid; e < n * nw; e += kUpdThreads) s_bits[e] = 0u;
+    __syncthreads();
+    for (int q = 0; q < n; ++q) {
+        const int j = s_sp[q].iy;
+        const int nr = nrows[j];
+        const int32_t *rl = rows + (size_t)j * K;
+        for (int r = tid; r < nr; r += kUpdThreads) {
+            const int i = rl[r];
+            atomicOr(&s_bits[q * nw + (i >> 5)], 1u << (i & 31));
+        }
+    }
+    // covering spikes of this lane's column, even rank (phase 0) first
     const int col = s0 + lane;
-    for (int ph = 0; ph < 2; ++ph) {
+    int qa = -1, qb = -1;
+    bool bad = false;
+    for (int ph = 0; ph < 2; ++ph)
         for (int q = 0; q < n; ++q) {
             if ((s_rank[q] & 1) != ph) continue;
-            const int x = s_sp[q].x, j = s_sp[q].iy;
-            const float a = s_sp[q].amp;
-            const bool inwin = col < s1 && col >= x - nt && col <= x + nt;
-            const int nr = nrows[j];
-            const int32_t *rl = rows + (size_t)j * K;
-            const float *cj = ctc + (size_t)j * K * ldl + (col - x + nt);
-            if (inwin) {
-                for (int r = wave; r < nr; r += 4) {
-                    const int i = rl[r];
-                    float *bp = B + (size_t)i * ldb + col;
-                    const float prod = a * cj[(size_t)i * ldl];
-                    *bp = *bp - prod;
-                }
+            const int x = s_sp[q].x;
+            if (col >= s1 || col < x - nt || col > x + nt) continue;
+            if (qa < 0) {
+                qa = q;
+            } else if (qb < 0 && (s_rank[qa] & 1) != ph) {
+                qb = q;
+            } else {
+                bad = true;     // two spikes of one phase cover the column: exact ties
             }
         }
-        __syncthreads();
+    if (bad) s_bad = 1;
+    __syncthreads();
+    if (s_bad) {
+        if (tid == 0) {
+            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
+            atomicExch(&state[ST_DONE], 1);
+        }
+        return;
     }
-    slab_colmax(B, ldb, nm, K, s0, T, nt, cfmax, imax, s_val, s_idx);
-    // Xres[:, x-nt/2 .. x+nt/2] -= amp * (U[id]^T W) for owned spikes
+    LaneSpikes ls;
+    if (qa >= 0) {
+        const Slot a = s_sp[qa];
+        ls.aa = a.amp;
+        ls.ca = ctc + (size_t)a.iy * K * ldl + (col - a.x + nt);
+        ls.ma = s_bits + qa * nw;
+    }
+    if (qb >= 0) {
+        const Slot b = s_sp[qb];
+        ls.ab = b.amp;
+        ls.cb = ctc + (size_t)b.iy * K * ldl + (col - b.x + nt);
+        ls.mb = s_bits + qb * nw;
+    }
+    column_pass(B, ldb, nm, K, ldl, s0, T, nt, qa >= 0, ls, cfmax, imax, s_val, s_idx);
+}
+
+// Xres[:, x-nt/2 .. x+nt/2] -= amp * (U[id]^T W) for every spike recorded in
+// iteration `it`.  Work item = (spike, group of 32 channels); lane = lag,
+// wave w takes channels w, w+4, ... of the group.  Channels where the
+// template is exactly zero are skipped (subtracting amp * 0 is a no-op).
+// Windows of one iteration's spikes are disjoint (they are > nt apart).
+__global__ __launch_bounds__(256) void peel_xres_kernel(
+    int it, float *__restrict__ Xres, int ldx, const float *__restrict__ U,
+    const float *__restrict__ W, int P, int C, int nt, const int *__restrict__ state,
+    const int64_t *__restrict__ st, const float *__restrict__ amps) {
+    if (state[ST_ITERS] != it + 1) return;
+    const int k0 = state[ST_KBASE + it];
+    const int ns = state[ST_KBASE + it + 1] - k0;
+    const int ngroups = (C + 31) >> 5;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int h = nt / 2;
-    for (int q = 0; q < n; ++q) {
-        const Slot sl = s_sp[q];
-        if (sl.x < s0 || sl.x >= s1) continue;
-        const float *uj = U + (size_t)sl.iy * P * C;
-        for (int c = wave; c < C; c += 4) {
-            float *xr = Xres + (size_t)c * ldx + sl.x - h;
-            for (int k = lane; k < nt; k += 64) {
-                float tm = 0.f;
-                for (int p = 0; p < P; ++p)
-                    tm = __builtin_fmaf(uj[(size_t)p * C + c], W[p * nt + k], tm);
-                const float prod = sl.amp * tm;
-                xr[k] = xr[k] - prod;
+    for (int item = blockIdx.x; item < ns * ngroups; item += gridDim.x) {
+        const int s = k0 + item / ngroups;
+        const int cg = (item % ngroups) * 32;
+        const int x = (int)st[2 * (size_t)s];
+        const int j = (int)st[2 * (size_t)s + 1];
+        const float a = amps[s];
+        const float *uj = U + (size_t)j * P * C;
+        for (int k0l = 0; k0l < nt; k0l += 64) {
+            const int k = k0l + lane;
+            const bool kin = k < nt;
+            float wk[8];
+#pragma unroll
+            for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
+            float *xb = Xres + x - h + k;
+            float tm[8], xv[8];
+            bool nz[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int c = cg + wave + 4 * u;
+                tm[u] = 0.f;
+                nz[u] = false;
+                if (c < C) {
+#pragma unroll
+                    for (int p = 0; p < 8; ++p) {
+                        if (p < P) {
+                            const float uu = uj[(size_t)p * C + c];
+                            nz[u] |= uu != 0.f;
+                            tm[u] = __builtin_fmaf(uu, wk[p], tm[u]);
+                        }
+                    }
+                }
             }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (nz[u] && kin) xv[u] = xb[(size_t)(cg + wave + 4 * u) * ldx];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (nz[u] && kin) {
+                    const float prod = a * tm[u];
+                    xb[(size_t)(cg + wave + 4 * u) * ldx] = xv[u] - prod;
+                }
         }
     }
 }
@@ -387,9 +522,9 @@ int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const floa
              int64_t *st, float *amps, float *th_amps, int capacity, int32_t *info,
              void *workspace, size_t ws_bytes, void *stream) {
     if (!B || !Xres || !U || !W || !nm || !ctc || !rows || !nrows || !st || !amps ||
-        !th_amps || !info || K <= 0 || P <= 0 || C <= 0 || nt <= 0 || nt > kMaxNt ||
-        (nt % 2) == 0 || T < 2 * nt + 1 || ldb < T || ldx < T || ldl < 2 * nt + 1 ||
-        max_peels < 0 || capacity < 0)
+        !th_amps || !info || K <= 0 || K > kMaxK || P <= 0 || P > 8 || C <= 0 || nt <= 0 ||
+        nt > kMaxNt || (nt % 2) == 0 || T < 2 * nt + 1 || ldb < T || ldx < T ||
+        ldl < 2 * nt + 1 || max_peels < 0 || capacity < 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
     if (!workspace || ws_bytes < ksm_peel_workspace(T, max_peels))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
@@ -408,25 +543,35 @@ int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const floa
     const float th2 = (float)th2d;
     // columns below th2_lo can never satisfy the acceptance test
     const float th2_lo = (float)(th2d - fmax(4e-9, fabs(th2d) * 1e-6));
+    // the early-exit window scan is exact when every candidate is >= 2^-6
+    const int early_exit = th2_lo >= 0.015625f ? 1 : 0;
+    const size_t bits_shm = (size_t)kMaxTouch * ((K + 31) / 32) * sizeof(uint32_t);
 
     peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
     KSM_LAUNCH_CHECK("peel_init_kernel");
     {
         ksm::ProfileScope prof("colmax_all", s);
-        colmax_all_kernel<<<L.nslabs, 256, 0, s>>>(B, ldb, nm, K, T, nt, cfmax, imax);
+        colmax_all_kernel<<<L.nslabs, kUpdThreads, 0, s>>>(B, ldb, nm, K, ldl, T, nt, cfmax,
+                                                           imax);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
     for (int it = 0; it < max_peels; ++it) {
         {
             ksm::ProfileScope prof("peel_select", s);
-            peel_select_kernel<<<L.nchunks, 256, 0, s>>>(cfmax, imax, B, ldb, nm, T, nt, th2,
-                                                         th2_lo, state, chunk_count, slots);
+            peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(cfmax, imax, B, ldb, nm, T, nt, th2,
+                                                            th2_lo, early_exit, state,
+                                                            chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", s);
-            peel_update_kernel<<<L.nslabs, 256, 0, s>>>(
-                it, B, ldb, Xres, ldx, U, W, nm, ctc, ldl, rows, nrows, K, P, C, T, nt,
-                L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax, imax);
+            peel_update_kernel<<<L.nslabs, kUpdThreads, bits_shm, s>>>(
+                it, B, ldb, nm, ctc, ldl, rows, nrows, K, T, nt, L.nchunks, chunk_count, slots,
+                state, st, amps, th_amps, capacity, cfmax, imax);
+        }
+        {
+            ksm::ProfileScope prof("peel_xres", s);
+            peel_xres_kernel<<<kXresGrid, 256, 0, s>>>(it, Xres, ldx, U, W, P, C, nt, state, st,
+                                                       amps);
         }
     }
     KSM_LAUNCH_CHECK("peel loop");

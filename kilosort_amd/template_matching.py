@@ -34,6 +34,9 @@ logger = logging.getLogger(__name__)
 
 ST_CAPACITY = 100000          # per-batch spike buffer of run_matching (ref :186)
 
+# peel iterations that accepted spikes in the last run_matching call
+last_peel_iterations = 0
+
 
 def _lags(nt: int) -> int:
     return 2 * nt + 1
@@ -235,6 +238,8 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
         raise RuntimeError(f"more than {ST_CAPACITY} spikes in one batch "
                            "(the reference's per-batch buffer overflows too)")
     _lib.check(status, "ksm_peel (device)")
+    global last_peel_iterations
+    last_peel_iterations = iters
     logger.debug("run_matching: %d spikes in %d peel iterations", k, iters)
     return st[:k], amps[:k], th_amps[:k], Xres
 
