@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-peel", action="store_true",
+                    help="also time the peel kernels with events (perturbs the timing)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the matching path is sharded by batch "
@@ -115,7 +117,8 @@ def main():
     torch.cuda.synchronize()
 
     lib.ksm_profile_reset()
-    lib.ksm_profile_enable(1)
+    # projection kernels only by default: no events inside the peel loop
+    lib.ksm_profile_enable(3 if args.profile_peel else 1)
     nsp = 0
     iters = 0
     log(f"timing {args.steps} steps")
@@ -131,8 +134,10 @@ def main():
     lib.ksm_profile_enable(0)
 
     kern = {}
-    for name in ("conv_pc", "gemm_f32", "colmax_all", "peel_select", "peel_update",
-                 "peel_xres"):
+    names = ["conv_pc", "gemm_f32"]
+    if args.profile_peel:
+        names += ["colmax_all", "peel_select", "peel_update"]
+    for name in names:
         ms, n = _lib.profile_query(name)
         kern[name] = (ms, n)
     lib.ksm_profile_reset()

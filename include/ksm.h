@@ -153,13 +153,14 @@ int ksm_features(const float *Xres, int ldx, const float *W,
 
 /*
  * Per-kernel timing (benchmark and test use; no reference counterpart).
- * When enabled, ksm_project and ksm_peel bracket their main kernels with HIP
- * event pairs on the launch stream.  ksm_profile_query synchronises those
- * events and returns the summed milliseconds and launch count of one kernel
- * ("conv_pc", "gemm_f32", "colmax_all", "peel_select", "peel_update",
- * "peel_xres").
+ * ksm_profile_enable(mask) turns on HIP event pairs on the launch stream
+ * around the kernels of the selected groups: bit 0 = projection
+ * ("conv_pc", "gemm_f32"), bit 1 = peel ("colmax_all", "peel_select",
+ * "peel_update"); 0 turns timing off.  ksm_profile_query synchronises the
+ * recorded events and returns the summed milliseconds and launch count of
+ * one kernel name.
  */
-int ksm_profile_enable(int on);
+int ksm_profile_enable(int mask);
 int ksm_profile_reset(void);
 int ksm_profile_query(const char *name, double *total_ms, int *launches);
 

@@ -11,18 +11,16 @@
 //   state               done flag, status, iteration count, and kbase[it]
 //                        (spikes recorded before iteration it)
 //
-// One iteration is three launches on the caller's stream:
+// One iteration is two launches on the caller's stream:
 //   peel_select  window max, threshold and ordered compaction into slots;
 //   peel_update  each block owns a slab of kSlab columns of B for all
 //                templates.  In a single pass over the rows it subtracts
 //                amp * ctc of the (at most two) spikes whose +-nt window
 //                covers each column -- the even-ranked spike first, as the
 //                reference's two interleaved phases do -- and recomputes
-//                cfmax/imax of its columns; it also records the spikes whose
-//                column lies in the slab into st/amps/th_amps;
-//   peel_xres    subtracts amp * (U^T W) of the iteration's spikes from the
-//                residual, spike x channel-group parallel (the peel itself
-//                never reads the residual).
+//                cfmax/imax of its columns; for the spikes whose column lies
+//                in the slab it records st/amps/th_amps and subtracts
+//                amp * (U^T W) from the residual.
 // Once an iteration accepts nothing the done flag makes every later launch
 // return at once, so the host never waits inside the loop.
 #include "ksm_common.h"
@@ -39,7 +37,6 @@ constexpr int kUnroll = 8;          // rows in flight per wave
 constexpr int kMaxTouch = 16;       // spikes touching one slab (<= 3 without ties)
 constexpr int kMaxNt = 255;
 constexpr int kMaxK = 8192;
-constexpr int kXresGrid = 1024;
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
 
@@ -256,7 +253,9 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
 }
 
 __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
-    int it, float *__restrict__ B, int ldb, const float *__restrict__ nm,
+    int it, float *__restrict__ B, int ldb, float *__restrict__ Xres, int ldx,
+    const float *__restrict__ U, const float *__restrict__ W, const int *__restrict__ crange,
+    int P, int C, const float *__restrict__ nm,
     const float *__restrict__ ctc, int ldl, const int32_t *__restrict__ rows,
     const int32_t *__restrict__ nrows, int K, int T, int nt, int nchunks,
     const int *__restrict__ chunk_count, const Slot *__restrict__ slots, int *__restrict__ state,
@@ -399,65 +398,75 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         ls.mb = s_bits + qb * nw;
     }
     column_pass(B, ldb, nm, K, ldl, s0, T, nt, qa >= 0, ls, cfmax, imax, s_val, s_idx);
-}
 
-// Xres[:, x-nt/2 .. x+nt/2] -= amp * (U[id]^T W) for every spike recorded in
-// iteration `it`.  Work item = (spike, group of 32 channels); lane = lag,
-// wave w takes channels w, w+4, ... of the group.  Channels where the
-// template is exactly zero are skipped (subtracting amp * 0 is a no-op).
-// Windows of one iteration's spikes are disjoint (they are > nt apart).
-__global__ __launch_bounds__(256) void peel_xres_kernel(
-    int it, float *__restrict__ Xres, int ldx, const float *__restrict__ U,
-    const float *__restrict__ W, int P, int C, int nt, const int *__restrict__ state,
-    const int64_t *__restrict__ st, const float *__restrict__ amps) {
-    if (state[ST_ITERS] != it + 1) return;
-    const int k0 = state[ST_KBASE + it];
-    const int ns = state[ST_KBASE + it + 1] - k0;
-    const int ngroups = (C + 31) >> 5;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // Xres[:, x-nt/2 .. x+nt/2] -= amp * (U[id]^T W) for the spikes owned by
+    // this slab (their windows are disjoint: accepted spikes are > nt apart).
+    // Lane = lag; the 8 waves take channels of the template's nonzero range
+    // (outside it the template is exactly zero and subtracting is a no-op).
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int h = nt / 2;
-    for (int item = blockIdx.x; item < ns * ngroups; item += gridDim.x) {
-        const int s = k0 + item / ngroups;
-        const int cg = (item % ngroups) * 32;
-        const int x = (int)st[2 * (size_t)s];
-        const int j = (int)st[2 * (size_t)s + 1];
-        const float a = amps[s];
-        const float *uj = U + (size_t)j * P * C;
-        for (int k0l = 0; k0l < nt; k0l += 64) {
-            const int k = k0l + lane;
+    for (int q = 0; q < n; ++q) {
+        const Slot sl = s_sp[q];
+        if (sl.x < s0 || sl.x >= s1) continue;
+        const int clo = crange[2 * sl.iy], chi = crange[2 * sl.iy + 1];
+        const float *uj = U + (size_t)sl.iy * P * C;
+        for (int kb0 = 0; kb0 < nt; kb0 += 64) {
+            const int k = kb0 + lane;
             const bool kin = k < nt;
             float wk[8];
 #pragma unroll
             for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
-            float *xb = Xres + x - h + k;
-            float tm[8], xv[8];
-            bool nz[8];
+            float *xb = Xres + sl.x - h + k;
+            for (int c0 = clo + wv; c0 < chi; c0 += kUpdWaves * 8) {
+                float tm[8], xv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int c = cg + wave + 4 * u;
-                tm[u] = 0.f;
-                nz[u] = false;
-                if (c < C) {
+                for (int u = 0; u < 8; ++u) {
+                    const int c = c0 + kUpdWaves * u;
+                    tm[u] = 0.f;
+                    if (c < chi) {
 #pragma unroll
-                    for (int p = 0; p < 8; ++p) {
-                        if (p < P) {
-                            const float uu = uj[(size_t)p * C + c];
-                            nz[u] |= uu != 0.f;
-                            tm[u] = __builtin_fmaf(uu, wk[p], tm[u]);
-                        }
+                        for (int p = 0; p < 8; ++p)
+                            if (p < P) tm[u] = __builtin_fmaf(uj[(size_t)p * C + c], wk[p], tm[u]);
+                        if (kin) xv[u] = xb[(size_t)c * ldx];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int c = c0 + kUpdWaves * u;
+                    if (c < chi && kin) {
+                        const float prod = sl.amp * tm[u];
+                        xb[(size_t)c * ldx] = xv[u] - prod;
                     }
                 }
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (nz[u] && kin) xv[u] = xb[(size_t)(cg + wave + 4 * u) * ldx];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (nz[u] && kin) {
-                    const float prod = a * tm[u];
-                    xb[(size_t)(cg + wave + 4 * u) * ldx] = xv[u] - prod;
-                }
         }
+    }
+}
+
+// crange[2j], crange[2j+1] = first and one-past-last channel where template
+// j has a nonzero component (an empty range is [0, 0)).
+__global__ __launch_bounds__(256) void channel_range_kernel(const float *__restrict__ U, int K,
+                                                            int P, int C,
+                                                            int *__restrict__ crange) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= K) return;
+    const float *u = U + (size_t)j * P * C;
+    int lo = C, hi = 0;
+    for (int c0 = 0; c0 < C; c0 += 64) {
+        const int c = c0 + lane;
+        bool nz = false;
+        if (c < C)
+            for (int p = 0; p < P; ++p) nz |= u[(size_t)p * C + c] != 0.f;
+        const unsigned long long m = __ballot(nz);
+        if (m) {
+            lo = min(lo, c0 + (int)__ffsll((long long)m) - 1);
+            hi = c0 + 64 - __clzll((long long)m);
+        }
+    }
+    if (lane == 0) {
+        crange[2 * j] = lo < hi ? lo : 0;
+        crange[2 * j + 1] = hi;
     }
 }
 
@@ -513,7 +522,8 @@ size_t ksm_peel_workspace(int T, int max_peels) {
     const size_t Tp = ksm::align_up(T, kSlab);
     return ksm::align_up(Tp * 4, 256) * 2 + ksm::align_up(L.nchunks * 4, 256) +
            ksm::align_up((size_t)L.nchunks * kSlotCap * sizeof(Slot), 256) +
-           ksm::align_up((size_t)L.n_state * 4, 256) + 256;
+           ksm::align_up((size_t)L.n_state * 4, 256) + ksm::align_up(2 * (size_t)kMaxK * 4, 256) +
+           256;
 }
 
 int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const float *W,
@@ -537,6 +547,7 @@ int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const floa
     int *chunk_count = cv.take<int>(L.nchunks);
     Slot *slots = cv.take<Slot>((size_t)L.nchunks * kSlotCap);
     int *state = cv.take<int>(L.n_state);
+    int *crange = cv.take<int>(2 * (size_t)K);
 
     // Th^2 as the reference forms it (Python float, compared in float32)
     const double th2d = (double)Th * (double)Th;
@@ -549,29 +560,26 @@ int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const floa
 
     peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
     KSM_LAUNCH_CHECK("peel_init_kernel");
+    channel_range_kernel<<<(K + 3) / 4, 256, 0, s>>>(U, K, P, C, crange);
+    KSM_LAUNCH_CHECK("channel_range_kernel");
     {
-        ksm::ProfileScope prof("colmax_all", s);
+        ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
         colmax_all_kernel<<<L.nslabs, kUpdThreads, 0, s>>>(B, ldb, nm, K, ldl, T, nt, cfmax,
                                                            imax);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
     for (int it = 0; it < max_peels; ++it) {
         {
-            ksm::ProfileScope prof("peel_select", s);
+            ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
             peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(cfmax, imax, B, ldb, nm, T, nt, th2,
                                                             th2_lo, early_exit, state,
                                                             chunk_count, slots);
         }
         {
-            ksm::ProfileScope prof("peel_update", s);
+            ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
             peel_update_kernel<<<L.nslabs, kUpdThreads, bits_shm, s>>>(
-                it, B, ldb, nm, ctc, ldl, rows, nrows, K, T, nt, L.nchunks, chunk_count, slots,
-                state, st, amps, th_amps, capacity, cfmax, imax);
-        }
-        {
-            ksm::ProfileScope prof("peel_xres", s);
-            peel_xres_kernel<<<kXresGrid, 256, 0, s>>>(it, Xres, ldx, U, W, P, C, nt, state, st,
-                                                       amps);
+                it, B, ldb, Xres, ldx, U, W, crange, P, C, nm, ctc, ldl, rows, nrows, K, T, nt,
+                L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax, imax);
         }
     }
     KSM_LAUNCH_CHECK("peel loop");

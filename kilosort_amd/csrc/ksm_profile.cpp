@@ -1,7 +1,7 @@
 // Optional per-kernel timing with HIP events on the launch stream.
 //
-// When enabled, the entry points bracket their main kernels with event
-// pairs; ksm_profile_query synchronises the recorded stop events and returns
+// When a kernel's group bit is enabled, the entry points bracket it with
+// event pairs; ksm_profile_query synchronises the recorded stop events and returns
 // the summed elapsed time of one kernel name.  Disabled by default, so the
 // normal path records nothing.
 #include <cstring>
@@ -20,7 +20,7 @@ struct Record {
 };
 
 std::mutex g_mu;
-bool g_on = false;
+unsigned g_mask = 0;
 std::vector<Record> g_records;
 std::vector<hipEvent_t> g_free;
 
@@ -39,9 +39,10 @@ hipEvent_t take_event() {
 
 namespace ksm {
 
-ProfileScope::ProfileScope(const char *name, hipStream_t s) : name_(name), stream_(s) {
+ProfileScope::ProfileScope(const char *name, unsigned group, hipStream_t s)
+    : name_(name), stream_(s) {
     std::lock_guard<std::mutex> lk(g_mu);
-    if (!g_on) return;
+    if (!(g_mask & group)) return;
     start_ = take_event();
     if (start_) hipEventRecord(start_, s);
 }
@@ -59,9 +60,9 @@ ProfileScope::~ProfileScope() {
 
 extern "C" {
 
-int ksm_profile_enable(int on) {
+int ksm_profile_enable(int mask) {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_on = on != 0;
+    g_mask = (unsigned)mask;
     return KSM_OK;
 }
 

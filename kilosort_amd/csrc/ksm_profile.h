@@ -5,9 +5,13 @@
 
 namespace ksm {
 
+// Profiling groups (bit mask of ksm_profile_enable).
+constexpr unsigned kProfProject = 1u;   // conv_pc, gemm_*
+constexpr unsigned kProfPeel = 2u;      // colmax_all, peel_*
+
 class ProfileScope {
   public:
-    ProfileScope(const char *name, hipStream_t s);
+    ProfileScope(const char *name, unsigned group, hipStream_t s);
     ~ProfileScope();
     ProfileScope(const ProfileScope &) = delete;
     ProfileScope &operator=(const ProfileScope &) = delete;

@@ -267,11 +267,11 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, int C, 
     float *At = cv.take<float>((size_t)L.Rp * L.Kp);
 
     if (P == 6 && nt == 61) {
-        ksm::ProfileScope prof("conv_pc", s);
+        ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
         dim3 grid((L.ldc + kConvSpan - 1) / kConvSpan, C);
         conv_pc_kernel<6, 61><<<grid, kConvThreads, 0, s>>>(X, ldx, W, C, T, L.ldc, Bc);
     } else {
-        ksm::ProfileScope prof("conv_pc", s);
+        ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
         dim3 grid((L.ldc + 255) / 256, C);
         conv_pc_generic<<<grid, 256, 0, s>>>(X, ldx, W, P, nt, C, T, L.ldc, Bc);
     }
@@ -284,7 +284,7 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, int C, 
     transpose_pad_kernel<<<512, 256, 0, s>>>(U, K, L.R, L.Kp, L.Rp, At);
     KSM_LAUNCH_CHECK("transpose_pad_kernel");
     const int nTm = L.Kp / GBM, nTn = L.ldc / GBN;
-    ksm::ProfileScope prof("gemm_f32", s);
+    ksm::ProfileScope prof("gemm_f32", ksm::kProfProject, s);
     gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, B, ldb, K, nTm, nTn);
     KSM_LAUNCH_CHECK("gemm_f32_mfma");
     return KSM_OK;
