@@ -96,6 +96,35 @@ int ksm_prepare_matching(const float *U, const float *W, int K, int P, int C,
 int ksm_ctc_rows(const float *ctc, int K, int L, int ldl, int32_t *rows,
                  int32_t *nrows, void *stream);
 
+/*
+ * Peel order of the templates: sorted by peak channel (argmax over channels
+ * of sum_p U[j,p,c]^2), stable in the template id.  perm[r] = template at
+ * position r, rank[j] = position of template j (int32 [K] each).  With
+ * channels in probe order this makes the nonzero rows of every template's
+ * Gram column a short contiguous range of positions.  No reference
+ * counterpart: an internal layout choice of the peel.
+ */
+int ksm_template_order(const float *U, int K, int P, int C, int32_t *perm,
+                       int32_t *rank, void *stream);
+
+/*
+ * Row ranges of the compact Gram: lohi[2r], lohi[2r+1] = [lo, hi) = smallest
+ * range of positions covering every template i with a nonzero ctc[i, perm[r], :]
+ * (from ksm_ctc_rows), and off int64 [K+1]: off[r] = sum_{r'<r} L*(hi-lo),
+ * off[K] = total floats of the compact Gram.
+ */
+int ksm_gram_ranges(const int32_t *rows, const int32_t *nrows, const int32_t *rank,
+                    int K, int L, int32_t *lohi, int64_t *off, void *stream);
+
+/*
+ * Compact lag-major Gram for the peel:
+ *   cc[off[r] + l*(hi-lo) + (q-lo)] = reference ctc[perm[q], perm[r], l],
+ * q in [lo, hi) of position r, l = 0..L-1 (L = 2*nt+1).
+ */
+int ksm_gram_compact(const float *ctc, int ldl, const int32_t *perm,
+                     const int32_t *lohi, const int64_t *off, int K, int L,
+                     float *cc, void *stream);
+
 /* Workspace bytes of ksm_project for the given sizes and precision. */
 size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision);
 
@@ -104,13 +133,15 @@ size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision);
  * each channel followed by the projection on the spatial components.
  * Replaces template_matching.py:180-181 (run_matching):
  *   B[i,t] = sum_p sum_c U[i,p,c] * sum_k W[p,k] * X[c, t + k - nt/2]
- * (zero padding outside [0, T)).  B is float32 [rows >= K][ldb], ldb >= T
- * rounded up to 128; columns >= T are written with values that the peel
- * never reads.
+ * (zero padding outside [0, T)).  Written column-major in peel order:
+ *   Bt[t*ldbt + r] = B[perm[r], t]   (perm = NULL: identity),
+ * Bt float32 with T rounded up to 128 rows (rows >= T hold scores of the zero
+ * padding), ldbt >= K and a multiple of 4.
  */
 int ksm_project(const float *X, int ldx, const float *W, const float *U,
-                int C, int T, int K, int P, int nt, float *B, int ldb,
-                int precision, void *workspace, size_t ws_bytes, void *stream);
+                const int32_t *perm, int C, int T, int K, int P, int nt, float *Bt,
+                int ldbt, int precision, void *workspace, size_t ws_bytes,
+                void *stream);
 
 /* Workspace bytes of ksm_peel. */
 size_t ksm_peel_workspace(int T, int max_peels);
@@ -124,19 +155,21 @@ size_t ksm_peel_workspace(int T, int max_peels);
  *   th_amps = sqrt(Cmax); subtract amps * (U^T W) from Xres and
  *   amps * ctc[:, id, :] from B in two interleaved phases (even-ranked
  *   spikes first), stopping early when nothing is accepted.
- * B (from ksm_project) and Xres (a copy of X) are updated in place.
+ * Bt (from ksm_project with the same perm) and Xres (a copy of X) are updated
+ * in place; nm float32 [K] by template id (ksm_template_norms); perm/rank from
+ * ksm_template_order; cc/lohi/off from ksm_gram_ranges + ksm_gram_compact.
  * Outputs: st int64 [capacity][2] (time column, template id), amps and
  * th_amps float32 [capacity], and info int32 [4] on the device:
  *   info[0] = spike count k, info[1] = ksm_status of the device run,
  *   info[2] = iterations that accepted spikes.
  * Spikes are ordered as the reference orders them: by iteration, then time.
  */
-int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U,
-             const float *W, const float *nm, const float *ctc, int ldl,
-             const int32_t *rows, const int32_t *nrows, int K, int P, int C,
-             int T, int nt, float Th, int max_peels, int64_t *st, float *amps,
-             float *th_amps, int capacity, int32_t *info, void *workspace,
-             size_t ws_bytes, void *stream);
+int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
+             const float *W, const float *nm, const int32_t *perm,
+             const int32_t *rank, const float *cc, const int32_t *lohi,
+             const int64_t *off, int K, int P, int C, int T, int nt, float Th,
+             int max_peels, int64_t *st, float *amps, float *th_amps, int capacity,
+             int32_t *info, void *workspace, size_t ws_bytes, void *stream);
 
 /*
  * PC features of each spike on the nC channels nearest its template's peak

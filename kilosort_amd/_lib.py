@@ -38,10 +38,13 @@ SIGNATURES = {
     "ksm_prepare_matching_workspace": (_sz, [_i, _i, _i, _i]),
     "ksm_prepare_matching": (_i, [_p, _p, _i, _i, _i, _i, _p, _i, _p, _sz, _p]),
     "ksm_ctc_rows": (_i, [_p, _i, _i, _i, _p, _p, _p]),
+    "ksm_template_order": (_i, [_p, _i, _i, _i, _p, _p, _p]),
+    "ksm_gram_ranges": (_i, [_p, _p, _p, _i, _i, _p, _p, _p]),
+    "ksm_gram_compact": (_i, [_p, _i, _p, _p, _p, _i, _i, _p, _p]),
     "ksm_project_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
-    "ksm_project": (_i, [_p, _i, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
+    "ksm_project": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
     "ksm_peel_workspace": (_sz, [_i, _i]),
-    "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _i, _p, _p, _i, _i, _i, _i, _i, _f,
+    "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
                       _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "ksm_profile_enable": (_i, [_i]),

@@ -3,8 +3,14 @@
 // Reference: kilosort/template_matching.py:183-244 (peel loop of
 // run_matching) and :85-86 (features in extract).
 //
+// Layout.  Scores are column-major, Bt[t][r] (ksm_project), over templates in
+// peel order r (perm[r] = template id, rank[id] = r; templates sorted by peak
+// channel), so one time column is one contiguous row of Bt and the rows a
+// spike's Gram update touches form one contiguous range [lo, hi).  The Gram
+// is the compact lag-major array of ksm_gram_compact.
+//
 // Device state per batch (carved from the caller's workspace):
-//   cfmax[T], imax[T]   best template score and id of every column
+//   cfmax[T], imax[T]   best score and template id of every column
 //                        (Cf = relu(B)^2/nm; columns < nt and >= T-nt are 0)
 //   slots               accepted spikes of the current iteration, per chunk
 //                        of kChunk columns, in increasing time
@@ -13,14 +19,14 @@
 //
 // One iteration is two launches on the caller's stream:
 //   peel_select  window max, threshold and ordered compaction into slots;
-//   peel_update  each block owns a slab of kSlab columns of B for all
-//                templates.  In a single pass over the rows it subtracts
-//                amp * ctc of the (at most two) spikes whose +-nt window
-//                covers each column -- the even-ranked spike first, as the
-//                reference's two interleaved phases do -- and recomputes
-//                cfmax/imax of its columns; for the spikes whose column lies
-//                in the slab it records st/amps/th_amps and subtracts
-//                amp * (U^T W) from the residual.
+//   peel_update  one block per slab of kSlab columns.  A wave takes one time
+//                column at a time (lanes over templates): it subtracts
+//                amp * ctc of the spikes whose +-nt window covers the column
+//                -- the even-ranked spike first, as the reference's two
+//                interleaved phases do -- writes the changed scores back, and
+//                reduces the column's best score across lanes.  The block
+//                also records the spikes whose column lies in the slab into
+//                st/amps/th_amps and subtracts their templates from Xres.
 // Once an iteration accepts nothing the done flag makes every later launch
 // return at once, so the host never waits inside the loop.
 #include "ksm_common.h"
@@ -30,13 +36,13 @@ namespace {
 
 constexpr int kChunk = 256;         // columns per select block (one per thread)
 constexpr int kSlotCap = 16;        // accepted spikes per chunk (>= 256/62 + ties)
-constexpr int kSlab = 64;           // columns per update block (one per lane)
-constexpr int kUpdWaves = 8;        // waves per update block (rows split 8 ways)
+constexpr int kSlab = 64;           // columns per update block
+constexpr int kUpdWaves = 8;        // waves per update block
 constexpr int kUpdThreads = kUpdWaves * 64;
-constexpr int kUnroll = 8;          // rows in flight per wave
+constexpr int kSlots = 12;          // 64-row slots of a column held in registers
 constexpr int kMaxTouch = 16;       // spikes touching one slab (<= 3 without ties)
 constexpr int kMaxNt = 255;
-constexpr int kMaxK = 8192;
+constexpr int kMaxK = 16384;
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
 
@@ -45,6 +51,13 @@ struct Slot {
     int iy;      // template id
     float amp;   // B[iy, x] / nm[iy]
     float tha;   // sqrt(Cmax[x])
+};
+
+// A touching spike as the column pass needs it.
+struct SpikeRef {
+    int x, lo, hi, w;       // column, Gram row range [lo, hi), w = hi - lo
+    float amp;
+    const float *cc;        // compact Gram block of the spike's template
 };
 
 __device__ __forceinline__ int wave_sum_i(int v) {
@@ -57,7 +70,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
 }
 
 // torch.max semantics for one reduction step: NaN wins, then larger value,
-// then the smaller index on equal values.
+// then the smaller template id on equal values.
 __device__ __forceinline__ bool beats(float v, int i, float bv, int bi) {
     const bool vn = v != v, bn = bv != bv;
     if (vn || bn) return vn && (!bn || i < bi);
@@ -70,90 +83,74 @@ __device__ __forceinline__ float score(float b, float n) {
     return (r * r) / n;
 }
 
-// Per-lane description of the (at most two) spikes whose ctc update covers
-// the lane's column, in application order.
-struct LaneSpikes {
-    const float *ca = nullptr, *cb = nullptr;        // ctc row base at this column
-    const uint32_t *ma = nullptr, *mb = nullptr;     // nonzero-row bitmaps
-    float aa = 0.f, ab = 0.f;
-};
-
-// One pass over rows wave, wave+kUpdWaves, ... of the lane's column: apply
-// the covering spikes' ctc updates (products rounded before the subtract),
-// store changed values, and track the best score (first index on ties).
-// Writes cfmax/imax of the slab's columns where `need` is set.
-__device__ __forceinline__ void column_pass(float *__restrict__ B, int ldb,
-                                            const float *__restrict__ nm, int K, int ldl, int s0,
-                                            int T, int nt, bool need, LaneSpikes ls,
-                            float *__restrict__ cfmax, int *__restrict__ imax, float *s_val,
-                            int *s_idx) {
+// One time column t (wave-uniform): apply the covering spikes' Gram updates
+// (products rounded before the subtract; spike a before spike b), store the
+// changed scores, and return the column's best (score, template id) in every
+// lane.  s_nm / s_pid: per-position norms and template ids in LDS.
+__device__ __forceinline__ void column_pass(float *__restrict__ Bt, int ldbt, int t, int K,
+                                            int nt, const float *s_nm, const int *s_pid,
+                                            const SpikeRef *a, const SpikeRef *b, float &best,
+                                            int &bid) {
     const int lane = threadIdx.x & 63;
-    // wave index as a scalar: row indices, nm[i] and the row base addresses
-    // of B then live in scalar registers
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int col = s0 + lane;
-    const bool live = col >= nt && col < T - nt;
-    float best = 0.f;
-    int bi = 0x7fffffff;
-    if (need && col < T) {
-        float *bc = B + col;
-        for (int i0 = wave; i0 < K; i0 += kUpdWaves * kUnroll) {
-            float b[kUnroll], va[kUnroll], vb[kUnroll];
-            bool ha[kUnroll], hb[kUnroll];
+    float *col = Bt + (size_t)t * ldbt;
+    const float *ca = a ? a->cc + (size_t)(t - a->x + nt) * a->w - a->lo : nullptr;
+    const float *cb = b ? b->cc + (size_t)(t - b->x + nt) * b->w - b->lo : nullptr;
+    best = -INFINITY;
+    bid = 0x7fffffff;
+    for (int r0 = 0; r0 < K; r0 += 64 * kSlots) {
+        float v[kSlots], va[kSlots], vb[kSlots];
 #pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const int i = i0 + kUpdWaves * u;
-                const bool in = i < K;
-                ha[u] = in && ls.ma && ((ls.ma[i >> 5] >> (i & 31)) & 1u);
-                hb[u] = in && ls.mb && ((ls.mb[i >> 5] >> (i & 31)) & 1u);
-                b[u] = in ? bc[(size_t)i * ldb] : 0.f;
-                va[u] = ha[u] ? ls.ca[(size_t)i * ldl] : 0.f;
-                vb[u] = hb[u] ? ls.cb[(size_t)i * ldl] : 0.f;
+        for (int k = 0; k < kSlots; ++k) {
+            const int r = r0 + 64 * k + lane;
+            const bool in = r < K;
+            const bool ha = a && r >= a->lo && r < a->hi;
+            const bool hb = b && r >= b->lo && r < b->hi;
+            v[k] = in ? col[r] : 0.f;
+            va[k] = ha ? ca[r] : 0.f;
+            vb[k] = hb ? cb[r] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k) {
+            const int r = r0 + 64 * k + lane;
+            if (r >= K) continue;
+            const bool ha = a && r >= a->lo && r < a->hi;
+            const bool hb = b && r >= b->lo && r < b->hi;
+            if (ha) {
+                const float prod = a->amp * va[k];
+                v[k] = v[k] - prod;
             }
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-                const int i = i0 + kUpdWaves * u;
-                if (i >= K) continue;
-                if (ha[u]) {
-                    const float prod = ls.aa * va[u];
-                    b[u] = b[u] - prod;
-                }
-                if (hb[u]) {
-                    const float prod = ls.ab * vb[u];
-                    b[u] = b[u] - prod;
-                }
-                if (ha[u] || hb[u]) bc[(size_t)i * ldb] = b[u];
-                if (live) {
-                    const float cf = score(b[u], nm[i]);
-                    if (bi == 0x7fffffff || cf > best || (cf != cf && best == best)) {
-                        best = cf;
-                        bi = i;
-                    }
-                }
+            if (hb) {
+                const float prod = b->amp * vb[k];
+                v[k] = v[k] - prod;
+            }
+            if (ha || hb) col[r] = v[k];
+            const float cf = score(v[k], s_nm[r]);
+            const int id = s_pid[r];
+            if (beats(cf, id, best, bid)) {
+                best = cf;
+                bid = id;
             }
         }
     }
-    s_val[threadIdx.x] = best;
-    s_idx[threadIdx.x] = bi;
-    __syncthreads();
-    if (wave == 0 && need && col < T) {
-        float v = s_val[lane];
-        int vi = s_idx[lane];
 #pragma unroll
-        for (int w = 1; w < kUpdWaves; ++w) {
-            const float o = s_val[w * 64 + lane];
-            const int oi = s_idx[w * 64 + lane];
-            if (beats(o, oi, v, vi)) {
-                v = o;
-                vi = oi;
-            }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bid, o, 64);
+        if (beats(ov, oi, best, bid)) {
+            best = ov;
+            bid = oi;
         }
-        if (!live) {
-            v = 0.f;
-            vi = 0;
-        }
-        cfmax[col] = v;
-        imax[col] = vi;
+    }
+}
+
+// Per-position norms and template ids into LDS.
+__device__ __forceinline__ void load_positions(const float *__restrict__ nm,
+                                               const int32_t *__restrict__ perm, int K,
+                                               float *s_nm, int *s_pid) {
+    for (int r = threadIdx.x; r < K; r += blockDim.x) {
+        const int id = perm[r];
+        s_pid[r] = id;
+        s_nm[r] = nm[id];
     }
 }
 
@@ -161,23 +158,45 @@ __global__ void peel_init_kernel(int *state, int n_state) {
     for (int e = threadIdx.x; e < n_state; e += blockDim.x) state[e] = 0;
 }
 
+// Best score of every column (first pass, no updates).
 __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
-    float *__restrict__ B, int ldb, const float *__restrict__ nm, int K, int ldl, int T, int nt,
-    float *__restrict__ cfmax, int *__restrict__ imax) {
-    __shared__ float s_val[kUpdThreads];
-    __shared__ int s_idx[kUpdThreads];
-    LaneSpikes none;
-    column_pass(B, ldb, nm, K, ldl, blockIdx.x * kSlab, T, nt, true, none, cfmax, imax, s_val,
-                s_idx);
+    float *__restrict__ Bt, int ldbt, const float *__restrict__ nm,
+    const int32_t *__restrict__ perm, int K, int T, int nt, float *__restrict__ cfmax,
+    int *__restrict__ imax) {
+    extern __shared__ unsigned char smem[];
+    float *s_nm = reinterpret_cast<float *>(smem);
+    int *s_pid = reinterpret_cast<int *>(s_nm + K);
+    load_positions(nm, perm, K, s_nm, s_pid);
+    __syncthreads();
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int s0 = blockIdx.x * kSlab;
+    for (int c = wv; c < kSlab; c += kUpdWaves) {
+        const int t = s0 + c;
+        if (t >= T) break;
+        float best;
+        int bid;
+        if (t >= nt && t < T - nt) {
+            column_pass(Bt, ldbt, t, K, nt, s_nm, s_pid, nullptr, nullptr, best, bid);
+        } else {
+            best = 0.f;
+            bid = 0;
+        }
+        if (lane == 0) {
+            cfmax[t] = best;
+            imax[t] = bid;
+        }
+    }
 }
 
 // Accepts columns whose score is the maximum of its +-nt window and exceeds
 // Th^2: (Cmax > Th^2) & (|Cmax - Cfmax| < 1e-9) with Cmax the window max
 // (template_matching.py:205-212), and records them in time order.
 __global__ __launch_bounds__(kChunk) void peel_select_kernel(
-    const float *__restrict__ cfmax, const int *__restrict__ imax, const float *__restrict__ B,
-    int ldb, const float *__restrict__ nm, int T, int nt, float th2, float th2_lo, int early_exit,
-    int *__restrict__ state, int *__restrict__ chunk_count, Slot *__restrict__ slots) {
+    const float *__restrict__ cfmax, const int *__restrict__ imax, const float *__restrict__ Bt,
+    int ldbt, const float *__restrict__ nm, const int32_t *__restrict__ rank, int T, int nt,
+    float th2, float th2_lo, int early_exit, int *__restrict__ state,
+    int *__restrict__ chunk_count, Slot *__restrict__ slots) {
     if (state[ST_DONE]) return;
     __shared__ float win[kChunk + 2 * kMaxNt];
     __shared__ int s_wc[kChunk / 64];
@@ -237,7 +256,7 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
             Slot sl;
             sl.x = x;
             sl.iy = iy;
-            sl.amp = B[(size_t)iy * ldb + x] / nm[iy];
+            sl.amp = Bt[(size_t)x * ldbt + rank[iy]] / nm[iy];
             sl.tha = sqrtf(cmax);
             slots[(size_t)blockIdx.x * kSlotCap + pos] = sl;
         }
@@ -253,28 +272,31 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
 }
 
 __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
-    int it, float *__restrict__ B, int ldb, float *__restrict__ Xres, int ldx,
+    int it, float *__restrict__ Bt, int ldbt, float *__restrict__ Xres, int ldx,
     const float *__restrict__ U, const float *__restrict__ W, const int *__restrict__ crange,
-    int P, int C, const float *__restrict__ nm,
-    const float *__restrict__ ctc, int ldl, const int32_t *__restrict__ rows,
-    const int32_t *__restrict__ nrows, int K, int T, int nt, int nchunks,
-    const int *__restrict__ chunk_count, const Slot *__restrict__ slots, int *__restrict__ state,
-    int64_t *__restrict__ st, float *__restrict__ amps, float *__restrict__ th_amps, int capacity,
-    float *__restrict__ cfmax, int *__restrict__ imax) {
+    int P, int C, const float *__restrict__ nm, const int32_t *__restrict__ perm,
+    const int32_t *__restrict__ rank, const float *__restrict__ cc,
+    const int32_t *__restrict__ lohi, const int64_t *__restrict__ off, int K, int T, int nt,
+    int nchunks, const int *__restrict__ chunk_count, const Slot *__restrict__ slots,
+    int *__restrict__ state, int64_t *__restrict__ st, float *__restrict__ amps,
+    float *__restrict__ th_amps, int capacity, float *__restrict__ cfmax,
+    int *__restrict__ imax) {
     if (state[ST_DONE]) return;
-    extern __shared__ uint32_t s_bits[];            // [n][nw] nonzero-row bitmaps
+    extern __shared__ unsigned char smem[];
+    float *s_nm = reinterpret_cast<float *>(smem);
+    int *s_pid = reinterpret_cast<int *>(s_nm + K);
     __shared__ Slot s_sp[kMaxTouch];
+    __shared__ SpikeRef s_ref[kMaxTouch];
     __shared__ int s_rank[kMaxTouch];
     __shared__ int s_total, s_n, s_bad;
-    __shared__ float s_val[kUpdThreads];
-    __shared__ int s_idx[kUpdThreads];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int s0 = blockIdx.x * kSlab;
     const int s1 = min(s0 + kSlab, T);
     const int lo_x = s0 - nt, hi_x = s1 - 1 + nt;
     const int c_lo = max(lo_x, 0) / kChunk;
     const int c_hi = min(hi_x, T - 1) / kChunk;
-    if (wave == 0) {
+    if (wv == 0) {
         int tot = 0, pre = 0;
         for (int c = lane; c < nchunks; c += 64) {
             const int v = chunk_count[c];
@@ -306,6 +328,13 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
             s_n = n;
             s_bad = 0;
         }
+    } else {
+        // the other waves stage the per-position norms and ids meanwhile
+        for (int r = tid - 64; r < K; r += kUpdThreads - 64) {
+            const int id = perm[r];
+            s_pid[r] = id;
+            s_nm[r] = nm[id];
+        }
     }
     __syncthreads();
     const int total = s_total;
@@ -334,76 +363,66 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         }
         return;
     }
-    // record the spikes whose column lies in this slab
     if (tid < n) {
         const Slot sl = s_sp[tid];
         if (sl.x >= s0 && sl.x < s1) {
+            // record the spikes whose column lies in this slab
             const size_t pos = (size_t)kb + s_rank[tid];
             st[2 * pos] = sl.x;
             st[2 * pos + 1] = sl.iy;
             amps[pos] = sl.amp;
             th_amps[pos] = sl.tha;
         }
+        const int jr = rank[sl.iy];
+        SpikeRef rf;
+        rf.x = sl.x;
+        rf.lo = lohi[2 * jr];
+        rf.hi = lohi[2 * jr + 1];
+        rf.w = rf.hi - rf.lo;
+        rf.amp = sl.amp;
+        rf.cc = cc + off[jr];
+        s_ref[tid] = rf;
     }
-    // nonzero-row bitmaps of the touching spikes' templates
-    const int nw = (K + 31) >> 5;
-    for (int e = tid; e < n * nw; e += kUpdThreads) s_bits[e] = 0u;
     __syncthreads();
-    for (int q = 0; q < n; ++q) {
-        const int j = s_sp[q].iy;
-        const int nr = nrows[j];
-        const int32_t *rl = rows + (size_t)j * K;
-        for (int r = tid; r < nr; r += kUpdThreads) {
-            const int i = rl[r];
-            atomicOr(&s_bits[q * nw + (i >> 5)], 1u << (i & 31));
-        }
-    }
-    // covering spikes of this lane's column, even rank (phase 0) first
-    const int col = s0 + lane;
-    int qa = -1, qb = -1;
-    bool bad = false;
-    for (int ph = 0; ph < 2; ++ph)
-        for (int q = 0; q < n; ++q) {
-            if ((s_rank[q] & 1) != ph) continue;
-            const int x = s_sp[q].x;
-            if (col >= s1 || col < x - nt || col > x + nt) continue;
-            if (qa < 0) {
-                qa = q;
-            } else if (qb < 0 && (s_rank[qa] & 1) != ph) {
-                qb = q;
-            } else {
-                bad = true;     // two spikes of one phase cover the column: exact ties
+    // time columns of the slab, one per wave at a time
+    for (int c = wv; c < kSlab; c += kUpdWaves) {
+        const int t = s0 + c;
+        if (t >= s1) break;
+        int qa = -1, qb = -1;
+        bool bad = false;
+        for (int ph = 0; ph < 2; ++ph)
+            for (int q = 0; q < n; ++q) {
+                if ((s_rank[q] & 1) != ph) continue;
+                const int x = s_ref[q].x;
+                if (t < x - nt || t > x + nt) continue;
+                if (qa < 0) {
+                    qa = q;
+                } else if (qb < 0 && (s_rank[qa] & 1) != ph) {
+                    qb = q;
+                } else {
+                    bad = true;     // two spikes of one phase cover the column: exact ties
+                }
             }
+        if (bad) {
+            if (lane == 0) s_bad = 1;
+            continue;
         }
-    if (bad) s_bad = 1;
-    __syncthreads();
-    if (s_bad) {
-        if (tid == 0) {
-            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
-            atomicExch(&state[ST_DONE], 1);
+        if (qa < 0) continue;       // no spike window covers this column
+        float best;
+        int bid;
+        column_pass(Bt, ldbt, t, K, nt, s_nm, s_pid, &s_ref[qa], qb >= 0 ? &s_ref[qb] : nullptr,
+                    best, bid);
+        if (lane == 0) {
+            const bool live = t >= nt && t < T - nt;
+            cfmax[t] = live ? best : 0.f;
+            imax[t] = live ? bid : 0;
         }
-        return;
     }
-    LaneSpikes ls;
-    if (qa >= 0) {
-        const Slot a = s_sp[qa];
-        ls.aa = a.amp;
-        ls.ca = ctc + (size_t)a.iy * K * ldl + (col - a.x + nt);
-        ls.ma = s_bits + qa * nw;
-    }
-    if (qb >= 0) {
-        const Slot b = s_sp[qb];
-        ls.ab = b.amp;
-        ls.cb = ctc + (size_t)b.iy * K * ldl + (col - b.x + nt);
-        ls.mb = s_bits + qb * nw;
-    }
-    column_pass(B, ldb, nm, K, ldl, s0, T, nt, qa >= 0, ls, cfmax, imax, s_val, s_idx);
 
     // Xres[:, x-nt/2 .. x+nt/2] -= amp * (U[id]^T W) for the spikes owned by
     // this slab (their windows are disjoint: accepted spikes are > nt apart).
-    // Lane = lag; the 8 waves take channels of the template's nonzero range
+    // Lane = lag; the waves take channels of the template's nonzero range
     // (outside it the template is exactly zero and subtracting is a no-op).
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int h = nt / 2;
     for (int q = 0; q < n; ++q) {
         const Slot sl = s_sp[q];
@@ -417,12 +436,13 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
 #pragma unroll
             for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
             float *xb = Xres + sl.x - h + k;
-            for (int c0 = clo + wv; c0 < chi; c0 += kUpdWaves * 8) {
-                float tm[8], xv[8];
+            for (int c0 = clo + wv; c0 < chi; c0 += kUpdWaves * 4) {
+                float tm[4], xv[4];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     const int c = c0 + kUpdWaves * u;
                     tm[u] = 0.f;
+                    xv[u] = 0.f;
                     if (c < chi) {
 #pragma unroll
                         for (int p = 0; p < 8; ++p)
@@ -431,7 +451,7 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
                     }
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     const int c = c0 + kUpdWaves * u;
                     if (c < chi && kin) {
                         const float prod = sl.amp * tm[u];
@@ -441,10 +461,15 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
             }
         }
     }
+    __syncthreads();
+    if (s_bad && tid == 0) {
+        atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
+        atomicExch(&state[ST_DONE], 1);
+    }
 }
 
 // crange[2j], crange[2j+1] = first and one-past-last channel where template
-// j has a nonzero component (an empty range is [0, 0)).
+// j has a nonzero component (an empty range is [0, 0)).  Wave per template.
 __global__ __launch_bounds__(256) void channel_range_kernel(const float *__restrict__ U, int K,
                                                             int P, int C,
                                                             int *__restrict__ crange) {
@@ -526,15 +551,15 @@ size_t ksm_peel_workspace(int T, int max_peels) {
            256;
 }
 
-int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const float *W,
-             const float *nm, const float *ctc, int ldl, const int32_t *rows,
-             const int32_t *nrows, int K, int P, int C, int T, int nt, float Th, int max_peels,
-             int64_t *st, float *amps, float *th_amps, int capacity, int32_t *info,
-             void *workspace, size_t ws_bytes, void *stream) {
-    if (!B || !Xres || !U || !W || !nm || !ctc || !rows || !nrows || !st || !amps ||
-        !th_amps || !info || K <= 0 || K > kMaxK || P <= 0 || P > 8 || C <= 0 || nt <= 0 ||
-        nt > kMaxNt || (nt % 2) == 0 || T < 2 * nt + 1 || ldb < T || ldx < T ||
-        ldl < 2 * nt + 1 || max_peels < 0 || capacity < 0)
+int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
+             const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
+             const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
+             float Th, int max_peels, int64_t *st, float *amps, float *th_amps, int capacity,
+             int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
+    if (!Bt || !Xres || !U || !W || !nm || !perm || !rank || !cc || !lohi || !off || !st ||
+        !amps || !th_amps || !info || K <= 0 || K > kMaxK || ldbt < K || P <= 0 || P > 8 ||
+        C <= 0 || nt <= 0 || nt > kMaxNt || (nt % 2) == 0 || T < 2 * nt + 1 || ldx < T ||
+        max_peels < 0 || capacity < 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
     if (!workspace || ws_bytes < ksm_peel_workspace(T, max_peels))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
@@ -556,7 +581,7 @@ int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const floa
     const float th2_lo = (float)(th2d - fmax(4e-9, fabs(th2d) * 1e-6));
     // the early-exit window scan is exact when every candidate is >= 2^-6
     const int early_exit = th2_lo >= 0.015625f ? 1 : 0;
-    const size_t bits_shm = (size_t)kMaxTouch * ((K + 31) / 32) * sizeof(uint32_t);
+    const size_t pos_shm = (size_t)K * (sizeof(float) + sizeof(int));
 
     peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
     KSM_LAUNCH_CHECK("peel_init_kernel");
@@ -564,22 +589,23 @@ int ksm_peel(float *B, int ldb, float *Xres, int ldx, const float *U, const floa
     KSM_LAUNCH_CHECK("channel_range_kernel");
     {
         ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
-        colmax_all_kernel<<<L.nslabs, kUpdThreads, 0, s>>>(B, ldb, nm, K, ldl, T, nt, cfmax,
-                                                           imax);
+        colmax_all_kernel<<<L.nslabs, kUpdThreads, pos_shm, s>>>(Bt, ldbt, nm, perm, K, T, nt,
+                                                                 cfmax, imax);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
     for (int it = 0; it < max_peels; ++it) {
         {
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
-            peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(cfmax, imax, B, ldb, nm, T, nt, th2,
-                                                            th2_lo, early_exit, state,
+            peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(cfmax, imax, Bt, ldbt, nm, rank, T,
+                                                            nt, th2, th2_lo, early_exit, state,
                                                             chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
-            peel_update_kernel<<<L.nslabs, kUpdThreads, bits_shm, s>>>(
-                it, B, ldb, Xres, ldx, U, W, crange, P, C, nm, ctc, ldl, rows, nrows, K, T, nt,
-                L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax, imax);
+            peel_update_kernel<<<L.nslabs, kUpdThreads, pos_shm, s>>>(
+                it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, nm, perm, rank, cc, lohi, off, K, T,
+                nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax,
+                imax);
         }
     }
     KSM_LAUNCH_CHECK("peel loop");

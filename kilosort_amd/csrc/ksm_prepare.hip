@@ -289,3 +289,190 @@ int ksm_ctc_rows(const float *ctc, int K, int L, int ldl, int32_t *rows, int32_t
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Peel-side template order and the compact Gram.
+//
+// The peel keeps the scores column-major (Bt[t][r]) over templates sorted by
+// peak channel, so that the rows a spike's ctc update touches form one short
+// contiguous range.  The compact Gram stores, for the template at position r,
+// only the rows [lo_r, hi_r) of that range, lag-major:
+//   cc[off[r] + l*(hi_r - lo_r) + (q - lo_r)] = ctc[q', perm[r], l],
+// with q' = perm[q] the template at position q.
+
+namespace {
+
+// Wave per template: peak channel = argmax_c sum_p U[j,p,c]^2 (first index).
+__global__ __launch_bounds__(1024) void order_kernel(const float *__restrict__ U, int K, int P,
+                                                     int C, int32_t *__restrict__ perm,
+                                                     int32_t *__restrict__ rank) {
+    extern __shared__ int s_mem[];
+    int *s_key = s_mem;          // [K]
+    int *s_hist = s_mem + K;     // [C]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int j = wave; j < K; j += nwave) {
+        const float *u = U + (size_t)j * P * C;
+        float best = -1.f;
+        int arg = 0;
+        for (int c0 = 0; c0 < C; c0 += 64) {
+            const int c = c0 + lane;
+            float s = -1.f;
+            if (c < C) {
+                s = 0.f;
+                for (int p = 0; p < P; ++p) {
+                    const float v = u[(size_t)p * C + c];
+                    s = s + v * v;
+                }
+            }
+            float v = s;
+            int a = c;
+            for (int o = 32; o > 0; o >>= 1) {
+                const float ov = __shfl_xor(v, o, 64);
+                const int oa = __shfl_xor(a, o, 64);
+                if (ov > v || (ov == v && oa < a)) {
+                    v = ov;
+                    a = oa;
+                }
+            }
+            if (v > best) {
+                best = v;
+                arg = a;
+            }
+        }
+        if (lane == 0) s_key[j] = arg;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // stable counting sort by peak channel
+        for (int c = 0; c < C; ++c) s_hist[c] = 0;
+        for (int j = 0; j < K; ++j) s_hist[s_key[j]]++;
+        int run = 0;
+        for (int c = 0; c < C; ++c) {
+            const int v = s_hist[c];
+            s_hist[c] = run;
+            run += v;
+        }
+        for (int j = 0; j < K; ++j) {
+            const int pos = s_hist[s_key[j]]++;
+            perm[pos] = j;
+            rank[j] = pos;
+        }
+    }
+}
+
+// Block per firing template j (original id): range of positions of its
+// nonzero Gram rows.
+__global__ __launch_bounds__(256) void gram_range_kernel(const int32_t *__restrict__ rows,
+                                                         const int32_t *__restrict__ nrows,
+                                                         const int32_t *__restrict__ rank, int K,
+                                                         int32_t *__restrict__ lohi) {
+    __shared__ int s_lo[256], s_hi[256];
+    const int j = blockIdx.x;
+    const int nr = nrows[j];
+    int lo = K, hi = 0;
+    for (int q = threadIdx.x; q < nr; q += 256) {
+        const int r = rank[rows[(size_t)j * K + q]];
+        lo = min(lo, r);
+        hi = max(hi, r + 1);
+    }
+    s_lo[threadIdx.x] = lo;
+    s_hi[threadIdx.x] = hi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s_lo[threadIdx.x] = min(s_lo[threadIdx.x], s_lo[threadIdx.x + o]);
+            s_hi[threadIdx.x] = max(s_hi[threadIdx.x], s_hi[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int jr = rank[j];
+        const bool empty = s_lo[0] >= s_hi[0];
+        lohi[2 * jr] = empty ? 0 : s_lo[0];
+        lohi[2 * jr + 1] = empty ? 0 : s_hi[0];
+    }
+}
+
+// off[r] = sum_{r' < r} L * (hi - lo), off[K] = total (one block).
+__global__ __launch_bounds__(1024) void gram_offsets_kernel(const int32_t *__restrict__ lohi,
+                                                            int K, int L,
+                                                            int64_t *__restrict__ off) {
+    __shared__ int64_t s_part[1024];
+    const int seg = (K + 1023) / 1024;
+    const int r0 = threadIdx.x * seg, r1 = min(K, r0 + seg);
+    int64_t sum = 0;
+    for (int r = r0; r < r1; ++r) sum += (int64_t)L * (lohi[2 * r + 1] - lohi[2 * r]);
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < 1024; ++t) {
+            const int64_t v = s_part[t];
+            s_part[t] = run;
+            run += v;
+        }
+        off[K] = run;
+    }
+    __syncthreads();
+    int64_t run = s_part[threadIdx.x];
+    for (int r = r0; r < r1; ++r) {
+        off[r] = run;
+        run += (int64_t)L * (lohi[2 * r + 1] - lohi[2 * r]);
+    }
+}
+
+// Block per position r: gather its row range of the Gram, lag-major.
+__global__ __launch_bounds__(256) void gram_compact_kernel(const float *__restrict__ ctc, int ldl,
+                                                           const int32_t *__restrict__ perm,
+                                                           const int32_t *__restrict__ lohi,
+                                                           const int64_t *__restrict__ off, int K,
+                                                           int L, float *__restrict__ cc) {
+    const int r = blockIdx.x;
+    const int j = perm[r];
+    const int lo = lohi[2 * r], w = lohi[2 * r + 1] - lo;
+    float *dst = cc + off[r];
+    const float *src = ctc + (size_t)j * K * ldl;
+    const int n = L * w;
+    for (int e = threadIdx.x; e < n; e += 256) {
+        const int l = e / w, q = e % w;
+        dst[e] = src[(size_t)perm[lo + q] * ldl + l];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ksm_template_order(const float *U, int K, int P, int C, int32_t *perm, int32_t *rank,
+                       void *stream) {
+    if (!U || !perm || !rank || K <= 0 || P <= 0 || C <= 0 || (size_t)(K + C) * 4 > 150 * 1024)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_template_order: bad arguments");
+    order_kernel<<<1, 1024, (size_t)(K + C) * sizeof(int), ksm::as_stream(stream)>>>(U, K, P, C,
+                                                                                    perm, rank);
+    KSM_LAUNCH_CHECK("order_kernel");
+    return KSM_OK;
+}
+
+int ksm_gram_ranges(const int32_t *rows, const int32_t *nrows, const int32_t *rank, int K, int L,
+                    int32_t *lohi, int64_t *off, void *stream) {
+    if (!rows || !nrows || !rank || !lohi || !off || K <= 0 || L <= 0)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_gram_ranges: bad arguments");
+    hipStream_t s = ksm::as_stream(stream);
+    gram_range_kernel<<<K, 256, 0, s>>>(rows, nrows, rank, K, lohi);
+    KSM_LAUNCH_CHECK("gram_range_kernel");
+    gram_offsets_kernel<<<1, 1024, 0, s>>>(lohi, K, L, off);
+    KSM_LAUNCH_CHECK("gram_offsets_kernel");
+    return KSM_OK;
+}
+
+int ksm_gram_compact(const float *ctc, int ldl, const int32_t *perm, const int32_t *lohi,
+                     const int64_t *off, int K, int L, float *cc, void *stream) {
+    if (!ctc || !perm || !lohi || !off || !cc || K <= 0 || L <= 0 || ldl < L)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_gram_compact: bad arguments");
+    gram_compact_kernel<<<K, 256, 0, ksm::as_stream(stream)>>>(ctc, ldl, perm, lohi, off, K, L,
+                                                               cc);
+    KSM_LAUNCH_CHECK("gram_compact_kernel");
+    return KSM_OK;
+}
+
+}  // extern "C"

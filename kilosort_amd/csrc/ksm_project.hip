@@ -103,15 +103,17 @@ __global__ __launch_bounds__(256) void conv_pc_generic(const float *__restrict__
     }
 }
 
-// At[r][Kp] = U[k][r] (k < K), 0 for K <= k < Kp or r >= R.
-__global__ void transpose_pad_kernel(const float *__restrict__ U, int K, int R, int Kp, int Rp,
-                                     float *__restrict__ At) {
+// At[r][Kp] = U[perm[k]][r] (k < K), 0 for K <= k < Kp or r >= R
+// (perm == nullptr: identity).
+__global__ void transpose_pad_kernel(const float *__restrict__ U, const int32_t *__restrict__ perm,
+                                     int K, int R, int Kp, int Rp, float *__restrict__ At) {
     const size_t n = (size_t)Rp * Kp;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x) {
         const int k = (int)(e % Kp);
         const int r = (int)(e / Kp);
-        At[e] = (k < K && r < R) ? U[(size_t)k * R + r] : 0.f;
+        const int src = (k < K && perm) ? perm[k] : k;
+        At[e] = (k < K && r < R) ? U[(size_t)src * R + r] : 0.f;
     }
 }
 
@@ -207,16 +209,29 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
         __syncthreads();
     }
     // D layout of 32x32 MFMA: lane l, register r -> row 8*(r/4) + 4*(l/32) + r%4,
-    // column l%32.
+    // column l%32.  The scores are stored transposed, Bt[n][m]: registers
+    // 4g..4g+3 of a lane are four consecutive templates of one time column.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            for (int g = 0; g < 4; ++g) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * g + 4 * (lane >> 5);
                 const int n = n0 + wn * 64 + j * 32 + (lane & 31);
-                if (m < K) Bout[(size_t)m * ldb + n] = acc[i][j][r];
+                float *dst = Bout + (size_t)n * ldb + m;
+                if (m + 3 < K) {
+                    float4 v;
+                    v.x = acc[i][j][4 * g + 0];
+                    v.y = acc[i][j][4 * g + 1];
+                    v.z = acc[i][j][4 * g + 2];
+                    v.w = acc[i][j][4 * g + 3];
+                    *reinterpret_cast<float4 *>(dst) = v;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (m + e < K) dst[e] = acc[i][j][4 * g + e];
+                }
             }
 }
 
@@ -248,17 +263,18 @@ size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) 
     return ksm::align_up(L.bc_bytes, 256) + ksm::align_up(L.at_bytes, 256) + 512;
 }
 
-int ksm_project(const float *X, int ldx, const float *W, const float *U, int C, int T, int K,
-                int P, int nt, float *B, int ldb, int precision, void *workspace,
-                size_t ws_bytes, void *stream) {
-    if (!X || !W || !U || !B || C <= 0 || T <= 0 || K <= 0 || P <= 0 || nt <= 0 ||
+int ksm_project(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
+                int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
+                void *workspace, size_t ws_bytes, void *stream) {
+    if (!X || !W || !U || !Bt || C <= 0 || T <= 0 || K <= 0 || P <= 0 || nt <= 0 ||
         (nt % 2) == 0 || ldx < T)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: bad arguments");
     if (precision != KSM_PREC_FP32)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: precision %d not built", precision);
     const ProjectLayout L = project_layout(C, T, K, P);
-    if (ldb < L.ldc)
-        return ksm::fail(KSM_ERR_INVALID, "ksm_project: ldb %d < %d", ldb, L.ldc);
+    if (ldbt < K || (ldbt % 4) != 0)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_project: ldbt %d must be >= K and a multiple of 4",
+                         ldbt);
     if (!workspace || ws_bytes < ksm_project_workspace(C, T, K, P, nt, precision))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_project: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
@@ -281,11 +297,11 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, int C, 
                                             (size_t)(L.Rp - L.R) * L.ldc);
         KSM_LAUNCH_CHECK("zero_rows_kernel");
     }
-    transpose_pad_kernel<<<512, 256, 0, s>>>(U, K, L.R, L.Kp, L.Rp, At);
+    transpose_pad_kernel<<<512, 256, 0, s>>>(U, perm, K, L.R, L.Kp, L.Rp, At);
     KSM_LAUNCH_CHECK("transpose_pad_kernel");
     const int nTm = L.Kp / GBM, nTn = L.ldc / GBN;
     ksm::ProfileScope prof("gemm_f32", ksm::kProfProject, s);
-    gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, B, ldb, K, nTm, nTn);
+    gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, Bt, ldbt, K, nTm, nTn);
     KSM_LAUNCH_CHECK("gemm_f32_mfma");
     return KSM_OK;
 }

@@ -84,16 +84,25 @@ class _Workspace:
 _ws = _Workspace()
 
 
+class _Gram:
+    """Peel-side form of one (templates, Gram) pair."""
+    __slots__ = ("base", "ldl", "rows", "nrows", "perm", "rank", "lohi", "off", "cc")
+
+
 class _GramCache:
-    """Per-Gram derived data: nonzero-row lists (and a native-layout copy when
-    the caller passes the reference's contiguous layout)."""
+    """Per-(U, ctc) derived data: the Gram in firing-template-major storage
+    (copied once when the caller passes the reference's contiguous layout),
+    its nonzero-row lists, the peel order of the templates (sorted by peak
+    channel) and the compact lag-major Gram over that order."""
 
     def __init__(self):
         self.key = None
         self.val = None
 
-    def get(self, ctc, K, nt, device):
-        key = (ctc.data_ptr(), ctc._version, tuple(ctc.shape), tuple(ctc.stride()))
+    def get(self, ctc, U, nt, device):
+        K, P, C = U.shape
+        key = (ctc.data_ptr(), ctc._version, tuple(ctc.shape), tuple(ctc.stride()),
+               U.data_ptr(), U._version, tuple(U.shape))
         if self.key == key:
             return self.val
         L = _lags(nt)
@@ -101,22 +110,39 @@ class _GramCache:
             raise ValueError(f"ctc must be [{K}, {K}, {L}] (got {tuple(ctc.shape)})")
         s = ctc.stride()
         if ctc.dtype == torch.float32 and s[2] == 1 and s[1] == K * s[0] and s[0] >= L \
-                and s[0] % 4 == 0:
+                and s[0] % 4 == 0 and ctc.device == device:
             base, ldl = ctc, s[0]            # firing-template-major storage
         else:
             ldl = _ldl(nt)
             store = torch.zeros((K, K, ldl), dtype=torch.float32, device=device)
             store[:, :, :L] = ctc.to(device, torch.float32).permute(1, 0, 2)
             base = store[:, :, :L].permute(1, 0, 2)
-        rows = torch.empty((K, K), dtype=torch.int32, device=device)
-        nrows = torch.empty((K,), dtype=torch.int32, device=device)
         lib = _lib.load()
-        _lib.check(lib.ksm_ctc_rows(base.data_ptr(), K, L, ldl, rows.data_ptr(),
-                                    nrows.data_ptr(), _lib.stream_of(device)),
-                   "ksm_ctc_rows")
+        stream = _lib.stream_of(device)
+        g = _Gram()
+        g.base, g.ldl = base, ldl
+        i32 = dict(dtype=torch.int32, device=device)
+        g.rows = torch.empty((K, K), **i32)
+        g.nrows = torch.empty((K,), **i32)
+        _lib.check(lib.ksm_ctc_rows(base.data_ptr(), K, L, ldl, g.rows.data_ptr(),
+                                    g.nrows.data_ptr(), stream), "ksm_ctc_rows")
+        g.perm = torch.empty((K,), **i32)
+        g.rank = torch.empty((K,), **i32)
+        _lib.check(lib.ksm_template_order(U.data_ptr(), K, P, C, g.perm.data_ptr(),
+                                          g.rank.data_ptr(), stream), "ksm_template_order")
+        g.lohi = torch.empty((K, 2), **i32)
+        g.off = torch.empty((K + 1,), dtype=torch.int64, device=device)
+        _lib.check(lib.ksm_gram_ranges(g.rows.data_ptr(), g.nrows.data_ptr(), g.rank.data_ptr(),
+                                       K, L, g.lohi.data_ptr(), g.off.data_ptr(), stream),
+                   "ksm_gram_ranges")
+        total = int(g.off[K].item())
+        g.cc = torch.empty((max(total, 1),), dtype=torch.float32, device=device)
+        _lib.check(lib.ksm_gram_compact(base.data_ptr(), ldl, g.perm.data_ptr(),
+                                        g.lohi.data_ptr(), g.off.data_ptr(), K, L,
+                                        g.cc.data_ptr(), stream), "ksm_gram_compact")
         self.key = key
-        self.val = (base, ldl, rows, nrows)
-        return self.val
+        self.val = g
+        return g
 
 
 _grams = _GramCache()
@@ -207,18 +233,20 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     lib = _lib.load()
     stream = _lib.stream_of(device)
 
-    base, ldl, rows, nrows = _grams.get(ctc, K, nt, device)
+    g = _grams.get(ctc, U, nt, device)
     nm = torch.empty((K,), dtype=torch.float32, device=device)
     _lib.check(lib.ksm_template_norms(U.data_ptr(), K, P, C, nm.data_ptr(), stream),
                "ksm_template_norms")
 
-    ldb = (T + 127) // 128 * 128
-    B = _ws.get(device, "B", K * ldb * 4)
+    # column-major scores over the peel order: Bt[t][r] = B[perm[r], t]
+    ldbt = (K + 63) // 64 * 64
+    rows_t = (T + 127) // 128 * 128
+    Bt = _ws.get(device, "Bt", rows_t * ldbt * 4)
     pbytes = lib.ksm_project_workspace(C, T, K, P, nt, prec)
     pws = _ws.get(device, "project", pbytes)
-    _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), C, T, K, P, nt,
-                               B.data_ptr(), ldb, prec, pws.data_ptr(), pbytes, stream),
-               "ksm_project")
+    _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), g.perm.data_ptr(),
+                               C, T, K, P, nt, Bt.data_ptr(), ldbt, prec, pws.data_ptr(),
+                               pbytes, stream), "ksm_project")
 
     Xres = X.clone()
     st = torch.empty((ST_CAPACITY, 2), dtype=torch.int64, device=device)
@@ -227,11 +255,12 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     info = torch.zeros((4,), dtype=torch.int32, device=device)
     kbytes = lib.ksm_peel_workspace(T, max_peels)
     kws = _ws.get(device, "peel", kbytes)
-    _lib.check(lib.ksm_peel(B.data_ptr(), ldb, Xres.data_ptr(), T, U.data_ptr(), W.data_ptr(),
-                            nm.data_ptr(), base.data_ptr(), ldl, rows.data_ptr(),
-                            nrows.data_ptr(), K, P, C, T, nt, Th, max_peels, st.data_ptr(),
-                            amps.data_ptr(), th_amps.data_ptr(), ST_CAPACITY, info.data_ptr(),
-                            kws.data_ptr(), kbytes, stream),
+    _lib.check(lib.ksm_peel(Bt.data_ptr(), ldbt, Xres.data_ptr(), T, U.data_ptr(), W.data_ptr(),
+                            nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),
+                            g.cc.data_ptr(), g.lohi.data_ptr(), g.off.data_ptr(), K, P, C, T,
+                            nt, Th, max_peels, st.data_ptr(), amps.data_ptr(),
+                            th_amps.data_ptr(), ST_CAPACITY, info.data_ptr(), kws.data_ptr(),
+                            kbytes, stream),
                "ksm_peel")
     k, status, iters, _ = (int(v) for v in info.cpu())
     if status == _lib.KSM_ERR_CAPACITY:

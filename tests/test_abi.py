@@ -66,8 +66,8 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     st = lib.ksm_template_norms(None, 0, 6, 10, None, None)
     assert st == _lib.KSM_ERR_INVALID
     assert b"ksm_template_norms" in lib.ksm_last_error()
-    st = lib.ksm_peel(None, 0, None, 0, None, None, None, None, 0, None, None, 1, 6, 4, 10, 61,
-                      8.0, 100, None, None, None, 10, None, None, 0, None)
+    st = lib.ksm_peel(None, 0, None, 0, None, None, None, None, None, None, None, None, 1, 6, 4,
+                      10, 61, 8.0, 100, None, None, None, 10, None, None, 0, None)
     assert st == _lib.KSM_ERR_INVALID
     with pytest.raises(ValueError):
         _lib.check(st, "ksm_peel")

@@ -108,19 +108,37 @@ def test_prepare_matching_matches_closed_form():
 
 
 def test_prepare_matching_sparse_rows_are_exact():
-    say("ctc row lists on a long probe")
+    say("ctc row lists, peel order and compact Gram on a long probe")
     rec = synth.make_recording(32, 383, 64, 300, with_data=False)
     ops = make_ops(rec, 300)
     U = torch.from_numpy(rec.U).to(DEV)
     ctc = tm.prepare_matching(ops, U)
-    base, ldl, rows, nrows = tm._grams.get(ctc, 64, 61, DEV)
+    g = tm._grams.get(ctc, U, 61, DEV)
     got = ctc.cpu().numpy()
     nz = np.abs(got).sum(2) != 0                      # [i, j]
-    nrows = nrows.cpu().numpy()
-    rows = rows.cpu().numpy()
+    nrows = g.nrows.cpu().numpy()
+    rows = g.rows.cpu().numpy()
     for j in range(64):
         np.testing.assert_array_equal(rows[j, :nrows[j]], np.nonzero(nz[:, j])[0])
     assert nrows.min() < 64                            # sparsity exists on 3.8 mm
+    # peel order: a permutation sorted by peak channel, stable in the id
+    perm = g.perm.cpu().numpy()
+    rank = g.rank.cpu().numpy()
+    np.testing.assert_array_equal(np.sort(perm), np.arange(64))
+    np.testing.assert_array_equal(rank[perm], np.arange(64))
+    peak = np.argmax((rec.U.astype(np.float32) ** 2).sum(1), 1)
+    np.testing.assert_array_equal(perm, np.argsort(peak, kind="stable"))
+    # compact Gram: every nonzero row lies in [lo, hi) and is copied exactly
+    lohi = g.lohi.cpu().numpy()
+    off = g.off.cpu().numpy()
+    cc = g.cc.cpu().numpy()
+    for r in range(64):
+        j = perm[r]
+        lo, hi = lohi[r]
+        assert set(rank[np.nonzero(nz[:, j])[0]]) <= set(range(lo, hi))
+        blk = cc[off[r]:off[r] + 123 * (hi - lo)].reshape(123, hi - lo)
+        np.testing.assert_array_equal(blk, got[perm[lo:hi], j, :].T)
+    assert off[64] < 64 * 64 * 123                      # smaller than the dense Gram
 
 
 def test_projection_matches_float64():
@@ -136,14 +154,15 @@ def test_projection_matches_float64():
     Xd = X.to(DEV).contiguous()
     Ud = torch.from_numpy(rec.U).to(DEV)
     Wd = ops["wPCA"]
-    ldb = (T + 127) // 128 * 128
-    B = torch.empty((K, ldb), device=DEV)
+    ldbt = (K + 63) // 64 * 64
+    Bt = torch.empty(((T + 127) // 128 * 128, ldbt), device=DEV)
     nb = lib.ksm_project_workspace(C, T, K, 6, 61, 0)
     ws = torch.empty(nb, dtype=torch.uint8, device=DEV)
-    _lib.check(lib.ksm_project(Xd.data_ptr(), T, Wd.data_ptr(), Ud.data_ptr(), C, T, K, 6, 61,
-                               B.data_ptr(), ldb, 0, ws.data_ptr(), nb,
+    _lib.check(lib.ksm_project(Xd.data_ptr(), T, Wd.data_ptr(), Ud.data_ptr(), None, C, T, K, 6,
+                               61, Bt.data_ptr(), ldbt, 0, ws.data_ptr(), nb,
                                _lib.stream_of(DEV)), "ksm_project")
-    got = B[:, :T].cpu().double().numpy()
+    B = Bt[:T, :K].T
+    got = B.cpu().double().numpy()
     ops64 = cpu_ops(ops)
     ops64["wPCA"] = ops64["wPCA"].double()
     ref = O.project(ops64, X.double(), torch.from_numpy(rec.U).double()).numpy()
