@@ -57,7 +57,7 @@ struct Slot {
 struct SpikeRef {
     int x, lo, hi, w;       // column, Gram row range [lo, hi), w = hi - lo
     float amp;
-    const float *cc;        // compact Gram block of the spike's template
+    long long off;          // start of the template's block in the compact Gram
 };
 
 __device__ __forceinline__ int wave_sum_i(int v) {
@@ -83,54 +83,86 @@ __device__ __forceinline__ float score(float b, float n) {
     return (r * r) / n;
 }
 
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ long long uni(long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// One covering spike, wave-uniform, in scalar registers: the Gram row of
+// this column for positions [lo, hi) is cc[base + lo .. base + hi).
+struct ColSpike {
+    long long base;
+    int lo, hi;
+    float amp;
+};
+
+__device__ __forceinline__ ColSpike col_spike(const SpikeRef &rf, int t, int nt) {
+    ColSpike cs;
+    cs.lo = uni(rf.lo);
+    cs.hi = uni(rf.hi);
+    cs.amp = uni(rf.amp);
+    cs.base = uni((long long)(rf.off + (long long)(t - rf.x + nt) * rf.w - rf.lo));
+    return cs;
+}
+
 // One time column t (wave-uniform): apply the covering spikes' Gram updates
-// (products rounded before the subtract; spike a before spike b), store the
-// changed scores, and return the column's best (score, template id) in every
-// lane.  s_nm / s_pid: per-position norms and template ids in LDS.
+// (products rounded before the subtract; spike a before spike b; an absent
+// spike has an empty range), store the changed scores, and return the
+// column's best (score, template id) in every lane.  s_nm / s_pid:
+// per-position norms and template ids in LDS.
 __device__ __forceinline__ void column_pass(float *__restrict__ Bt, int ldbt, int t, int K,
-                                            int nt, const float *s_nm, const int *s_pid,
-                                            const SpikeRef *a, const SpikeRef *b, float &best,
+                                            const float *__restrict__ cc, const float *s_nm,
+                                            const int *s_pid, ColSpike a, ColSpike b, float &best,
                                             int &bid) {
     const int lane = threadIdx.x & 63;
     float *col = Bt + (size_t)t * ldbt;
-    const float *ca = a ? a->cc + (size_t)(t - a->x + nt) * a->w - a->lo : nullptr;
-    const float *cb = b ? b->cc + (size_t)(t - b->x + nt) * b->w - b->lo : nullptr;
     best = -INFINITY;
     bid = 0x7fffffff;
+    // Loads are branch-free (indices clamped into valid ranges, values
+    // discarded where not needed) and all stores come after all loads, so no
+    // wait on the load counter has to cover a store of the same column.
+    const bool a_any = a.hi > a.lo, b_any = b.hi > b.lo;
     for (int r0 = 0; r0 < K; r0 += 64 * kSlots) {
         float v[kSlots], va[kSlots], vb[kSlots];
 #pragma unroll
         for (int k = 0; k < kSlots; ++k) {
             const int r = r0 + 64 * k + lane;
-            const bool in = r < K;
-            const bool ha = a && r >= a->lo && r < a->hi;
-            const bool hb = b && r >= b->lo && r < b->hi;
-            v[k] = in ? col[r] : 0.f;
-            va[k] = ha ? ca[r] : 0.f;
-            vb[k] = hb ? cb[r] : 0.f;
+            const int ra = a_any ? min(max(r, a.lo), a.hi - 1) : 0;
+            const int rb = b_any ? min(max(r, b.lo), b.hi - 1) : 0;
+            v[k] = col[min(r, K - 1)];
+            va[k] = cc[a.base + ra];
+            vb[k] = cc[b.base + rb];
         }
+        bool dirty[kSlots];
 #pragma unroll
         for (int k = 0; k < kSlots; ++k) {
             const int r = r0 + 64 * k + lane;
-            if (r >= K) continue;
-            const bool ha = a && r >= a->lo && r < a->hi;
-            const bool hb = b && r >= b->lo && r < b->hi;
-            if (ha) {
-                const float prod = a->amp * va[k];
-                v[k] = v[k] - prod;
-            }
-            if (hb) {
-                const float prod = b->amp * vb[k];
-                v[k] = v[k] - prod;
-            }
-            if (ha || hb) col[r] = v[k];
-            const float cf = score(v[k], s_nm[r]);
-            const int id = s_pid[r];
-            if (beats(cf, id, best, bid)) {
-                best = cf;
-                bid = id;
+            const bool ha = r >= a.lo && r < a.hi;
+            const bool hb = r >= b.lo && r < b.hi;
+            const float pa = a.amp * va[k];
+            const float pb = b.amp * vb[k];
+            float x = v[k];
+            x = ha ? x - pa : x;
+            x = hb ? x - pb : x;
+            v[k] = x;
+            dirty[k] = (ha || hb) && r < K;
+            if (r < K) {
+                const float cf = score(x, s_nm[r]);
+                const int id = s_pid[r];
+                if (beats(cf, id, best, bid)) {
+                    best = cf;
+                    bid = id;
+                }
             }
         }
+#pragma unroll
+        for (int k = 0; k < kSlots; ++k)
+            if (dirty[k]) col[r0 + 64 * k + lane] = v[k];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -177,7 +209,8 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
         float best;
         int bid;
         if (t >= nt && t < T - nt) {
-            column_pass(Bt, ldbt, t, K, nt, s_nm, s_pid, nullptr, nullptr, best, bid);
+            const ColSpike none = {0, 0, 0, 0.f};
+            column_pass(Bt, ldbt, t, K, Bt, s_nm, s_pid, none, none, best, bid);
         } else {
             best = 0.f;
             bid = 0;
@@ -380,7 +413,7 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         rf.hi = lohi[2 * jr + 1];
         rf.w = rf.hi - rf.lo;
         rf.amp = sl.amp;
-        rf.cc = cc + off[jr];
+        rf.off = off[jr];
         s_ref[tid] = rf;
     }
     __syncthreads();
@@ -410,8 +443,10 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         if (qa < 0) continue;       // no spike window covers this column
         float best;
         int bid;
-        column_pass(Bt, ldbt, t, K, nt, s_nm, s_pid, &s_ref[qa], qb >= 0 ? &s_ref[qb] : nullptr,
-                    best, bid);
+        const ColSpike none = {0, 0, 0, 0.f};
+        const ColSpike ca = col_spike(s_ref[qa], t, nt);
+        const ColSpike cb = qb >= 0 ? col_spike(s_ref[qb], t, nt) : none;
+        column_pass(Bt, ldbt, t, K, cc, s_nm, s_pid, ca, cb, best, bid);
         if (lane == 0) {
             const bool live = t >= nt && t < T - nt;
             cfmax[t] = live ? best : 0.f;
