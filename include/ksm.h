@@ -108,10 +108,11 @@ int ksm_template_order(const float *U, int K, int P, int C, int32_t *perm,
                        int32_t *rank, void *stream);
 
 /*
- * Row ranges of the compact Gram: lohi[2r], lohi[2r+1] = [lo, hi) = smallest
- * range of positions covering every template i with a nonzero ctc[i, perm[r], :]
- * (from ksm_ctc_rows), and off int64 [K+1]: off[r] = sum_{r'<r} L*(hi-lo),
- * off[K] = total floats of the compact Gram.
+ * Row ranges of the compact Gram: lohi[2r], lohi[2r+1] = [lo, hi) = the
+ * smallest range of positions covering every template i with a nonzero
+ * ctc[i, perm[r], :] (from ksm_ctc_rows), widened to multiples of 4 (hi may
+ * exceed K by up to 3; those rows are zero), and off int64 [K+1]:
+ * off[r] = sum_{r'<r} L*(hi-lo), off[K] = total floats of the compact Gram.
  */
 int ksm_gram_ranges(const int32_t *rows, const int32_t *nrows, const int32_t *rank,
                     int K, int L, int32_t *lohi, int64_t *off, void *stream);
@@ -119,7 +120,7 @@ int ksm_gram_ranges(const int32_t *rows, const int32_t *nrows, const int32_t *ra
 /*
  * Compact lag-major Gram for the peel:
  *   cc[off[r] + l*(hi-lo) + (q-lo)] = reference ctc[perm[q], perm[r], l],
- * q in [lo, hi) of position r, l = 0..L-1 (L = 2*nt+1).
+ * q in [lo, hi) of position r (0 for q >= K), l = 0..L-1 (L = 2*nt+1).
  */
 int ksm_gram_compact(const float *ctc, int ldl, const int32_t *perm,
                      const int32_t *lohi, const int64_t *off, int K, int L,

@@ -39,7 +39,7 @@ constexpr int kSlotCap = 16;        // accepted spikes per chunk (>= 256/62 + ti
 constexpr int kSlab = 64;           // columns per update block
 constexpr int kUpdWaves = 8;        // waves per update block
 constexpr int kUpdThreads = kUpdWaves * 64;
-constexpr int kSlots = 12;          // 64-row slots of a column held in registers
+constexpr int kSlots4 = 3;          // 256-position slots of a column held in registers
 constexpr int kMaxTouch = 16;       // spikes touching one slab (<= 3 without ties)
 constexpr int kMaxNt = 255;
 constexpr int kMaxK = 16384;
@@ -123,46 +123,53 @@ __device__ __forceinline__ void column_pass(float *__restrict__ Bt, int ldbt, in
     float *col = Bt + (size_t)t * ldbt;
     best = -INFINITY;
     bid = 0x7fffffff;
-    // Loads are branch-free (indices clamped into valid ranges, values
-    // discarded where not needed) and all stores come after all loads, so no
-    // wait on the load counter has to cover a store of the same column.
+    // Each lane holds 4 consecutive positions (16-byte loads; ldbt, the Gram
+    // ranges and their offsets are multiples of 4).  Loads are branch-free
+    // (indices clamped into valid ranges, values discarded where not needed)
+    // and all stores come after all loads, so no wait on the load counter has
+    // to cover a store of the same column.
     const bool a_any = a.hi > a.lo, b_any = b.hi > b.lo;
-    for (int r0 = 0; r0 < K; r0 += 64 * kSlots) {
-        float v[kSlots], va[kSlots], vb[kSlots];
+    for (int r0 = 0; r0 < K; r0 += 256 * kSlots4) {
+        float4 v[kSlots4], va[kSlots4], vb[kSlots4];
 #pragma unroll
-        for (int k = 0; k < kSlots; ++k) {
-            const int r = r0 + 64 * k + lane;
-            const int ra = a_any ? min(max(r, a.lo), a.hi - 1) : 0;
-            const int rb = b_any ? min(max(r, b.lo), b.hi - 1) : 0;
-            v[k] = col[min(r, K - 1)];
-            va[k] = cc[a.base + ra];
-            vb[k] = cc[b.base + rb];
+        for (int k = 0; k < kSlots4; ++k) {
+            const int r = r0 + 256 * k + 4 * lane;
+            const int ra = a_any ? min(max(r, a.lo), a.hi - 4) : 0;
+            const int rb = b_any ? min(max(r, b.lo), b.hi - 4) : 0;
+            v[k] = *reinterpret_cast<const float4 *>(col + min(r, ldbt - 4));
+            va[k] = *reinterpret_cast<const float4 *>(cc + a.base + ra);
+            vb[k] = *reinterpret_cast<const float4 *>(cc + b.base + rb);
         }
-        bool dirty[kSlots];
+        bool dirty[kSlots4];
 #pragma unroll
-        for (int k = 0; k < kSlots; ++k) {
-            const int r = r0 + 64 * k + lane;
+        for (int k = 0; k < kSlots4; ++k) {
+            const int r = r0 + 256 * k + 4 * lane;
             const bool ha = r >= a.lo && r < a.hi;
             const bool hb = r >= b.lo && r < b.hi;
-            const float pa = a.amp * va[k];
-            const float pb = b.amp * vb[k];
-            float x = v[k];
-            x = ha ? x - pa : x;
-            x = hb ? x - pb : x;
-            v[k] = x;
-            dirty[k] = (ha || hb) && r < K;
-            if (r < K) {
-                const float cf = score(x, s_nm[r]);
-                const int id = s_pid[r];
-                if (beats(cf, id, best, bid)) {
-                    best = cf;
-                    bid = id;
+            float x[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            const float ca[4] = {va[k].x, va[k].y, va[k].z, va[k].w};
+            const float cb[4] = {vb[k].x, vb[k].y, vb[k].z, vb[k].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float pa = a.amp * ca[e];
+                const float pb = b.amp * cb[e];
+                x[e] = ha ? x[e] - pa : x[e];
+                x[e] = hb ? x[e] - pb : x[e];
+                if (r + e < K) {
+                    const float cf = score(x[e], s_nm[r + e]);
+                    const int id = s_pid[r + e];
+                    if (beats(cf, id, best, bid)) {
+                        best = cf;
+                        bid = id;
+                    }
                 }
             }
+            v[k] = make_float4(x[0], x[1], x[2], x[3]);
+            dirty[k] = ha || hb;
         }
 #pragma unroll
-        for (int k = 0; k < kSlots; ++k)
-            if (dirty[k]) col[r0 + 64 * k + lane] = v[k];
+        for (int k = 0; k < kSlots4; ++k)
+            if (dirty[k]) *reinterpret_cast<float4 *>(col + r0 + 256 * k + 4 * lane) = v[k];
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -592,7 +599,8 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
              float Th, int max_peels, int64_t *st, float *amps, float *th_amps, int capacity,
              int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
     if (!Bt || !Xres || !U || !W || !nm || !perm || !rank || !cc || !lohi || !off || !st ||
-        !amps || !th_amps || !info || K <= 0 || K > kMaxK || ldbt < K || P <= 0 || P > 8 ||
+        !amps || !th_amps || !info || K <= 0 || K > kMaxK || ldbt < K || (ldbt % 4) != 0 ||
+        P <= 0 || P > 8 ||
         C <= 0 || nt <= 0 || nt > kMaxNt || (nt % 2) == 0 || T < 2 * nt + 1 || ldx < T ||
         max_peels < 0 || capacity < 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");

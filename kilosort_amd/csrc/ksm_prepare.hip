@@ -386,10 +386,12 @@ __global__ __launch_bounds__(256) void gram_range_kernel(const int32_t *__restri
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        // widened to multiples of 4 positions so that the peel can read the
+        // compact Gram with aligned 16-byte loads (the extra rows are zero)
         const int jr = rank[j];
         const bool empty = s_lo[0] >= s_hi[0];
-        lohi[2 * jr] = empty ? 0 : s_lo[0];
-        lohi[2 * jr + 1] = empty ? 0 : s_hi[0];
+        lohi[2 * jr] = empty ? 0 : (s_lo[0] & ~3);
+        lohi[2 * jr + 1] = empty ? 0 : ((s_hi[0] + 3) & ~3);
     }
 }
 
@@ -434,8 +436,8 @@ __global__ __launch_bounds__(256) void gram_compact_kernel(const float *__restri
     const float *src = ctc + (size_t)j * K * ldl;
     const int n = L * w;
     for (int e = threadIdx.x; e < n; e += 256) {
-        const int l = e / w, q = e % w;
-        dst[e] = src[(size_t)perm[lo + q] * ldl + l];
+        const int l = e / w, q = lo + e % w;
+        dst[e] = q < K ? src[(size_t)perm[q] * ldl + l] : 0.f;
     }
 }
 

@@ -136,7 +136,7 @@ class _GramCache:
                                        K, L, g.lohi.data_ptr(), g.off.data_ptr(), stream),
                    "ksm_gram_ranges")
         total = int(g.off[K].item())
-        g.cc = torch.empty((max(total, 1),), dtype=torch.float32, device=device)
+        g.cc = torch.zeros((max(total, 4),), dtype=torch.float32, device=device)
         _lib.check(lib.ksm_gram_compact(base.data_ptr(), ldl, g.perm.data_ptr(),
                                         g.lohi.data_ptr(), g.off.data_ptr(), K, L,
                                         g.cc.data_ptr(), stream), "ksm_gram_compact")
