@@ -42,7 +42,7 @@ constexpr int kUpdThreads = kUpdWaves * 64;
 constexpr int kSlots4 = 3;          // 256-position slots of a column held in registers
 constexpr int kMaxTouch = 16;       // spikes touching one slab (<= 3 without ties)
 constexpr int kMaxNt = 255;
-constexpr int kMaxK = 16384;
+constexpr int kMaxK = 8192;
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
 
@@ -115,10 +115,23 @@ __device__ __forceinline__ ColSpike col_spike(const SpikeRef &rf, int t, int nt)
 // spike has an empty range), store the changed scores, and return the
 // column's best (score, template id) in every lane.  s_nm / s_pid:
 // per-position norms and template ids in LDS.
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Column pass.  The scores are first ranked with a multiply by the rounded
+// reciprocal of nm (within 3 ulp of the IEEE quotient); only positions whose
+// approximate score is within 2^-20 of the column's approximate maximum can
+// hold the exact maximum, and only those get the exact division and the
+// (score, template id) comparison.  A NaN anywhere in the column, or a column
+// maximum below 2^-100, sends the whole column through the exact comparison
+// (NaN wins, as in torch.max).
 __device__ __forceinline__ void column_pass(float *__restrict__ Bt, int ldbt, int t, int K,
                                             const float *__restrict__ cc, const float *s_nm,
-                                            const int *s_pid, ColSpike a, ColSpike b, float &best,
-                                            int &bid) {
+                                            const float *s_inv, const int *s_pid, ColSpike a,
+                                            ColSpike b, float &best, int &bid) {
     const int lane = threadIdx.x & 63;
     float *col = Bt + (size_t)t * ldbt;
     best = -INFINITY;
@@ -140,36 +153,69 @@ __device__ __forceinline__ void column_pass(float *__restrict__ Bt, int ldbt, in
             va[k] = *reinterpret_cast<const float4 *>(cc + a.base + ra);
             vb[k] = *reinterpret_cast<const float4 *>(cc + b.base + rb);
         }
+        float x[kSlots4][4];
         bool dirty[kSlots4];
+        float amax = 0.f;
+        bool nan_any = false;
 #pragma unroll
         for (int k = 0; k < kSlots4; ++k) {
             const int r = r0 + 256 * k + 4 * lane;
             const bool ha = r >= a.lo && r < a.hi;
             const bool hb = r >= b.lo && r < b.hi;
-            float x[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            const float bv[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
             const float ca[4] = {va[k].x, va[k].y, va[k].z, va[k].w};
             const float cb[4] = {vb[k].x, vb[k].y, vb[k].z, vb[k].w};
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float pa = a.amp * ca[e];
                 const float pb = b.amp * cb[e];
-                x[e] = ha ? x[e] - pa : x[e];
-                x[e] = hb ? x[e] - pb : x[e];
+                float y = bv[e];
+                y = ha ? y - pa : y;
+                y = hb ? y - pb : y;
+                x[k][e] = y;
                 if (r + e < K) {
-                    const float cf = score(x[e], s_nm[r + e]);
-                    const int id = s_pid[r + e];
-                    if (beats(cf, id, best, bid)) {
-                        best = cf;
-                        bid = id;
-                    }
+                    const float rr = y > 0.f ? y : 0.f;
+                    const float ap = (rr * rr) * s_inv[r + e];
+                    nan_any |= ap != ap;
+                    amax = fmaxf(amax, ap);
                 }
             }
-            v[k] = make_float4(x[0], x[1], x[2], x[3]);
             dirty[k] = ha || hb;
         }
 #pragma unroll
         for (int k = 0; k < kSlots4; ++k)
-            if (dirty[k]) *reinterpret_cast<float4 *>(col + r0 + 256 * k + 4 * lane) = v[k];
+            if (dirty[k])
+                *reinterpret_cast<float4 *>(col + r0 + 256 * k + 4 * lane) =
+                    make_float4(x[k][0], x[k][1], x[k][2], x[k][3]);
+        const float amax_w = wave_max(amax);
+        // below 2^-100 the relative error bound of the approximation no
+        // longer holds (subnormal products); compare exactly there too
+        const bool exact_all = __ballot(nan_any) != 0ull || amax_w < 0x1p-100f;
+        const float thr = amax_w * (1.0f - 0x1p-20f);
+#pragma unroll
+        for (int k = 0; k < kSlots4; ++k) {
+            const int r = r0 + 256 * k + 4 * lane;
+            bool cand[4];
+            bool any = false;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float rr = x[k][e] > 0.f ? x[k][e] : 0.f;
+                const float ap = (rr * rr) * s_inv[r + e < K ? r + e : 0];
+                cand[e] = (r + e < K) && (exact_all || ap >= thr);
+                any |= cand[e];
+            }
+            if (__ballot(any) == 0ull) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                if (!cand[e]) continue;
+                const float cf = score(x[k][e], s_nm[r + e]);
+                const int id = s_pid[r + e];
+                if (beats(cf, id, best, bid)) {
+                    best = cf;
+                    bid = id;
+                }
+            }
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -182,14 +228,17 @@ __device__ __forceinline__ void column_pass(float *__restrict__ Bt, int ldbt, in
     }
 }
 
-// Per-position norms and template ids into LDS.
+// Per-position norms, their rounded reciprocals and template ids into LDS.
 __device__ __forceinline__ void load_positions(const float *__restrict__ nm,
                                                const int32_t *__restrict__ perm, int K,
-                                               float *s_nm, int *s_pid) {
-    for (int r = threadIdx.x; r < K; r += blockDim.x) {
+                                               float *s_nm, float *s_inv, int *s_pid, int first,
+                                               int stride) {
+    for (int r = first; r < K; r += stride) {
         const int id = perm[r];
+        const float n = nm[id];
         s_pid[r] = id;
-        s_nm[r] = nm[id];
+        s_nm[r] = n;
+        s_inv[r] = 1.0f / n;
     }
 }
 
@@ -204,8 +253,9 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
     int *__restrict__ imax) {
     extern __shared__ unsigned char smem[];
     float *s_nm = reinterpret_cast<float *>(smem);
-    int *s_pid = reinterpret_cast<int *>(s_nm + K);
-    load_positions(nm, perm, K, s_nm, s_pid);
+    float *s_inv = s_nm + K;
+    int *s_pid = reinterpret_cast<int *>(s_inv + K);
+    load_positions(nm, perm, K, s_nm, s_inv, s_pid, threadIdx.x, blockDim.x);
     __syncthreads();
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -217,7 +267,7 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
         int bid;
         if (t >= nt && t < T - nt) {
             const ColSpike none = {0, 0, 0, 0.f};
-            column_pass(Bt, ldbt, t, K, Bt, s_nm, s_pid, none, none, best, bid);
+            column_pass(Bt, ldbt, t, K, Bt, s_nm, s_inv, s_pid, none, none, best, bid);
         } else {
             best = 0.f;
             bid = 0;
@@ -324,7 +374,8 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     if (state[ST_DONE]) return;
     extern __shared__ unsigned char smem[];
     float *s_nm = reinterpret_cast<float *>(smem);
-    int *s_pid = reinterpret_cast<int *>(s_nm + K);
+    float *s_inv = s_nm + K;
+    int *s_pid = reinterpret_cast<int *>(s_inv + K);
     __shared__ Slot s_sp[kMaxTouch];
     __shared__ SpikeRef s_ref[kMaxTouch];
     __shared__ int s_rank[kMaxTouch];
@@ -369,12 +420,8 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
             s_bad = 0;
         }
     } else {
-        // the other waves stage the per-position norms and ids meanwhile
-        for (int r = tid - 64; r < K; r += kUpdThreads - 64) {
-            const int id = perm[r];
-            s_pid[r] = id;
-            s_nm[r] = nm[id];
-        }
+        // the other waves stage the per-position tables meanwhile
+        load_positions(nm, perm, K, s_nm, s_inv, s_pid, tid - 64, kUpdThreads - 64);
     }
     __syncthreads();
     const int total = s_total;
@@ -453,7 +500,7 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         const ColSpike none = {0, 0, 0, 0.f};
         const ColSpike ca = col_spike(s_ref[qa], t, nt);
         const ColSpike cb = qb >= 0 ? col_spike(s_ref[qb], t, nt) : none;
-        column_pass(Bt, ldbt, t, K, cc, s_nm, s_pid, ca, cb, best, bid);
+        column_pass(Bt, ldbt, t, K, cc, s_nm, s_inv, s_pid, ca, cb, best, bid);
         if (lane == 0) {
             const bool live = t >= nt && t < T - nt;
             cfmax[t] = live ? best : 0.f;
@@ -624,7 +671,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     const float th2_lo = (float)(th2d - fmax(4e-9, fabs(th2d) * 1e-6));
     // the early-exit window scan is exact when every candidate is >= 2^-6
     const int early_exit = th2_lo >= 0.015625f ? 1 : 0;
-    const size_t pos_shm = (size_t)K * (sizeof(float) + sizeof(int));
+    const size_t pos_shm = (size_t)K * (2 * sizeof(float) + sizeof(int));
 
     peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
     KSM_LAUNCH_CHECK("peel_init_kernel");
