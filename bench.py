@@ -9,7 +9,13 @@ A step = one batch through the hot path exactly as ``extract`` runs it:
 run_matching (temporal-PC correlation, projection, peel) + PC features + the
 device-to-host copies of st/th_amps/tF.  Prints one JSON line.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp32]
+Every form (fp32: the parity form; bf16x3, bf16: projection on BF16 MFMA) is
+timed the same way; `value` is the --precision form.  A separate pass with
+events around each kernel gives per-kernel times for the roofline objects, and
+batch 0 is matched by the CPU oracle in FP32 (the timed CPU baseline) and
+FP64 (spike agreement and amplitude error of every form).
+
+    python bench.py [--steps K] [--warmup W] [--precision fp32|bf16x3|bf16]
 """
 from __future__ import annotations
 
@@ -27,7 +33,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "matched channel·samples/s, FP32 vs BF16-MFMA, + spike agreement % at 383ch"
 C, K, NT, NTAP, P = 383, 700, 60000, 61, 6
-PEAK_FP32_MFMA = 157.3e12
+FORMS = ("fp32", "bf16x3", "bf16")
+PEAK_MFMA = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15}   # dense MFMA peaks
+GEMM_KERNEL = {"fp32": "gemm_f32", "bf16x3": "gemm_bf16", "bf16": "gemm_bf16"}
 PEAK_HBM = 8.0e12
 
 
@@ -52,8 +60,10 @@ def make_workload(n_batches, seed, device):
     return rec, ops, batches
 
 
-def cpu_baseline(ops, X, U, spikes_gpu):
-    """Oracle (torch-CPU restatement) on one batch: run_matching + features."""
+def oracle_runs(ops, X, U, need64=True):
+    """Oracle (torch-CPU restatement) on one batch: FP32 run_matching + features
+    (timed: the CPU baseline) and an FP64 run_matching (the accuracy yardstick
+    of the precision forms)."""
     from oracle import tm_oracle as O
     cores = os.cpu_count() or 1
     allowed = os.environ.get("OMP_NUM_THREADS")
@@ -70,14 +80,101 @@ def cpu_baseline(ops, X, U, spikes_gpu):
     st, amps, th, Xres = O.run_matching(ops_c, Xc, Uc, ctc)
     O.features(ops_c, Xres, st, amps, iCC, iU, Ucc)
     dt = time.perf_counter() - t0
-    ref = set(map(tuple, st.numpy().tolist()))
-    got = set(map(tuple, spikes_gpu.tolist()))
-    agree = len(ref & got) / max(len(ref), len(got), 1)
-    return dict(value=C * NT / dt, unit="channel·samples/s", cores=cores, kind="port",
-                sample="1 padded batch (383 ch x 60122 samples, 700 templates): oracle "
-                       "run_matching + features, torch CPU float32",
-                seconds=dt), dict(fp32_gpu_vs_cpu_pct=100.0 * agree, n_cpu=len(ref),
-                                  n_gpu=len(got))
+    cpu = dict(value=C * NT / dt, unit="channel·samples/s", cores=cores, kind="port",
+               sample="1 padded batch (383 ch x 60122 samples, 700 templates): oracle "
+                      "run_matching + features, torch CPU float32",
+               seconds=dt)
+    ref32 = (st.numpy(), amps.numpy()[:, 0])
+    ref64 = None
+    if need64:
+        ops64 = dict(ops_c)
+        ops64["wPCA"] = ops_c["wPCA"].double()
+        t0 = time.perf_counter()
+        st64, amps64, _, _ = O.run_matching(ops64, Xc.double(), Uc.double(),
+                                            O.prepare_matching(ops64, Uc.double()))
+        log(f"fp64 oracle {time.perf_counter() - t0:.1f}s")
+        ref64 = (st64.numpy(), amps64.numpy()[:, 0])
+    return cpu, ref32, ref64
+
+
+def agreement(st, amps, ref):
+    """Exact (time, template) agreement over max(n, n_ref); +-1-sample agreement
+    (same template); median relative amplitude error over exact matches."""
+    rst, ramps = ref
+    a = {tuple(p): i for i, p in enumerate(np.asarray(st).tolist())}
+    b = {tuple(p): i for i, p in enumerate(np.asarray(rst).tolist())}
+    n = max(len(a), len(b), 1)
+    both = [(a[p], b[p]) for p in a.keys() & b.keys()]
+    near = 0
+    for (t, i) in a:
+        if (t, i) in b or (t - 1, i) in b or (t + 1, i) in b:
+            near += 1
+    if both:
+        gi, ri = np.array(both).T
+        rel = np.abs(amps[gi].astype(np.float64) - ramps[ri]) / np.abs(ramps[ri])
+        med = float(np.median(rel))
+    else:
+        med = None
+    return dict(exact_pct=100.0 * len(both) / n, within1_pct=100.0 * near / n,
+                n=len(a), n_ref=len(b), median_rel_amp_err=med)
+
+
+def time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, steps,
+              warmup):
+    """Untimed warmup, then `steps` batches end to end (no events in the loop)."""
+    o = dict(ops, matching_precision=precision)
+    nb = len(batches)
+    for i in range(warmup):
+        tm._extract_batch(o, batches[i % nb], U, ctc, iCC, iU, Ucc, device)
+    torch.cuda.synchronize()
+    nsp = iters = 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        st, th, tf = tm._extract_batch(o, batches[(warmup + i) % nb], U, ctc, iCC, iU, Ucc,
+                                       device)
+        nsp += len(st)
+        iters += tm.last_peel_iterations
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return dict(value=C * NT * steps / elapsed, ms_per_step=elapsed / steps * 1e3,
+                spikes_per_batch=nsp / steps, peel_iterations_per_batch=iters / steps)
+
+
+PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_rescore",
+            "peel_update"]
+
+
+def profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, nbatch):
+    """A separate pass with library events around every kernel: per-kernel
+    totals (ms) and launch counts per batch, and the spike ids found."""
+    o = dict(ops, matching_precision=precision)
+    torch.cuda.synchronize()
+    lib.ksm_profile_reset()
+    lib.ksm_profile_enable(3)
+    ids = []
+    for i in range(nbatch):
+        st, th, tf = tm._extract_batch(o, batches[i % len(batches)], U, ctc, iCC, iU, Ucc,
+                                       device)
+        ids.append(st[:, 1].numpy())
+    torch.cuda.synchronize()
+    lib.ksm_profile_enable(0)
+    from kilosort_amd import _lib
+    kern = {}
+    for name in PROFILED:
+        ms, n = _lib.profile_query(name)
+        if n:
+            kern[name] = dict(ms_per_batch=ms / nbatch, launches_per_batch=n / nbatch,
+                              avg_launch_ms=ms / n)
+    lib.ksm_profile_reset()
+    return kern, np.concatenate(ids) if ids else np.zeros(0, np.int64)
+
+
+def peel_update_bytes(ids, nnzK, nnzC, nbatch):
+    """Algorithmic HBM bytes of the subtraction per batch: for each accepted
+    spike of template j, 12 B per (template i, lag) of its nonzero Gram rows
+    (ctc read + score read-modify-write over 2nt+1 lags) and 8 B per
+    (channel, lag) of its nonzero channels (residual read-modify-write)."""
+    return float((12 * (2 * NTAP + 1) * nnzK[ids] + 8 * NTAP * nnzC[ids]).sum()) / nbatch
 
 
 def main():
@@ -85,11 +182,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--precision", default="fp32", choices=FORMS,
+                    help="form timed for `value`; every form is also timed under `forms`")
+    ap.add_argument("--forms", default=",".join(FORMS),
+                    help="comma list of forms to time beside `value` ('' for none)")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-peel", action="store_true",
-                    help="also time the peel kernels with events (perturbs the timing)")
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU oracle runs (cpu_baseline and FP64 agreement)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the matching path is sharded by batch "
@@ -105,69 +204,76 @@ def main():
     n_distinct = min(args.steps + args.warmup, 12)
     log(f"generating {n_distinct} synthetic batches")
     rec, ops, batches = make_workload(n_distinct, args.seed, device)
-    ops["matching_precision"] = args.precision
     U = torch.from_numpy(rec.U).to(device)
     iCC, _, iU, Ucc = tm.prepare_extract(ops["xc"], ops["yc"], U, 10, 100.0, device=device)
     ctc = tm.prepare_matching(ops, U)
+    nnzK = (ctc != 0).any(dim=2).sum(dim=0).cpu().numpy().astype(np.int64)
+    nnzC = (U != 0).any(dim=1).sum(dim=1).cpu().numpy().astype(np.int64)
     torch.cuda.synchronize()
 
-    log(f"warmup {args.warmup}")
-    for i in range(args.warmup):
-        tm._extract_batch(ops, batches[i % n_distinct], U, ctc, iCC, iU, Ucc, device)
-    torch.cuda.synchronize()
+    forms = [f for f in args.forms.split(",") if f]
+    if args.precision not in forms:
+        forms.insert(0, args.precision)
+    results = {}
+    for f in forms:
+        log(f"timing {f}: warmup {args.warmup}, {args.steps} steps")
+        results[f] = time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, f,
+                               args.steps, args.warmup)
+        log(f"{f}: {results[f]['ms_per_step']:.3f} ms/step")
+    main_res = results[args.precision]
 
-    lib.ksm_profile_reset()
-    # projection kernels only by default: no events inside the peel loop
-    lib.ksm_profile_enable(3 if args.profile_peel else 1)
-    nsp = 0
-    iters = 0
-    log(f"timing {args.steps} steps")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        st, th, tf = tm._extract_batch(ops, batches[(args.warmup + i) % n_distinct], U, ctc,
-                                       iCC, iU, Ucc, device)
-        nsp += len(st)
-        iters += tm.last_peel_iterations
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    lib.ksm_profile_enable(0)
-
-    kern = {}
-    names = ["conv_pc", "gemm_f32"]
-    if args.profile_peel:
-        names += ["colmax_all", "peel_select", "peel_update"]
-    for name in names:
-        ms, n = _lib.profile_query(name)
-        kern[name] = (ms, n)
-    lib.ksm_profile_reset()
+    nprof = min(n_distinct, 4)
+    kern, ids = profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device,
+                             args.precision, nprof)
+    log("kernels per batch (ms, launches): " + ", ".join(
+        f"{k}={v['ms_per_batch']:.3f}/{v['launches_per_batch']:.0f}" for k, v in kern.items()))
     T = NT + 2 * NTAP
-    gemm_ms, gemm_n = kern["gemm_f32"]
-    gemm_avg = gemm_ms / max(gemm_n, 1) / 1e3
+    upd = kern["peel_update"]
+    upd_bytes = peel_update_bytes(ids, nnzK, nnzC, nprof)
+    upd_achieved = upd_bytes / (upd["ms_per_batch"] / 1e3)
+    roofline = {"kernel": "peel_update_kernel (subtraction + column maxima, per batch)",
+                "bound": "hbm", "achieved": upd_achieved / 1e9, "peak": PEAK_HBM / 1e9,
+                "unit": "GB/s", "frac": upd_achieved / PEAK_HBM, "traffic": None,
+                "bytes_per_batch": upd_bytes,
+                "bytes_per_launch": upd_bytes / upd["launches_per_batch"],
+                "avg_launch_ms": upd["avg_launch_ms"],
+                "launches_per_batch": upd["launches_per_batch"]}
+    gname = GEMM_KERNEL[args.precision]
+    g = kern[gname]
     flops_gemm = 2.0 * K * P * C * T
-    achieved = flops_gemm / gemm_avg if gemm_avg > 0 else 0.0
-    ms_step = elapsed / args.steps * 1e3
-    value = C * NT * args.steps / elapsed
-    log("kernel totals over the timed steps (ms, launches): " +
-        ", ".join(f"{k}={v[0]:.2f}/{v[1]}" for k, v in kern.items()))
+    g_ach = flops_gemm / (g["avg_launch_ms"] / 1e3)
+    peak = PEAK_MFMA[args.precision]
+    roofline_projection = {"kernel": f"{gname} (projection GEMM)", "bound": "mfma",
+                           "achieved": g_ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                           "frac": g_ach / peak, "flops_per_launch": flops_gemm,
+                           "avg_launch_ms": g["avg_launch_ms"]}
 
     cpu = None
-    agreement = None
+    agree = None
     if not args.no_cpu_baseline:
         Xs = batches[0]
-        stg = tm.run_matching(ops, Xs, U, ctc, device=device)[0].cpu().numpy()
-        log("cpu baseline (oracle on one batch)")
-        cpu, agreement = cpu_baseline(ops, Xs, U, stg)
+        log("cpu baseline (oracle on one batch) and FP64 oracle")
+        cpu, ref32, ref64 = oracle_runs(ops, Xs, U)
         log(f"cpu baseline {cpu['seconds']:.1f}s on {cpu['cores']} threads")
+        agree = {}
+        for f in forms:
+            o = dict(ops, matching_precision=f)
+            st, amps, _, _ = tm.run_matching(o, Xs, U, ctc, device=device)
+            st = st.cpu().numpy()
+            amps = amps.cpu().numpy()[:, 0]
+            agree[f] = {"vs_fp64_oracle": agreement(st, amps, ref64),
+                        "vs_fp32_oracle": agreement(st, amps, ref32)}
+            log(f"{f}: vs fp64 {agree[f]['vs_fp64_oracle']['exact_pct']:.3f} %, "
+                f"vs fp32 {agree[f]['vs_fp32_oracle']['exact_pct']:.3f} %")
 
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": main_res["value"],
         "unit": "channel·samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": main_res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -177,16 +283,14 @@ def main():
                                "Th=8, max_peels=100, Poisson 1-20 Hz, 20% overlaps",
                    "channels": C, "templates": K, "batch_samples": NT,
                    "precision": args.precision, "distinct_batches": n_distinct},
-        "roofline": {"kernel": "gemm_f32_mfma (projection)", "bound": "mfma",
-                     "achieved": achieved / 1e12, "peak": PEAK_FP32_MFMA / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA,
-                     "traffic": None,
-                     "flops_per_launch": flops_gemm, "avg_launch_ms": gemm_avg * 1e3},
-        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in kern.items()},
-        "spikes_per_batch": nsp / args.steps,
-        "peel_iterations_per_batch": iters / args.steps,
+        "roofline": roofline,
+        "roofline_projection": roofline_projection,
+        "kernels_per_batch": kern,
+        "spikes_per_batch": main_res["spikes_per_batch"],
+        "peel_iterations_per_batch": main_res["peel_iterations_per_batch"],
+        "forms": {f: {k: results[f][k] for k in ("value", "ms_per_step")} for f in forms},
+        "agreement_batch0": agree,
         "cpu_baseline": cpu,
-        "agreement": agreement,
     }
     print(json.dumps(out), flush=True)
 

@@ -40,8 +40,9 @@ typedef enum ksm_status {
 } ksm_status;
 
 typedef enum ksm_precision {
-    KSM_PREC_FP32 = 0,          /* FP32 conv + FP32 MFMA projection          */
-    KSM_PREC_BF16 = 1           /* BF16 MFMA projection, FP32 accumulation   */
+    KSM_PREC_FP32 = 0,          /* FP32 correlation + FP32 MFMA projection   */
+    KSM_PREC_BF16 = 1,          /* BF16 operands on MFMA, FP32 accumulation  */
+    KSM_PREC_BF16X3 = 2         /* split BF16 (hi*hi + hi*lo + lo*hi) on MFMA */
 } ksm_precision;
 
 /* Library version (major*10000 + minor*100 + patch). */
@@ -159,6 +160,7 @@ size_t ksm_peel_workspace(int T, int max_peels);
  * Bt (from ksm_project with the same perm) and Xres (a copy of X) are updated
  * in place; nm float32 [K] by template id (ksm_template_norms); perm/rank from
  * ksm_template_order; cc/lohi/off from ksm_gram_ranges + ksm_gram_compact.
+ * flags: KSM_PEEL_RESCORE (below) or 0.
  * Outputs: st int64 [capacity][2] (time column, template id), amps and
  * th_amps float32 [capacity], and info int32 [4] on the device:
  *   info[0] = spike count k, info[1] = ksm_status of the device run,
@@ -169,8 +171,14 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
              const float *W, const float *nm, const int32_t *perm,
              const int32_t *rank, const float *cc, const int32_t *lohi,
              const int64_t *off, int K, int P, int C, int T, int nt, float Th,
-             int max_peels, int64_t *st, float *amps, float *th_amps, int capacity,
-             int32_t *info, void *workspace, size_t ws_bytes, void *stream);
+             int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
+             int capacity, int32_t *info, void *workspace, size_t ws_bytes,
+             void *stream);
+
+/* ksm_peel flags. */
+#define KSM_PEEL_RESCORE 1  /* re-score every accepted spike in FP32 from the
+                               residual (for scores from a BF16 projection):
+                               amps, th_amps and the score entry become FP32 */
 
 /*
  * PC features of each spike on the nC channels nearest its template's peak

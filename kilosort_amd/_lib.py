@@ -22,6 +22,9 @@ KSM_ERR_TIES = 5
 
 PREC_FP32 = 0
 PREC_BF16 = 1
+PREC_BF16X3 = 2
+
+PEEL_RESCORE = 1
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -45,7 +48,7 @@ SIGNATURES = {
     "ksm_project": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
     "ksm_peel_workspace": (_sz, [_i, _i]),
     "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
-                      _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
+                      _i, _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "ksm_profile_enable": (_i, [_i]),
     "ksm_profile_reset": (_i, []),

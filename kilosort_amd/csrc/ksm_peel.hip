@@ -557,6 +557,72 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     }
 }
 
+// FP32 re-scoring of the accepted spikes (projection forms that compute B in
+// reduced precision).  For the spike in slot q of chunk c: the score of its
+// template at its column, recomputed from the current residual,
+//   B = sum_c sum_k Xres[c, x + k - nt/2] * (sum_p U[id,p,c] W[p,k]),
+// replaces the slot's amplitude (B/nm) and th_amp (sqrt(relu(B)^2/nm)) and
+// the score entry itself, so that the subtraction and everything recorded
+// for the spike use FP32 values.  Block per slot: lanes = lags, waves split
+// the template's nonzero channel range.
+__global__ __launch_bounds__(256) void peel_rescore_kernel(
+    const float *__restrict__ Xres, int ldx, const float *__restrict__ U,
+    const float *__restrict__ W, const int *__restrict__ crange, int P, int C, int nt,
+    const float *__restrict__ nm, const int32_t *__restrict__ rank, float *__restrict__ Bt,
+    int ldbt, const int *__restrict__ state, const int *__restrict__ chunk_count,
+    Slot *__restrict__ slots) {
+    if (state[ST_DONE]) return;
+    const int c = blockIdx.x / kSlotCap, q = blockIdx.x % kSlotCap;
+    if (q >= chunk_count[c]) return;
+    __shared__ float s_part[4];
+    Slot *sl = slots + (size_t)c * kSlotCap + q;
+    const int x = sl->x, id = sl->iy;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int h = nt / 2;
+    const int clo = crange[2 * id], chi = crange[2 * id + 1];
+    const float *uj = U + (size_t)id * P * C;
+    float part = 0.f;
+    for (int kb0 = 0; kb0 < nt; kb0 += 64) {
+        const int k = kb0 + lane;
+        const bool kin = k < nt;
+        float wk[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
+        const float *xb = Xres + x - h + (kin ? k : 0);
+        for (int c0 = clo + wv; c0 < chi; c0 += 4 * 8) {
+            float xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int ch = c0 + 4 * u;
+                xv[u] = (ch < chi) ? xb[(size_t)ch * ldx] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int ch = c0 + 4 * u;
+                if (ch < chi) {
+                    float tm = 0.f;
+#pragma unroll
+                    for (int p = 0; p < 8; ++p)
+                        if (p < P) tm = __builtin_fmaf(uj[(size_t)p * C + ch], wk[p], tm);
+                    part = __builtin_fmaf(kin ? xv[u] : 0.f, tm, part);
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    if (lane == 0) s_part[wv] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float b = (s_part[0] + s_part[1]) + (s_part[2] + s_part[3]);
+        const float n = nm[id];
+        const float cf = score(b, n);
+        sl->amp = b / n;
+        sl->tha = sqrtf(cf);
+        Bt[(size_t)x * ldbt + rank[id]] = b;
+    }
+}
+
 // crange[2j], crange[2j+1] = first and one-past-last channel where template
 // j has a nonzero component (an empty range is [0, 0)).  Wave per template.
 __global__ __launch_bounds__(256) void channel_range_kernel(const float *__restrict__ U, int K,
@@ -643,8 +709,8 @@ size_t ksm_peel_workspace(int T, int max_peels) {
 int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
              const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
              const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
-             float Th, int max_peels, int64_t *st, float *amps, float *th_amps, int capacity,
-             int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
+             float Th, int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
+             int capacity, int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
     if (!Bt || !Xres || !U || !W || !nm || !perm || !rank || !cc || !lohi || !off || !st ||
         !amps || !th_amps || !info || K <= 0 || K > kMaxK || ldbt < K || (ldbt % 4) != 0 ||
         P <= 0 || P > 8 ||
@@ -689,6 +755,11 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
             peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(cfmax, imax, Bt, ldbt, nm, rank, T,
                                                             nt, th2, th2_lo, early_exit, state,
                                                             chunk_count, slots);
+        }
+        if (flags & KSM_PEEL_RESCORE) {
+            ksm::ProfileScope prof("peel_rescore", ksm::kProfPeel, s);
+            peel_rescore_kernel<<<L.nchunks * kSlotCap, 256, 0, s>>>(
+                Xres, ldx, U, W, crange, P, C, nt, nm, rank, Bt, ldbt, state, chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);

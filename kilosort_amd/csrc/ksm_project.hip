@@ -9,6 +9,7 @@
 // as [K][P*C] is the left GEMM operand without reshuffling.
 #include "ksm_common.h"
 #include "ksm_profile.h"
+#include "ksm_project_internal.h"
 
 namespace {
 
@@ -257,8 +258,9 @@ extern "C" {
 
 size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) {
     (void)nt;
-    (void)precision;
     if (C <= 0 || T <= 0 || K <= 0 || P <= 0) return 0;
+    if (precision == KSM_PREC_BF16 || precision == KSM_PREC_BF16X3)
+        return ksm::project_bf16_workspace(C, T, K, P, precision);
     const ProjectLayout L = project_layout(C, T, K, P);
     return ksm::align_up(L.bc_bytes, 256) + ksm::align_up(L.at_bytes, 256) + 512;
 }
@@ -269,8 +271,9 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     if (!X || !W || !U || !Bt || C <= 0 || T <= 0 || K <= 0 || P <= 0 || nt <= 0 ||
         (nt % 2) == 0 || ldx < T)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: bad arguments");
-    if (precision != KSM_PREC_FP32)
-        return ksm::fail(KSM_ERR_INVALID, "ksm_project: precision %d not built", precision);
+    if (precision != KSM_PREC_FP32 && precision != KSM_PREC_BF16 &&
+        precision != KSM_PREC_BF16X3)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_project: unknown precision %d", precision);
     const ProjectLayout L = project_layout(C, T, K, P);
     if (ldbt < K || (ldbt % 4) != 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: ldbt %d must be >= K and a multiple of 4",
@@ -278,6 +281,9 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     if (!workspace || ws_bytes < ksm_project_workspace(C, T, K, P, nt, precision))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_project: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
+    if (precision != KSM_PREC_FP32)
+        return ksm::project_bf16(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision,
+                                 workspace, s);
     ksm::Carver cv(workspace);
     float *Bc = cv.take<float>((size_t)L.Rp * L.ldc);
     float *At = cv.take<float>((size_t)L.Rp * L.Kp);

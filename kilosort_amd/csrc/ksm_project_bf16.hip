@@ -1,0 +1,301 @@
+// BF16 forms of the projection B = U * (W (*) X) on v_mfma_f32_32x32x16_bf16
+// with FP32 accumulation.
+//
+// Reference: kilosort/template_matching.py:180-181 (computed there in FP32).
+//
+// The temporal-PC correlation is computed in FP32 (VALU) and stored as
+// BF16, time-major and channel-major: Bh[t][c*P + p] (and the rounding
+// remainder Bl = bf16(x - Bh) for the split form), so that both GEMM operands
+// are contiguous along the contraction index r = c*P + p.  U is stored the
+// same way per template position: Uh[m][c*P + p] = bf16(U[perm[m], p, c]).
+//
+//   KSM_PREC_BF16     B ~ Uh.Bh                       (one product)
+//   KSM_PREC_BF16X3   B ~ Uh.Bh + Uh.Bl + Ul.Bh       (split operands; the
+//                     dropped Ul.Bl term is ~2^-16 relative)
+// The three products of the split form are one GEMM over a contraction three
+// times as deep: [Uh | Uh | Ul] . [Bh | Bl | Bh].
+#include "ksm_common.h"
+#include "ksm_profile.h"
+#include "ksm_project_internal.h"
+
+namespace {
+
+typedef unsigned short bf16_t;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// float -> bf16, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ bf16_t to_bf16(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((unsigned)h << 16); }
+
+constexpr int kCvCh = 64;     // channels per correlation block (one per lane)
+constexpr int kCvTT = 8;      // time samples per thread
+constexpr int kCvT = 4 * kCvTT;  // 32 time samples per block (4 waves)
+
+// Bh[t][c*P + p] (and Bl) for t < Tp, c < Cp (zero for t >= T or c >= C).
+template <int P, int NTAP, bool SPLIT>
+__global__ __launch_bounds__(256) void conv_pc_bf16_kernel(
+    const float *__restrict__ X, int ldx, const float *__restrict__ W, int C, int T, int Tp,
+    int ldr, bf16_t *__restrict__ Bh, bf16_t *__restrict__ Bl) {
+    constexpr int H = NTAP / 2;
+    constexpr int SPAN = kCvT + NTAP - 1;
+    constexpr int LD = SPAN | 1;                 // odd row stride: no bank conflicts
+    __shared__ float xs[kCvCh * LD];
+    const int t0 = blockIdx.x * kCvT;
+    const int c0 = blockIdx.y * kCvCh;
+    for (int e = threadIdx.x; e < kCvCh * SPAN; e += 256) {
+        const int cl = e / SPAN, j = e % SPAN;
+        const int c = c0 + cl, t = t0 - H + j;
+        xs[cl * LD + j] = (c < C && t >= 0 && t < T) ? X[(size_t)c * ldx + t] : 0.f;
+    }
+    __syncthreads();
+    const int cl = threadIdx.x & 63, tg = threadIdx.x >> 6;
+    const float *xr = xs + cl * LD + tg * kCvTT;
+    float win[kCvTT + NTAP - 1];
+#pragma unroll
+    for (int e = 0; e < kCvTT + NTAP - 1; ++e) win[e] = xr[e];
+    float acc[kCvTT][P];
+#pragma unroll
+    for (int u = 0; u < kCvTT; ++u)
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[u][p] = 0.f;
+#pragma unroll
+    for (int k = 0; k < NTAP; ++k) {
+        float w[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) w[p] = W[p * NTAP + k];
+#pragma unroll
+        for (int u = 0; u < kCvTT; ++u)
+#pragma unroll
+            for (int p = 0; p < P; ++p) acc[u][p] = __builtin_fmaf(win[u + k], w[p], acc[u][p]);
+    }
+    const int c = c0 + cl;
+    if (c * P >= ldr) return;
+#pragma unroll
+    for (int u = 0; u < kCvTT; ++u) {
+        const int t = t0 + tg * kCvTT + u;
+        if (t >= Tp) break;
+        const bool live = t < T && c < C;
+        bf16_t *dh = Bh + (size_t)t * ldr + c * P;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const float x = live ? acc[u][p] : 0.f;
+            const bf16_t h = to_bf16(x);
+            dh[p] = h;
+            if (SPLIT) Bl[(size_t)t * ldr + c * P + p] = to_bf16(x - from_bf16(h));
+        }
+    }
+}
+
+// Uh[m][c*P + p] = bf16(U[perm[m], p, c]) (Ul = remainder), zero for m >= K or
+// c >= C.
+__global__ void u_bf16_kernel(const float *__restrict__ U, const int32_t *__restrict__ perm,
+                              int K, int P, int C, int Kp, int ldr, bf16_t *__restrict__ Uh,
+                              bf16_t *__restrict__ Ul) {
+    const size_t n = (size_t)Kp * ldr;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int m = (int)(e / ldr), r = (int)(e % ldr);
+        const int c = r / P, p = r % P;
+        float x = 0.f;
+        if (m < K && c < C) x = U[((size_t)(perm ? perm[m] : m) * P + p) * C + c];
+        const bf16_t h = to_bf16(x);
+        Uh[e] = h;
+        if (Ul) Ul[e] = to_bf16(x - from_bf16(h));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// TN GEMM: Bout[n][m] = sum_r A[m][r] * B[n][r] over nseg contraction segments
+// (segment s uses A = As_[s], B = Bs_[s]).  Block tile 128 (m) x 128 (n),
+// depth step 64; 4 waves each own 64x64 as 2x2 tiles of 32x32x16 MFMAs.
+// LDS rows are padded by 16 bytes so that the 16-byte operand reads of 32
+// lanes fall in distinct banks.
+
+constexpr int HBM_ = 128, HBN = 128, HBK = 64;
+constexpr int HLD = HBK + 8;      // padded LDS row (bf16 elements)
+
+struct Segs {
+    const bf16_t *a[3];
+    const bf16_t *b[3];
+};
+
+__device__ __forceinline__ int xcd_remap_h(int bid, int nb) {
+    const int xcd = bid & 7, pos = bid >> 3;
+    const int q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+}
+
+__global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int ldr, float *__restrict__ Bout,
+                                                      int ldb, int K, int nTm, int nTn) {
+    __shared__ bf16_t As[2][HBM_ * HLD];
+    __shared__ bf16_t Bs[2][HBN * HLD];
+    const int logical = xcd_remap_h(blockIdx.x, nTm * nTn);
+    const int tm = logical % nTm, tn = logical / nTm;
+    const int m0 = tm * HBM_, n0 = tn * HBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // global -> LDS: thread owns row tid/2, 32 consecutive elements (4 x 16 B)
+    const int grow = tid >> 1, gcol = (tid & 1) * 32;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    const int nk_seg = ldr / HBK;
+    const int nk = nk_seg * nseg;
+    // k step kt reads segment kt / nk_seg at depth (kt % nk_seg) * HBK
+    const bf16_t *ap = sg.a[0] + (size_t)(m0 + grow) * ldr + gcol;
+    const bf16_t *bp = sg.b[0] + (size_t)(n0 + grow) * ldr + gcol;
+    uint4 ra0 = reinterpret_cast<const uint4 *>(ap)[0], ra1 = reinterpret_cast<const uint4 *>(ap)[1];
+    uint4 ra2 = reinterpret_cast<const uint4 *>(ap)[2], ra3 = reinterpret_cast<const uint4 *>(ap)[3];
+    uint4 rb0 = reinterpret_cast<const uint4 *>(bp)[0], rb1 = reinterpret_cast<const uint4 *>(bp)[1];
+    uint4 rb2 = reinterpret_cast<const uint4 *>(bp)[2], rb3 = reinterpret_cast<const uint4 *>(bp)[3];
+    {
+        uint4 *la = reinterpret_cast<uint4 *>(&As[0][grow * HLD + gcol]);
+        uint4 *lb = reinterpret_cast<uint4 *>(&Bs[0][grow * HLD + gcol]);
+        la[0] = ra0; la[1] = ra1; la[2] = ra2; la[3] = ra3;
+        lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
+    }
+    __syncthreads();
+    const int kr = (lane >> 5) * 8, kc = lane & 31;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            const int s = (kt + 1) / nk_seg, k0 = ((kt + 1) % nk_seg) * HBK;
+            const bf16_t *sa = s == 0 ? sg.a[0] : (s == 1 ? sg.a[1] : sg.a[2]);
+            const bf16_t *sb = s == 0 ? sg.b[0] : (s == 1 ? sg.b[1] : sg.b[2]);
+            const uint4 *pa = reinterpret_cast<const uint4 *>(sa + (size_t)(m0 + grow) * ldr + k0 + gcol);
+            const uint4 *pb = reinterpret_cast<const uint4 *>(sb + (size_t)(n0 + grow) * ldr + k0 + gcol);
+            ra0 = pa[0]; ra1 = pa[1]; ra2 = pa[2]; ra3 = pa[3];
+            rb0 = pb[0]; rb1 = pb[1]; rb2 = pb[2]; rb3 = pb[3];
+        }
+#pragma unroll
+        for (int kk = 0; kk < HBK; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(
+                    &As[buf][(wm * 64 + i * 32 + kc) * HLD + kk + kr]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8 *>(
+                    &Bs[buf][(wn * 64 + j * 32 + kc) * HLD + kk + kr]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0,
+                                                                        0, 0);
+        }
+        if (more) {
+            uint4 *la = reinterpret_cast<uint4 *>(&As[buf ^ 1][grow * HLD + gcol]);
+            uint4 *lb = reinterpret_cast<uint4 *>(&Bs[buf ^ 1][grow * HLD + gcol]);
+            la[0] = ra0; la[1] = ra1; la[2] = ra2; la[3] = ra3;
+            lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
+        }
+        __syncthreads();
+    }
+    // transposed store Bout[n][m]: registers 4g..4g+3 are 4 consecutive m
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * g + 4 * (lane >> 5);
+                const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+                float *dst = Bout + (size_t)n * ldb + m;
+                if (m + 3 < K) {
+                    float4 v;
+                    v.x = acc[i][j][4 * g + 0];
+                    v.y = acc[i][j][4 * g + 1];
+                    v.z = acc[i][j][4 * g + 2];
+                    v.w = acc[i][j][4 * g + 3];
+                    *reinterpret_cast<float4 *>(dst) = v;
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (m + e < K) dst[e] = acc[i][j][4 * g + e];
+                }
+            }
+}
+
+struct HLayout {
+    int ldr, Kp, Tp;
+    size_t bc_elems, u_elems;
+};
+
+HLayout h_layout(int C, int T, int K, int P) {
+    HLayout L;
+    L.ldr = (int)ksm::align_up((size_t)P * C, 192);   // multiple of P=6 and of 64
+    if (L.ldr % P) L.ldr = (int)ksm::align_up((size_t)P * C, (size_t)P * HBK);
+    L.Kp = (int)ksm::align_up(K, HBM_);
+    L.Tp = (int)ksm::align_up(T, HBN);
+    L.bc_elems = (size_t)L.Tp * L.ldr;
+    L.u_elems = (size_t)L.Kp * L.ldr;
+    return L;
+}
+
+}  // namespace
+
+namespace ksm {
+
+size_t project_bf16_workspace(int C, int T, int K, int P, int precision) {
+    const HLayout L = h_layout(C, T, K, P);
+    const int terms = precision == KSM_PREC_BF16X3 ? 2 : 1;
+    return terms * (align_up(L.bc_elems * 2, 256) + align_up(L.u_elems * 2, 256)) + 512;
+}
+
+int project_bf16(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
+                 int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
+                 void *workspace, hipStream_t s) {
+    if (P != 6 || nt != 61)
+        return fail(KSM_ERR_INVALID, "ksm_project: BF16 forms are built for P=6, nt=61");
+    const HLayout L = h_layout(C, T, K, P);
+    const bool split = precision == KSM_PREC_BF16X3;
+    Carver cv(workspace);
+    bf16_t *Bh = cv.take<bf16_t>(L.bc_elems);
+    bf16_t *Uh = cv.take<bf16_t>(L.u_elems);
+    bf16_t *Bl = split ? cv.take<bf16_t>(L.bc_elems) : nullptr;
+    bf16_t *Ul = split ? cv.take<bf16_t>(L.u_elems) : nullptr;
+    {
+        ProfileScope prof("conv_pc", kProfProject, s);
+        dim3 grid(L.Tp / kCvT, (L.ldr / P + kCvCh - 1) / kCvCh);
+        if (split)
+            conv_pc_bf16_kernel<6, 61, true><<<grid, 256, 0, s>>>(X, ldx, W, C, T, L.Tp, L.ldr, Bh,
+                                                                  Bl);
+        else
+            conv_pc_bf16_kernel<6, 61, false><<<grid, 256, 0, s>>>(X, ldx, W, C, T, L.Tp, L.ldr,
+                                                                   Bh, Bl);
+    }
+    KSM_LAUNCH_CHECK("conv_pc_bf16_kernel");
+    u_bf16_kernel<<<512, 256, 0, s>>>(U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
+    KSM_LAUNCH_CHECK("u_bf16_kernel");
+    Segs sg;
+    sg.a[0] = Uh;
+    sg.b[0] = Bh;
+    sg.a[1] = Uh;
+    sg.b[1] = split ? Bl : Bh;
+    sg.a[2] = split ? Ul : Uh;
+    sg.b[2] = Bh;
+    const int nTm = L.Kp / HBM_, nTn = L.Tp / HBN;
+    {
+        ProfileScope prof("gemm_bf16", kProfProject, s);
+        gemm_bf16_mfma<<<nTm * nTn, 256, 0, s>>>(sg, split ? 3 : 1, L.ldr, Bt, ldbt, K, nTm, nTn);
+    }
+    KSM_LAUNCH_CHECK("gemm_bf16_mfma");
+    return KSM_OK;
+}
+
+}  // namespace ksm

@@ -1,0 +1,16 @@
+// Entry points shared between the projection sources.
+#pragma once
+
+#include "ksm_common.h"
+
+namespace ksm {
+
+// Workspace bytes of the BF16 projection forms.
+size_t project_bf16_workspace(int C, int T, int K, int P, int precision);
+
+// BF16 / split-BF16 projection into Bt (see ksm_project_bf16.hip).
+int project_bf16(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
+                 int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
+                 void *workspace, hipStream_t s);
+
+}  // namespace ksm

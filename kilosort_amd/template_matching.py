@@ -19,7 +19,17 @@ nearest-channel table, which the reference also computes on the host with
 numpy (its unstable ``argsort`` tie order is reproduced by making the same
 call).  There is no CPU path: a CPU ``device`` or a missing library raises.
 
-Optional key ``ops['matching_precision']``: ``'fp32'`` (default).
+Optional key ``ops['matching_precision']``:
+
+* ``'fp32'`` (default): FP32 correlation and FP32 MFMA projection -- the
+  parity form;
+* ``'bf16'``: BF16 operands on MFMA with FP32 accumulation;
+* ``'bf16x3'``: split BF16 operands (hi*hi + hi*lo + lo*hi) on MFMA.
+
+In the BF16 forms every accepted spike is re-scored in FP32 from the residual
+before it is recorded and subtracted, so amplitudes, th_amps and the
+subtraction use FP32 scores; the accept/reject decisions use the
+reduced-precision scores.
 """
 from __future__ import annotations
 
@@ -204,11 +214,14 @@ def prepare_matching(ops, U):
     return store[:, :, :L].permute(1, 0, 2)
 
 
+_PRECISIONS = {'fp32': _lib.PREC_FP32, 'bf16': _lib.PREC_BF16, 'bf16x3': _lib.PREC_BF16X3}
+
+
 def _precision(ops):
     name = str(ops.get('matching_precision', 'fp32')).lower()
-    if name == 'fp32':
-        return _lib.PREC_FP32
-    raise ValueError(f"unknown matching_precision {name!r}")
+    if name not in _PRECISIONS:
+        raise ValueError(f"unknown matching_precision {name!r} (one of {sorted(_PRECISIONS)})")
+    return _PRECISIONS[name]
 
 
 def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
@@ -258,7 +271,9 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     _lib.check(lib.ksm_peel(Bt.data_ptr(), ldbt, Xres.data_ptr(), T, U.data_ptr(), W.data_ptr(),
                             nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),
                             g.cc.data_ptr(), g.lohi.data_ptr(), g.off.data_ptr(), K, P, C, T,
-                            nt, Th, max_peels, st.data_ptr(), amps.data_ptr(),
+                            nt, Th, max_peels,
+                            0 if prec == _lib.PREC_FP32 else _lib.PEEL_RESCORE,
+                            st.data_ptr(), amps.data_ptr(),
                             th_amps.data_ptr(), ST_CAPACITY, info.data_ptr(), kws.data_ptr(),
                             kbytes, stream),
                "ksm_peel")
