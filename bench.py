@@ -223,10 +223,13 @@ def main():
     main_res = results[args.precision]
 
     nprof = min(n_distinct, 4)
-    kern, ids = profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device,
-                             args.precision, nprof)
-    log("kernels per batch (ms, launches): " + ", ".join(
-        f"{k}={v['ms_per_batch']:.3f}/{v['launches_per_batch']:.0f}" for k, v in kern.items()))
+    prof = {}
+    for f in forms:
+        prof[f] = profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, f, nprof)
+        log(f"{f} kernels per batch (ms, launches): " + ", ".join(
+            f"{k}={v['ms_per_batch']:.3f}/{v['launches_per_batch']:.0f}"
+            for k, v in prof[f][0].items()))
+    kern, ids = prof[args.precision]
     T = NT + 2 * NTAP
     upd = kern["peel_update"]
     upd_bytes = peel_update_bytes(ids, nnzK, nnzC, nprof)
@@ -288,7 +291,10 @@ def main():
         "kernels_per_batch": kern,
         "spikes_per_batch": main_res["spikes_per_batch"],
         "peel_iterations_per_batch": main_res["peel_iterations_per_batch"],
-        "forms": {f: {k: results[f][k] for k in ("value", "ms_per_step")} for f in forms},
+        "forms": {f: dict(value=results[f]["value"], ms_per_step=results[f]["ms_per_step"],
+                          kernel_ms_per_batch={k: v["ms_per_batch"]
+                                               for k, v in prof[f][0].items()})
+                  for f in forms},
         "agreement_batch0": agree,
         "cpu_baseline": cpu,
     }

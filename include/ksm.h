@@ -146,7 +146,7 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U,
                 void *stream);
 
 /* Workspace bytes of ksm_peel. */
-size_t ksm_peel_workspace(int T, int max_peels);
+size_t ksm_peel_workspace(int K, int T, int max_peels);
 
 /*
  * Matching pursuit (the peel).  Replaces template_matching.py:183-244:

@@ -46,7 +46,7 @@ SIGNATURES = {
     "ksm_gram_compact": (_i, [_p, _i, _p, _p, _p, _i, _i, _p, _p]),
     "ksm_project_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
     "ksm_project": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
-    "ksm_peel_workspace": (_sz, [_i, _i]),
+    "ksm_peel_workspace": (_sz, [_i, _i, _i]),
     "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
                       _i, _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),

@@ -266,7 +266,7 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     amps = torch.empty((ST_CAPACITY, 1), dtype=torch.float32, device=device)
     th_amps = torch.empty((ST_CAPACITY, 1), dtype=torch.float32, device=device)
     info = torch.zeros((4,), dtype=torch.int32, device=device)
-    kbytes = lib.ksm_peel_workspace(T, max_peels)
+    kbytes = lib.ksm_peel_workspace(K, T, max_peels)
     kws = _ws.get(device, "peel", kbytes)
     _lib.check(lib.ksm_peel(Bt.data_ptr(), ldbt, Xres.data_ptr(), T, U.data_ptr(), W.data_ptr(),
                             nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),

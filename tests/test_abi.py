@@ -54,7 +54,7 @@ def test_status_strings_and_workspace_queries():
     assert lib.ksm_status_string(0) == b"ok"
     assert lib.ksm_status_string(2) == b"spike capacity exceeded"
     assert lib.ksm_project_workspace(383, 60122, 700, 6, 61, 0) > 383 * 6 * 60122 * 4
-    assert lib.ksm_peel_workspace(60122, 100) > 60122 * 8
+    assert lib.ksm_peel_workspace(700, 60122, 100) > 60122 * 11 * 8
     assert lib.ksm_project_workspace(0, 10, 10, 6, 61, 0) == 0
 
 
