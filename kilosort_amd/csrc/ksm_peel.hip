@@ -16,7 +16,7 @@
 //                        is the larger score, then the smaller id; NaN wins)
 //   cfmax[T], imax[T]   best score and template id of every column
 //                        (Cf = relu(B)^2/nm; columns < nt and >= T-nt are 0)
-//   pnm, pinv, ppid     norm, rounded reciprocal and template id per position
+//   pnm, ppid           norm and template id per position
 //   slots               accepted spikes of the current iteration, per chunk
 //                        of kChunk columns, in increasing time
 //   state               done flag, status, iteration count, and kbase[it]
@@ -42,14 +42,14 @@ namespace {
 
 constexpr int kChunk = 256;         // columns per select block (one per thread)
 constexpr int kSlotCap = 16;        // accepted spikes per chunk (>= 256/62 + ties)
-constexpr int kGroup = 64;          // positions per group (16 lanes x 4)
+constexpr int kGroup = 32;          // positions per group (8 lanes x 4)
+constexpr int kPassGroups = 8;      // groups a wave reduces per pass
 constexpr int kMaxK = 8192;
-constexpr int kMaxG = kMaxK / kGroup;
 constexpr int kMaxNt = 255;
 constexpr int kMaxChunks = 4096;    // T <= kChunk * kMaxChunks
 constexpr int kUpdThreads = 256;    // update / first-pass block (4 waves)
-constexpr int kXresItems = 4;       // residual work items per spike
-constexpr int kXresCh = 8;          // channels per residual step
+constexpr int kXresItems = 8;       // residual work items per spike
+constexpr int kXresCh = 16;         // channels per residual step (loads first)
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
 
@@ -58,6 +58,13 @@ struct Slot {
     int iy;      // template id
     float amp;   // B[iy, x] / nm[iy]
     float tha;   // sqrt(Cmax[x])
+};
+
+// A template's Gram block: rows [lo, hi) of the peel order at
+// cc[off + lag * (hi - lo) + (r - lo)].
+struct GramRef {
+    int lo, hi;
+    long long off;
 };
 
 __device__ __forceinline__ int wave_sum_i(int v) {
@@ -121,171 +128,248 @@ struct ColSpike {
     float amp;
 };
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+// Cross-lane steps on DPP (no LDS round trip).  quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1] and row_half_mirror applied with a commutative
+// reduction leave every lane of an 8-lane group holding the group's result;
+// row_mirror then joins the two groups of a 16-lane row.
+template <int CTRL> __device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xf, 0xf, false);
 }
-
-// Bits [glo, ghi] of 64-bit word w of a group mask (empty if glo > ghi).
-__device__ __forceinline__ u64 range_bits(int glo, int ghi, int w) {
-    const int lo = max(glo, 64 * w), hi = min(ghi, 64 * w + 63);
-    if (lo > hi) return 0ull;
-    return (~0ull >> (63 - (hi - lo))) << (lo - 64 * w);
+template <int CTRL> __device__ __forceinline__ u64 dpp_max_step(u64 k) {
+    const unsigned lo = dpp_u32<CTRL>((unsigned)k);
+    const unsigned hi = dpp_u32<CTRL>((unsigned)(k >> 32));
+    const u64 o = ((u64)hi << 32) | lo;
+    return o > k ? o : k;
 }
-
-// Column pass over position groups.  Applies the covering spikes' Gram
-// updates to column t (products rounded before the subtract; spike a before
-// spike b; an absent spike has an empty range), stores the changed scores,
-// re-reduces every group that a spike touches (all groups when `all`), stores
-// those group keys, and returns the column's best key (wave-uniform).
-//
-// 16 lanes take one group (4 consecutive positions each: 16-byte loads; ldbt,
-// the Gram ranges and their offsets are multiples of 4), so a wave reduces
-// four groups per round.  Within a group the scores are first ranked with a
-// multiply by the rounded reciprocal of nm (within 3 ulp of the IEEE
-// quotient); only positions whose approximate score is within 2^-20 of the
-// group's approximate maximum can hold the exact maximum, and only those get
-// the exact division.  A NaN in the group, or a group maximum below 2^-100,
-// sends the whole group through the exact division.
-__device__ __forceinline__ u64 column_groups(float *__restrict__ Bt, int ldbt, int t, int K,
-                                             int G, const float *__restrict__ cc,
-                                             const float *__restrict__ pnm,
-                                             const float *__restrict__ pinv,
-                                             const int *__restrict__ ppid, ColSpike a,
-                                             ColSpike b, u64 *__restrict__ gk, bool all,
-                                             u64 *s_key) {
-    const int lane = threadIdx.x & 63;
-    const int sg = lane >> 4, sl = lane & 15;
-    float *col = Bt + (size_t)t * ldbt;
-    const bool a_any = a.hi > a.lo, b_any = b.hi > b.lo;
-    // groups to reduce (uniform masks over up to 128 groups)
-    u64 m0, m1;
-    if (all) {
-        m0 = range_bits(0, G - 1, 0);
-        m1 = range_bits(0, G - 1, 1);
-    } else {
-        const int ga0 = a_any ? a.lo / kGroup : 1, ga1 = a_any ? (a.hi - 1) / kGroup : 0;
-        const int gb0 = b_any ? b.lo / kGroup : 1, gb1 = b_any ? (b.hi - 1) / kGroup : 0;
-        m0 = range_bits(ga0, ga1, 0) | range_bits(gb0, gb1, 0);
-        m1 = range_bits(ga0, ga1, 1) | range_bits(gb0, gb1, 1);
-    }
-    const u64 t0 = m0, t1 = m1;
-    // keys of the groups left as they are (read before any store of this column)
-    u64 old0 = 0ull, old1 = 0ull;
-    if (!all) {
-        if (lane < G && !((t0 >> lane) & 1ull)) old0 = gk[lane];
-        if (lane + 64 < G && !((t1 >> lane) & 1ull)) old1 = gk[lane + 64];
-    }
-    while (m0 | m1) {
-        int gs[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (m0) {
-                gs[j] = __builtin_ctzll(m0);
-                m0 &= m0 - 1;
-            } else if (m1) {
-                gs[j] = 64 + __builtin_ctzll(m1);
-                m1 &= m1 - 1;
-            } else {
-                gs[j] = -1;
-            }
-        }
-        const int g = sg == 0 ? gs[0] : sg == 1 ? gs[1] : sg == 2 ? gs[2] : gs[3];
-        if (g < 0) continue;
-        const int r = g * kGroup + 4 * sl;
-        const int ra = a_any ? min(max(r, a.lo), a.hi - 4) : 0;
-        const int rb = b_any ? min(max(r, b.lo), b.hi - 4) : 0;
-        const float4 v = *reinterpret_cast<const float4 *>(col + min(r, ldbt - 4));
-        const float4 va = *reinterpret_cast<const float4 *>(cc + a.base + ra);
-        const float4 vb = *reinterpret_cast<const float4 *>(cc + b.base + rb);
-        const float4 n4 = *reinterpret_cast<const float4 *>(pnm + r);
-        const float4 i4 = *reinterpret_cast<const float4 *>(pinv + r);
-        const int4 p4 = *reinterpret_cast<const int4 *>(ppid + r);
-        const bool ha = r >= a.lo && r < a.hi;
-        const bool hb = r >= b.lo && r < b.hi;
-        const float bv[4] = {v.x, v.y, v.z, v.w};
-        const float ca[4] = {va.x, va.y, va.z, va.w};
-        const float cb[4] = {vb.x, vb.y, vb.z, vb.w};
-        const float nv[4] = {n4.x, n4.y, n4.z, n4.w};
-        const float iv[4] = {i4.x, i4.y, i4.z, i4.w};
-        const int pv[4] = {p4.x, p4.y, p4.z, p4.w};
-        float y[4], ap[4];
-        float amax = 0.f;
-        bool nan_any = false;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const float pa = a.amp * ca[e];
-            const float pb = b.amp * cb[e];
-            float x = bv[e];
-            x = ha ? x - pa : x;
-            x = hb ? x - pb : x;
-            y[e] = x;
-            const float rr = relu(x);
-            ap[e] = (rr * rr) * iv[e];
-            if (r + e < K) {
-                nan_any |= ap[e] != ap[e];
-                amax = fmaxf(amax, ap[e]);
-            }
-        }
-        if (ha || hb) *reinterpret_cast<float4 *>(col + r) = make_float4(y[0], y[1], y[2], y[3]);
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
-        const unsigned long long nb = __ballot(nan_any);
-        const bool exact_all = ((nb >> (16 * sg)) & 0xffffull) != 0ull || amax < 0x1p-100f;
-        const float thr = amax * (1.0f - 0x1p-20f);
-        u64 key = 0ull;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            if (r + e < K && (exact_all || ap[e] >= thr)) {
-                const u64 k = make_key(score(y[e], nv[e]), pv[e]);
-                key = k > key ? k : key;
-            }
-        }
-#pragma unroll
-        for (int o = 8; o > 0; o >>= 1) {
-            const u64 k = shfl_xor_u64(key, o);
-            key = k > key ? k : key;
-        }
-        if (sl == 0) {
-            gk[g] = key;
-            s_key[g] = key;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+__device__ __forceinline__ u64 group_max_u64(u64 k) {
+    k = dpp_max_step<0xb1>(k);
+    k = dpp_max_step<0x4e>(k);
+    k = dpp_max_step<0x141>(k);
+    return k;
+}
+// Maximum over the wave (uniform result): 16-lane rows on DPP, then the four
+// row results through scalar registers.
+__device__ __forceinline__ u64 wave_max_u64(u64 k) {
+    k = group_max_u64(k);
+    k = dpp_max_step<0x140>(k);
     u64 best = 0ull;
-    if (lane < G) best = ((t0 >> lane) & 1ull) ? s_key[lane] : old0;
-    if (lane + 64 < G) {
-        const u64 k = ((t1 >> lane) & 1ull) ? s_key[lane + 64] : old1;
-        best = k > best ? k : best;
-    }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const u64 k = shfl_xor_u64(best, o);
-        best = k > best ? k : best;
+    for (int r = 0; r < 4; ++r) {
+        const unsigned lo = __builtin_amdgcn_readlane((unsigned)k, 16 * r);
+        const unsigned hi = __builtin_amdgcn_readlane((unsigned)(k >> 32), 16 * r);
+        const u64 o = ((u64)hi << 32) | lo;
+        best = o > best ? o : best;
     }
     return best;
 }
 
-// Per-position norms, rounded reciprocals and template ids, padded with
-// (1, 0, 0) to G * kGroup positions.
+// The groups a column pass reduces, as up to three intervals: the first
+// spike's groups, then the second spike's groups below and above them.
+struct GroupList {
+    int lo0, n0, lo1, n1, lo2, n2;
+    __device__ __forceinline__ int count() const { return n0 + n1 + n2; }
+    __device__ __forceinline__ int at(int j) const {
+        return j < n0 ? lo0 + j : j < n0 + n1 ? lo1 + (j - n0) : lo2 + (j - n0 - n1);
+    }
+    __device__ __forceinline__ bool has(int g) const {
+        return (g >= lo0 && g < lo0 + n0) || (g >= lo1 && g < lo1 + n1) ||
+               (g >= lo2 && g < lo2 + n2);
+    }
+};
+
+__device__ __forceinline__ GroupList group_list(const ColSpike &a, const ColSpike &b, int G,
+                                                bool all) {
+    GroupList L = {0, 0, 0, 0, 0, 0};
+    if (all) {
+        L.n0 = G;
+        return L;
+    }
+    int a0 = 0, a1 = -1;
+    if (a.hi > a.lo) {
+        a0 = a.lo / kGroup;
+        a1 = (a.hi - 1) / kGroup;
+        L.lo0 = a0;
+        L.n0 = a1 - a0 + 1;
+    }
+    if (b.hi > b.lo) {
+        const int b0 = b.lo / kGroup, b1 = (b.hi - 1) / kGroup;
+        if (a1 < a0) {
+            L.lo1 = b0;
+            L.n1 = b1 - b0 + 1;
+        } else {
+            const int h1 = min(b1, a0 - 1);
+            if (h1 >= b0) {
+                L.lo1 = b0;
+                L.n1 = h1 - b0 + 1;
+            }
+            const int l2 = max(b0, a1 + 1);
+            if (b1 >= l2) {
+                L.lo2 = l2;
+                L.n2 = b1 - l2 + 1;
+            }
+        }
+    }
+    return L;
+}
+
+// A column to update: its index and the spikes covering it, the one
+// subtracted first in `a` (an absent spike has an empty range).
+struct ColJob {
+    int t;
+    ColSpike a, b;
+};
+
+// Loads of one pass (kPassGroups groups) over one column: lanes 8 g .. 8 g + 7
+// take group g of the pass, 4 consecutive positions each (16-byte loads;
+// ldbt, the Gram ranges and their offsets are multiples of 4).  An absent
+// group reads group 0 and is discarded.
+struct ColPass {
+    float4 v, va, vb;
+    int gg;
+};
+
+__device__ __forceinline__ void pass_load(ColPass &ps, const ColJob &J, const GroupList &L,
+                                          int j0, const float *__restrict__ Bt, int ldbt,
+                                          const float *__restrict__ cc) {
+    const int lane = threadIdx.x & 63;
+    const int j = j0 + (lane >> 3);
+    ps.gg = j < L.count() ? L.at(j) : -1;
+    const int r = (ps.gg < 0 ? 0 : ps.gg) * kGroup + 4 * (lane & 7);
+    const bool a_any = J.a.hi > J.a.lo, b_any = J.b.hi > J.b.lo;
+    const int ra = a_any ? min(max(r, J.a.lo), J.a.hi - 4) : 0;
+    const int rb = b_any ? min(max(r, J.b.lo), J.b.hi - 4) : 0;
+    ps.v = *reinterpret_cast<const float4 *>(Bt + (size_t)J.t * ldbt + min(r, ldbt - 4));
+    ps.va = *reinterpret_cast<const float4 *>(cc + J.a.base + ra);
+    ps.vb = *reinterpret_cast<const float4 *>(cc + J.b.base + rb);
+}
+
+// Applies the Gram updates (products rounded before the subtract; a before
+// b) in place in ps.v and reduces the lane's group to its key (exact score
+// relu(b)^2/nm; larger score, then smaller template id; NaN wins).  s_nm,
+// s_pid: per-position norms and template ids.
+__device__ __forceinline__ u64 pass_compute(ColPass &ps, const ColJob &J, int K, const float *s_nm,
+                                            const int *s_pid) {
+    const int r = (ps.gg < 0 ? 0 : ps.gg) * kGroup + 4 * (threadIdx.x & 7);
+    const bool ha = r >= J.a.lo && r < J.a.hi;
+    const bool hb = r >= J.b.lo && r < J.b.hi;
+    const float4 n4 = *reinterpret_cast<const float4 *>(s_nm + r);
+    const int4 p4 = *reinterpret_cast<const int4 *>(s_pid + r);
+    float bv[4] = {ps.v.x, ps.v.y, ps.v.z, ps.v.w};
+    const float ca[4] = {ps.va.x, ps.va.y, ps.va.z, ps.va.w};
+    const float cb[4] = {ps.vb.x, ps.vb.y, ps.vb.z, ps.vb.w};
+    const float nv[4] = {n4.x, n4.y, n4.z, n4.w};
+    const int pv[4] = {p4.x, p4.y, p4.z, p4.w};
+    u64 kk = 0ull;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float pa = J.a.amp * ca[e];
+        const float pb = J.b.amp * cb[e];
+        float x = bv[e];
+        x = ha ? x - pa : x;
+        x = hb ? x - pb : x;
+        bv[e] = x;
+        const u64 k = r + e < K ? make_key(score(x, nv[e]), pv[e]) : 0ull;
+        kk = k > kk ? k : kk;
+    }
+    ps.v = make_float4(bv[0], bv[1], bv[2], bv[3]);
+    return group_max_u64(ps.gg < 0 ? 0ull : kk);
+}
+
+__device__ __forceinline__ void pass_store(const ColPass &ps, const ColJob &J,
+                                           float *__restrict__ Bt, int ldbt,
+                                           u64 *__restrict__ gk, u64 key, u64 &acc) {
+    const int sl = threadIdx.x & 7;
+    const int r = (ps.gg < 0 ? 0 : ps.gg) * kGroup + 4 * sl;
+    const bool dirty = ps.gg >= 0 && ((r >= J.a.lo && r < J.a.hi) || (r >= J.b.lo && r < J.b.hi));
+    if (dirty) *reinterpret_cast<float4 *>(Bt + (size_t)J.t * ldbt + r) = ps.v;
+    if (ps.gg >= 0 && sl == 0) gk[ps.gg] = key;
+    acc = key > acc ? key : acc;
+}
+
+// Keys of the groups a column pass leaves as they are (read before any store).
+__device__ __forceinline__ u64 kept_keys(const GroupList &L, int G, const u64 *__restrict__ gk) {
+    const int lane = threadIdx.x & 63;
+    u64 acc = 0ull;
+    for (int g = lane; g < G; g += 64)
+        if (!L.has(g)) {
+            const u64 k = gk[g];
+            acc = k > acc ? k : acc;
+        }
+    return acc;
+}
+
+// A column in flight: its job, group list, the kept keys and the loads of
+// its first pass.
+struct ColWork {
+    ColJob J;
+    GroupList L;
+    u64 acc;
+    ColPass ps;
+};
+
+__device__ __forceinline__ void col_begin(ColWork &w, const ColJob &J, bool all, int G,
+                                          const float *__restrict__ Bt, int ldbt,
+                                          const float *__restrict__ cc,
+                                          const u64 *__restrict__ gkey) {
+    w.J = J;
+    w.L = group_list(J.a, J.b, G, all);
+    w.acc = all ? 0ull : kept_keys(w.L, G, gkey + (size_t)J.t * G);
+    pass_load(w.ps, J, w.L, 0, Bt, ldbt, cc);
+}
+
+// Finishes a column whose first pass is loaded: computes it, then -- when
+// `has_next` -- issues the next column's loads before this column's stores
+// (a store issued ahead of a load would hold that load's wait), then stores,
+// runs any further passes (a column with more than kPassGroups groups), and
+// writes the column's best (score, template id).
+__device__ __forceinline__ void col_finish(ColWork &w, bool has_next, ColWork &next,
+                                           const ColJob &nJ, bool all, float *__restrict__ Bt,
+                                           int ldbt, int K,
+                                           int G, const float *__restrict__ cc,
+                                           const float *s_nm, const int *s_pid,
+                                           u64 *__restrict__ gkey, float *__restrict__ cfmax,
+                                           int *__restrict__ imax) {
+    u64 *gk = gkey + (size_t)w.J.t * G;
+    const u64 key = pass_compute(w.ps, w.J, K, s_nm, s_pid);
+    if (has_next) col_begin(next, nJ, all, G, Bt, ldbt, cc, gkey);
+    pass_store(w.ps, w.J, Bt, ldbt, gk, key, w.acc);
+    const int N = w.L.count();
+    for (int j0 = kPassGroups; j0 < N; j0 += kPassGroups) {
+        ColPass ps;
+        pass_load(ps, w.J, w.L, j0, Bt, ldbt, cc);
+        const u64 k2 = pass_compute(ps, w.J, K, s_nm, s_pid);
+        pass_store(ps, w.J, Bt, ldbt, gk, k2, w.acc);
+    }
+    const u64 best = wave_max_u64(w.acc);
+    if ((threadIdx.x & 63) == 0) {
+        cfmax[w.J.t] = key_score(best);
+        imax[w.J.t] = key_id(best);
+    }
+}
+
+// Per-position norms and template ids (padded to Kg) into LDS.
+__device__ __forceinline__ void stage_tables(const float *__restrict__ pnm,
+                                             const int *__restrict__ ppid, int Kg, float *s_nm,
+                                             int *s_pid) {
+    for (int r = threadIdx.x; r < Kg; r += blockDim.x) {
+        s_nm[r] = pnm[r];
+        s_pid[r] = ppid[r];
+    }
+}
+
+// Per-position norms and template ids, padded with (1, 0) to G * kGroup
+// positions.
 __global__ void position_tables_kernel(const float *__restrict__ nm,
                                        const int32_t *__restrict__ perm, int K, int Kg,
-                                       float *__restrict__ pnm, float *__restrict__ pinv,
-                                       int *__restrict__ ppid) {
+                                       float *__restrict__ pnm, int *__restrict__ ppid) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= Kg) return;
     if (r < K) {
         const int id = perm[r];
-        const float n = nm[id];
-        pnm[r] = n;
-        pinv[r] = 1.0f / n;
+        pnm[r] = nm[id];
         ppid[r] = id;
     } else {
         pnm[r] = 1.f;
-        pinv[r] = 0.f;
         ppid[r] = 0;
     }
 }
@@ -295,27 +379,41 @@ __global__ void peel_init_kernel(int *state, int n_state) {
 }
 
 // Group keys and best score of every live column (first pass, no updates);
-// one wave per column.
+// a block takes kFirstCols columns, a wave every fourth one, pipelined.
+constexpr int kFirstCols = 64;
 __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
     float *__restrict__ Bt, int ldbt, const float *__restrict__ pnm,
-    const float *__restrict__ pinv, const int *__restrict__ ppid, int K, int G, int T, int nt,
+    const int *__restrict__ ppid, int K, int G, int Kg, int T, int nt,
     float *__restrict__ cfmax, int *__restrict__ imax, u64 *__restrict__ gkey) {
-    __shared__ u64 s_key[kUpdThreads / 64][kMaxG];
+    extern __shared__ float s_tab[];        // s_nm[Kg], s_pid[Kg]
+    float *s_nm = s_tab;
+    int *s_pid = reinterpret_cast<int *>(s_tab + Kg);
+    stage_tables(pnm, ppid, Kg, s_nm, s_pid);
+    __syncthreads();
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (kUpdThreads / 64) + wv;
-    if (t >= T) return;
-    if (t >= nt && t < T - nt) {
-        const ColSpike none = {0, 0, 0, 0.f};
-        const u64 best = column_groups(Bt, ldbt, t, K, G, Bt, pnm, pinv, ppid, none, none,
-                                       gkey + (size_t)t * G, true, s_key[wv]);
-        if (lane == 0) {
-            cfmax[t] = key_score(best);
-            imax[t] = key_id(best);
+    const ColSpike none = {0, 0, 0, 0.f};
+    const int t0 = blockIdx.x * kFirstCols;
+    const int tend = min(t0 + kFirstCols, T);
+    for (int t = t0 + wv; t < tend; t += kUpdThreads / 64)
+        if (lane == 0 && (t < nt || t >= T - nt)) {
+            cfmax[t] = 0.f;
+            imax[t] = 0;
         }
-    } else if (lane == 0) {
-        cfmax[t] = 0.f;
-        imax[t] = 0;
+    // live columns of the block handled by this wave
+    const int lo = max(t0, nt), hi = min(tend, T - nt);
+    int t = lo + wv;
+    if (t >= hi) return;
+    ColWork cur, nxt;
+    col_begin(cur, ColJob{t, none, none}, true, G, Bt, ldbt, Bt, gkey);
+    for (;;) {
+        const int tn = t + kUpdThreads / 64;
+        const bool more = tn < hi;
+        col_finish(cur, more, nxt, ColJob{tn, none, none}, true, Bt, ldbt, K, G, Bt, s_nm,
+                   s_pid, gkey, cfmax, imax);
+        if (!more) break;
+        cur = nxt;
+        t = tn;
     }
 }
 
@@ -328,7 +426,7 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
 //   b = sum_c sum_k Xres[c, x + k - nt/2] * (sum_p U[id,p,c] W[p,k]),
 // and b replaces the score entry, the amplitude (b/nm) and th_amp
 // (sqrt(relu(b)^2/nm)), so that what is recorded and subtracted is FP32.
-// Work split: lanes = lags, wave w takes channels clo+w, clo+w+4, ... of the
+// Work split: lanes = lags, wave w takes blocks of 16 channels of the
 // template's nonzero channel range; partial sums meet in LDS.
 __global__ __launch_bounds__(kChunk) void peel_select_kernel(
     const float *__restrict__ cfmax, const int *__restrict__ imax, float *__restrict__ Bt,
@@ -412,53 +510,56 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
         }
         chunk_count[blockIdx.x] = count < kSlotCap ? count : kSlotCap;
     }
-    if (!rescore || count == 0) return;      // block-uniform
     const int n = count < kSlotCap ? count : kSlotCap;
     __syncthreads();
     const int h = nt / 2;
-    for (int q = 0; q < n; ++q) {
+    for (int q = 0; q < (rescore ? n : 0); ++q) {
         const int sx = uni(s_sx[q]), id = uni(s_sid[q]);
         const int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
         const float *uj = U + (size_t)id * P * C;
         float part = 0.f;
-        for (int kb0 = 0; kb0 < nt; kb0 += 64) {
-            const int k = kb0 + lane;
-            const bool kin = k < nt;
-            float wk[8];
+        // wave w: channel blocks [cs, cs + 16) with cs = clo + 16 w + 64 i; lane l
+        // holds U[id, :, cs + l], broadcast per channel with readlane
+        for (int cs = clo + 16 * uni(wave); cs < chi; cs += 64) {
+            const int nch = min(chi - cs, 16);
+            float ul[8];
 #pragma unroll
-            for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
-            const float *xb = Xres + sx - h + (kin ? k : 0);
-            // wave w: channels [cb, cb + 16) with cb = clo + 16 w + 64 i (uniform,
-            // so the template coefficients are scalar loads); loads are clamped
-            // to the last channel and masked, with no branch between them
-            for (int cb = clo + 16 * uni(wave); cb < chi; cb += 64) {
-                float xv[16], tm[16];
+            for (int p = 0; p < 8; ++p)
+                ul[p] = (p < P && lane < nch) ? uj[(size_t)p * C + cs + lane] : 0.f;
+            for (int kb0 = 0; kb0 < nt; kb0 += 64) {
+                const int k = kb0 + lane;
+                const bool kin = k < nt;
+                float wk[8];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const int ch = cb + u < chi ? cb + u : chi - 1;
-                    xv[u] = xb[(size_t)ch * ldx];
-                    tm[u] = 0.f;
-                }
+                for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
+                const float *xb = Xres + sx - h + (kin ? k : 0) + (size_t)cs * ldx;
+                for (int cb = 0; cb < nch; cb += 16) {
+                    float xv[16];
 #pragma unroll
-                for (int p = 0; p < 8; ++p) {
-                    if (p < P) {
+                    for (int u = 0; u < 16; ++u) {
+                        const int c = cb + u < nch ? cb + u : nch - 1;
+                        xv[u] = xb[(size_t)c * ldx];
+                    }
 #pragma unroll
-                        for (int u = 0; u < 16; ++u) {
-                            const int ch = cb + u < chi ? cb + u : chi - 1;
-                            tm[u] = __builtin_fmaf(uj[(size_t)p * C + ch], wk[p], tm[u]);
-                        }
+                    for (int u = 0; u < 16; ++u) {
+                        const int c = cb + u < nch ? cb + u : nch - 1;
+                        float tm = 0.f;
+#pragma unroll
+                        for (int p = 0; p < 8; ++p)
+                            if (p < P)
+                                tm = __builtin_fmaf(__int_as_float(__builtin_amdgcn_readlane(
+                                                        __float_as_int(ul[p]), c)),
+                                                    wk[p], tm);
+                        part = __builtin_fmaf((kin && cb + u < nch) ? xv[u] : 0.f, tm, part);
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < 16; ++u)
-                    part = __builtin_fmaf((kin && cb + u < chi) ? xv[u] : 0.f, tm[u], part);
             }
         }
         for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
         if (lane == 0) s_part[q][wave] = part;
     }
     __syncthreads();
-    if (tid < n) {
+    if (rescore && tid < n) {
         const int sx = s_sx[tid], id = s_sid[tid];
         const float b = (s_part[tid][0] + s_part[tid][1]) + (s_part[tid][2] + s_part[tid][3]);
         const float nmv = nm[id];
@@ -467,6 +568,7 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
         sl->tha = sqrtf(score(b, nmv));
         Bt[(size_t)sx * ldbt + rank[id]] = b;
     }
+
 }
 
 // Spike q of the current iteration (time order) from the per-chunk slots;
@@ -489,17 +591,16 @@ struct SpikeG {
     long long off;
 };
 
-__device__ __forceinline__ SpikeG spike_gram(const Slot &sl, const int32_t *__restrict__ rank,
-                                             const int32_t *__restrict__ lohi,
-                                             const int64_t *__restrict__ off) {
-    const int jr = uni(rank[uni(sl.iy)]);
-    SpikeG g;
-    g.x = uni(sl.x);
-    g.lo = uni(lohi[2 * jr]);
-    g.hi = uni(lohi[2 * jr + 1]);
-    g.amp = uni(sl.amp);
-    g.off = uni((long long)off[jr]);
-    return g;
+__device__ __forceinline__ SpikeG spike_gram(const Slot &sl, const GramRef *__restrict__ gref) {
+    const int id = uni(sl.iy);
+    const GramRef g = gref[id];
+    SpikeG s;
+    s.x = uni(sl.x);
+    s.amp = uni(sl.amp);
+    s.lo = uni(g.lo);
+    s.hi = uni(g.hi);
+    s.off = uni((long long)g.off);
+    return s;
 }
 
 // The spike's Gram row for column t (lag t - x + nt).
@@ -512,21 +613,88 @@ __device__ __forceinline__ ColSpike col_spike(const SpikeG &g, int t, int nt) {
     return cs;
 }
 
+// Walks a wave's range of column items (spike q, lag d), skipping columns
+// that are never scored again (< nt, >= T - nt) or owned by the previous
+// spike, and yields the next column with its covering spikes.
+struct ColCursor {
+    int i, i1, q, d;        // item i = q * per_spike + d
+    int per_spike, ncol, nt, T, S, nchunks;
+    int cq;
+    SpikeG gq, gn;
+    bool has_p, has_n, has_nn;
+    int x_p, x_nn;
+};
+
+// Returns false at the end of the range; sets ties when three spikes fall
+// within 2nt+1 columns (only exact score ties place accepted spikes so close).
+__device__ __forceinline__ bool next_column(ColCursor &c, const int *pre,
+                                            const Slot *__restrict__ slots,
+                                            const GramRef *__restrict__ gref, ColJob &J,
+                                            bool &ties) {
+    for (; c.i < c.i1; ++c.i) {
+        const int q = c.q, d = c.d;
+        if (++c.d == c.per_spike) {
+            c.d = 0;
+            ++c.q;
+        }
+        if (d >= c.ncol) continue;
+        if (q != c.cq) {
+            c.cq = q;
+            c.gq = spike_gram(spike_at(q, pre, c.nchunks, slots), gref);
+            c.has_p = q > 0;
+            if (c.has_p) c.x_p = uni(spike_at(q - 1, pre, c.nchunks, slots).x);
+            c.has_n = q + 1 < c.S;
+            c.has_nn = q + 2 < c.S;
+            if (c.has_nn) c.x_nn = uni(spike_at(q + 2, pre, c.nchunks, slots).x);
+            if (c.has_n) c.gn = spike_gram(spike_at(q + 1, pre, c.nchunks, slots), gref);
+        }
+        const int t = c.gq.x - c.nt + d;
+        if (t < c.nt || t >= c.T - c.nt) continue;      // never scored again
+        if (c.has_p && t <= c.x_p + c.nt) continue;     // spike q-1 owns it
+        if (c.has_nn && t >= c.x_nn - c.nt) {
+            ties = true;
+            continue;
+        }
+        const ColSpike none = {0, 0, 0, 0.f};
+        const ColSpike sq_ = col_spike(c.gq, t, c.nt);
+        const ColSpike sn_ = (c.has_n && t >= c.gn.x - c.nt) ? col_spike(c.gn, t, c.nt) : none;
+        // the even-ranked spike of the pair is subtracted first
+        const bool q_even = (q & 1) == 0;
+        J.t = t;
+        J.a = q_even ? sq_ : sn_;
+        J.b = q_even ? sn_ : sq_;
+        ++c.i;
+        return true;
+    }
+    return false;
+}
+
+// One iteration's subtraction.  Work items per spike q (time order): 2nt+1
+// columns of its window, then kXresItems residual items; each wave takes one
+// contiguous range of items.  A column is owned by the earliest spike whose
+// window covers it; its owner applies that spike and the next one if it
+// covers the column too (the even-ranked of the two first, as the
+// reference's two interleaved phases do).  A wave runs its columns as a
+// pipeline (the next column's loads are issued before the current column's
+// stores), then its residual items: each subtracts amp * (U[id]^T W) from
+// its share of the template's nonzero channels in Xres[:, x-nt/2 .. x+nt/2]
+// (the windows of one iteration's spikes are disjoint).  The block also
+// records the spikes into st / amps / th_amps.
 __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int it, float *__restrict__ Bt, int ldbt, float *__restrict__ Xres, int ldx,
     const float *__restrict__ U, const float *__restrict__ W, const int *__restrict__ crange,
-    int P, int C, const float *__restrict__ pnm, const float *__restrict__ pinv,
-    const int *__restrict__ ppid, const int32_t *__restrict__ rank,
-    const float *__restrict__ cc, const int32_t *__restrict__ lohi,
-    const int64_t *__restrict__ off, int K, int G, int T, int nt, int nchunks,
-    const int *__restrict__ chunk_count, const Slot *__restrict__ slots,
+    int P, int C, const float *__restrict__ pnm, const int *__restrict__ ppid,
+    const GramRef *__restrict__ gref, const float *__restrict__ cc, int K, int G, int Kg, int T,
+    int nt, int nchunks, const int *__restrict__ chunk_count, const Slot *__restrict__ slots,
     int *__restrict__ state, int64_t *__restrict__ st, float *__restrict__ amps,
     float *__restrict__ th_amps, int capacity, float *__restrict__ cfmax,
     int *__restrict__ imax, u64 *__restrict__ gkey) {
     if (state[ST_DONE]) return;
-    extern __shared__ int s_pre[];          // nchunks + 1 entries
+    extern __shared__ float s_tab[];        // s_nm[Kg], s_pid[Kg], s_pre[nchunks + 1]
+    float *s_nm = s_tab;
+    int *s_pid = reinterpret_cast<int *>(s_tab + Kg);
+    int *s_pre = s_pid + Kg;
     __shared__ int s_wsum[kUpdThreads / 64];
-    __shared__ u64 s_key[kUpdThreads / 64][kMaxG];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
@@ -552,7 +720,6 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     }
     const int S = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
     if (tid == 0) s_pre[nchunks] = S;
-    __syncthreads();
 
     if (S == 0) {
         if (blockIdx.x == 0 && tid == 0) atomicExch(&state[ST_DONE], 1);
@@ -570,114 +737,140 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         state[ST_KBASE + it + 1] = kb + S;
         state[ST_ITERS] = it + 1;
     }
-
-    // work items: per spike, 2nt+1 columns then kXresItems residual items;
-    // each wave takes one contiguous range of items
     const int ncol = 2 * nt + 1;
     const int per_spike = ncol + kXresItems;
-    const long long n_items = (long long)S * per_spike;
-    const long long nwaves = (long long)gridDim.x * (kUpdThreads / 64);
-    const long long gw = (long long)blockIdx.x * (kUpdThreads / 64) + wv;
-    const long long per = (n_items + nwaves - 1) / nwaves;
-    const long long i0 = gw * per, i1 = min(n_items, i0 + per);
-    int cq = -1;
-    Slot sq = {0, 0, 0.f, 0.f};
-    SpikeG gq = {0, 0, 0, 0.f, 0}, gn = gq;
-    bool has_p = false, has_n = false, has_nn = false;
-    int x_p = 0, x_nn = 0;
-    const int h = nt / 2;
-    for (long long i = i0; i < i1; ++i) {
-        const int q = (int)(i / per_spike), d = (int)(i % per_spike);
-        if (q != cq) {
-            cq = q;
-            sq = spike_at(q, s_pre, nchunks, slots);
-            has_p = q > 0;
-            if (has_p) x_p = uni(spike_at(q - 1, s_pre, nchunks, slots).x);
-            has_n = q + 1 < S;
-            has_nn = q + 2 < S;
-            if (has_nn) x_nn = uni(spike_at(q + 2, s_pre, nchunks, slots).x);
-            gq = spike_gram(sq, rank, lohi, off);
-            if (has_n) gn = spike_gram(spike_at(q + 1, s_pre, nchunks, slots), rank, lohi, off);
+    const int n_items = S * per_spike;          // < 2^31: S <= nchunks * kSlotCap
+    const int nwaves = gridDim.x * (kUpdThreads / 64);
+    const int per = (n_items + nwaves - 1) / nwaves;
+    if ((long long)blockIdx.x * (kUpdThreads / 64) * per >= n_items) return;   // no work here
+    stage_tables(pnm, ppid, Kg, s_nm, s_pid);
+    __syncthreads();
+
+    // record this iteration's spikes: block b takes spikes b, b + grid, ...
+    for (int q = blockIdx.x * kUpdThreads + tid; q < S; q += gridDim.x * kUpdThreads) {
+        const Slot sl = spike_at(q, s_pre, nchunks, slots);
+        const size_t pos = (size_t)kb + q;
+        st[2 * pos] = sl.x;
+        st[2 * pos + 1] = sl.iy;
+        amps[pos] = sl.amp;
+        th_amps[pos] = sl.tha;
+    }
+
+    const int gw = blockIdx.x * (kUpdThreads / 64) + wv;
+    const int i0 = (int)min((long long)gw * per, (long long)n_items);
+    const int i1 = min(n_items, i0 + per);
+    if (i0 >= i1) return;
+
+    // columns, pipelined
+    {
+        ColCursor cur;
+        cur.i = i0;
+        cur.i1 = i1;
+        cur.q = i0 / per_spike;
+        cur.d = i0 % per_spike;
+        cur.per_spike = per_spike;
+        cur.ncol = ncol;
+        cur.nt = nt;
+        cur.T = T;
+        cur.S = S;
+        cur.nchunks = nchunks;
+        cur.cq = -1;
+        cur.has_p = cur.has_n = cur.has_nn = false;
+        cur.x_p = cur.x_nn = 0;
+        bool ties = false;
+        ColJob J, nJ;
+        if (next_column(cur, s_pre, slots, gref, J, ties)) {
+            ColWork a, b;
+            col_begin(a, J, false, G, Bt, ldbt, cc, gkey);
+            for (;;) {
+                const bool more = next_column(cur, s_pre, slots, gref, nJ, ties);
+                col_finish(a, more, b, nJ, false, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
+                           imax);
+                if (!more) break;
+                a = b;
+            }
         }
-        if (d < ncol) {
-            // column t, owned by the earliest spike whose window covers it
-            const int t = gq.x - nt + d;
-            if (t < nt || t >= T - nt) continue;        // never scored again
-            if (has_p && t <= x_p + nt) continue;       // spike q-1 owns it
-            const bool with_n = has_n && t >= gn.x - nt;
-            if (has_nn && t >= x_nn - nt) {
-                // three spikes within 2nt+1 columns: only exact score ties
-                // place accepted spikes this close
-                if (lane == 0) {
-                    atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
-                    atomicExch(&state[ST_DONE], 1);
-                }
-                continue;
-            }
-            const ColSpike none = {0, 0, 0, 0.f};
-            const ColSpike cq_ = col_spike(gq, t, nt);
-            const ColSpike cn_ = with_n ? col_spike(gn, t, nt) : none;
-            // the even-ranked spike of the pair is subtracted first
-            const bool q_even = (q & 1) == 0;
-            const u64 best = column_groups(Bt, ldbt, t, K, G, cc, pnm, pinv, ppid,
-                                           q_even ? cq_ : cn_, q_even ? cn_ : cq_,
-                                           gkey + (size_t)t * G, false, s_key[wv]);
-            if (lane == 0) {
-                cfmax[t] = key_score(best);
-                imax[t] = key_id(best);
-            }
-        } else {
-            // residual: Xres[:, x-nt/2 .. x+nt/2] -= amp * (U[id]^T W) over the
-            // template's nonzero channel range (the windows of one iteration's
-            // spikes are disjoint); lanes = lags, kXresCh channels per step
-            const int k = d - ncol;
-            const int id = uni(sq.iy), x = uni(sq.x);
-            const float amp = uni(sq.amp);
-            if (k == 0 && lane == 0) {
-                const size_t pos = (size_t)kb + q;
-                st[2 * pos] = x;
-                st[2 * pos + 1] = id;
-                amps[pos] = amp;
-                th_amps[pos] = sq.tha;
-            }
-            const int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
-            const float *uj = U + (size_t)id * P * C;
-            for (int kb0 = 0; kb0 < nt; kb0 += 64) {
-                const int kk = kb0 + lane;
-                const bool kin = kk < nt;
-                float wk[8];
+        if (ties && lane == 0) {
+            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
+            atomicExch(&state[ST_DONE], 1);
+        }
+    }
+
+    // residual items of the range: item k of spike q takes channels
+    // [cl0, cl1) of the template's nonzero range (at most 64: lane l holds
+    // U[id, :, cl0 + l], broadcast per channel with readlane)
+    const int h = nt / 2;
+    const int qa = i0 / per_spike, qb = (i1 - 1) / per_spike;
+    for (int q = qa; q <= qb; ++q) {
+        const int k0 = max(i0 - q * per_spike - ncol, 0);
+        const int k1 = min(i1 - q * per_spike - ncol, kXresItems);
+        if (k0 >= k1) continue;
+        const Slot rs = spike_at(q, s_pre, nchunks, slots);
+        const int id = uni(rs.iy), x = uni(rs.x);
+        const float amp = uni(rs.amp);
+        const int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
+        const int cw = (chi - clo + kXresItems - 1) / kXresItems;
+        const float *uj = U + (size_t)id * P * C;
+        for (int k = k0; k < k1; ++k) {
+            const int cl0 = clo + k * cw, cl1 = min(chi, cl0 + cw);
+            for (int cs = cl0; cs < cl1; cs += 64) {
+                const int nch = min(cl1 - cs, 64);
+                float ul[8];
 #pragma unroll
-                for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + kk] : 0.f;
-                float *xb = Xres + x - h + (kin ? kk : 0);
-                for (int cb = clo + kXresCh * k; cb < chi; cb += kXresCh * kXresItems) {
-                    float tm[kXresCh], xv[kXresCh];
+                for (int p = 0; p < 8; ++p)
+                    ul[p] = (p < P && lane < nch) ? uj[(size_t)p * C + cs + lane] : 0.f;
+                for (int kb0 = 0; kb0 < nt; kb0 += 64) {
+                    const int kk = kb0 + lane;
+                    const bool kin = kk < nt;
+                    float wk[8];
 #pragma unroll
-                    for (int u = 0; u < kXresCh; ++u) {
-                        const int c = cb + u < chi ? cb + u : chi - 1;
-                        xv[u] = xb[(size_t)c * ldx];
-                        tm[u] = 0.f;
-                    }
+                    for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + kk] : 0.f;
+                    float *xb = Xres + x - h + (kin ? kk : 0) + (size_t)cs * ldx;
+                    for (int cb = 0; cb < nch; cb += kXresCh) {
+                        // all loads of the step first (clamped), then compute and
+                        // masked stores
+                        float xv[kXresCh];
 #pragma unroll
-                    for (int p = 0; p < 8; ++p) {
-                        if (p < P) {
-#pragma unroll
-                            for (int u = 0; u < kXresCh; ++u) {
-                                const int c = cb + u < chi ? cb + u : chi - 1;
-                                tm[u] = __builtin_fmaf(uj[(size_t)p * C + c], wk[p], tm[u]);
-                            }
+                        for (int u = 0; u < kXresCh; ++u) {
+                            const int c = cb + u < nch ? cb + u : nch - 1;
+                            xv[u] = xb[(size_t)c * ldx];
                         }
-                    }
 #pragma unroll
-                    for (int u = 0; u < kXresCh; ++u) {
-                        if (cb + u < chi && kin) {
-                            const float prod = amp * tm[u];
-                            xb[(size_t)(cb + u) * ldx] = xv[u] - prod;
+                        for (int u = 0; u < kXresCh; ++u) {
+                            const int c = cb + u < nch ? cb + u : nch - 1;
+                            float tm = 0.f;
+#pragma unroll
+                            for (int p = 0; p < 8; ++p)
+                                if (p < P)
+                                    tm = __builtin_fmaf(
+                                        __int_as_float(__builtin_amdgcn_readlane(
+                                            __float_as_int(ul[p]), c)),
+                                        wk[p], tm);
+                            if (cb + u < nch && kin) {
+                                const float prod = amp * tm;
+                                xb[(size_t)c * ldx] = xv[u] - prod;
+                            }
                         }
                     }
                 }
             }
         }
     }
+}
+
+// Gram block of every template, by template id.
+__global__ void gram_refs_kernel(const int32_t *__restrict__ rank,
+                                 const int32_t *__restrict__ lohi,
+                                 const int64_t *__restrict__ off, int K,
+                                 GramRef *__restrict__ gref) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= K) return;
+    const int jr = rank[j];
+    GramRef g;
+    g.lo = lohi[2 * jr];
+    g.hi = lohi[2 * jr + 1];
+    g.off = off[jr];
+    gref[j] = g;
 }
 
 // crange[2j], crange[2j+1] = first and one-past-last channel where template
@@ -780,8 +973,9 @@ size_t ksm_peel_workspace(int K, int T, int max_peels) {
     if (K <= 0 || T <= 0 || max_peels < 0) return 0;
     const PeelLayout L = peel_layout(K, T, max_peels);
     return ksm::align_up((size_t)T * L.G * 8, 256) + ksm::align_up((size_t)T * 4, 256) * 2 +
-           ksm::align_up((size_t)L.Kg * 4, 256) * 3 + ksm::align_up(L.nchunks * 4, 256) +
+           ksm::align_up((size_t)L.Kg * 4, 256) * 2 + ksm::align_up(L.nchunks * 4, 256) +
            ksm::align_up((size_t)L.nchunks * kSlotCap * sizeof(Slot), 256) +
+           ksm::align_up((size_t)K * sizeof(GramRef), 256) +
            ksm::align_up((size_t)L.n_state * 4, 256) + ksm::align_up(2 * (size_t)K * 4, 256) +
            256;
 }
@@ -806,10 +1000,10 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     float *cfmax = cv.take<float>(T);
     int *imax = cv.take<int>(T);
     float *pnm = cv.take<float>(L.Kg);
-    float *pinv = cv.take<float>(L.Kg);
     int *ppid = cv.take<int>(L.Kg);
     int *chunk_count = cv.take<int>(L.nchunks);
     Slot *slots = cv.take<Slot>((size_t)L.nchunks * kSlotCap);
+    GramRef *gref = cv.take<GramRef>(K);
     int *state = cv.take<int>(L.n_state);
     int *crange = cv.take<int>(2 * (size_t)K);
 
@@ -820,20 +1014,22 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     const float th2_lo = (float)(th2d - fmax(4e-9, fabs(th2d) * 1e-6));
     // the early-exit window scan is exact when every candidate is >= 2^-6
     const int early_exit = th2_lo >= 0.015625f ? 1 : 0;
-    const size_t upd_dyn = (L.nchunks + 1) * sizeof(int);
+    const size_t tab_dyn = (size_t)L.Kg * (sizeof(float) + sizeof(int));
+    const size_t upd_dyn = tab_dyn + (size_t)(L.nchunks + 1) * sizeof(int);
     const int grid = update_grid(upd_dyn);
 
     peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
     KSM_LAUNCH_CHECK("peel_init_kernel");
     channel_range_kernel<<<(K + 3) / 4, 256, 0, s>>>(U, K, P, C, crange);
     KSM_LAUNCH_CHECK("channel_range_kernel");
-    position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, pinv,
-                                                               ppid);
+    position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, ppid);
+    KSM_LAUNCH_CHECK("position_tables_kernel");
+    gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, K, gref);
     KSM_LAUNCH_CHECK("position_tables_kernel");
     {
         ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
-        colmax_all_kernel<<<(T + 3) / 4, kUpdThreads, 0, s>>>(Bt, ldbt, pnm, pinv, ppid, K,
-                                                              L.G, T, nt, cfmax, imax, gkey);
+        colmax_all_kernel<<<(T + kFirstCols - 1) / kFirstCols, kUpdThreads, tab_dyn, s>>>(
+            Bt, ldbt, pnm, ppid, K, L.G, L.Kg, T, nt, cfmax, imax, gkey);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
     for (int it = 0; it < max_peels; ++it) {
@@ -847,9 +1043,9 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
             peel_update_kernel<<<grid, kUpdThreads, upd_dyn, s>>>(
-                it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, pinv, ppid, rank, cc, lohi,
-                off, K, L.G, T, nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps,
-                capacity, cfmax, imax, gkey);
+                it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, gref, cc, K, L.G, L.Kg, T,
+                nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax,
+                imax, gkey);
         }
     }
     KSM_LAUNCH_CHECK("peel loop");
