@@ -35,6 +35,8 @@
 //                group keys into the column's best score.
 // Once an iteration accepts nothing the done flag makes every later launch
 // return at once, so the host never waits inside the loop.
+#include <chrono>
+
 #include "ksm_common.h"
 #include "ksm_profile.h"
 
@@ -103,6 +105,11 @@ __device__ __forceinline__ u64 make_key(float cf, int id) {
 }
 __device__ __forceinline__ float key_score(u64 k) { return __uint_as_float((unsigned)(k >> 32)); }
 __device__ __forceinline__ int key_id(u64 k) { return (int)(~(unsigned)k); }
+
+// A progress word for the host, in pinned host memory (system scope).
+__device__ __forceinline__ void publish(u64 *p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int o) {
     const unsigned lo = __shfl_xor((unsigned)v, o, 64);
@@ -688,8 +695,13 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int nt, int nchunks, const int *__restrict__ chunk_count, const Slot *__restrict__ slots,
     int *__restrict__ state, int64_t *__restrict__ st, float *__restrict__ amps,
     float *__restrict__ th_amps, int capacity, float *__restrict__ cfmax,
-    int *__restrict__ imax, u64 *__restrict__ gkey) {
-    if (state[ST_DONE]) return;
+    int *__restrict__ imax, u64 *__restrict__ gkey, u64 *hflags, unsigned seq) {
+    if (state[ST_DONE]) {
+        if (hflags && blockIdx.x == 0 && threadIdx.x == 0) publish(hflags + 1, seq);
+        return;
+    }
+    if (hflags && blockIdx.x == 0 && threadIdx.x == 0)
+        publish(hflags, ((u64)seq << 32) | (unsigned)it);
     extern __shared__ float s_tab[];        // s_nm[Kg], s_pid[Kg], s_pre[nchunks + 1]
     float *s_nm = s_tab;
     int *s_pid = reinterpret_cast<int *>(s_tab + Kg);
@@ -722,7 +734,10 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     if (tid == 0) s_pre[nchunks] = S;
 
     if (S == 0) {
-        if (blockIdx.x == 0 && tid == 0) atomicExch(&state[ST_DONE], 1);
+        if (blockIdx.x == 0 && tid == 0) {
+            atomicExch(&state[ST_DONE], 1);
+            if (hflags) publish(hflags + 1, seq);
+        }
         return;
     }
     const int kb = state[ST_KBASE + it];
@@ -965,6 +980,61 @@ int update_grid(size_t dyn) {
     return c_grid;
 }
 
+// Host-side end of the peel loop.  The update kernel of iteration it writes
+// (seq, it) to flags[0] when it starts and seq to flags[1] once an iteration
+// has accepted nothing; the host keeps at most kLag iterations enqueued ahead
+// of the device and stops enqueuing once the done word carries its call's
+// sequence number (words from earlier calls carry older numbers).  The flags
+// live in pinned host memory, one pair per host thread.
+constexpr int kLag = 6;
+
+struct HostFlags {
+    u64 *host = nullptr;    // host view
+    u64 *dev = nullptr;     // device view
+    unsigned seq = 0;
+};
+
+HostFlags &host_flags() {
+    static thread_local HostFlags f;
+    static thread_local bool tried = false;
+    if (!tried) {
+        tried = true;
+        void *p = nullptr;
+        if (hipHostMalloc(&p, 256,
+                          hipHostMallocCoherent | hipHostMallocMapped | hipHostMallocPortable) ==
+            hipSuccess) {
+            void *d = nullptr;
+            if (hipHostGetDevicePointer(&d, p, 0) == hipSuccess) {
+                f.host = static_cast<u64 *>(p);
+                f.dev = static_cast<u64 *>(d);
+                f.host[0] = 0;
+                f.host[1] = 0;
+            } else {
+                (void)hipHostFree(p);
+            }
+        }
+        (void)hipGetLastError();
+    }
+    return f;
+}
+
+// Waits until the device has started iteration >= want or finished the loop
+// (true: finished).  Gives up after a bounded wait (sets *gave_up).
+bool wait_progress(const HostFlags &f, unsigned seq, int want, bool *gave_up) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 0;; ++spin) {
+        const u64 done = __atomic_load_n(f.host + 1, __ATOMIC_ACQUIRE);
+        if ((unsigned)done == seq) return true;
+        const u64 pr = __atomic_load_n(f.host, __ATOMIC_ACQUIRE);
+        if ((unsigned)(pr >> 32) == seq && (int)(unsigned)pr >= want) return false;
+        if ((spin & 1023) == 1023 &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+            *gave_up = true;
+            return false;
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1032,7 +1102,18 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
             Bt, ldbt, pnm, ppid, K, L.G, L.Kg, T, nt, cfmax, imax, gkey);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
+    // stop enqueuing once the device reports the end (not under stream capture)
+    HostFlags &hf = host_flags();
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    const bool capturing =
+        hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
+    (void)hipGetLastError();
+    const bool poll = hf.host != nullptr && !capturing;
+    if (poll && ++hf.seq == 0u) ++hf.seq;      // 0 is never a live sequence number
+    const unsigned seq = poll ? hf.seq : 0u;
+    bool gave_up = false;
     for (int it = 0; it < max_peels; ++it) {
+        if (poll && !gave_up && it >= kLag && wait_progress(hf, seq, it - kLag, &gave_up)) break;
         {
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
             peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(
@@ -1045,7 +1126,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
             peel_update_kernel<<<grid, kUpdThreads, upd_dyn, s>>>(
                 it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, gref, cc, K, L.G, L.Kg, T,
                 nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax,
-                imax, gkey);
+                imax, gkey, poll ? hf.dev : nullptr, seq);
         }
     }
     KSM_LAUNCH_CHECK("peel loop");

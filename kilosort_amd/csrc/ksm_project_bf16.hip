@@ -76,18 +76,40 @@ __global__ __launch_bounds__(256) void conv_pc_bf16_kernel(
     }
     const int c = c0 + cl;
     if (c * P >= ldr) return;
+    static_assert(P % 2 == 0, "pairs of bf16 are stored as 32-bit words");
+    // the P values of (t, c) are 2P consecutive bytes at a 4-byte aligned
+    // offset (c P 2 bytes, P even): stored as P/2 32-bit words in one go
 #pragma unroll
     for (int u = 0; u < kCvTT; ++u) {
         const int t = t0 + tg * kCvTT + u;
         if (t >= Tp) break;
         const bool live = t < T && c < C;
-        bf16_t *dh = Bh + (size_t)t * ldr + c * P;
+        unsigned wh[P / 2], wl[P / 2];
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-            const float x = live ? acc[u][p] : 0.f;
-            const bf16_t h = to_bf16(x);
-            dh[p] = h;
-            if (SPLIT) Bl[(size_t)t * ldr + c * P + p] = to_bf16(x - from_bf16(h));
+        for (int p = 0; p < P; p += 2) {
+            const float x0 = live ? acc[u][p] : 0.f, x1 = live ? acc[u][p + 1] : 0.f;
+            const bf16_t h0 = to_bf16(x0), h1 = to_bf16(x1);
+            wh[p / 2] = (unsigned)h0 | ((unsigned)h1 << 16);
+            if (SPLIT) {
+                const bf16_t l0 = to_bf16(x0 - from_bf16(h0)), l1 = to_bf16(x1 - from_bf16(h1));
+                wl[p / 2] = (unsigned)l0 | ((unsigned)l1 << 16);
+            }
+        }
+        if constexpr (P == 6) {
+            *reinterpret_cast<uint3 *>(Bh + (size_t)t * ldr + c * P) =
+                make_uint3(wh[0], wh[1], wh[2]);
+            if (SPLIT)
+                *reinterpret_cast<uint3 *>(Bl + (size_t)t * ldr + c * P) =
+                    make_uint3(wl[0], wl[1], wl[2]);
+        } else {
+            unsigned *dh = reinterpret_cast<unsigned *>(Bh + (size_t)t * ldr + c * P);
+#pragma unroll
+            for (int k = 0; k < P / 2; ++k) dh[k] = wh[k];
+            if (SPLIT) {
+                unsigned *dl = reinterpret_cast<unsigned *>(Bl + (size_t)t * ldr + c * P);
+#pragma unroll
+                for (int k = 0; k < P / 2; ++k) dl[k] = wl[k];
+            }
         }
     }
 }
