@@ -51,6 +51,8 @@ constexpr int kMaxNt = 255;
 constexpr int kMaxChunks = 4096;    // T <= kChunk * kMaxChunks
 constexpr int kUpdThreads = 256;    // update / first-pass block (4 waves)
 constexpr int kXresItems = 8;       // residual work items per spike
+constexpr int kRsTasks = 32;        // re-scoring tasks (channel blocks) per spike
+constexpr int kRsThreads = 1024;    // select block when re-scoring (16 waves)
 constexpr int kXresCh = 16;         // channels per residual step (loads first)
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
@@ -433,9 +435,11 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
 //   b = sum_c sum_k Xres[c, x + k - nt/2] * (sum_p U[id,p,c] W[p,k]),
 // and b replaces the score entry, the amplitude (b/nm) and th_amp
 // (sqrt(relu(b)^2/nm)), so that what is recorded and subtracted is FP32.
-// Work split: lanes = lags, wave w takes blocks of 16 channels of the
-// template's nonzero channel range; partial sums meet in LDS.
-__global__ __launch_bounds__(kChunk) void peel_select_kernel(
+// Work split: lanes = lags; the spikes' nonzero channel ranges are cut into
+// blocks of 16 (or more) channels, one task per (spike, block), spread over
+// the block's waves; per-task sums meet in LDS and are added in block order.
+template <int NTH>
+__global__ __launch_bounds__(NTH) void peel_select_kernel(
     const float *__restrict__ cfmax, const int *__restrict__ imax, float *__restrict__ Bt,
     int ldbt, const float *__restrict__ nm, const int32_t *__restrict__ rank, int T, int nt,
     float th2, float th2_lo, int early_exit, int rescore, const float *__restrict__ Xres,
@@ -446,10 +450,12 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
     __shared__ float win[kChunk + 2 * kMaxNt];
     __shared__ int s_wc[kChunk / 64];
     __shared__ int s_sx[kSlotCap], s_sid[kSlotCap];
-    __shared__ float s_part[kSlotCap][kChunk / 64];
+    __shared__ int s_tb[kSlotCap + 1], s_clo[kSlotCap], s_cw[kSlotCap];
+    __shared__ float s_part[kSlotCap][kRsTasks];
+    constexpr int NW = NTH / 64;
     const int t0 = blockIdx.x * kChunk;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < kChunk + 2 * nt; e += kChunk) {
+    for (int e = tid; e < kChunk + 2 * nt; e += NTH) {
         const int x = t0 - nt + e;
         win[e] = (x >= 0 && x < T) ? cfmax[x] : -INFINITY;
     }
@@ -457,7 +463,7 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
     const int x = t0 + tid;
     bool acc = false;
     float cmax = 0.f;
-    if (x >= nt && x < T - nt) {
+    if (tid < kChunk && x >= nt && x < T - nt) {
         const int e0 = tid + nt;
         const float v = win[e0];
         if (v >= th2_lo) {
@@ -488,7 +494,7 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
         }
     }
     const unsigned long long m = __ballot(acc);
-    if (lane == 0) s_wc[wave] = __popcll(m);
+    if (lane == 0 && wave < kChunk / 64) s_wc[wave] = __popcll(m);
     __syncthreads();
     int off = 0, count = 0;
 #pragma unroll
@@ -519,16 +525,41 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
     }
     const int n = count < kSlotCap ? count : kSlotCap;
     __syncthreads();
+    if (!rescore || n == 0) return;          // block-uniform
+    // tasks: (spike q, channel block b); spike q's nonzero channel range is
+    // cut into s_tb[q+1] - s_tb[q] <= kRsTasks blocks of s_cw[q] channels
+    if (tid < n) {
+        const int id = s_sid[tid];
+        const int clo = crange[2 * id], chi = crange[2 * id + 1];
+        int nb = (chi - clo + 15) / 16;
+        nb = nb < kRsTasks ? nb : kRsTasks;
+        const int cw = nb > 0 ? ((chi - clo + nb - 1) / nb + 15) / 16 * 16 : 16;
+        s_clo[tid] = clo;
+        s_cw[tid] = cw;
+        s_tb[tid + 1] = nb > 0 ? (chi - clo + cw - 1) / cw : 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s_tb[0] = 0;
+        for (int q = 0; q < n; ++q) s_tb[q + 1] += s_tb[q];
+    }
+    __syncthreads();
     const int h = nt / 2;
-    for (int q = 0; q < (rescore ? n : 0); ++q) {
-        const int sx = uni(s_sx[q]), id = uni(s_sid[q]);
-        const int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
+    const int ntask = s_tb[n];
+    for (int j = uni(wave); j < ntask; j += NW) {
+        int q = 0;
+        while (s_tb[q + 1] <= j) ++q;
+        q = uni(q);
+        const int bi = j - s_tb[q];
+        const int id = uni(s_sid[q]), sx = uni(s_sx[q]);
+        const int chi = uni(crange[2 * id + 1]);
+        const int cw = uni(s_cw[q]);
+        const int ca = uni(s_clo[q]) + bi * cw, cz = min(chi, ca + cw);
         const float *uj = U + (size_t)id * P * C;
         float part = 0.f;
-        // wave w: channel blocks [cs, cs + 16) with cs = clo + 16 w + 64 i; lane l
-        // holds U[id, :, cs + l], broadcast per channel with readlane
-        for (int cs = clo + 16 * uni(wave); cs < chi; cs += 64) {
-            const int nch = min(chi - cs, 16);
+        for (int cs = ca; cs < cz; cs += 16) {
+            const int nch = min(cz - cs, 16);
+            // lane l < nch holds U[id, :, cs + l], broadcast with readlane
             float ul[8];
 #pragma unroll
             for (int p = 0; p < 8; ++p)
@@ -540,42 +571,40 @@ __global__ __launch_bounds__(kChunk) void peel_select_kernel(
 #pragma unroll
                 for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
                 const float *xb = Xres + sx - h + (kin ? k : 0) + (size_t)cs * ldx;
-                for (int cb = 0; cb < nch; cb += 16) {
-                    float xv[16];
+                float xv[16];
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const int c = cb + u < nch ? cb + u : nch - 1;
-                        xv[u] = xb[(size_t)c * ldx];
-                    }
+                for (int u = 0; u < 16; ++u) {
+                    const int c = u < nch ? u : nch - 1;
+                    xv[u] = xb[(size_t)c * ldx];
+                }
 #pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const int c = cb + u < nch ? cb + u : nch - 1;
-                        float tm = 0.f;
+                for (int u = 0; u < 16; ++u) {
+                    const int c = u < nch ? u : nch - 1;
+                    float tm = 0.f;
 #pragma unroll
-                        for (int p = 0; p < 8; ++p)
-                            if (p < P)
-                                tm = __builtin_fmaf(__int_as_float(__builtin_amdgcn_readlane(
-                                                        __float_as_int(ul[p]), c)),
-                                                    wk[p], tm);
-                        part = __builtin_fmaf((kin && cb + u < nch) ? xv[u] : 0.f, tm, part);
-                    }
+                    for (int p = 0; p < 8; ++p)
+                        if (p < P)
+                            tm = __builtin_fmaf(__int_as_float(__builtin_amdgcn_readlane(
+                                                    __float_as_int(ul[p]), c)),
+                                                wk[p], tm);
+                    part = __builtin_fmaf((kin && u < nch) ? xv[u] : 0.f, tm, part);
                 }
             }
         }
         for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
-        if (lane == 0) s_part[q][wave] = part;
+        if (lane == 0) s_part[q][bi] = part;
     }
     __syncthreads();
-    if (rescore && tid < n) {
+    if (tid < n) {
         const int sx = s_sx[tid], id = s_sid[tid];
-        const float b = (s_part[tid][0] + s_part[tid][1]) + (s_part[tid][2] + s_part[tid][3]);
+        float b = 0.f;
+        for (int bi = 0; bi < s_tb[tid + 1] - s_tb[tid]; ++bi) b += s_part[tid][bi];
         const float nmv = nm[id];
         Slot *sl = slots + (size_t)blockIdx.x * kSlotCap + tid;
         sl->amp = b / nmv;
         sl->tha = sqrtf(score(b, nmv));
         Bt[(size_t)sx * ldbt + rank[id]] = b;
     }
-
 }
 
 // Spike q of the current iteration (time order) from the per-chunk slots;
@@ -1116,10 +1145,14 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         if (poll && !gave_up && it >= kLag && wait_progress(hf, seq, it - kLag, &gave_up)) break;
         {
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
-            peel_select_kernel<<<L.nchunks, kChunk, 0, s>>>(
-                cfmax, imax, Bt, ldbt, nm, rank, T, nt, th2, th2_lo, early_exit,
-                (flags & KSM_PEEL_RESCORE) ? 1 : 0, Xres, ldx, U, W, crange, P, C, state,
-                chunk_count, slots);
+            if (flags & KSM_PEEL_RESCORE)
+                peel_select_kernel<kRsThreads><<<L.nchunks, kRsThreads, 0, s>>>(
+                    cfmax, imax, Bt, ldbt, nm, rank, T, nt, th2, th2_lo, early_exit, 1, Xres,
+                    ldx, U, W, crange, P, C, state, chunk_count, slots);
+            else
+                peel_select_kernel<kChunk><<<L.nchunks, kChunk, 0, s>>>(
+                    cfmax, imax, Bt, ldbt, nm, rank, T, nt, th2, th2_lo, early_exit, 0, Xres,
+                    ldx, U, W, crange, P, C, state, chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
