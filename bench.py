@@ -121,27 +121,33 @@ def agreement(st, amps, ref):
 
 def time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, steps,
               warmup):
-    """Untimed warmup, then `steps` batches end to end (no events in the loop)."""
+    """Untimed warmup, then `steps` batches end to end -- matching, features
+    and the host copies of st / th_amps / tF, the copies of one batch
+    overlapping the next batch's device work -- with no events in the loop."""
     o = dict(ops, matching_precision=precision)
     nb = len(batches)
+    pipe = tm._BatchPipeline(o, U, ctc, iCC, iU, Ucc, device)
     for i in range(warmup):
-        tm._extract_batch(o, batches[i % nb], U, ctc, iCC, iU, Ucc, device)
+        pipe.submit(batches[i % nb])
+    pipe.flush()
     torch.cuda.synchronize()
     nsp = iters = 0
     t0 = time.perf_counter()
     for i in range(steps):
-        st, th, tf = tm._extract_batch(o, batches[(warmup + i) % nb], U, ctc, iCC, iU, Ucc,
-                                       device)
-        nsp += len(st)
+        res = pipe.submit(batches[(warmup + i) % nb])
         iters += tm.last_peel_iterations
+        if res is not None:
+            nsp += len(res[1])
+    res = pipe.flush()
+    nsp += len(res[1])
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     return dict(value=C * NT * steps / elapsed, ms_per_step=elapsed / steps * 1e3,
                 spikes_per_batch=nsp / steps, peel_iterations_per_batch=iters / steps)
 
 
-PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_rescore",
-            "peel_update"]
+PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_update",
+            "features"]
 
 
 def profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, nbatch):

@@ -187,6 +187,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
  *   tF[s,n,p] = sum_k Xres[iCC[n, iU[id_s]], t_s + k - nt/2] W[p,k]
  *             + amps[s] * Ucc[n, id_s, p]
  * st int64 [nsp][2] as produced by ksm_peel; output tF float32 [nsp][nC][P].
+ * Limits: nC <= 32, nt <= 255.
  */
 int ksm_features(const float *Xres, int ldx, const float *W,
                  const int64_t *st, const float *amps, int nsp,

@@ -953,22 +953,42 @@ __global__ void peel_finish_kernel(const int *__restrict__ state, int32_t *__res
 }
 
 // tF[s][n][p] = sum_k Xres[iCC[n, iU[id]], x + k - nt/2] W[p,k] + amps[s] Ucc[n,id,p]
-__global__ __launch_bounds__(256) void features_kernel(
+// One wave per spike: lanes = lags load the spike's nC residual rows
+// (coalesced) into LDS, then lane j < nC*P computes feature (n, p) = (j / P,
+// j % P) from LDS.  W sits in LDS for the whole block.
+constexpr int kFeatWaves = 4;
+constexpr int kFeatMaxNC = 32;
+__global__ __launch_bounds__(kFeatWaves * 64) void features_kernel(
     const float *__restrict__ Xres, int ldx, const float *__restrict__ W,
     const int64_t *__restrict__ st, const float *__restrict__ amps, int nsp,
     const int64_t *__restrict__ iCC, const int64_t *__restrict__ iU, const float *__restrict__ Ucc,
     int nC, int K, int P, int C, int nt, float *__restrict__ tF) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nsp * nC) return;
-    const int s = g / nC, n = g % nC;
+    extern __shared__ float f_smem[];
+    float *s_w = f_smem;                                  // [P][nt]
+    float *s_x = f_smem + P * nt + (threadIdx.x >> 6) * nC * nt;   // this wave: [nC][nt]
+    for (int e = threadIdx.x; e < P * nt; e += blockDim.x) s_w[e] = W[e];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int s = blockIdx.x * kFeatWaves + (threadIdx.x >> 6);
+    if (s >= nsp) return;
     const int x = (int)st[2 * s];
     const int id = (int)st[2 * s + 1];
-    const int ch = (int)iCC[(size_t)n * C + iU[id]];
-    const float *xr = Xres + (size_t)ch * ldx + x - nt / 2;
+    const int iu = (int)iU[id];
+    for (int n = 0; n < nC; ++n) {
+        const int ch = (int)iCC[(size_t)n * C + iu];
+        const float *xr = Xres + (size_t)ch * ldx + x - nt / 2;
+        for (int k = lane; k < nt; k += 64) s_x[n * nt + k] = xr[k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float a = amps[s];
-    for (int p = 0; p < P; ++p) {
+    for (int j = lane; j < nC * P; j += 64) {
+        const int n = j / P, p = j % P;
+        const float *xs = s_x + n * nt;
+        const float *ws = s_w + p * nt;
         float f = 0.f;
-        for (int k = 0; k < nt; ++k) f = __builtin_fmaf(xr[k], W[p * nt + k], f);
+        for (int k = 0; k < nt; ++k) f = __builtin_fmaf(xs[k], ws[k], f);
         const float own = a * Ucc[((size_t)n * K + id) * P + p];
         tF[((size_t)s * nC + n) * P + p] = f + own;
     }
@@ -1171,14 +1191,17 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
 int ksm_features(const float *Xres, int ldx, const float *W, const int64_t *st,
                  const float *amps, int nsp, const int64_t *iCC, const int64_t *iU,
                  const float *Ucc, int nC, int K, int P, int C, int nt, float *tF, void *stream) {
-    if (nsp < 0 || nC <= 0 || K <= 0 || P <= 0 || C <= 0 || nt <= 0 || nC > C)
+    if (nsp < 0 || nC <= 0 || K <= 0 || P <= 0 || C <= 0 || nt <= 0 || nC > C ||
+        nC > kFeatMaxNC || nt > kMaxNt)
         return ksm::fail(KSM_ERR_INVALID, "ksm_features: bad arguments");
     if (nsp == 0) return KSM_OK;
     if (!Xres || !W || !st || !amps || !iCC || !iU || !Ucc || !tF)
         return ksm::fail(KSM_ERR_INVALID, "ksm_features: null pointer");
-    const int n = nsp * nC;
-    features_kernel<<<(n + 255) / 256, 256, 0, ksm::as_stream(stream)>>>(
-        Xres, ldx, W, st, amps, nsp, iCC, iU, Ucc, nC, K, P, C, nt, tF);
+    const size_t smem = (size_t)(P * nt + kFeatWaves * nC * nt) * sizeof(float);
+    ksm::ProfileScope prof("features", ksm::kProfPeel, ksm::as_stream(stream));
+    features_kernel<<<(nsp + kFeatWaves - 1) / kFeatWaves, kFeatWaves * 64, smem,
+                      ksm::as_stream(stream)>>>(Xres, ldx, W, st, amps, nsp, iCC, iU, Ucc, nC,
+                                                K, P, C, nt, tF);
     KSM_LAUNCH_CHECK("features_kernel");
     return KSM_OK;
 }

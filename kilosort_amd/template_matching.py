@@ -305,6 +305,72 @@ def _features(ops, Xres, st, amps, iCC, iU, Ucc, device):
     return tF
 
 
+class _BatchPipeline:
+    """run_matching + PC features over a sequence of batches, with each
+    batch's host copies (st, th_amps, tF) on a second stream into pinned
+    double buffers, so that they overlap the next batch's device work.
+
+    ``submit(X, tag)`` launches batch X and returns the host results of the
+    previous batch as ``(tag, st int64 [k, 2], th_amps float32 [k, 1],
+    tF float32 [k, nC, P])`` -- views into pinned buffers that stay valid
+    until the next ``submit`` -- or None after the first batch; ``flush()``
+    returns the last batch's results.
+    """
+
+    def __init__(self, ops, U, ctc, iCC, iU, Ucc, device):
+        self.ops, self.U, self.ctc = ops, U, ctc
+        self.iCC, self.iU, self.Ucc = iCC, iU, Ucc
+        self.device = device
+        self.copy_stream = torch.cuda.Stream(device=device)
+        self.bufs = [None, None]
+        self.slot = 0
+        self.pending = None
+
+    def _host(self, slot, k, nC, P):
+        b = self.bufs[slot]
+        if b is None or b[0].shape[0] < k:
+            cap = max(2 * k, 4096)
+            b = (torch.empty((cap, 2), dtype=torch.int64, pin_memory=True),
+                 torch.empty((cap, 1), dtype=torch.float32, pin_memory=True),
+                 torch.empty((cap, nC, P), dtype=torch.float32, pin_memory=True))
+            self.bufs[slot] = b
+        return b
+
+    def submit(self, X, tag=None):
+        stt, amps, th, Xres = run_matching(self.ops, X, self.U, self.ctc, device=self.device)
+        xfeat = _features(self.ops, Xres, stt, amps, self.iCC, self.iU, self.Ucc, self.device)
+        k = stt.shape[0]
+        nC, _, P = self.Ucc.shape
+        hb = self._host(self.slot, k, nC, P)
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(self.copy_stream):
+            self.copy_stream.wait_event(ready)
+            if k:
+                hb[0][:k].copy_(stt, non_blocking=True)
+                hb[1][:k].copy_(th, non_blocking=True)
+                hb[2][:k].copy_(xfeat, non_blocking=True)
+            done = torch.cuda.Event()
+            done.record(self.copy_stream)
+        prev = self.pending
+        # the device tensors stay referenced until their copies are done
+        self.pending = (tag, done, k, self.slot, (stt, th, xfeat))
+        self.slot ^= 1
+        return self._finish(prev)
+
+    def _finish(self, p):
+        if p is None:
+            return None
+        tag, done, k, slot, _refs = p
+        done.synchronize()
+        hb = self.bufs[slot]
+        return tag, hb[0][:k], hb[1][:k], hb[2][:k]
+
+    def flush(self):
+        p, self.pending = self.pending, None
+        return self._finish(p)
+
+
 def _extract_batch(ops, X, U, ctc, iCC, iU, Ucc, device):
     """run_matching + features of one padded batch; host copies of
     (st int64 [k, 2], th_amps float32 [k, 1], tF float32 [k, nC, P])."""
@@ -337,29 +403,40 @@ def extract(ops, bfile, U, device=torch.device('cuda'), progress_bar=None):
     k = 0
     n_batches = int(bfile.n_batches)
     ibatch, X, stt_h = -1, None, None
+    pipe = _BatchPipeline(ops, U, ctc, iCC, iU, Ucc, device)
+
+    def consume(res):
+        nonlocal st, tF, k, stt_h
+        jb, stt_h, th_h, xfeat_h = res
+        if jb == 0:
+            # times before the recording start (trough alignment) are dropped
+            neg = (stt_h[:, 0] - nt - nt // 2 + ops['nt0min']) < 0
+            stt_h, th_h, xfeat_h = stt_h[~neg], th_h[~neg], xfeat_h[~neg]
+        nsp = len(stt_h)
+        if k + nsp > st.shape[0]:
+            st = np.concatenate((st, np.zeros_like(st)), 0)
+            tF = torch.cat((tF, torch.zeros_like(tF)), 0)
+        t_shift = jb * bfile.batch_downsampling * (ops['batch_size'])
+        sd = stt_h.double()
+        st[k:k + nsp, 0] = ((sd[:, 0] - nt) + t_shift).numpy() - nt // 2 + ops['nt0min']
+        st[k:k + nsp, 1] = sd[:, 1].numpy()
+        st[k:k + nsp, 2] = th_h.double().numpy().reshape(-1)
+        tF[k:k + nsp] = xfeat_h
+        k += nsp
+        if progress_bar is not None:
+            progress_bar.emit(int((jb + 1) / n_batches * 100))
+
     try:
         for ibatch in range(n_batches):
             if ibatch % 100 == 0:
                 logger.debug("extract: batch %d of %d", ibatch, n_batches)
             X = bfile.padded_batch_to_torch(ibatch, ops)
-            stt_h, th_h, xfeat_h = _extract_batch(ops, X, U, ctc, iCC, iU, Ucc, device)
-            if ibatch == 0:
-                # times before the recording start (trough alignment) are dropped
-                neg = (stt_h[:, 0] - nt - nt // 2 + ops['nt0min']) < 0
-                stt_h, th_h, xfeat_h = stt_h[~neg], th_h[~neg], xfeat_h[~neg]
-            nsp = len(stt_h)
-            if k + nsp > st.shape[0]:
-                st = np.concatenate((st, np.zeros_like(st)), 0)
-                tF = torch.cat((tF, torch.zeros_like(tF)), 0)
-            t_shift = ibatch * bfile.batch_downsampling * (ops['batch_size'])
-            sd = stt_h.double()
-            st[k:k + nsp, 0] = ((sd[:, 0] - nt) + t_shift).numpy() - nt // 2 + ops['nt0min']
-            st[k:k + nsp, 1] = sd[:, 1].numpy()
-            st[k:k + nsp, 2] = th_h.double().numpy().reshape(-1)
-            tF[k:k + nsp] = xfeat_h
-            k += nsp
-            if progress_bar is not None:
-                progress_bar.emit(int((ibatch + 1) / n_batches * 100))
+            res = pipe.submit(X, tag=ibatch)
+            if res is not None:
+                consume(res)
+        res = pipe.flush()
+        if res is not None:
+            consume(res)
     except Exception:
         logger.exception(f'Error in template_matching.extract on batch {ibatch}')
         logger.debug(f'X shape: {None if X is None else tuple(X.shape)}')
