@@ -651,18 +651,42 @@ __device__ __forceinline__ ColSpike col_spike(const SpikeG &g, int t, int nt) {
 
 // Walks a wave's range of column items (spike q, lag d), skipping columns
 // that are never scored again (< nt, >= T - nt) or owned by the previous
-// spike, and yields the next column with its covering spikes.
+// spike, and yields the next column with the subtractions it takes.
+//
+// The reference subtracts each phase (even-ranked spikes, then odd-ranked)
+// with one indexed write, so where windows of one phase overlap -- only exact
+// score ties place accepted spikes within 2nt+1 columns of each other -- the
+// last spike's write is the one that remains (torch's non-accumulating
+// index_put on the CPU).  A column covered by spikes q .. q+m therefore takes
+// at most two subtractions: the last even-ranked and the last odd-ranked of
+// them, even first.  Up to kMaxCover spikes may cover one column.
+constexpr int kMaxCover = 4;
 struct ColCursor {
     int i, i1, q, d;        // item i = q * per_spike + d
     int per_spike, ncol, nt, T, S, nchunks;
     int cq;
-    SpikeG gq, gn;
-    bool has_p, has_n, has_nn;
-    int x_p, x_nn;
+    bool has_p, over;       // over: a (kMaxCover+1)-th spike reaches q's window
+    int x_p, x_over, ncov;  // spikes q .. q+ncov-1 reach q's window
+    int x2, x3;             // columns of spikes q+2, q+3 (when ncov > 2, 3)
+    SpikeG g0, g1;          // spikes q, q+1; spikes q+2, q+3 in s_far (ties only)
+    SpikeG *s_far;          // this wave's LDS slots
 };
 
-// Returns false at the end of the range; sets ties when three spikes fall
-// within 2nt+1 columns (only exact score ties place accepted spikes so close).
+__device__ __forceinline__ SpikeG cover_at(const ColCursor &c, int j) {
+    if (j == 0) return c.g0;
+    if (j == 1) return c.g1;
+    const SpikeG f = c.s_far[j - 2];
+    SpikeG r;
+    r.x = uni(f.x);
+    r.lo = uni(f.lo);
+    r.hi = uni(f.hi);
+    r.amp = uni(f.amp);
+    r.off = uni(f.off);
+    return r;
+}
+
+// Returns false at the end of the range; sets ties when more than kMaxCover
+// spikes cover one column.
 __device__ __forceinline__ bool next_column(ColCursor &c, const int *pre,
                                             const Slot *__restrict__ slots,
                                             const GramRef *__restrict__ gref, ColJob &J,
@@ -676,45 +700,152 @@ __device__ __forceinline__ bool next_column(ColCursor &c, const int *pre,
         if (d >= c.ncol) continue;
         if (q != c.cq) {
             c.cq = q;
-            c.gq = spike_gram(spike_at(q, pre, c.nchunks, slots), gref);
+            c.g0 = spike_gram(spike_at(q, pre, c.nchunks, slots), gref);
             c.has_p = q > 0;
             if (c.has_p) c.x_p = uni(spike_at(q - 1, pre, c.nchunks, slots).x);
-            c.has_n = q + 1 < c.S;
-            c.has_nn = q + 2 < c.S;
-            if (c.has_nn) c.x_nn = uni(spike_at(q + 2, pre, c.nchunks, slots).x);
-            if (c.has_n) c.gn = spike_gram(spike_at(q + 1, pre, c.nchunks, slots), gref);
+            // later spikes whose windows reach this spike's window
+            const int reach = c.g0.x + 2 * c.nt;
+            c.ncov = 1;
+            c.over = false;
+            for (int j = 1; j <= kMaxCover && q + j < c.S; ++j) {
+                const Slot sj = spike_at(q + j, pre, c.nchunks, slots);
+                if (uni(sj.x) > reach) break;
+                if (j == kMaxCover) {
+                    c.over = true;
+                    c.x_over = uni(sj.x);
+                    break;
+                }
+                const SpikeG gj = spike_gram(sj, gref);
+                if (j == 1) {
+                    c.g1 = gj;
+                } else {
+                    if ((threadIdx.x & 63) == 0) c.s_far[j - 2] = gj;
+                    __builtin_amdgcn_wave_barrier();
+                    if (j == 2) c.x2 = gj.x;
+                    else c.x3 = gj.x;
+                }
+                c.ncov = j + 1;
+            }
         }
-        const int t = c.gq.x - c.nt + d;
+        const int t = c.g0.x - c.nt + d;
         if (t < c.nt || t >= c.T - c.nt) continue;      // never scored again
-        if (c.has_p && t <= c.x_p + c.nt) continue;     // spike q-1 owns it
-        if (c.has_nn && t >= c.x_nn - c.nt) {
+        if (c.has_p && t <= c.x_p + c.nt) continue;     // an earlier spike owns it
+        if (c.over && t >= c.x_over - c.nt) {
             ties = true;
             continue;
         }
+        int m = 0;                                      // spikes q .. q+m cover t
+        if (c.ncov > 1 && t >= c.g1.x - c.nt) m = 1;
+        if (c.ncov > 2 && t >= c.x2 - c.nt) m = 2;
+        if (c.ncov > 3 && t >= c.x3 - c.nt) m = 3;
         const ColSpike none = {0, 0, 0, 0.f};
-        const ColSpike sq_ = col_spike(c.gq, t, c.nt);
-        const ColSpike sn_ = (c.has_n && t >= c.gn.x - c.nt) ? col_spike(c.gn, t, c.nt) : none;
-        // the even-ranked spike of the pair is subtracted first
-        const bool q_even = (q & 1) == 0;
+        const ColSpike sl = col_spike(cover_at(c, m), t, c.nt);
+        const ColSpike sp = m > 0 ? col_spike(cover_at(c, m - 1), t, c.nt) : none;
+        // the last even-ranked spike is subtracted first
+        const bool last_even = ((q + m) & 1) == 0;
         J.t = t;
-        J.a = q_even ? sq_ : sn_;
-        J.b = q_even ? sn_ : sq_;
+        J.a = last_even ? sl : sp;
+        J.b = last_even ? sp : sl;
         ++c.i;
         return true;
     }
     return false;
 }
 
+// Residual subtraction for a cluster of spikes whose windows overlap (gaps of
+// at most 2h = nt - 1 samples; only exact score ties produce them): spikes
+// q .. q+m-1.  As for B, each sample takes the last even-ranked and then the
+// last odd-ranked of the spikes covering it.  Work item k of the cluster's
+// first spike takes its share of the union of the templates' channel ranges;
+// lanes = samples, with per-lane template coefficients.
+constexpr int kMaxCluster = 8;
+struct ClusterMember {
+    int x, id;
+    float amp;
+};
+
+__device__ void xres_cluster(int q, int k, int S, const int *pre, int nchunks,
+                             const Slot *__restrict__ slots, float *__restrict__ Xres, int ldx,
+                             const float *__restrict__ U, const float *__restrict__ W,
+                             const int *__restrict__ crange, int P, int C, int nt,
+                             int *__restrict__ state, ClusterMember *s_cl) {
+    const int lane = threadIdx.x & 63;
+    const int h = nt / 2;
+    int m = 0, clo = C, chi = 0, prev_x = 0;
+    for (int j = 0; j < kMaxCluster && q + j < S; ++j) {
+        const Slot sj = spike_at(q + j, pre, nchunks, slots);
+        const int x = uni(sj.x), id = uni(sj.iy);
+        if (j > 0 && x - prev_x > 2 * h) break;
+        if (lane == 0) s_cl[j] = ClusterMember{x, id, uni(sj.amp)};
+        clo = min(clo, uni(crange[2 * id]));
+        chi = max(chi, uni(crange[2 * id + 1]));
+        prev_x = x;
+        m = j + 1;
+    }
+    if (m == kMaxCluster && q + m < S &&
+        uni(spike_at(q + m, pre, nchunks, slots).x) - prev_x <= 2 * h) {
+        if (lane == 0) {
+            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
+            atomicExch(&state[ST_DONE], 1);
+        }
+        return;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (chi <= clo) return;
+    const int cw = (chi - clo + kXresItems - 1) / kXresItems;
+    const int cl0 = clo + k * cw, cl1 = min(chi, cl0 + cw);
+    const int x0 = s_cl[0].x, x1 = s_cl[m - 1].x;
+    for (int s0 = x0 - h; s0 <= x1 + h; s0 += 64) {
+        const int sm = s0 + lane;
+        // covering members form a run; the last two of it are the last
+        // even-ranked and the last odd-ranked
+        int jl = -1, jf = m;
+        for (int j = 0; j < m; ++j) {
+            const int dx = sm - s_cl[j].x;
+            if (dx >= -h && dx <= h) {
+                jl = j;
+                jf = min(jf, j);
+            }
+        }
+        if (sm > x1 + h) jl = -1;
+        const int jp = (jl > jf) ? jl - 1 : -1;
+        const bool l_even = ((q + jl) & 1) == 0;
+        const int je = jl < 0 ? -1 : (l_even ? jl : jp);
+        const int jo = jl < 0 ? -1 : (l_even ? jp : jl);
+        const ClusterMember me = s_cl[je < 0 ? 0 : je], mo = s_cl[jo < 0 ? 0 : jo];
+        const int ke = sm - me.x + h, ko = sm - mo.x + h;
+        for (int c = cl0; c < cl1; ++c) {
+            if (jl < 0) continue;
+            float xv = Xres[(size_t)c * ldx + sm];
+            if (je >= 0) {
+                float tm = 0.f;
+                for (int p = 0; p < P; ++p)
+                    tm = __builtin_fmaf(U[((size_t)me.id * P + p) * C + c], W[p * nt + ke], tm);
+                const float prod = me.amp * tm;
+                xv = xv - prod;
+            }
+            if (jo >= 0) {
+                float tm = 0.f;
+                for (int p = 0; p < P; ++p)
+                    tm = __builtin_fmaf(U[((size_t)mo.id * P + p) * C + c], W[p * nt + ko], tm);
+                const float prod = mo.amp * tm;
+                xv = xv - prod;
+            }
+            Xres[(size_t)c * ldx + sm] = xv;
+        }
+    }
+}
+
 // One iteration's subtraction.  Work items per spike q (time order): 2nt+1
 // columns of its window, then kXresItems residual items; each wave takes one
 // contiguous range of items.  A column is owned by the earliest spike whose
-// window covers it; its owner applies that spike and the next one if it
-// covers the column too (the even-ranked of the two first, as the
-// reference's two interleaved phases do).  A wave runs its columns as a
+// window covers it; its owner applies the subtractions next_column yields
+// (the reference's two interleaved phases).  A wave runs its columns as a
 // pipeline (the next column's loads are issued before the current column's
 // stores), then its residual items: each subtracts amp * (U[id]^T W) from
 // its share of the template's nonzero channels in Xres[:, x-nt/2 .. x+nt/2]
-// (the windows of one iteration's spikes are disjoint).  The block also
+// (windows of one iteration's spikes are disjoint unless exact ties put
+// spikes closer than nt: those go through xres_cluster).  The block also
 // records the spikes into st / amps / th_amps.
 __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int it, float *__restrict__ Bt, int ldbt, float *__restrict__ Xres, int ldx,
@@ -736,6 +867,8 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int *s_pid = reinterpret_cast<int *>(s_tab + Kg);
     int *s_pre = s_pid + Kg;
     __shared__ int s_wsum[kUpdThreads / 64];
+    __shared__ ClusterMember s_cl[kUpdThreads / 64][kMaxCluster];
+    __shared__ SpikeG s_far[kUpdThreads / 64][kMaxCover - 2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
@@ -819,8 +952,10 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         cur.S = S;
         cur.nchunks = nchunks;
         cur.cq = -1;
-        cur.has_p = cur.has_n = cur.has_nn = false;
-        cur.x_p = cur.x_nn = 0;
+        cur.has_p = cur.over = false;
+        cur.x_p = cur.x_over = cur.x2 = cur.x3 = 0;
+        cur.ncov = 1;
+        cur.s_far = s_far[wv];
         bool ties = false;
         ColJob J, nJ;
         if (next_column(cur, s_pre, slots, gref, J, ties)) {
@@ -851,6 +986,15 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         if (k0 >= k1) continue;
         const Slot rs = spike_at(q, s_pre, nchunks, slots);
         const int id = uni(rs.iy), x = uni(rs.x);
+        // windows closer than nt samples (exact ties): the cluster's first
+        // spike handles the whole cluster
+        if (q > 0 && x - uni(spike_at(q - 1, s_pre, nchunks, slots).x) < nt) continue;
+        if (q + 1 < S && uni(spike_at(q + 1, s_pre, nchunks, slots).x) - x < nt) {
+            for (int k = k0; k < k1; ++k)
+                xres_cluster(q, k, S, s_pre, nchunks, slots, Xres, ldx, U, W, crange, P, C, nt,
+                             state, s_cl[wv]);
+            continue;
+        }
         const float amp = uni(rs.amp);
         const int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
         const int cw = (chi - clo + kXresItems - 1) / kXresItems;

@@ -68,13 +68,25 @@ def project(ops, X, U):
 
 
 def run_matching(ops, X, U, ctc, capacity=100000, record=None):
-    """Matching pursuit over one padded batch (``template_matching.py:170-244``).
+    """Matching pursuit over one padded batch (``template_matching.py:170-244``):
+    the projection B, then ``peel``."""
+    nm = template_norms(U)
+    B = project(ops, X, U)
+    return peel(ops, B, X, U, ctc, nm, capacity=capacity, record=record)
+
+
+def peel(ops, B, X, U, ctc, nm, capacity=100000, record=None):
+    """The peel loop of ``run_matching`` (``template_matching.py:183-244``)
+    from given scores B [K, T] (modified in place) and norms nm.
 
     Per iteration: Cf = relu(B)^2 / nm with the first and last nt columns
     zeroed; best template per column; a window max of width 2*nt+1; accept
     columns that exceed Th^2 and equal the window max; record (time, id),
     amps = B/nm and th_amps = sqrt(Cmax); subtract the accepted spikes from the
-    residual and their ctc columns from B in two interleaved phases.
+    residual and their ctc columns from B in two interleaved phases.  Where
+    the windows of one phase overlap (only exact score ties place accepted
+    spikes that close) the indexed subtraction keeps the last spike's write,
+    as torch's non-accumulating index_put does on the CPU.
 
     ``record`` (a list) receives per-iteration decision data for the margin
     report when given.
@@ -84,9 +96,6 @@ def run_matching(ops, X, U, ctc, capacity=100000, record=None):
     nt = ops["nt"]
     max_peels = ops["max_peels"]
     W = ops["wPCA"].contiguous()
-
-    nm = template_norms(U)
-    B = project(ops, X, U)
 
     lag_b = torch.arange(-nt, nt + 1)
     lag_x = torch.arange(-(nt // 2), nt // 2 + 1)
