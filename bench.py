@@ -175,6 +175,21 @@ def profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision,
     return kern, np.concatenate(ids) if ids else np.zeros(0, np.int64)
 
 
+TRAFFIC = os.path.join(ROOT, "profiles", "traffic_383ch_700t.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    runs on this workload (FETCH_SIZE + WRITE_SIZE, calibrated with a plain
+    copy); None when absent."""
+    try:
+        with open(TRAFFIC) as fh:
+            k = json.load(fh)["kernels"][kernel]
+        return k["read_bytes"] + k["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def peel_update_bytes(ids, nnzK, nnzC, nbatch):
     """Algorithmic HBM bytes of the subtraction per batch: for each accepted
     spike of template j, 12 B per (template i, lag) of its nonzero Gram rows
@@ -242,7 +257,9 @@ def main():
     upd_achieved = upd_bytes / (upd["ms_per_batch"] / 1e3)
     roofline = {"kernel": "peel_update_kernel (subtraction + column maxima, per batch)",
                 "bound": "hbm", "achieved": upd_achieved / 1e9, "peak": PEAK_HBM / 1e9,
-                "unit": "GB/s", "frac": upd_achieved / PEAK_HBM, "traffic": None,
+                "unit": "GB/s", "frac": upd_achieved / PEAK_HBM,
+                "traffic": measured_traffic("peel_update_kernel"),
+                "traffic_source": os.path.relpath(TRAFFIC, ROOT),
                 "bytes_per_batch": upd_bytes,
                 "bytes_per_launch": upd_bytes / upd["launches_per_batch"],
                 "avg_launch_ms": upd["avg_launch_ms"],
@@ -255,6 +272,7 @@ def main():
     roofline_projection = {"kernel": f"{gname} (projection GEMM)", "bound": "mfma",
                            "achieved": g_ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
                            "frac": g_ach / peak, "flops_per_launch": flops_gemm,
+                           "traffic": measured_traffic(f"{gname}_mfma"),
                            "avg_launch_ms": g["avg_launch_ms"]}
 
     cpu = None
