@@ -307,52 +307,73 @@ __device__ __forceinline__ u64 kept_keys(const GroupList &L, int G, const u64 *_
     return acc;
 }
 
-// A column in flight: its job, group list, the kept keys and the loads of
-// its first pass.
-struct ColWork {
+// A pass in flight: its column job, the column's group list, the pass's
+// first group j0, the kept keys (first pass of a column) and its loads.
+struct PassWork {
     ColJob J;
     GroupList L;
-    u64 acc;
+    int j0;
+    u64 kept;
     ColPass ps;
 };
 
-__device__ __forceinline__ void col_begin(ColWork &w, const ColJob &J, bool all, int G,
-                                          const float *__restrict__ Bt, int ldbt,
-                                          const float *__restrict__ cc,
-                                          const u64 *__restrict__ gkey) {
+__device__ __forceinline__ void pass_begin(PassWork &w, const ColJob &J, const GroupList &L,
+                                           int j0, bool all, int G,
+                                           const float *__restrict__ Bt, int ldbt,
+                                           const float *__restrict__ cc,
+                                           const u64 *__restrict__ gkey) {
     w.J = J;
-    w.L = group_list(J.a, J.b, G, all);
-    w.acc = all ? 0ull : kept_keys(w.L, G, gkey + (size_t)J.t * G);
-    pass_load(w.ps, J, w.L, 0, Bt, ldbt, cc);
+    w.L = L;
+    w.j0 = j0;
+    w.kept = (all || j0 > 0) ? 0ull : kept_keys(L, G, gkey + (size_t)J.t * G);
+    pass_load(w.ps, J, L, j0, Bt, ldbt, cc);
 }
 
-// Finishes a column whose first pass is loaded: computes it, then -- when
-// `has_next` -- issues the next column's loads before this column's stores
-// (a store issued ahead of a load would hold that load's wait), then stores,
-// runs any further passes (a column with more than kPassGroups groups), and
-// writes the column's best (score, template id).
-__device__ __forceinline__ void col_finish(ColWork &w, bool has_next, ColWork &next,
-                                           const ColJob &nJ, bool all, float *__restrict__ Bt,
-                                           int ldbt, int K,
-                                           int G, const float *__restrict__ cc,
-                                           const float *s_nm, const int *s_pid,
-                                           u64 *__restrict__ gkey, float *__restrict__ cfmax,
-                                           int *__restrict__ imax) {
-    u64 *gk = gkey + (size_t)w.J.t * G;
-    const u64 key = pass_compute(w.ps, w.J, K, s_nm, s_pid);
-    if (has_next) col_begin(next, nJ, all, G, Bt, ldbt, cc, gkey);
-    pass_store(w.ps, w.J, Bt, ldbt, gk, key, w.acc);
-    const int N = w.L.count();
-    for (int j0 = kPassGroups; j0 < N; j0 += kPassGroups) {
-        ColPass ps;
-        pass_load(ps, w.J, w.L, j0, Bt, ldbt, cc);
-        const u64 k2 = pass_compute(ps, w.J, K, s_nm, s_pid);
-        pass_store(ps, w.J, Bt, ldbt, gk, k2, w.acc);
-    }
-    const u64 best = wave_max_u64(w.acc);
-    if ((threadIdx.x & 63) == 0) {
-        cfmax[w.J.t] = key_score(best);
-        imax[w.J.t] = key_id(best);
+// Column passes as a pipeline over the columns `src` yields: each pass's
+// loads are issued before the previous pass's stores (a store issued ahead of
+// a load would hold that load's wait on gfx9, whose counter covers both), a
+// column with more than kPassGroups groups simply contributes several
+// consecutive passes, and after a column's last pass its best (score,
+// template id) -- over its re-reduced and kept group keys -- goes to
+// cfmax / imax.
+template <class Source>
+__device__ __forceinline__ void column_pipeline(Source &src, bool all, float *__restrict__ Bt,
+                                                int ldbt, int K, int G,
+                                                const float *__restrict__ cc, const float *s_nm,
+                                                const int *s_pid, u64 *__restrict__ gkey,
+                                                float *__restrict__ cfmax,
+                                                int *__restrict__ imax) {
+    ColJob J;
+    if (!src.next(J)) return;
+    GroupList L = group_list(J.a, J.b, G, all);
+    int j0 = 0;
+    PassWork a, b;
+    pass_begin(a, J, L, 0, all, G, Bt, ldbt, cc, gkey);
+    u64 acc = 0ull;
+    for (;;) {
+        const u64 key = pass_compute(a.ps, a.J, K, s_nm, s_pid);
+        // the next pass: the same column's next groups, or the next column
+        bool more = true;
+        if (j0 + kPassGroups < L.count()) {
+            j0 += kPassGroups;
+        } else if (src.next(J)) {
+            L = group_list(J.a, J.b, G, all);
+            j0 = 0;
+        } else {
+            more = false;
+        }
+        if (more) pass_begin(b, J, L, j0, all, G, Bt, ldbt, cc, gkey);
+        if (a.j0 == 0) acc = a.kept;
+        pass_store(a.ps, a.J, Bt, ldbt, gkey + (size_t)a.J.t * G, key, acc);
+        if (a.j0 + kPassGroups >= a.L.count()) {
+            const u64 best = wave_max_u64(acc);
+            if ((threadIdx.x & 63) == 0) {
+                cfmax[a.J.t] = key_score(best);
+                imax[a.J.t] = key_id(best);
+            }
+        }
+        if (!more) break;
+        a = b;
     }
 }
 
@@ -390,6 +411,18 @@ __global__ void peel_init_kernel(int *state, int n_state) {
 // Group keys and best score of every live column (first pass, no updates);
 // a block takes kFirstCols columns, a wave every fourth one, pipelined.
 constexpr int kFirstCols = 64;
+struct FirstPassColumns {
+    int t, hi, step;
+    __device__ __forceinline__ bool next(ColJob &J) {
+        if (t >= hi) return false;
+        const ColSpike none = {0, 0, 0, 0.f};
+        J.t = t;
+        J.a = none;
+        J.b = none;
+        t += step;
+        return true;
+    }
+};
 __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
     float *__restrict__ Bt, int ldbt, const float *__restrict__ pnm,
     const int *__restrict__ ppid, int K, int G, int Kg, int T, int nt,
@@ -410,20 +443,11 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
             imax[t] = 0;
         }
     // live columns of the block handled by this wave
-    const int lo = max(t0, nt), hi = min(tend, T - nt);
-    int t = lo + wv;
-    if (t >= hi) return;
-    ColWork cur, nxt;
-    col_begin(cur, ColJob{t, none, none}, true, G, Bt, ldbt, Bt, gkey);
-    for (;;) {
-        const int tn = t + kUpdThreads / 64;
-        const bool more = tn < hi;
-        col_finish(cur, more, nxt, ColJob{tn, none, none}, true, Bt, ldbt, K, G, Bt, s_nm,
-                   s_pid, gkey, cfmax, imax);
-        if (!more) break;
-        cur = nxt;
-        t = tn;
-    }
+    FirstPassColumns src;
+    src.t = max(t0, nt) + wv;
+    src.hi = min(tend, T - nt);
+    src.step = kUpdThreads / 64;
+    column_pipeline(src, true, Bt, ldbt, K, G, Bt, s_nm, s_pid, gkey, cfmax, imax);
 }
 
 // Accepts columns whose score is the maximum of its +-nt window and exceeds
@@ -649,33 +673,60 @@ __device__ __forceinline__ ColSpike col_spike(const SpikeG &g, int t, int nt) {
     return cs;
 }
 
-// Walks a wave's range of column items (spike q, lag d), skipping columns
-// that are never scored again (< nt, >= T - nt) or owned by the previous
-// spike, and yields the next column with the subtractions it takes.
-//
-// The reference subtracts each phase (even-ranked spikes, then odd-ranked)
-// with one indexed write, so where windows of one phase overlap -- only exact
-// score ties place accepted spikes within 2nt+1 columns of each other -- the
-// last spike's write is the one that remains (torch's non-accumulating
-// index_put on the CPU).  A column covered by spikes q .. q+m therefore takes
-// at most two subtractions: the last even-ranked and the last odd-ranked of
-// them, even first.  Up to kMaxCover spikes may cover one column.
+// Subtractions of a column.  The reference subtracts each phase
+// (even-ranked spikes, then odd-ranked) with one indexed write, so where
+// windows of one phase overlap -- only exact score ties place accepted spikes
+// within 2nt+1 columns of each other -- the last spike's write is the one that
+// remains (torch's non-accumulating index_put on the CPU).  A column covered
+// by spikes q .. q+m therefore takes at most two subtractions: the last
+// even-ranked and the last odd-ranked of them, even first.  Up to kMaxCover
+// spikes may cover one column.
 constexpr int kMaxCover = 4;
-struct ColCursor {
-    int i, i1, q, d;        // item i = q * per_spike + d
-    int per_spike, ncol, nt, T, S, nchunks;
-    int cq;
+
+// Spike q with the later spikes whose windows reach its window.
+struct SpikeCtx {
+    SpikeG g0, g1;          // spikes q, q+1 (g1 when ncov > 1)
+    int ncov;               // spikes q .. q+ncov-1 reach q's window
+    int x2, x3;             // columns of spikes q+2, q+3; their Gram refs in s_far
     bool has_p, over;       // over: a (kMaxCover+1)-th spike reaches q's window
-    int x_p, x_over, ncov;  // spikes q .. q+ncov-1 reach q's window
-    int x2, x3;             // columns of spikes q+2, q+3 (when ncov > 2, 3)
-    SpikeG g0, g1;          // spikes q, q+1; spikes q+2, q+3 in s_far (ties only)
-    SpikeG *s_far;          // this wave's LDS slots
+    int x_p, x_over;
 };
 
-__device__ __forceinline__ SpikeG cover_at(const ColCursor &c, int j) {
+__device__ __forceinline__ void spike_ctx(SpikeCtx &c, int q, int S, int nt, const int *pre,
+                                          int nchunks, const Slot *__restrict__ slots,
+                                          const GramRef *__restrict__ gref, SpikeG *s_far) {
+    c.g0 = spike_gram(spike_at(q, pre, nchunks, slots), gref);
+    c.has_p = q > 0;
+    c.x_p = c.has_p ? uni(spike_at(q - 1, pre, nchunks, slots).x) : 0;
+    const int reach = c.g0.x + 2 * nt;
+    c.ncov = 1;
+    c.over = false;
+    c.x2 = c.x3 = c.x_over = 0;
+    for (int j = 1; j <= kMaxCover && q + j < S; ++j) {
+        const Slot sj = spike_at(q + j, pre, nchunks, slots);
+        if (uni(sj.x) > reach) break;
+        if (j == kMaxCover) {
+            c.over = true;
+            c.x_over = uni(sj.x);
+            break;
+        }
+        const SpikeG gj = spike_gram(sj, gref);
+        if (j == 1) {
+            c.g1 = gj;
+        } else {
+            if ((threadIdx.x & 63) == 0) s_far[j - 2] = gj;
+            __builtin_amdgcn_wave_barrier();
+            if (j == 2) c.x2 = gj.x;
+            else c.x3 = gj.x;
+        }
+        c.ncov = j + 1;
+    }
+}
+
+__device__ __forceinline__ SpikeG cover_at(const SpikeCtx &c, int j, const SpikeG *s_far) {
     if (j == 0) return c.g0;
     if (j == 1) return c.g1;
-    const SpikeG f = c.s_far[j - 2];
+    const SpikeG f = s_far[j - 2];
     SpikeG r;
     r.x = uni(f.x);
     r.lo = uni(f.lo);
@@ -685,71 +736,206 @@ __device__ __forceinline__ SpikeG cover_at(const ColCursor &c, int j) {
     return r;
 }
 
-// Returns false at the end of the range; sets ties when more than kMaxCover
-// spikes cover one column.
-__device__ __forceinline__ bool next_column(ColCursor &c, const int *pre,
-                                            const Slot *__restrict__ slots,
-                                            const GramRef *__restrict__ gref, ColJob &J,
-                                            bool &ties) {
-    for (; c.i < c.i1; ++c.i) {
-        const int q = c.q, d = c.d;
-        if (++c.d == c.per_spike) {
-            c.d = 0;
-            ++c.q;
-        }
-        if (d >= c.ncol) continue;
-        if (q != c.cq) {
-            c.cq = q;
-            c.g0 = spike_gram(spike_at(q, pre, c.nchunks, slots), gref);
-            c.has_p = q > 0;
-            if (c.has_p) c.x_p = uni(spike_at(q - 1, pre, c.nchunks, slots).x);
-            // later spikes whose windows reach this spike's window
-            const int reach = c.g0.x + 2 * c.nt;
-            c.ncov = 1;
-            c.over = false;
-            for (int j = 1; j <= kMaxCover && q + j < c.S; ++j) {
-                const Slot sj = spike_at(q + j, pre, c.nchunks, slots);
-                if (uni(sj.x) > reach) break;
-                if (j == kMaxCover) {
-                    c.over = true;
-                    c.x_over = uni(sj.x);
-                    break;
-                }
-                const SpikeG gj = spike_gram(sj, gref);
-                if (j == 1) {
-                    c.g1 = gj;
-                } else {
-                    if ((threadIdx.x & 63) == 0) c.s_far[j - 2] = gj;
-                    __builtin_amdgcn_wave_barrier();
-                    if (j == 2) c.x2 = gj.x;
-                    else c.x3 = gj.x;
-                }
-                c.ncov = j + 1;
-            }
-        }
-        const int t = c.g0.x - c.nt + d;
-        if (t < c.nt || t >= c.T - c.nt) continue;      // never scored again
-        if (c.has_p && t <= c.x_p + c.nt) continue;     // an earlier spike owns it
-        if (c.over && t >= c.x_over - c.nt) {
-            ties = true;
-            continue;
-        }
-        int m = 0;                                      // spikes q .. q+m cover t
-        if (c.ncov > 1 && t >= c.g1.x - c.nt) m = 1;
-        if (c.ncov > 2 && t >= c.x2 - c.nt) m = 2;
-        if (c.ncov > 3 && t >= c.x3 - c.nt) m = 3;
-        const ColSpike none = {0, 0, 0, 0.f};
-        const ColSpike sl = col_spike(cover_at(c, m), t, c.nt);
-        const ColSpike sp = m > 0 ? col_spike(cover_at(c, m - 1), t, c.nt) : none;
-        // the last even-ranked spike is subtracted first
-        const bool last_even = ((q + m) & 1) == 0;
+// Per-lane constants of one pass (kPassGroups groups) over the columns of a
+// segment: lanes 8 g .. 8 g + 7 take group g of the pass, 4 positions each.
+struct SegPass {
+    int r, rb_, ra, rb;     // first position; B offset (clamped); Gram offsets (clamped)
+    bool in, ha, hb;        // lane has a group; its positions take a's / b's subtraction
+};
+
+__device__ __forceinline__ void seg_pass(SegPass &sp, const GroupList &L, int j0,
+                                         const ColSpike &a, const ColSpike &b, int ldbt) {
+    const int lane = threadIdx.x & 63;
+    const int j = j0 + (lane >> 3);
+    const int g = j < L.count() ? L.at(j) : -1;
+    sp.in = g >= 0;
+    sp.r = (g < 0 ? 0 : g) * kGroup + 4 * (lane & 7);
+    sp.rb_ = min(sp.r, ldbt - 4);
+    sp.ra = a.hi > a.lo ? min(max(sp.r, a.lo), a.hi - 4) : 0;
+    sp.rb = b.hi > b.lo ? min(max(sp.r, b.lo), b.hi - 4) : 0;
+    sp.ha = sp.in && sp.r >= a.lo && sp.r < a.hi;
+    sp.hb = sp.in && sp.r >= b.lo && sp.r < b.hi;
+}
+
+struct ColLoads {
+    float4 v0, va0, vb0, v1, va1, vb1;
+    u64 kept;
+};
+
+__device__ __forceinline__ void seg_load(ColLoads &l, int t, long long ba, long long bb,
+                                         bool two, const SegPass &s0, const SegPass &s1,
+                                         bool keep, const float *__restrict__ Bt, int ldbt,
+                                         const float *__restrict__ cc, int G,
+                                         const u64 *__restrict__ gkey) {
+    const float *col = Bt + (size_t)t * ldbt;
+    l.v0 = *reinterpret_cast<const float4 *>(col + s0.rb_);
+    l.va0 = *reinterpret_cast<const float4 *>(cc + ba + s0.ra);
+    l.vb0 = *reinterpret_cast<const float4 *>(cc + bb + s0.rb);
+    if (two) {
+        l.v1 = *reinterpret_cast<const float4 *>(col + s1.rb_);
+        l.va1 = *reinterpret_cast<const float4 *>(cc + ba + s1.ra);
+        l.vb1 = *reinterpret_cast<const float4 *>(cc + bb + s1.rb);
+    }
+    l.kept = keep ? gkey[(size_t)t * G + (threadIdx.x & 63)] : 0ull;
+}
+
+// Applies a then b to the lane's 4 positions (products rounded before the
+// subtract) in place and returns the lane group's key.
+__device__ __forceinline__ u64 seg_compute(float4 &v, const float4 &va, const float4 &vb,
+                                           const SegPass &sp, float amp_a, float amp_b, int K,
+                                           const float *s_nm, const int *s_pid) {
+    const float4 n4 = *reinterpret_cast<const float4 *>(s_nm + sp.r);
+    const int4 p4 = *reinterpret_cast<const int4 *>(s_pid + sp.r);
+    float bv[4] = {v.x, v.y, v.z, v.w};
+    const float ca[4] = {va.x, va.y, va.z, va.w};
+    const float cb[4] = {vb.x, vb.y, vb.z, vb.w};
+    const float nv[4] = {n4.x, n4.y, n4.z, n4.w};
+    const int pv[4] = {p4.x, p4.y, p4.z, p4.w};
+    u64 kk = 0ull;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const float pa = amp_a * ca[e];
+        const float pb = amp_b * cb[e];
+        float x = bv[e];
+        x = sp.ha ? x - pa : x;
+        x = sp.hb ? x - pb : x;
+        bv[e] = x;
+        const u64 k = sp.r + e < K ? make_key(score(x, nv[e]), pv[e]) : 0ull;
+        kk = k > kk ? k : kk;
+    }
+    v = make_float4(bv[0], bv[1], bv[2], bv[3]);
+    return group_max_u64(sp.in ? kk : 0ull);
+}
+
+__device__ __forceinline__ void seg_store(const float4 &v, const SegPass &sp, u64 key, int t,
+                                          float *__restrict__ Bt, int ldbt, int G,
+                                          u64 *__restrict__ gkey, u64 &acc) {
+    if (sp.ha || sp.hb) *reinterpret_cast<float4 *>(Bt + (size_t)t * ldbt + sp.r) = v;
+    if (sp.in && (threadIdx.x & 7) == 0) gkey[(size_t)t * G + sp.r / kGroup] = key;
+    acc = key > acc ? key : acc;
+}
+
+// Column t of a segment, whose loads are in `cur`: compute, issue the next
+// column's loads into `nxt`, store, and write the column's best key.
+__device__ __forceinline__ void seg_column(ColLoads &cur, ColLoads &nxt, int t, int t1,
+                                           long long &ba, long long &bb, int wa, int wb,
+                                           bool two, const SegPass &s0, const SegPass &s1,
+                                           bool keep, float amp_a, float amp_b,
+                                           float *__restrict__ Bt, int ldbt, int K, int G,
+                                           const float *__restrict__ cc, const float *s_nm,
+                                           const int *s_pid, u64 *__restrict__ gkey,
+                                           float *__restrict__ cfmax, int *__restrict__ imax) {
+    const u64 k0 = seg_compute(cur.v0, cur.va0, cur.vb0, s0, amp_a, amp_b, K, s_nm, s_pid);
+    u64 k1 = 0ull;
+    if (two) k1 = seg_compute(cur.v1, cur.va1, cur.vb1, s1, amp_a, amp_b, K, s_nm, s_pid);
+    ba += wa;
+    bb += wb;
+    if (t < t1) seg_load(nxt, t + 1, ba, bb, two, s0, s1, keep, Bt, ldbt, cc, G, gkey);
+    u64 acc = cur.kept;
+    seg_store(cur.v0, s0, k0, t, Bt, ldbt, G, gkey, acc);
+    if (two) seg_store(cur.v1, s1, k1, t, Bt, ldbt, G, gkey, acc);
+    const u64 best = wave_max_u64(acc);
+    if ((threadIdx.x & 63) == 0) {
+        cfmax[t] = key_score(best);
+        imax[t] = key_id(best);
+    }
+}
+
+// Columns t0 .. t1 of one segment: fixed covering spikes a (first) and b
+// (either may be absent: empty range), whose Gram rows for column t start at
+// a.base + (t - t0) wa and b.base + (t - t0) wb.  At most 2 passes.  The
+// columns run as a pipeline: the next column's loads are issued before the
+// current column's stores (a store issued ahead of a load would hold that
+// load's wait on gfx9, whose counter covers both).
+__device__ __forceinline__ void seg_columns(int t0, int t1, const ColSpike &a, const ColSpike &b,
+                                            int wa, int wb, const GroupList &L,
+                                            float *__restrict__ Bt, int ldbt, int K, int G,
+                                            const float *__restrict__ cc, const float *s_nm,
+                                            const int *s_pid, u64 *__restrict__ gkey,
+                                            float *__restrict__ cfmax, int *__restrict__ imax) {
+    const int lane = threadIdx.x & 63;
+    const bool two = L.count() > kPassGroups;     // uniform
+    SegPass s0, s1;
+    seg_pass(s0, L, 0, a, b, ldbt);
+    if (two) seg_pass(s1, L, kPassGroups, a, b, ldbt);
+    else s1 = s0;
+    const bool keep = lane < G && !L.has(lane);
+    long long ba = a.base, bb = b.base;
+    ColLoads cur, nxt;
+    seg_load(cur, t0, ba, bb, two, s0, s1, keep, Bt, ldbt, cc, G, gkey);
+    for (int t = t0;; ++t) {
+        seg_column(cur, nxt, t, t1, ba, bb, wa, wb, two, s0, s1, keep, a.amp, b.amp, Bt, ldbt, K,
+                   G, cc, s_nm, s_pid, gkey, cfmax, imax);
+        if (t >= t1) break;
+        cur = nxt;
+    }
+}
+
+// Columns of one segment with more than 2 passes (rare: far-apart Gram
+// ranges of two covering spikes), through the general pass pipeline.
+struct SegColumnsGeneral {
+    int t, t1, wa, wb;
+    ColSpike a, b;
+    __device__ __forceinline__ bool next(ColJob &J) {
+        if (t > t1) return false;
         J.t = t;
-        J.a = last_even ? sl : sp;
-        J.b = last_even ? sp : sl;
-        ++c.i;
+        J.a = a;
+        J.b = b;
+        ++t;
+        a.base += wa;
+        b.base += wb;
         return true;
     }
-    return false;
+};
+
+// The column items [d_lo, d_hi] of spike q (time order): the columns it owns
+// (live, not covered by spike q-1), cut into segments of fixed covering
+// spikes.
+__device__ __forceinline__ void spike_columns(int q, int d_lo, int d_hi, const SpikeCtx &c,
+                                              const SpikeG *s_far, int nt, int T,
+                                              float *__restrict__ Bt, int ldbt, int K, int G,
+                                              const float *__restrict__ cc, const float *s_nm,
+                                              const int *s_pid, u64 *__restrict__ gkey,
+                                              float *__restrict__ cfmax,
+                                              int *__restrict__ imax, bool &ties) {
+    int tl = c.g0.x - nt + d_lo, th = c.g0.x - nt + d_hi;
+    tl = max(tl, nt);
+    th = min(th, T - nt - 1);
+    if (c.has_p) tl = max(tl, c.x_p + nt + 1);
+    if (c.over && th >= c.x_over - nt) {
+        ties = true;
+        th = c.x_over - nt - 1;
+    }
+    const int bnd[4] = {tl, c.ncov > 1 ? c.g1.x - nt : th + 1, c.ncov > 2 ? c.x2 - nt : th + 1,
+                        c.ncov > 3 ? c.x3 - nt : th + 1};
+    for (int m = 0; m < kMaxCover; ++m) {
+        const int s0 = max(tl, bnd[m]);
+        const int s1 = min(th, m + 1 < kMaxCover ? bnd[m + 1] - 1 : th);
+        if (s0 > s1) continue;
+        const ColSpike none = {0, 0, 0, 0.f};
+        const SpikeG gl = cover_at(c, m, s_far);
+        const ColSpike sl = col_spike(gl, s0, nt);
+        ColSpike sp = none;
+        int wp = 0;
+        if (m > 0) {
+            const SpikeG gp = cover_at(c, m - 1, s_far);
+            sp = col_spike(gp, s0, nt);
+            wp = gp.hi - gp.lo;
+        }
+        const int wl = gl.hi - gl.lo;
+        // the last even-ranked spike is subtracted first
+        const bool last_even = ((q + m) & 1) == 0;
+        const ColSpike a = last_even ? sl : sp, b = last_even ? sp : sl;
+        const int wa = last_even ? wl : wp, wb = last_even ? wp : wl;
+        const GroupList L = group_list(a, b, G, false);
+        if (L.count() <= 2 * kPassGroups) {
+            seg_columns(s0, s1, a, b, wa, wb, L, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
+                        imax);
+        } else {
+            SegColumnsGeneral src{s0, s1, wa, wb, a, b};
+            column_pipeline(src, false, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax, imax);
+        }
+    }
 }
 
 // Residual subtraction for a cluster of spikes whose windows overlap (gaps of
@@ -837,8 +1023,9 @@ __device__ void xres_cluster(int q, int k, int S, const int *pre, int nchunks,
 }
 
 // One iteration's subtraction.  Work items per spike q (time order): 2nt+1
-// columns of its window, then kXresItems residual items; each wave takes one
-// contiguous range of items.  A column is owned by the earliest spike whose
+// columns of its window, which the waves take in contiguous ranges, and
+// kXresItems residual items, dealt round-robin over all waves so that no
+// wave gets more than about one.  A column is owned by the earliest spike whose
 // window covers it; its owner applies the subtractions next_column yields
 // (the reference's two interleaved phases).  A wave runs its columns as a
 // pipeline (the next column's loads are issued before the current column's
@@ -915,11 +1102,14 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         state[ST_ITERS] = it + 1;
     }
     const int ncol = 2 * nt + 1;
-    const int per_spike = ncol + kXresItems;
-    const int n_items = S * per_spike;          // < 2^31: S <= nchunks * kSlotCap
+    // column items (spike, lag): contiguous ranges per wave; residual items
+    // (spike, channel share): dealt round-robin over all waves
+    const int n_items = S * ncol;               // < 2^31: S <= nchunks * kSlotCap
+    const int n_xres = S * kXresItems;
     const int nwaves = gridDim.x * (kUpdThreads / 64);
     const int per = (n_items + nwaves - 1) / nwaves;
-    if ((long long)blockIdx.x * (kUpdThreads / 64) * per >= n_items) return;   // no work here
+    const int gw0 = blockIdx.x * (kUpdThreads / 64);
+    if ((long long)gw0 * per >= n_items && gw0 >= n_xres) return;   // no work here
     stage_tables(pnm, ppid, Kg, s_nm, s_pid);
     __syncthreads();
 
@@ -933,41 +1123,21 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         th_amps[pos] = sl.tha;
     }
 
-    const int gw = blockIdx.x * (kUpdThreads / 64) + wv;
+    const int gw = gw0 + wv;
     const int i0 = (int)min((long long)gw * per, (long long)n_items);
     const int i1 = min(n_items, i0 + per);
-    if (i0 >= i1) return;
 
-    // columns, pipelined
-    {
-        ColCursor cur;
-        cur.i = i0;
-        cur.i1 = i1;
-        cur.q = i0 / per_spike;
-        cur.d = i0 % per_spike;
-        cur.per_spike = per_spike;
-        cur.ncol = ncol;
-        cur.nt = nt;
-        cur.T = T;
-        cur.S = S;
-        cur.nchunks = nchunks;
-        cur.cq = -1;
-        cur.has_p = cur.over = false;
-        cur.x_p = cur.x_over = cur.x2 = cur.x3 = 0;
-        cur.ncov = 1;
-        cur.s_far = s_far[wv];
+    // columns: the wave's items, spike by spike
+    if (i0 < i1) {
         bool ties = false;
-        ColJob J, nJ;
-        if (next_column(cur, s_pre, slots, gref, J, ties)) {
-            ColWork a, b;
-            col_begin(a, J, false, G, Bt, ldbt, cc, gkey);
-            for (;;) {
-                const bool more = next_column(cur, s_pre, slots, gref, nJ, ties);
-                col_finish(a, more, b, nJ, false, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
-                           imax);
-                if (!more) break;
-                a = b;
-            }
+        const int q0 = i0 / ncol, q1 = (i1 - 1) / ncol;
+        for (int q = q0; q <= q1; ++q) {
+            const int d_lo = q == q0 ? i0 % ncol : 0;
+            const int d_hi = q == q1 ? (i1 - 1) % ncol : ncol - 1;
+            SpikeCtx c;
+            spike_ctx(c, q, S, nt, s_pre, nchunks, slots, gref, s_far[wv]);
+            spike_columns(q, d_lo, d_hi, c, s_far[wv], nt, T, Bt, ldbt, K, G, cc, s_nm, s_pid,
+                          gkey, cfmax, imax, ties);
         }
         if (ties && lane == 0) {
             atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
@@ -975,15 +1145,13 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         }
     }
 
-    // residual items of the range: item k of spike q takes channels
-    // [cl0, cl1) of the template's nonzero range (at most 64: lane l holds
-    // U[id, :, cl0 + l], broadcast per channel with readlane)
+    // residual items j = gw, gw + nwaves, ...: item k of spike q takes
+    // channels [cl0, cl1) of the template's nonzero range (at most 64 at a
+    // time: lane l holds U[id, :, cl0 + l], broadcast per channel with readlane)
     const int h = nt / 2;
-    const int qa = i0 / per_spike, qb = (i1 - 1) / per_spike;
-    for (int q = qa; q <= qb; ++q) {
-        const int k0 = max(i0 - q * per_spike - ncol, 0);
-        const int k1 = min(i1 - q * per_spike - ncol, kXresItems);
-        if (k0 >= k1) continue;
+    for (int j = gw; j < n_xres; j += nwaves) {
+        const int q = j / kXresItems;
+        const int k0 = j % kXresItems, k1 = k0 + 1;
         const Slot rs = spike_at(q, s_pre, nchunks, slots);
         const int id = uni(rs.iy), x = uni(rs.x);
         // windows closer than nt samples (exact ties): the cluster's first
