@@ -58,15 +58,19 @@ constexpr int kXresCh = 16;         // channels per residual step (loads first)
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
 
 struct Slot {
-    int x;       // column in the padded batch
-    int iy;      // template id
-    float amp;   // B[iy, x] / nm[iy]
-    float tha;   // sqrt(Cmax[x])
+    int x;          // column in the padded batch
+    int iy;         // template id
+    float amp;      // B[iy, x] / nm[iy]
+    float tha;      // sqrt(Cmax[x])
+    int lo, hi;     // the template's Gram block (see GramRef)
+    long long off;
 };
 
-// A template's Gram block: rows [lo, hi) of the peel order at
-// cc[off + lag * (hi - lo) + (r - lo)].
+// Per template id: its position in the peel order, its norm and its Gram
+// block: rows [lo, hi) of the peel order at cc[off + lag * (hi - lo) + (r - lo)].
 struct GramRef {
+    int rank;
+    float nm;
     int lo, hi;
     long long off;
 };
@@ -452,7 +456,9 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
 
 // Accepts columns whose score is the maximum of its +-nt window and exceeds
 // Th^2: (Cmax > Th^2) & (|Cmax - Cfmax| < 1e-9) with Cmax the window max
-// (template_matching.py:205-212), and records them in time order.
+// (template_matching.py:205-212), and records them in time order.  The window
+// max of the chunk's columns comes from maxima of 2^j consecutive values built
+// by doubling in LDS (max_pool1d semantics: -inf padding, NaN propagates).
 //
 // With `rescore` set (scores from a reduced-precision projection), each
 // accepted spike is then re-scored in FP32 from the residual,
@@ -462,16 +468,21 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
 // Work split: lanes = lags; the spikes' nonzero channel ranges are cut into
 // blocks of 16 (or more) channels, one task per (spike, block), spread over
 // the block's waves; per-task sums meet in LDS and are added in block order.
+// NaN-propagating maximum (max_pool1d semantics).
+__device__ __forceinline__ float nan_max(float a, float b) {
+    return a != a ? a : (b != b ? b : fmaxf(a, b));
+}
+
 template <int NTH>
 __global__ __launch_bounds__(NTH) void peel_select_kernel(
     const float *__restrict__ cfmax, const int *__restrict__ imax, float *__restrict__ Bt,
-    int ldbt, const float *__restrict__ nm, const int32_t *__restrict__ rank, int T, int nt,
-    float th2, float th2_lo, int early_exit, int rescore, const float *__restrict__ Xres,
-    int ldx, const float *__restrict__ U, const float *__restrict__ W,
-    const int *__restrict__ crange, int P, int C, int *__restrict__ state,
-    int *__restrict__ chunk_count, Slot *__restrict__ slots) {
+    int ldbt, const GramRef *__restrict__ gref, int T, int nt, float th2, int rescore,
+    const float *__restrict__ Xres, int ldx, const float *__restrict__ U,
+    const float *__restrict__ W, const int *__restrict__ crange, int P, int C,
+    int *__restrict__ state, int *__restrict__ chunk_count, Slot *__restrict__ slots) {
     if (state[ST_DONE]) return;
     __shared__ float win[kChunk + 2 * kMaxNt];
+    __shared__ float win2[kChunk + 2 * kMaxNt];
     __shared__ int s_wc[kChunk / 64];
     __shared__ int s_sx[kSlotCap], s_sid[kSlotCap];
     __shared__ int s_tb[kSlotCap + 1], s_clo[kSlotCap], s_cw[kSlotCap];
@@ -479,43 +490,35 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
     constexpr int NW = NTH / 64;
     const int t0 = blockIdx.x * kChunk;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int e = tid; e < kChunk + 2 * nt; e += NTH) {
-        const int x = t0 - nt + e;
-        win[e] = (x >= 0 && x < T) ? cfmax[x] : -INFINITY;
+    const int x = t0 + tid;
+    const int my_id = (tid < kChunk && x < T) ? imax[x] : 0;     // issued with the window
+    const int WL = kChunk + 2 * nt;
+    for (int e = tid; e < WL; e += NTH) {
+        const int xe = t0 - nt + e;
+        win[e] = (xe >= 0 && xe < T) ? cfmax[xe] : -INFINITY;
     }
     __syncthreads();
-    const int x = t0 + tid;
+    const float v = tid < kChunk ? win[tid + nt] : 0.f;     // Cfmax of column x
+    // window max of width L = 2nt+1 (max_pool1d, -inf padding): maxima of
+    // 2^k consecutive values by doubling, then two overlapping 2^k windows
+    const int L = 2 * nt + 1;
+    const int k = 31 - __clz(L);
+    float *cur = win, *nxt = win2;
+    for (int j = 0; j < k; ++j) {
+        const int half = 1 << j;
+        for (int e = tid; e < WL; e += NTH)
+            nxt[e] = e + half < WL ? nan_max(cur[e], cur[e + half]) : cur[e];
+        __syncthreads();
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
     bool acc = false;
     float cmax = 0.f;
     if (tid < kChunk && x >= nt && x < T - nt) {
-        const int e0 = tid + nt;
-        const float v = win[e0];
-        if (v >= th2_lo) {
-            if (early_exit) {
-                // With v >= 2^-6 any larger neighbour differs from v by at
-                // least one ulp of v (> 1e-9), so the column is rejected as
-                // soon as one is found; scanning outwards finds it within a
-                // few samples for columns that are not a peak.
-                bool ok = true;
-                for (int d = 1; d <= nt; ++d) {
-                    const float a = win[e0 - d], b = win[e0 + d];
-                    if (a > v || b > v || a != a || b != b) {
-                        ok = false;
-                        break;
-                    }
-                }
-                cmax = v;
-                acc = ok && (v > th2);
-            } else {
-                cmax = v;
-                for (int d = -nt; d <= nt; ++d) {
-                    const float o = win[e0 + d];
-                    if (o > cmax || o != o) cmax = o;
-                    if (cmax != cmax) break;
-                }
-                acc = (cmax > th2) && (fabsf(cmax - v) < 1e-9f);
-            }
-        }
+        // the reference's test: (Cmax > Th^2) & (|Cmax - Cfmax| < 1e-9)
+        cmax = nan_max(cur[tid], cur[tid + L - (1 << k)]);
+        acc = (cmax > th2) && (fabsf(cmax - v) < 1e-9f);
     }
     const unsigned long long m = __ballot(acc);
     if (lane == 0 && wave < kChunk / 64) s_wc[wave] = __popcll(m);
@@ -529,12 +532,16 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
     if (acc) {
         const int pos = off + __popcll(m & lanemask_lt(lane));
         if (pos < kSlotCap) {
-            const int iy = imax[x];
+            const int iy = my_id;
+            const GramRef g = gref[iy];
             Slot sl;
             sl.x = x;
             sl.iy = iy;
-            sl.amp = Bt[(size_t)x * ldbt + rank[iy]] / nm[iy];
+            sl.amp = Bt[(size_t)x * ldbt + g.rank] / g.nm;
             sl.tha = sqrtf(cmax);
+            sl.lo = g.lo;
+            sl.hi = g.hi;
+            sl.off = g.off;
             slots[(size_t)blockIdx.x * kSlotCap + pos] = sl;
             s_sx[pos] = x;
             s_sid[pos] = iy;
@@ -623,11 +630,11 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
         const int sx = s_sx[tid], id = s_sid[tid];
         float b = 0.f;
         for (int bi = 0; bi < s_tb[tid + 1] - s_tb[tid]; ++bi) b += s_part[tid][bi];
-        const float nmv = nm[id];
+        const GramRef g = gref[id];
         Slot *sl = slots + (size_t)blockIdx.x * kSlotCap + tid;
-        sl->amp = b / nmv;
-        sl->tha = sqrtf(score(b, nmv));
-        Bt[(size_t)sx * ldbt + rank[id]] = b;
+        sl->amp = b / g.nm;
+        sl->tha = sqrtf(score(b, g.nm));
+        Bt[(size_t)sx * ldbt + g.rank] = b;
     }
 }
 
@@ -651,15 +658,13 @@ struct SpikeG {
     long long off;
 };
 
-__device__ __forceinline__ SpikeG spike_gram(const Slot &sl, const GramRef *__restrict__ gref) {
-    const int id = uni(sl.iy);
-    const GramRef g = gref[id];
+__device__ __forceinline__ SpikeG spike_gram(const Slot &sl) {
     SpikeG s;
     s.x = uni(sl.x);
     s.amp = uni(sl.amp);
-    s.lo = uni(g.lo);
-    s.hi = uni(g.hi);
-    s.off = uni((long long)g.off);
+    s.lo = uni(sl.lo);
+    s.hi = uni(sl.hi);
+    s.off = uni((long long)sl.off);
     return s;
 }
 
@@ -694,8 +699,8 @@ struct SpikeCtx {
 
 __device__ __forceinline__ void spike_ctx(SpikeCtx &c, int q, int S, int nt, const int *pre,
                                           int nchunks, const Slot *__restrict__ slots,
-                                          const GramRef *__restrict__ gref, SpikeG *s_far) {
-    c.g0 = spike_gram(spike_at(q, pre, nchunks, slots), gref);
+                                          SpikeG *s_far) {
+    c.g0 = spike_gram(spike_at(q, pre, nchunks, slots));
     c.has_p = q > 0;
     c.x_p = c.has_p ? uni(spike_at(q - 1, pre, nchunks, slots).x) : 0;
     const int reach = c.g0.x + 2 * nt;
@@ -710,7 +715,7 @@ __device__ __forceinline__ void spike_ctx(SpikeCtx &c, int q, int S, int nt, con
             c.x_over = uni(sj.x);
             break;
         }
-        const SpikeG gj = spike_gram(sj, gref);
+        const SpikeG gj = spike_gram(sj);
         if (j == 1) {
             c.g1 = gj;
         } else {
@@ -1038,7 +1043,7 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int it, float *__restrict__ Bt, int ldbt, float *__restrict__ Xres, int ldx,
     const float *__restrict__ U, const float *__restrict__ W, const int *__restrict__ crange,
     int P, int C, const float *__restrict__ pnm, const int *__restrict__ ppid,
-    const GramRef *__restrict__ gref, const float *__restrict__ cc, int K, int G, int Kg, int T,
+    const float *__restrict__ cc, int K, int G, int Kg, int T,
     int nt, int nchunks, const int *__restrict__ chunk_count, const Slot *__restrict__ slots,
     int *__restrict__ state, int64_t *__restrict__ st, float *__restrict__ amps,
     float *__restrict__ th_amps, int capacity, float *__restrict__ cfmax,
@@ -1059,11 +1064,15 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-    // exclusive prefix of the per-chunk spike counts (thread: consecutive chunks)
+    // the loads of the prologue are issued together: the iteration's output
+    // base, the per-chunk spike counts and the per-position tables
+    const int kb = state[ST_KBASE + it];
     const int per_t = (nchunks + kUpdThreads - 1) / kUpdThreads;
     const int c0 = tid * per_t, c1 = min(c0 + per_t, nchunks);
     int mine = 0;
     for (int c = c0; c < c1; ++c) mine += chunk_count[c];
+    stage_tables(pnm, ppid, Kg, s_nm, s_pid);
+    // exclusive prefix of the per-chunk spike counts (thread: consecutive chunks)
     int incl = mine;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -1089,7 +1098,6 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         }
         return;
     }
-    const int kb = state[ST_KBASE + it];
     if (kb + S > capacity) {
         if (blockIdx.x == 0 && tid == 0) {
             atomicExch(&state[ST_STATUS], KSM_ERR_CAPACITY);
@@ -1110,7 +1118,6 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     const int per = (n_items + nwaves - 1) / nwaves;
     const int gw0 = blockIdx.x * (kUpdThreads / 64);
     if ((long long)gw0 * per >= n_items && gw0 >= n_xres) return;   // no work here
-    stage_tables(pnm, ppid, Kg, s_nm, s_pid);
     __syncthreads();
 
     // record this iteration's spikes: block b takes spikes b, b + grid, ...
@@ -1135,7 +1142,7 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
             const int d_lo = q == q0 ? i0 % ncol : 0;
             const int d_hi = q == q1 ? (i1 - 1) % ncol : ncol - 1;
             SpikeCtx c;
-            spike_ctx(c, q, S, nt, s_pre, nchunks, slots, gref, s_far[wv]);
+            spike_ctx(c, q, S, nt, s_pre, nchunks, slots, s_far[wv]);
             spike_columns(q, d_lo, d_hi, c, s_far[wv], nt, T, Bt, ldbt, K, G, cc, s_nm, s_pid,
                           gkey, cfmax, imax, ties);
         }
@@ -1217,12 +1224,14 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
 // Gram block of every template, by template id.
 __global__ void gram_refs_kernel(const int32_t *__restrict__ rank,
                                  const int32_t *__restrict__ lohi,
-                                 const int64_t *__restrict__ off, int K,
-                                 GramRef *__restrict__ gref) {
+                                 const int64_t *__restrict__ off, const float *__restrict__ nm,
+                                 int K, GramRef *__restrict__ gref) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= K) return;
     const int jr = rank[j];
     GramRef g;
+    g.rank = jr;
+    g.nm = nm[j];
     g.lo = lohi[2 * jr];
     g.hi = lohi[2 * jr + 1];
     g.off = off[jr];
@@ -1439,12 +1448,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     int *crange = cv.take<int>(2 * (size_t)K);
 
     // Th^2 as the reference forms it (Python float, compared in float32)
-    const double th2d = (double)Th * (double)Th;
-    const float th2 = (float)th2d;
-    // columns below th2_lo can never satisfy the acceptance test
-    const float th2_lo = (float)(th2d - fmax(4e-9, fabs(th2d) * 1e-6));
-    // the early-exit window scan is exact when every candidate is >= 2^-6
-    const int early_exit = th2_lo >= 0.015625f ? 1 : 0;
+    const float th2 = (float)((double)Th * (double)Th);
     const size_t tab_dyn = (size_t)L.Kg * (sizeof(float) + sizeof(int));
     const size_t upd_dyn = tab_dyn + (size_t)(L.nchunks + 1) * sizeof(int);
     const int grid = update_grid(upd_dyn);
@@ -1455,7 +1459,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     KSM_LAUNCH_CHECK("channel_range_kernel");
     position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, ppid);
     KSM_LAUNCH_CHECK("position_tables_kernel");
-    gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, K, gref);
+    gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, K, gref);
     KSM_LAUNCH_CHECK("position_tables_kernel");
     {
         ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
@@ -1479,17 +1483,17 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
             if (flags & KSM_PEEL_RESCORE)
                 peel_select_kernel<kRsThreads><<<L.nchunks, kRsThreads, 0, s>>>(
-                    cfmax, imax, Bt, ldbt, nm, rank, T, nt, th2, th2_lo, early_exit, 1, Xres,
-                    ldx, U, W, crange, P, C, state, chunk_count, slots);
+                    cfmax, imax, Bt, ldbt, gref, T, nt, th2, 1, Xres, ldx, U, W, crange, P, C,
+                    state, chunk_count, slots);
             else
                 peel_select_kernel<kChunk><<<L.nchunks, kChunk, 0, s>>>(
-                    cfmax, imax, Bt, ldbt, nm, rank, T, nt, th2, th2_lo, early_exit, 0, Xres,
-                    ldx, U, W, crange, P, C, state, chunk_count, slots);
+                    cfmax, imax, Bt, ldbt, gref, T, nt, th2, 0, Xres, ldx, U, W, crange, P, C,
+                    state, chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
             peel_update_kernel<<<grid, kUpdThreads, upd_dyn, s>>>(
-                it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, gref, cc, K, L.G, L.Kg, T,
+                it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, cc, K, L.G, L.Kg, T,
                 nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax,
                 imax, gkey, poll ? hf.dev : nullptr, seq);
         }
