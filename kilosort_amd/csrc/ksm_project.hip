@@ -126,11 +126,12 @@ __global__ void zero_rows_kernel(float *__restrict__ p, size_t n) {
 
 // ---------------------------------------------------------------------------
 // FP32 GEMM on v_mfma_f32_32x32x2f32: Bout[m][n] = sum_r At[r][m] * Bc[r][n].
-// Block tile 128x128, depth step 16, 4 waves each owning a 64x64 quadrant as
-// 2x2 MFMA tiles.  Operand tiles are double buffered in LDS with the next
-// step's global loads in flight during the MFMAs.
+// Block tile 64 (templates) x 256 (time), depth step 16, 4 waves each owning
+// 64x64 as 2x2 MFMA tiles (64-row template tiles pad 700 templates to 704,
+// not 768).  Operand tiles are double buffered in LDS with the next step's
+// global loads in flight during the MFMAs.
 
-constexpr int GBM = 128, GBN = 128, GBK = 16;
+constexpr int GBM = 64, GBN = 256, GBK = 16;
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Maps the hardware block id to a logical tile so that each XCD (blocks are
@@ -145,16 +146,18 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 
 __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
     const float *__restrict__ At, int Kp, const float *__restrict__ Bc, int ldc, int Rp,
-    float *__restrict__ Bout, int ldb, int K, int nTm, int nTn) {
+    float *__restrict__ Bout, int ldb, int K, int nmax, int nTm, int nTn) {
     __shared__ float As[2][GBK][GBM];
     __shared__ float Bs[2][GBK][GBN];
     const int logical = xcd_remap(blockIdx.x, nTm * nTn);
     const int tm = logical % nTm, tn = logical / nTm;
     const int m0 = tm * GBM, n0 = tn * GBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    // global->LDS mapping: 2 float4 per operand per thread (16 rows x 32 float4)
-    const int lr = tid >> 5, lc = (tid & 31) * 4;
+    const int wn = wave;
+    // global->LDS mapping: A 16 rows x 16 float4 (one per thread); B 16 rows x
+    // 64 float4 (four per thread, rows lb + 4 i)
+    const int la = tid >> 4, lca = (tid & 15) * 4;
+    const int lb = tid >> 6, lcb = (tid & 63) * 4;
 
     f32x16 acc[2][2];
 #pragma unroll
@@ -165,17 +168,19 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
     const int nk = Rp / GBK;
-    const float *ga = At + (size_t)lr * Kp + m0 + lc;
-    const float *gb = Bc + (size_t)lr * ldc + n0 + lc;
-    const size_t sa = (size_t)8 * Kp, sb = (size_t)8 * ldc;
-    float4 ra0 = *reinterpret_cast<const float4 *>(ga);
-    float4 ra1 = *reinterpret_cast<const float4 *>(ga + sa);
+    const float *ga = At + (size_t)la * Kp + m0 + lca;
+    const float *gb = Bc + (size_t)lb * ldc + n0 + lcb;
+    const size_t sb = (size_t)4 * ldc;
+    float4 ra = *reinterpret_cast<const float4 *>(ga);
     float4 rb0 = *reinterpret_cast<const float4 *>(gb);
     float4 rb1 = *reinterpret_cast<const float4 *>(gb + sb);
-    *reinterpret_cast<float4 *>(&As[0][lr][lc]) = ra0;
-    *reinterpret_cast<float4 *>(&As[0][lr + 8][lc]) = ra1;
-    *reinterpret_cast<float4 *>(&Bs[0][lr][lc]) = rb0;
-    *reinterpret_cast<float4 *>(&Bs[0][lr + 8][lc]) = rb1;
+    float4 rb2 = *reinterpret_cast<const float4 *>(gb + 2 * sb);
+    float4 rb3 = *reinterpret_cast<const float4 *>(gb + 3 * sb);
+    *reinterpret_cast<float4 *>(&As[0][la][lca]) = ra;
+    *reinterpret_cast<float4 *>(&Bs[0][lb][lcb]) = rb0;
+    *reinterpret_cast<float4 *>(&Bs[0][lb + 4][lcb]) = rb1;
+    *reinterpret_cast<float4 *>(&Bs[0][lb + 8][lcb]) = rb2;
+    *reinterpret_cast<float4 *>(&Bs[0][lb + 12][lcb]) = rb3;
     __syncthreads();
     const int kr = lane >> 5, kc = lane & 31;
     for (int kt = 0; kt < nk; ++kt) {
@@ -184,15 +189,16 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
         if (more) {
             ga += (size_t)GBK * Kp;
             gb += (size_t)GBK * ldc;
-            ra0 = *reinterpret_cast<const float4 *>(ga);
-            ra1 = *reinterpret_cast<const float4 *>(ga + sa);
+            ra = *reinterpret_cast<const float4 *>(ga);
             rb0 = *reinterpret_cast<const float4 *>(gb);
             rb1 = *reinterpret_cast<const float4 *>(gb + sb);
+            rb2 = *reinterpret_cast<const float4 *>(gb + 2 * sb);
+            rb3 = *reinterpret_cast<const float4 *>(gb + 3 * sb);
         }
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 2) {
-            const float a0 = As[buf][kk + kr][wm * 64 + kc];
-            const float a1 = As[buf][kk + kr][wm * 64 + 32 + kc];
+            const float a0 = As[buf][kk + kr][kc];
+            const float a1 = As[buf][kk + kr][32 + kc];
             const float b0 = Bs[buf][kk + kr][wn * 64 + kc];
             const float b1 = Bs[buf][kk + kr][wn * 64 + 32 + kc];
             acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
@@ -202,10 +208,11 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
         }
         if (more) {
             const int nb = buf ^ 1;
-            *reinterpret_cast<float4 *>(&As[nb][lr][lc]) = ra0;
-            *reinterpret_cast<float4 *>(&As[nb][lr + 8][lc]) = ra1;
-            *reinterpret_cast<float4 *>(&Bs[nb][lr][lc]) = rb0;
-            *reinterpret_cast<float4 *>(&Bs[nb][lr + 8][lc]) = rb1;
+            *reinterpret_cast<float4 *>(&As[nb][la][lca]) = ra;
+            *reinterpret_cast<float4 *>(&Bs[nb][lb][lcb]) = rb0;
+            *reinterpret_cast<float4 *>(&Bs[nb][lb + 4][lcb]) = rb1;
+            *reinterpret_cast<float4 *>(&Bs[nb][lb + 8][lcb]) = rb2;
+            *reinterpret_cast<float4 *>(&Bs[nb][lb + 12][lcb]) = rb3;
         }
         __syncthreads();
     }
@@ -218,8 +225,9 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int m = m0 + wm * 64 + i * 32 + 8 * g + 4 * (lane >> 5);
+                const int m = m0 + i * 32 + 8 * g + 4 * (lane >> 5);
                 const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+                if (n >= nmax) continue;
                 float *dst = Bout + (size_t)n * ldb + m;
                 if (m + 3 < K) {
                     float4 v;
@@ -307,7 +315,8 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     KSM_LAUNCH_CHECK("transpose_pad_kernel");
     const int nTm = L.Kp / GBM, nTn = L.ldc / GBN;
     ksm::ProfileScope prof("gemm_f32", ksm::kProfProject, s);
-    gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, Bt, ldbt, K, nTm, nTn);
+    gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, Bt, ldbt, K,
+                                            (int)ksm::align_up(T, 128), nTm, nTn);
     KSM_LAUNCH_CHECK("gemm_f32_mfma");
     return KSM_OK;
 }
