@@ -166,6 +166,7 @@ size_t ksm_peel_workspace(int K, int T, int max_peels);
  *   info[0] = spike count k, info[1] = ksm_status of the device run,
  *   info[2] = iterations that accepted spikes.
  * Spikes are ordered as the reference orders them: by iteration, then time.
+ * Limits: K <= 8192, P <= 8, C <= 65535, odd nt <= 255.
  */
 int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
              const float *W, const float *nm, const int32_t *perm,

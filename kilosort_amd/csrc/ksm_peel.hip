@@ -48,6 +48,7 @@ constexpr int kGroup = 32;          // positions per group (8 lanes x 4)
 constexpr int kPassGroups = 8;      // groups a wave reduces per pass
 constexpr int kMaxK = 8192;
 constexpr int kMaxNt = 255;
+constexpr int kMaxC = 65535;        // channel ranges are packed in 16 bits
 constexpr int kMaxChunks = 4096;    // T <= kChunk * kMaxChunks
 constexpr int kUpdThreads = 256;    // update / first-pass block (4 waves)
 constexpr int kXresItems = 8;       // residual work items per spike
@@ -62,7 +63,8 @@ struct Slot {
     int iy;         // template id
     float amp;      // B[iy, x] / nm[iy]
     float tha;      // sqrt(Cmax[x])
-    int lo, hi;     // the template's Gram block (see GramRef)
+    unsigned lohi;  // the template's Gram block rows lo | hi << 16 (see GramRef)
+    unsigned crg;   // its nonzero channels clo | chi << 16
     long long off;
 };
 
@@ -73,6 +75,7 @@ struct GramRef {
     float nm;
     int lo, hi;
     long long off;
+    int clo, chi;   // nonzero channels [clo, chi) of the template
 };
 
 __device__ __forceinline__ int wave_sum_i(int v) {
@@ -539,8 +542,8 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
             sl.iy = iy;
             sl.amp = Bt[(size_t)x * ldbt + g.rank] / g.nm;
             sl.tha = sqrtf(cmax);
-            sl.lo = g.lo;
-            sl.hi = g.hi;
+            sl.lohi = (unsigned)g.lo | ((unsigned)g.hi << 16);
+            sl.crg = (unsigned)g.clo | ((unsigned)g.chi << 16);
             sl.off = g.off;
             slots[(size_t)blockIdx.x * kSlotCap + pos] = sl;
             s_sx[pos] = x;
@@ -658,13 +661,39 @@ struct SpikeG {
     long long off;
 };
 
-__device__ __forceinline__ SpikeG spike_gram(const Slot &sl) {
+// Lane l < n of the wave loads spike q0 + l (lanes outside [0, S): x = -1),
+// so that a spike's neighbours arrive with one load.
+__device__ __forceinline__ Slot spike_lanes(int q0, int n, int S, const int *pre, int nchunks,
+                                            const Slot *__restrict__ slots) {
+    const int lane = threadIdx.x & 63;
+    const int q = q0 + lane;
+    Slot sl = {-1, 0, 0.f, 0.f, 0u, 0u, 0ll};
+    if (lane < n && q >= 0 && q < S) sl = spike_at(q, pre, nchunks, slots);
+    return sl;
+}
+
+__device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ unsigned rl(unsigned v, int l) {
+    return (unsigned)__builtin_amdgcn_readlane((int)v, l);
+}
+__device__ __forceinline__ float rl(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ long long rl(long long v, int l) {
+    const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, l);
+    const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)((unsigned long long)v >> 32), l);
+    return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// The spike held by lane l.
+__device__ __forceinline__ SpikeG spike_gram(const Slot &w, int l) {
     SpikeG s;
-    s.x = uni(sl.x);
-    s.amp = uni(sl.amp);
-    s.lo = uni(sl.lo);
-    s.hi = uni(sl.hi);
-    s.off = uni((long long)sl.off);
+    s.x = rl(w.x, l);
+    s.amp = rl(w.amp, l);
+    const unsigned lh = rl(w.lohi, l);
+    s.lo = (int)(lh & 0xffffu);
+    s.hi = (int)(lh >> 16);
+    s.off = rl(w.off, l);
     return s;
 }
 
@@ -700,22 +729,24 @@ struct SpikeCtx {
 __device__ __forceinline__ void spike_ctx(SpikeCtx &c, int q, int S, int nt, const int *pre,
                                           int nchunks, const Slot *__restrict__ slots,
                                           SpikeG *s_far) {
-    c.g0 = spike_gram(spike_at(q, pre, nchunks, slots));
+    // lanes 0 .. kMaxCover + 1: spikes q - 1 .. q + kMaxCover
+    const Slot w = spike_lanes(q - 1, kMaxCover + 2, S, pre, nchunks, slots);
+    c.g0 = spike_gram(w, 1);
     c.has_p = q > 0;
-    c.x_p = c.has_p ? uni(spike_at(q - 1, pre, nchunks, slots).x) : 0;
+    c.x_p = c.has_p ? rl(w.x, 0) : 0;
     const int reach = c.g0.x + 2 * nt;
     c.ncov = 1;
     c.over = false;
     c.x2 = c.x3 = c.x_over = 0;
     for (int j = 1; j <= kMaxCover && q + j < S; ++j) {
-        const Slot sj = spike_at(q + j, pre, nchunks, slots);
-        if (uni(sj.x) > reach) break;
+        const int xj = rl(w.x, j + 1);
+        if (xj > reach) break;
         if (j == kMaxCover) {
             c.over = true;
-            c.x_over = uni(sj.x);
+            c.x_over = xj;
             break;
         }
-        const SpikeG gj = spike_gram(sj);
+        const SpikeG gj = spike_gram(w, j + 1);
         if (j == 1) {
             c.g1 = gj;
         } else {
@@ -1159,19 +1190,21 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     for (int j = gw; j < n_xres; j += nwaves) {
         const int q = j / kXresItems;
         const int k0 = j % kXresItems, k1 = k0 + 1;
-        const Slot rs = spike_at(q, s_pre, nchunks, slots);
-        const int id = uni(rs.iy), x = uni(rs.x);
+        // lanes 0, 1, 2: spikes q - 1, q, q + 1
+        const Slot rw = spike_lanes(q - 1, 3, S, s_pre, nchunks, slots);
+        const int id = rl(rw.iy, 1), x = rl(rw.x, 1);
         // windows closer than nt samples (exact ties): the cluster's first
         // spike handles the whole cluster
-        if (q > 0 && x - uni(spike_at(q - 1, s_pre, nchunks, slots).x) < nt) continue;
-        if (q + 1 < S && uni(spike_at(q + 1, s_pre, nchunks, slots).x) - x < nt) {
+        if (q > 0 && x - rl(rw.x, 0) < nt) continue;
+        if (q + 1 < S && rl(rw.x, 2) - x < nt) {
             for (int k = k0; k < k1; ++k)
                 xres_cluster(q, k, S, s_pre, nchunks, slots, Xres, ldx, U, W, crange, P, C, nt,
                              state, s_cl[wv]);
             continue;
         }
-        const float amp = uni(rs.amp);
-        const int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
+        const float amp = rl(rw.amp, 1);
+        const unsigned crg = rl(rw.crg, 1);
+        const int clo = (int)(crg & 0xffffu), chi = (int)(crg >> 16);
         const int cw = (chi - clo + kXresItems - 1) / kXresItems;
         const float *uj = U + (size_t)id * P * C;
         for (int k = k0; k < k1; ++k) {
@@ -1225,7 +1258,8 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
 __global__ void gram_refs_kernel(const int32_t *__restrict__ rank,
                                  const int32_t *__restrict__ lohi,
                                  const int64_t *__restrict__ off, const float *__restrict__ nm,
-                                 int K, GramRef *__restrict__ gref) {
+                                 const int *__restrict__ crange, int K,
+                                 GramRef *__restrict__ gref) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= K) return;
     const int jr = rank[j];
@@ -1235,6 +1269,8 @@ __global__ void gram_refs_kernel(const int32_t *__restrict__ rank,
     g.lo = lohi[2 * jr];
     g.hi = lohi[2 * jr + 1];
     g.off = off[jr];
+    g.clo = crange[2 * j];
+    g.chi = crange[2 * j + 1];
     gref[j] = g;
 }
 
@@ -1427,7 +1463,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
              int capacity, int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
     if (!Bt || !Xres || !U || !W || !nm || !perm || !rank || !cc || !lohi || !off || !st ||
         !amps || !th_amps || !info || K <= 0 || K > kMaxK || ldbt < K || (ldbt % 4) != 0 ||
-        P <= 0 || P > 8 || C <= 0 || nt <= 0 || nt > kMaxNt || (nt % 2) == 0 ||
+        P <= 0 || P > 8 || C <= 0 || C > kMaxC || nt <= 0 || nt > kMaxNt || (nt % 2) == 0 ||
         T < 2 * nt + 1 || T > kChunk * kMaxChunks || ldx < T || max_peels < 0 ||
         capacity < 0 || (flags & ~KSM_PEEL_RESCORE) != 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
@@ -1459,8 +1495,8 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     KSM_LAUNCH_CHECK("channel_range_kernel");
     position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, ppid);
     KSM_LAUNCH_CHECK("position_tables_kernel");
-    gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, K, gref);
-    KSM_LAUNCH_CHECK("position_tables_kernel");
+    gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, crange, K, gref);
+    KSM_LAUNCH_CHECK("gram_refs_kernel");
     {
         ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
         colmax_all_kernel<<<(T + kFirstCols - 1) / kFirstCols, kUpdThreads, tab_dyn, s>>>(
