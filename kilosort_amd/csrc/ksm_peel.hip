@@ -482,8 +482,12 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
     int ldbt, const GramRef *__restrict__ gref, int T, int nt, float th2, int rescore,
     const float *__restrict__ Xres, int ldx, const float *__restrict__ U,
     const float *__restrict__ W, const int *__restrict__ crange, int P, int C,
-    int *__restrict__ state, int *__restrict__ chunk_count, Slot *__restrict__ slots) {
-    if (state[ST_DONE]) return;
+    const int32_t *__restrict__ rank, int K, int *__restrict__ state,
+    int *__restrict__ chunk_count, Slot *__restrict__ slots) {
+    // every load of the prologue is issued together: the done flag, the
+    // window, the column's template and the rank table
+    const int done = state[ST_DONE];
+    extern __shared__ int s_rank[];          // position in the peel order, by id
     __shared__ float win[kChunk + 2 * kMaxNt];
     __shared__ float win2[kChunk + 2 * kMaxNt];
     __shared__ int s_wc[kChunk / 64];
@@ -500,7 +504,9 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
         const int xe = t0 - nt + e;
         win[e] = (xe >= 0 && xe < T) ? cfmax[xe] : -INFINITY;
     }
+    for (int e = tid; e < K; e += NTH) s_rank[e] = rank[e];
     __syncthreads();
+    if (done) return;                        // block-uniform
     const float v = tid < kChunk ? win[tid + nt] : 0.f;     // Cfmax of column x
     // window max of width L = 2nt+1 (max_pool1d, -inf padding): maxima of
     // 2^k consecutive values by doubling, then two overlapping 2^k windows
@@ -536,11 +542,12 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
         const int pos = off + __popcll(m & lanemask_lt(lane));
         if (pos < kSlotCap) {
             const int iy = my_id;
+            const float b = Bt[(size_t)x * ldbt + s_rank[iy]];
             const GramRef g = gref[iy];
             Slot sl;
             sl.x = x;
             sl.iy = iy;
-            sl.amp = Bt[(size_t)x * ldbt + g.rank] / g.nm;
+            sl.amp = b / g.nm;
             sl.tha = sqrtf(cmax);
             sl.lohi = (unsigned)g.lo | ((unsigned)g.hi << 16);
             sl.crg = (unsigned)g.clo | ((unsigned)g.chi << 16);
@@ -798,21 +805,39 @@ struct ColLoads {
     u64 kept;
 };
 
+// 16- and 8-byte accesses at base + i elements.
+__device__ __forceinline__ float4 ld4(const float *base, int i) {
+    return *reinterpret_cast<const float4 *>(base + i);
+}
+__device__ __forceinline__ void st4(float *base, int i, const float4 &v) {
+    *reinterpret_cast<float4 *>(base + i) = v;
+}
+__device__ __forceinline__ u64 ld8(const u64 *base, int i) { return base[i]; }
+__device__ __forceinline__ void st8(u64 *base, int i, u64 v) { base[i] = v; }
+
+// Key of group `lane` if it is outside L (read before any store); the
+// segment paths run only where G <= 64.  keep: lane < G and lane not in L.
+__device__ __forceinline__ u64 seg_kept(bool keep, const GroupList &L, int G,
+                                        const u64 *__restrict__ gk) {
+    return keep ? ld8(gk, threadIdx.x & 63) : 0ull;
+}
+
 __device__ __forceinline__ void seg_load(ColLoads &l, int t, long long ba, long long bb,
                                          bool two, const SegPass &s0, const SegPass &s1,
-                                         bool keep, const float *__restrict__ Bt, int ldbt,
+                                         bool keep, const GroupList &L,
+                                         const float *__restrict__ Bt, int ldbt,
                                          const float *__restrict__ cc, int G,
                                          const u64 *__restrict__ gkey) {
     const float *col = Bt + (size_t)t * ldbt;
-    l.v0 = *reinterpret_cast<const float4 *>(col + s0.rb_);
-    l.va0 = *reinterpret_cast<const float4 *>(cc + ba + s0.ra);
-    l.vb0 = *reinterpret_cast<const float4 *>(cc + bb + s0.rb);
+    l.v0 = ld4(col, s0.rb_);
+    l.va0 = ld4(cc + ba, s0.ra);
+    l.vb0 = ld4(cc + bb, s0.rb);
     if (two) {
-        l.v1 = *reinterpret_cast<const float4 *>(col + s1.rb_);
-        l.va1 = *reinterpret_cast<const float4 *>(cc + ba + s1.ra);
-        l.vb1 = *reinterpret_cast<const float4 *>(cc + bb + s1.rb);
+        l.v1 = ld4(col, s1.rb_);
+        l.va1 = ld4(cc + ba, s1.ra);
+        l.vb1 = ld4(cc + bb, s1.rb);
     }
-    l.kept = keep ? gkey[(size_t)t * G + (threadIdx.x & 63)] : 0ull;
+    l.kept = seg_kept(keep, L, G, gkey + (size_t)t * G);
 }
 
 // Applies a then b to the lane's 4 positions (products rounded before the
@@ -846,8 +871,8 @@ __device__ __forceinline__ u64 seg_compute(float4 &v, const float4 &va, const fl
 __device__ __forceinline__ void seg_store(const float4 &v, const SegPass &sp, u64 key, int t,
                                           float *__restrict__ Bt, int ldbt, int G,
                                           u64 *__restrict__ gkey, u64 &acc) {
-    if (sp.ha || sp.hb) *reinterpret_cast<float4 *>(Bt + (size_t)t * ldbt + sp.r) = v;
-    if (sp.in && (threadIdx.x & 7) == 0) gkey[(size_t)t * G + sp.r / kGroup] = key;
+    if (sp.ha || sp.hb) st4(Bt + (size_t)t * ldbt, sp.r, v);
+    if (sp.in && (threadIdx.x & 7) == 0) st8(gkey + (size_t)t * G, sp.r / kGroup, key);
     acc = key > acc ? key : acc;
 }
 
@@ -856,8 +881,9 @@ __device__ __forceinline__ void seg_store(const float4 &v, const SegPass &sp, u6
 __device__ __forceinline__ void seg_column(ColLoads &cur, ColLoads &nxt, int t, int t1,
                                            long long &ba, long long &bb, int wa, int wb,
                                            bool two, const SegPass &s0, const SegPass &s1,
-                                           bool keep, float amp_a, float amp_b,
-                                           float *__restrict__ Bt, int ldbt, int K, int G,
+                                           bool keep, const GroupList &L, float amp_a,
+                                           float amp_b, float *__restrict__ Bt, int ldbt, int K,
+                                           int G,
                                            const float *__restrict__ cc, const float *s_nm,
                                            const int *s_pid, u64 *__restrict__ gkey,
                                            float *__restrict__ cfmax, int *__restrict__ imax) {
@@ -866,7 +892,7 @@ __device__ __forceinline__ void seg_column(ColLoads &cur, ColLoads &nxt, int t, 
     if (two) k1 = seg_compute(cur.v1, cur.va1, cur.vb1, s1, amp_a, amp_b, K, s_nm, s_pid);
     ba += wa;
     bb += wb;
-    if (t < t1) seg_load(nxt, t + 1, ba, bb, two, s0, s1, keep, Bt, ldbt, cc, G, gkey);
+    if (t < t1) seg_load(nxt, t + 1, ba, bb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
     u64 acc = cur.kept;
     seg_store(cur.v0, s0, k0, t, Bt, ldbt, G, gkey, acc);
     if (two) seg_store(cur.v1, s1, k1, t, Bt, ldbt, G, gkey, acc);
@@ -898,17 +924,82 @@ __device__ __forceinline__ void seg_columns(int t0, int t1, const ColSpike &a, c
     const bool keep = lane < G && !L.has(lane);
     long long ba = a.base, bb = b.base;
     ColLoads cur, nxt;
-    seg_load(cur, t0, ba, bb, two, s0, s1, keep, Bt, ldbt, cc, G, gkey);
+    seg_load(cur, t0, ba, bb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
     for (int t = t0;; ++t) {
-        seg_column(cur, nxt, t, t1, ba, bb, wa, wb, two, s0, s1, keep, a.amp, b.amp, Bt, ldbt, K,
+        seg_column(cur, nxt, t, t1, ba, bb, wa, wb, two, s0, s1, keep, L, a.amp, b.amp, Bt, ldbt, K,
                    G, cc, s_nm, s_pid, gkey, cfmax, imax);
         if (t >= t1) break;
         cur = nxt;
     }
 }
 
-// Columns of one segment with more than 2 passes (rare: far-apart Gram
-// ranges of two covering spikes), through the general pass pipeline.
+// Columns of one segment with 3 passes (two covering spikes with far-apart
+// Gram ranges), as two jobs per column -- passes 0 and 1, then pass 2 -- in
+// the same pipeline as above, so that a column costs two memory round trips.
+__device__ __forceinline__ void seg_columns3(int t0, int t1, const ColSpike &a, const ColSpike &b,
+                                             int wa, int wb, const GroupList &L,
+                                             float *__restrict__ Bt, int ldbt, int K, int G,
+                                             const float *__restrict__ cc, const float *s_nm,
+                                             const int *s_pid, u64 *__restrict__ gkey,
+                                             float *__restrict__ cfmax,
+                                             int *__restrict__ imax) {
+    const int lane = threadIdx.x & 63;
+    const bool keep = lane < G && !L.has(lane);
+    long long ba = a.base, bb = b.base;
+    SegPass s0, s1;
+    seg_pass(s0, L, 0, a, b, ldbt);
+    seg_pass(s1, L, kPassGroups, a, b, ldbt);
+    ColLoads cur, nxt;
+    seg_load(cur, t0, ba, bb, true, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
+    u64 acc = 0ull;
+    int t = t0, h = 0;
+    for (;;) {
+        // job h of column t: passes (0, 1) or pass 2
+        SegPass p, q;
+        if (h == 0) {
+            p = s0;
+            q = s1;
+        } else {
+            seg_pass(p, L, 2 * kPassGroups, a, b, ldbt);
+            q = p;
+        }
+        const u64 k0 = seg_compute(cur.v0, cur.va0, cur.vb0, p, a.amp, b.amp, K, s_nm, s_pid);
+        u64 k1 = 0ull;
+        if (h == 0) k1 = seg_compute(cur.v1, cur.va1, cur.vb1, q, a.amp, b.amp, K, s_nm, s_pid);
+        const int tn = h ? t + 1 : t, hn = h ^ 1;
+        if (h) {
+            ba += wa;
+            bb += wb;
+        }
+        const bool more = tn <= t1;
+        if (more) {
+            if (hn == 0) {
+                seg_load(nxt, tn, ba, bb, true, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
+            } else {
+                SegPass r2;
+                seg_pass(r2, L, 2 * kPassGroups, a, b, ldbt);
+                seg_load(nxt, tn, ba, bb, false, r2, r2, false, L, Bt, ldbt, cc, G, gkey);
+            }
+        }
+        if (h == 0) acc = cur.kept;
+        seg_store(cur.v0, p, k0, t, Bt, ldbt, G, gkey, acc);
+        if (h == 0) seg_store(cur.v1, q, k1, t, Bt, ldbt, G, gkey, acc);
+        if (h) {
+            const u64 best = wave_max_u64(acc);
+            if (lane == 0) {
+                cfmax[t] = key_score(best);
+                imax[t] = key_id(best);
+            }
+        }
+        if (!more) break;
+        cur = nxt;
+        t = tn;
+        h = hn;
+    }
+}
+
+// Columns of one segment with more than 3 passes (K > 768 only), or of any
+// segment where G > 64 (K > 2048), through the general pass pipeline.
 struct SegColumnsGeneral {
     int t, t1, wa, wb;
     ColSpike a, b;
@@ -964,9 +1055,12 @@ __device__ __forceinline__ void spike_columns(int q, int d_lo, int d_hi, const S
         const ColSpike a = last_even ? sl : sp, b = last_even ? sp : sl;
         const int wa = last_even ? wl : wp, wb = last_even ? wp : wl;
         const GroupList L = group_list(a, b, G, false);
-        if (L.count() <= 2 * kPassGroups) {
+        if (G <= 64 && L.count() <= 2 * kPassGroups) {
             seg_columns(s0, s1, a, b, wa, wb, L, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
                         imax);
+        } else if (G <= 64 && L.count() <= 3 * kPassGroups) {
+            seg_columns3(s0, s1, a, b, wa, wb, L, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
+                         imax);
         } else {
             SegColumnsGeneral src{s0, s1, wa, wb, a, b};
             column_pipeline(src, false, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax, imax);
@@ -1079,12 +1173,6 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int *__restrict__ state, int64_t *__restrict__ st, float *__restrict__ amps,
     float *__restrict__ th_amps, int capacity, float *__restrict__ cfmax,
     int *__restrict__ imax, u64 *__restrict__ gkey, u64 *hflags, unsigned seq) {
-    if (state[ST_DONE]) {
-        if (hflags && blockIdx.x == 0 && threadIdx.x == 0) publish(hflags + 1, seq);
-        return;
-    }
-    if (hflags && blockIdx.x == 0 && threadIdx.x == 0)
-        publish(hflags, ((u64)seq << 32) | (unsigned)it);
     extern __shared__ float s_tab[];        // s_nm[Kg], s_pid[Kg], s_pre[nchunks + 1]
     float *s_nm = s_tab;
     int *s_pid = reinterpret_cast<int *>(s_tab + Kg);
@@ -1095,8 +1183,10 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
-    // the loads of the prologue are issued together: the iteration's output
-    // base, the per-chunk spike counts and the per-position tables
+    // the loads of the prologue are issued together: the done flag, the
+    // iteration's output base, the per-chunk spike counts and the per-position
+    // tables
+    const int done = state[ST_DONE];
     const int kb = state[ST_KBASE + it];
     const int per_t = (nchunks + kUpdThreads - 1) / kUpdThreads;
     const int c0 = tid * per_t, c1 = min(c0 + per_t, nchunks);
@@ -1112,6 +1202,11 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     }
     if (lane == 63) s_wsum[wv] = incl;
     __syncthreads();
+    if (done) {                             // block-uniform
+        if (hflags && blockIdx.x == 0 && tid == 0) publish(hflags + 1, seq);
+        return;
+    }
+    if (hflags && blockIdx.x == 0 && tid == 0) publish(hflags, ((u64)seq << 32) | (unsigned)it);
     int base = 0;
     for (int w = 0; w < wv; ++w) base += s_wsum[w];
     int run = base + incl - mine;
@@ -1488,6 +1583,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     const size_t tab_dyn = (size_t)L.Kg * (sizeof(float) + sizeof(int));
     const size_t upd_dyn = tab_dyn + (size_t)(L.nchunks + 1) * sizeof(int);
     const int grid = update_grid(upd_dyn);
+    const size_t sel_dyn = (size_t)K * sizeof(int);
 
     peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
     KSM_LAUNCH_CHECK("peel_init_kernel");
@@ -1518,13 +1614,13 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         {
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
             if (flags & KSM_PEEL_RESCORE)
-                peel_select_kernel<kRsThreads><<<L.nchunks, kRsThreads, 0, s>>>(
+                peel_select_kernel<kRsThreads><<<L.nchunks, kRsThreads, sel_dyn, s>>>(
                     cfmax, imax, Bt, ldbt, gref, T, nt, th2, 1, Xres, ldx, U, W, crange, P, C,
-                    state, chunk_count, slots);
+                    rank, K, state, chunk_count, slots);
             else
-                peel_select_kernel<kChunk><<<L.nchunks, kChunk, 0, s>>>(
+                peel_select_kernel<kChunk><<<L.nchunks, kChunk, sel_dyn, s>>>(
                     cfmax, imax, Bt, ldbt, gref, T, nt, th2, 0, Xres, ldx, U, W, crange, P, C,
-                    state, chunk_count, slots);
+                    rank, K, state, chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);

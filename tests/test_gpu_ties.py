@@ -18,9 +18,9 @@ import pytest
 import torch
 
 from kilosort_amd import _lib, synth
-from kilosort_amd import template_matching as tm
 from kilosort_amd.batches import ArrayBatches
 from oracle import tm_oracle as O
+from tests.gpu_peel import gpu_peel
 
 pytestmark = pytest.mark.gpu
 
@@ -52,42 +52,6 @@ def _case(plants, seed=61):
     return ops, X, U, B
 
 
-def _gpu_peel(ops, X, U, B):
-    lib = _lib.load()
-    stream = _lib.stream_of(DEV)
-    K, P, C = U.shape
-    T = X.shape[1]
-    Ud = U.to(DEV)
-    Wd = ops["wPCA"].to(DEV)
-    ctc = tm.prepare_matching(dict(ops, wPCA=Wd), Ud)
-    g = tm._grams.get(ctc, Ud, 61, DEV)
-    nm = torch.empty((K,), dtype=torch.float32, device=DEV)
-    _lib.check(lib.ksm_template_norms(Ud.data_ptr(), K, P, C, nm.data_ptr(), stream), "norms")
-    ldbt = (K + 63) // 64 * 64
-    Bt = torch.zeros(((T + 127) // 128 * 128, ldbt), dtype=torch.float32, device=DEV)
-    perm = g.perm.long()
-    Bt[:T, :K] = B.to(DEV)[perm].T
-    Xres = X.to(DEV).clone()
-    cap = 10000
-    st = torch.empty((cap, 2), dtype=torch.int64, device=DEV)
-    amps = torch.empty((cap, 1), dtype=torch.float32, device=DEV)
-    th = torch.empty((cap, 1), dtype=torch.float32, device=DEV)
-    info = torch.zeros((4,), dtype=torch.int32, device=DEV)
-    nb = lib.ksm_peel_workspace(K, T, ops["max_peels"])
-    ws = torch.empty((nb,), dtype=torch.uint8, device=DEV)
-    _lib.check(lib.ksm_peel(Bt.data_ptr(), ldbt, Xres.data_ptr(), T, Ud.data_ptr(), Wd.data_ptr(),
-                            nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(), g.cc.data_ptr(),
-                            g.lohi.data_ptr(), g.off.data_ptr(), K, P, C, T, 61,
-                            float(ops["Th_learned"]), ops["max_peels"], 0, st.data_ptr(),
-                            amps.data_ptr(), th.data_ptr(), cap, info.data_ptr(), ws.data_ptr(),
-                            nb, stream), "ksm_peel")
-    k, status, iters, _ = (int(v) for v in info.cpu())
-    _lib.check(status, "ksm_peel (device)")
-    # the oracle gets the same Gram and norms
-    ctc_ref = ctc.cpu().contiguous()
-    return (st[:k].cpu(), amps[:k].cpu(), th[:k].cpu(), Xres.cpu()), ctc_ref, nm.cpu()
-
-
 CASES = {
     "pair": [(3, 1500, 40.0), (3, 1501, 40.0)],
     "pair_plus_third": [(3, 1500, 40.0), (3, 1501, 40.0), (7, 1564, 30.0)],
@@ -100,7 +64,7 @@ CASES = {
 def test_exact_ties_follow_the_reference(name):
     print(f"[gpu-ties] {name}", flush=True)
     ops, X, U, B = _case(CASES[name])
-    got, ctc_ref, nm = _gpu_peel(ops, X, U, B)
+    got, ctc_ref, nm = gpu_peel(ops, X, U, B, capacity=10000)
     ref = O.peel(ops, B.clone(), X, U, ctc_ref, nm, capacity=10000)
     st, amps, th, Xres = got
     assert len(ref[0]) >= len(CASES[name])
