@@ -74,13 +74,15 @@ def _check_U(U, device):
 
 
 class _Workspace:
-    """Grow-only device scratch buffers, one set per device."""
+    """Grow-only device scratch buffers, one set per (device, stream): calls
+    on different streams (e.g. batches run concurrently from several host
+    threads) never share scratch memory."""
 
     def __init__(self):
         self._bufs = {}
 
     def get(self, device, name, nbytes):
-        key = (str(device), name)
+        key = (str(device), torch.cuda.current_stream(device).cuda_stream, name)
         buf = self._bufs.get(key)
         if buf is None or buf.numel() < nbytes:
             buf = torch.empty(max(int(nbytes), 1), dtype=torch.uint8, device=device)
@@ -96,7 +98,8 @@ _ws = _Workspace()
 
 class _Gram:
     """Peel-side form of one (templates, Gram) pair."""
-    __slots__ = ("base", "ldl", "rows", "nrows", "perm", "rank", "lohi", "off", "cc")
+    __slots__ = ("base", "ldl", "rows", "nrows", "perm", "rank", "lohi", "off", "cc",
+                 "ready")
 
 
 class _GramCache:
@@ -114,6 +117,8 @@ class _GramCache:
         key = (ctc.data_ptr(), ctc._version, tuple(ctc.shape), tuple(ctc.stride()),
                U.data_ptr(), U._version, tuple(U.shape))
         if self.key == key:
+            # built on another stream: order this stream after it
+            torch.cuda.current_stream(device).wait_event(self.val.ready)
             return self.val
         L = _lags(nt)
         if tuple(ctc.shape) != (K, K, L):
@@ -150,6 +155,8 @@ class _GramCache:
         _lib.check(lib.ksm_gram_compact(base.data_ptr(), ldl, g.perm.data_ptr(),
                                         g.lohi.data_ptr(), g.off.data_ptr(), K, L,
                                         g.cc.data_ptr(), stream), "ksm_gram_compact")
+        g.ready = torch.cuda.Event()
+        g.ready.record(torch.cuda.current_stream(device))
         self.key = key
         self.val = g
         return g
