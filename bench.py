@@ -120,30 +120,36 @@ def agreement(st, amps, ref):
 
 
 def time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, steps,
-              warmup):
+              warmup, lanes=None):
     """Untimed warmup, then `steps` batches end to end -- matching, features
-    and the host copies of st / th_amps / tF, the copies of one batch
-    overlapping the next batch's device work -- with no events in the loop."""
+    and the host copies of st / th_amps / tF -- through the library's batch
+    pipeline (batches spread over its device streams, one host thread each),
+    with no events in the loop."""
     o = dict(ops, matching_precision=precision)
     nb = len(batches)
-    pipe = tm._BatchPipeline(o, U, ctc, iCC, iU, Ucc, device)
-    for i in range(warmup):
-        pipe.submit(batches[i % nb])
-    pipe.flush()
-    torch.cuda.synchronize()
-    nsp = iters = 0
-    t0 = time.perf_counter()
-    for i in range(steps):
-        res = pipe.submit(batches[(warmup + i) % nb])
-        iters += tm.last_peel_iterations
-        if res is not None:
+    pipe = tm._BatchPipeline(o, U, ctc, iCC, iU, Ucc, device, lanes=lanes)
+    n_lanes = len(pipe.lanes)
+    try:
+        for i in range(warmup):
+            pipe.submit(batches[i % nb])
+        pipe.flush()
+        torch.cuda.synchronize()
+        it0 = pipe.peel_iterations
+        nsp = 0
+        t0 = time.perf_counter()
+        for i in range(steps):
+            for res in pipe.submit(batches[(warmup + i) % nb]):
+                nsp += len(res[1])
+        for res in pipe.flush():
             nsp += len(res[1])
-    res = pipe.flush()
-    nsp += len(res[1])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        iters = pipe.peel_iterations - it0
+    finally:
+        pipe.close()
     return dict(value=C * NT * steps / elapsed, ms_per_step=elapsed / steps * 1e3,
-                spikes_per_batch=nsp / steps, peel_iterations_per_batch=iters / steps)
+                spikes_per_batch=nsp / steps, peel_iterations_per_batch=iters / steps,
+                streams=n_lanes)
 
 
 PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_update",
@@ -208,6 +214,8 @@ def main():
     ap.add_argument("--forms", default=",".join(FORMS),
                     help="comma list of forms to time beside `value` ('' for none)")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--streams", type=int, default=None,
+                    help="device streams of the batch pipeline (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU oracle runs (cpu_baseline and FP64 agreement)")
     args = ap.parse_args()
@@ -239,8 +247,9 @@ def main():
     for f in forms:
         log(f"timing {f}: warmup {args.warmup}, {args.steps} steps")
         results[f] = time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, f,
-                               args.steps, args.warmup)
-        log(f"{f}: {results[f]['ms_per_step']:.3f} ms/step")
+                               args.steps, args.warmup, lanes=args.streams)
+        log(f"{f}: {results[f]['ms_per_step']:.3f} ms/step on "
+            f"{results[f]['streams']} streams")
     main_res = results[args.precision]
 
     nprof = min(n_distinct, 4)
@@ -309,7 +318,8 @@ def main():
         "config": {"workload": "383 ch x 700 templates, NT=60000 (+2x61 pad) batches, "
                                "Th=8, max_peels=100, Poisson 1-20 Hz, 20% overlaps",
                    "channels": C, "templates": K, "batch_samples": NT,
-                   "precision": args.precision, "distinct_batches": n_distinct},
+                   "precision": args.precision, "distinct_batches": n_distinct,
+                   "streams": main_res["streams"]},
         "roofline": roofline,
         "roofline_projection": roofline_projection,
         "kernels_per_batch": kern,

@@ -30,10 +30,16 @@ In the BF16 forms every accepted spike is re-scored in FP32 from the residual
 before it is recorded and subtracted, so amplitudes, th_amps and the
 subtraction use FP32 scores; the accept/reject decisions use the
 reduced-precision scores.
+
+Optional key ``ops['matching_streams']`` (default ``MATCHING_STREAMS``):
+device streams over which ``extract`` spreads its batches, one host thread
+each; results are identical for any value and come back in batch order.
 """
 from __future__ import annotations
 
 import logging
+import queue
+import threading
 
 import numpy as np
 import torch
@@ -44,8 +50,13 @@ logger = logging.getLogger(__name__)
 
 ST_CAPACITY = 100000          # per-batch spike buffer of run_matching (ref :186)
 
-# peel iterations that accepted spikes in the last run_matching call
+# peel iterations that accepted spikes in the last run_matching call (of any
+# thread; _iters.n holds the calling thread's)
 last_peel_iterations = 0
+_iters = threading.local()
+
+# device streams (one host thread each) over which extract runs its batches
+MATCHING_STREAMS = 3
 
 
 def _lags(nt: int) -> int:
@@ -291,6 +302,7 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     _lib.check(status, "ksm_peel (device)")
     global last_peel_iterations
     last_peel_iterations = iters
+    _iters.n = iters
     logger.debug("run_matching: %d spikes in %d peel iterations", k, iters)
     return st[:k], amps[:k], th_amps[:k], Xres
 
@@ -312,26 +324,19 @@ def _features(ops, Xres, st, amps, iCC, iU, Ucc, device):
     return tF
 
 
-class _BatchPipeline:
-    """run_matching + PC features over a sequence of batches, with each
-    batch's host copies (st, th_amps, tF) on a second stream into pinned
-    double buffers, so that they overlap the next batch's device work.
+class _Lane:
+    """One device stream of a _BatchPipeline and the host thread that drives
+    it: run_matching + features per job, then the host copies of (st,
+    th_amps, tF) into one of two pinned buffer sets."""
 
-    ``submit(X, tag)`` launches batch X and returns the host results of the
-    previous batch as ``(tag, st int64 [k, 2], th_amps float32 [k, 1],
-    tF float32 [k, nC, P])`` -- views into pinned buffers that stay valid
-    until the next ``submit`` -- or None after the first batch; ``flush()``
-    returns the last batch's results.
-    """
-
-    def __init__(self, ops, U, ctc, iCC, iU, Ucc, device):
-        self.ops, self.U, self.ctc = ops, U, ctc
-        self.iCC, self.iU, self.Ucc = iCC, iU, Ucc
-        self.device = device
-        self.copy_stream = torch.cuda.Stream(device=device)
+    def __init__(self, pipe):
+        self.pipe = pipe
+        self.stream = torch.cuda.Stream(device=pipe.device)
+        self.jobs = queue.Queue()
+        self.results = queue.Queue()
         self.bufs = [None, None]
-        self.slot = 0
-        self.pending = None
+        self.thread = threading.Thread(target=self._run, daemon=True)
+        self.thread.start()
 
     def _host(self, slot, k, nC, P):
         b = self.bufs[slot]
@@ -343,39 +348,110 @@ class _BatchPipeline:
             self.bufs[slot] = b
         return b
 
+    def _run(self):
+        p = self.pipe
+        with torch.cuda.device(p.device), torch.cuda.stream(self.stream):
+            while True:
+                job = self.jobs.get()
+                if job is None:
+                    return
+                slot, X, tag, ready = job
+                try:
+                    self.stream.wait_event(ready)
+                    stt, amps, th, Xres = run_matching(p.ops, X, p.U, p.ctc, device=p.device)
+                    iters = _iters.n
+                    xfeat = _features(p.ops, Xres, stt, amps, p.iCC, p.iU, p.Ucc, p.device)
+                    k = stt.shape[0]
+                    nC, _, P = p.Ucc.shape
+                    hb = self._host(slot, k, nC, P)
+                    if k:
+                        hb[0][:k].copy_(stt, non_blocking=True)
+                        hb[1][:k].copy_(th, non_blocking=True)
+                        hb[2][:k].copy_(xfeat, non_blocking=True)
+                    done = torch.cuda.Event()
+                    done.record(self.stream)
+                    done.synchronize()
+                    del X, stt, amps, th, Xres, xfeat
+                    self.results.put((tag, hb[0][:k], hb[1][:k], hb[2][:k], iters))
+                except BaseException as e:      # handed to the submitting thread
+                    self.results.put(e)
+
+
+class _BatchPipeline:
+    """run_matching + PC features over a sequence of batches, on ``lanes``
+    device streams at once with one host thread each (the peel's host loop
+    follows its stream's progress), so that one batch's latency-bound late
+    peel iterations and small kernels overlap other batches' work.  Batch i
+    runs on lane i mod lanes; each lane copies its results to pinned host
+    buffers, two sets per lane.
+
+    ``submit(X, tag)`` queues batch X (a device tensor made on the caller's
+    current stream) and returns the batches finished so far that the lanes
+    need back, in submission order, as a list of ``(tag, st int64 [k, 2],
+    th_amps float32 [k, 1], tF float32 [k, nC, P])`` -- views into pinned
+    buffers that stay valid until the next ``submit`` or ``flush`` returns;
+    ``flush()`` returns all remaining batches and ``close()`` ends the
+    threads.  ``peel_iterations`` sums the peel iterations of the batches
+    returned.
+    """
+
+    def __init__(self, ops, U, ctc, iCC, iU, Ucc, device, lanes=None):
+        self.ops, self.U, self.ctc = ops, U, ctc
+        self.iCC, self.iU, self.Ucc = iCC, iU, Ucc
+        self.device = device
+        # the derived Gram once, on the caller's stream (lanes wait for it)
+        _grams.get(ctc, _check_U(U, device), int(ops['nt']), device)
+        n = MATCHING_STREAMS if lanes is None else int(lanes)
+        if n < 1:
+            raise ValueError("lanes must be >= 1")
+        self.lanes = [_Lane(self) for _ in range(n)]
+        self.n_sub = 0
+        self.n_done = 0
+        self.peel_iterations = 0
+
+    def _take(self):
+        lane = self.lanes[self.n_done % len(self.lanes)]
+        res = lane.results.get()
+        self.n_done += 1
+        if isinstance(res, BaseException):
+            raise res
+        tag, st, th, tf, iters = res
+        self.peel_iterations += iters
+        return tag, st, th, tf
+
     def submit(self, X, tag=None):
-        stt, amps, th, Xres = run_matching(self.ops, X, self.U, self.ctc, device=self.device)
-        xfeat = _features(self.ops, Xres, stt, amps, self.iCC, self.iU, self.Ucc, self.device)
-        k = stt.shape[0]
-        nC, _, P = self.Ucc.shape
-        hb = self._host(self.slot, k, nC, P)
+        n = len(self.lanes)
+        i = self.n_sub
+        out = []
+        # the lane's previous batch (i - n) must be back before it starts
+        # batch i, whose host buffers alternate with it
+        while self.n_done <= i - n:
+            out.append(self._take())
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.device))
-        with torch.cuda.stream(self.copy_stream):
-            self.copy_stream.wait_event(ready)
-            if k:
-                hb[0][:k].copy_(stt, non_blocking=True)
-                hb[1][:k].copy_(th, non_blocking=True)
-                hb[2][:k].copy_(xfeat, non_blocking=True)
-            done = torch.cuda.Event()
-            done.record(self.copy_stream)
-        prev = self.pending
-        # the device tensors stay referenced until their copies are done
-        self.pending = (tag, done, k, self.slot, (stt, th, xfeat))
-        self.slot ^= 1
-        return self._finish(prev)
-
-    def _finish(self, p):
-        if p is None:
-            return None
-        tag, done, k, slot, _refs = p
-        done.synchronize()
-        hb = self.bufs[slot]
-        return tag, hb[0][:k], hb[1][:k], hb[2][:k]
+        self.lanes[i % n].jobs.put(((i // n) % 2, X, tag, ready))
+        self.n_sub += 1
+        return out
 
     def flush(self):
-        p, self.pending = self.pending, None
-        return self._finish(p)
+        out = []
+        while self.n_done < self.n_sub:
+            out.append(self._take())
+        return out
+
+    def close(self):
+        for lane in self.lanes:
+            lane.jobs.put(None)
+        for lane in self.lanes:
+            lane.thread.join()
+        self.lanes = []
+
+    def __del__(self):
+        if getattr(self, "lanes", None):
+            try:
+                self.close()
+            except Exception:
+                pass
 
 
 def _extract_batch(ops, X, U, ctc, iCC, iU, Ucc, device):
@@ -410,7 +486,8 @@ def extract(ops, bfile, U, device=torch.device('cuda'), progress_bar=None):
     k = 0
     n_batches = int(bfile.n_batches)
     ibatch, X, stt_h = -1, None, None
-    pipe = _BatchPipeline(ops, U, ctc, iCC, iU, Ucc, device)
+    pipe = _BatchPipeline(ops, U, ctc, iCC, iU, Ucc, device,
+                          lanes=ops.get('matching_streams', MATCHING_STREAMS))
 
     def consume(res):
         nonlocal st, tF, k, stt_h
@@ -438,17 +515,18 @@ def extract(ops, bfile, U, device=torch.device('cuda'), progress_bar=None):
             if ibatch % 100 == 0:
                 logger.debug("extract: batch %d of %d", ibatch, n_batches)
             X = bfile.padded_batch_to_torch(ibatch, ops)
-            res = pipe.submit(X, tag=ibatch)
-            if res is not None:
+            for res in pipe.submit(X, tag=ibatch):
                 consume(res)
-        res = pipe.flush()
-        if res is not None:
+            X = None
+        for res in pipe.flush():
             consume(res)
     except Exception:
         logger.exception(f'Error in template_matching.extract on batch {ibatch}')
         logger.debug(f'X shape: {None if X is None else tuple(X.shape)}')
         logger.debug(f'stt shape: {None if stt_h is None else tuple(stt_h.shape)}')
         raise
+    finally:
+        pipe.close()
     isort = np.argsort(st[:k, 0])
     st = st[isort]
     tF = tF[isort]

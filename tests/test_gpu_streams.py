@@ -71,3 +71,21 @@ def test_concurrent_streams_match_sequential():
         np.testing.assert_array_equal(got[i][0], seq[i][0])
         np.testing.assert_array_equal(got[i][1], seq[i][1])
         np.testing.assert_array_equal(got[i][2], seq[i][2])
+
+
+@pytest.mark.parametrize("streams", [1, 2, 4])
+def test_extract_streams_identical(streams):
+    """extract over 5 batches gives the same st and tF for any number of
+    device streams (batches spread over lanes, results back in order)."""
+    say(f"extract C=64 K=64 5x20000 with matching_streams={streams} vs 3")
+    rec = synth.make_recording(82, 64, 64, 5 * NT, rate_range=(5.0, 30.0))
+    ops = dict(nt=61, nt0min=rec.nt0min, Th_learned=8, max_peels=100, batch_size=NT,
+               wPCA=torch.from_numpy(rec.wPCA).to(DEV), xc=rec.xc, yc=rec.yc,
+               settings=dict(nearest_chans=10, position_limit=100.0, n_pcs=6))
+    U = torch.from_numpy(rec.U).to(DEV)
+    bf = ArrayBatches(rec.data, NT=NT, nt=61, device=DEV)
+    st3, tF3, _ = tm.extract(dict(ops, matching_streams=3), bf, U, device=DEV)
+    st, tF, _ = tm.extract(dict(ops, matching_streams=streams), bf, U, device=DEV)
+    assert len(st3) > 100
+    np.testing.assert_array_equal(st, st3)
+    np.testing.assert_array_equal(tF.numpy(), tF3.numpy())
