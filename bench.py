@@ -196,6 +196,17 @@ def measured_traffic(kernel):
         return None
 
 
+def trace_avg_ms(kernel):
+    """Average duration (ms) of `kernel` in the committed rocprofv3 kernel
+    trace of this workload; None when absent."""
+    try:
+        with open(TRAFFIC) as fh:
+            v = json.load(fh)["kernels"][kernel]["trace_avg_us"]
+        return None if v is None else v / 1e3
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def peel_update_bytes(ids, nnzK, nnzC, nbatch):
     """Algorithmic HBM bytes of the subtraction per batch: for each accepted
     spike of template j, 12 B per (template i, lag) of its nonzero Gram rows
@@ -273,6 +284,14 @@ def main():
                 "bytes_per_launch": upd_bytes / upd["launches_per_batch"],
                 "avg_launch_ms": upd["avg_launch_ms"],
                 "launches_per_batch": upd["launches_per_batch"]}
+    # the committed kernel trace of the same workload: events stamped through
+    # hipExtLaunchKernel read a few microseconds longer per launch than the
+    # trace's dispatch timestamps (see the profile summary)
+    tr = trace_avg_ms("peel_update_kernel")
+    if tr:
+        roofline["rocprof_avg_launch_ms"] = tr
+        roofline["frac_at_rocprof_duration"] = (upd_bytes / upd["launches_per_batch"]) / (
+            tr / 1e3) / PEAK_HBM
     gname = GEMM_KERNEL[args.precision]
     g = kern[gname]
     flops_gemm = 2.0 * K * P * C * T

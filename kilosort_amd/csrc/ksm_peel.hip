@@ -1595,8 +1595,8 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     KSM_LAUNCH_CHECK("gram_refs_kernel");
     {
         ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
-        colmax_all_kernel<<<(T + kFirstCols - 1) / kFirstCols, kUpdThreads, tab_dyn, s>>>(
-            Bt, ldbt, pnm, ppid, K, L.G, L.Kg, T, nt, cfmax, imax, gkey);
+        prof.launch(colmax_all_kernel, dim3((T + kFirstCols - 1) / kFirstCols), dim3(kUpdThreads),
+                    (unsigned)tab_dyn, Bt, ldbt, pnm, ppid, K, L.G, L.Kg, T, nt, cfmax, imax, gkey);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
     // stop enqueuing once the device reports the end (not under stream capture)
@@ -1614,20 +1614,20 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         {
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
             if (flags & KSM_PEEL_RESCORE)
-                peel_select_kernel<kRsThreads><<<L.nchunks, kRsThreads, sel_dyn, s>>>(
-                    cfmax, imax, Bt, ldbt, gref, T, nt, th2, 1, Xres, ldx, U, W, crange, P, C,
-                    rank, K, state, chunk_count, slots);
+                prof.launch(peel_select_kernel<kRsThreads>, dim3(L.nchunks), dim3(kRsThreads),
+                            (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, 1, Xres,
+                            ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
             else
-                peel_select_kernel<kChunk><<<L.nchunks, kChunk, sel_dyn, s>>>(
-                    cfmax, imax, Bt, ldbt, gref, T, nt, th2, 0, Xres, ldx, U, W, crange, P, C,
-                    rank, K, state, chunk_count, slots);
+                prof.launch(peel_select_kernel<kChunk>, dim3(L.nchunks), dim3(kChunk),
+                            (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, 0, Xres,
+                            ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
-            peel_update_kernel<<<grid, kUpdThreads, upd_dyn, s>>>(
-                it, Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, cc, K, L.G, L.Kg, T,
-                nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity, cfmax,
-                imax, gkey, poll ? hf.dev : nullptr, seq);
+            prof.launch(peel_update_kernel, dim3(grid), dim3(kUpdThreads), (unsigned)upd_dyn, it,
+                        Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, cc, K, L.G, L.Kg, T,
+                        nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity,
+                        cfmax, imax, gkey, poll ? hf.dev : nullptr, seq);
         }
     }
     KSM_LAUNCH_CHECK("peel loop");
@@ -1647,9 +1647,8 @@ int ksm_features(const float *Xres, int ldx, const float *W, const int64_t *st,
         return ksm::fail(KSM_ERR_INVALID, "ksm_features: null pointer");
     const size_t smem = (size_t)(P * nt + kFeatWaves * nC * nt) * sizeof(float);
     ksm::ProfileScope prof("features", ksm::kProfPeel, ksm::as_stream(stream));
-    features_kernel<<<(nsp + kFeatWaves - 1) / kFeatWaves, kFeatWaves * 64, smem,
-                      ksm::as_stream(stream)>>>(Xres, ldx, W, st, amps, nsp, iCC, iU, Ucc, nC,
-                                                K, P, C, nt, tF);
+    prof.launch(features_kernel, dim3((nsp + kFeatWaves - 1) / kFeatWaves), dim3(kFeatWaves * 64),
+                (unsigned)smem, Xres, ldx, W, st, amps, nsp, iCC, iU, Ucc, nC, K, P, C, nt, tF);
     KSM_LAUNCH_CHECK("features_kernel");
     return KSM_OK;
 }

@@ -1,9 +1,10 @@
-// Optional per-kernel timing with HIP events on the launch stream.
+// Optional per-kernel timing with HIP events.
 //
-// When a kernel's group bit is enabled, the entry points bracket it with
-// event pairs; ksm_profile_query synchronises the recorded stop events and returns
-// the summed elapsed time of one kernel name.  Disabled by default, so the
-// normal path records nothing.
+// When a kernel's group bit is enabled, the entry points launch it with an
+// event pair that the runtime stamps with the kernel's start and end;
+// ksm_profile_query synchronises the stop events and returns the summed
+// elapsed time of one kernel name.  Disabled by default, so the normal path
+// records nothing.
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -44,16 +45,17 @@ ProfileScope::ProfileScope(const char *name, unsigned group, hipStream_t s)
     std::lock_guard<std::mutex> lk(g_mu);
     if (!(g_mask & group)) return;
     start_ = take_event();
-    if (start_) hipEventRecord(start_, s);
+    stop_ = take_event();
 }
 
 ProfileScope::~ProfileScope() {
-    if (!start_) return;
     std::lock_guard<std::mutex> lk(g_mu);
-    hipEvent_t stop = take_event();
-    if (!stop) return;
-    hipEventRecord(stop, stream_);
-    g_records.push_back(Record{name_, start_, stop});
+    if (start_ && stop_ && launched_) {
+        g_records.push_back(Record{name_, start_, stop_});
+        return;
+    }
+    if (start_) g_free.push_back(start_);
+    if (stop_) g_free.push_back(stop_);
 }
 
 }  // namespace ksm

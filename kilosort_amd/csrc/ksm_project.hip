@@ -299,11 +299,11 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     if (P == 6 && nt == 61) {
         ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
         dim3 grid((L.ldc + kConvSpan - 1) / kConvSpan, C);
-        conv_pc_kernel<6, 61><<<grid, kConvThreads, 0, s>>>(X, ldx, W, C, T, L.ldc, Bc);
+        prof.launch(conv_pc_kernel<6, 61>, grid, dim3(kConvThreads), 0u, X, ldx, W, C, T, L.ldc, Bc);
     } else {
         ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
         dim3 grid((L.ldc + 255) / 256, C);
-        conv_pc_generic<<<grid, 256, 0, s>>>(X, ldx, W, P, nt, C, T, L.ldc, Bc);
+        prof.launch(conv_pc_generic, grid, dim3(256), 0u, X, ldx, W, P, nt, C, T, L.ldc, Bc);
     }
     KSM_LAUNCH_CHECK("conv_pc_kernel");
     if (L.Rp > L.R) {
@@ -315,8 +315,8 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     KSM_LAUNCH_CHECK("transpose_pad_kernel");
     const int nTm = L.Kp / GBM, nTn = L.ldc / GBN;
     ksm::ProfileScope prof("gemm_f32", ksm::kProfProject, s);
-    gemm_f32_mfma<<<nTm * nTn, 256, 0, s>>>(At, L.Kp, Bc, L.ldc, L.Rp, Bt, ldbt, K,
-                                            (int)ksm::align_up(T, 128), nTm, nTn);
+    prof.launch(gemm_f32_mfma, dim3(nTm * nTn), dim3(256), 0u, At, L.Kp, Bc, L.ldc, L.Rp, Bt,
+                ldbt, K, (int)ksm::align_up(T, 128), nTm, nTn);
     KSM_LAUNCH_CHECK("gemm_f32_mfma");
     return KSM_OK;
 }

@@ -295,11 +295,11 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
         ProfileScope prof("conv_pc", kProfProject, s);
         dim3 grid(L.Tp / kCvT, (L.ldr / P + kCvCh - 1) / kCvCh);
         if (split)
-            conv_pc_bf16_kernel<6, 61, true><<<grid, 256, 0, s>>>(X, ldx, W, C, T, L.Tp, L.ldr, Bh,
-                                                                  Bl);
+            prof.launch(conv_pc_bf16_kernel<6, 61, true>, grid, dim3(256), 0u, X, ldx, W, C, T,
+                        L.Tp, L.ldr, Bh, Bl);
         else
-            conv_pc_bf16_kernel<6, 61, false><<<grid, 256, 0, s>>>(X, ldx, W, C, T, L.Tp, L.ldr,
-                                                                   Bh, Bl);
+            prof.launch(conv_pc_bf16_kernel<6, 61, false>, grid, dim3(256), 0u, X, ldx, W, C, T,
+                        L.Tp, L.ldr, Bh, Bl);
     }
     KSM_LAUNCH_CHECK("conv_pc_bf16_kernel");
     u_bf16_kernel<<<512, 256, 0, s>>>(U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
@@ -314,7 +314,8 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     const int nTm = L.Kp / HBM_, nTn = L.Tp / HBN;
     {
         ProfileScope prof("gemm_bf16", kProfProject, s);
-        gemm_bf16_mfma<<<nTm * nTn, 256, 0, s>>>(sg, split ? 3 : 1, L.ldr, Bt, ldbt, K, nTm, nTn);
+        prof.launch(gemm_bf16_mfma, dim3(nTm * nTn), dim3(256), 0u, sg, split ? 3 : 1, L.ldr, Bt,
+                    ldbt, K, nTm, nTn);
     }
     KSM_LAUNCH_CHECK("gemm_bf16_mfma");
     return KSM_OK;
