@@ -39,6 +39,11 @@ typedef enum ksm_status {
     KSM_ERR_TIES = 5            /* exact score ties beyond slot capacity    */
 } ksm_status;
 
+typedef enum ksm_dtype {
+    KSM_DTYPE_F32 = 0,          /* float32 arrays                            */
+    KSM_DTYPE_F64 = 1           /* float64 arrays (gradient checks)          */
+} ksm_dtype;
+
 typedef enum ksm_precision {
     KSM_PREC_FP32 = 0,          /* FP32 correlation + FP32 MFMA projection   */
     KSM_PREC_BF16 = 1,          /* BF16 operands on MFMA, FP32 accumulation  */
@@ -207,6 +212,42 @@ int ksm_features(const float *Xres, int ldx, const float *W,
 int ksm_profile_enable(int mask);
 int ksm_profile_reset(void);
 int ksm_profile_query(const char *name, double *total_ms, int *launches);
+
+/*
+ * Template refinement: the residual of a fixed spike set and its gradients.
+ * The reference refines templates by averaging spike features
+ * (clustering_qr.py:524-527) and has no gradients; the model is its own
+ * subtraction (template_matching.py:235-236): spike s = (x_s, id_s) = st[s]
+ * with amplitude amps[s] contributes amps[s] * sum_p U[id,p,c] * W[p,k] at
+ * sample x_s - nt/2 + k.  Arrays are float32 or float64 (dtype); U [K][P][C],
+ * W [P][nt], X and R [C][ld*], st int64 [nsp][2], ids in [0, K).
+ */
+
+/* Workspace bytes of ksm_residual and ksm_residual_backward. */
+size_t ksm_residual_workspace(int nsp, int K, int P, int C, int T, int nt, int dtype);
+
+/*
+ * R = X - sum_s amps[s] T_{id_s}(. - x_s) (R may be NULL) and loss[0] =
+ * sum R^2 in float64 (loss may be NULL).  order: spike indices sorted by
+ * x (non-decreasing); contributions are summed in that order.
+ */
+int ksm_residual(const void *X, int ldx, const void *U, const void *W, const int64_t *st,
+                 const void *amps, const int32_t *order, int nsp, int K, int P, int C, int T,
+                 int nt, void *R, int ldr, double *loss, int dtype, void *workspace,
+                 size_t ws_bytes, void *stream);
+
+/*
+ * Gradients of scale/2 * sum R^2 (scale = 2 dL for the plain loss) with
+ * spike times and ids fixed, from the residual R of ksm_residual:
+ *   dU [K][P][C], dW [P][nt], damp [nsp] (by spike index).
+ * order_k: spike indices sorted by (id, x); seg [K+1]: template k's spikes are
+ * order_k[seg[k] .. seg[k+1]).  Deterministic (fixed summation orders).
+ */
+int ksm_residual_backward(const void *R, int ldr, const void *U, const void *W,
+                          const int64_t *st, const void *amps, const int32_t *order_k,
+                          const int32_t *seg, int nsp, int K, int P, int C, int T, int nt,
+                          double scale, void *dU, void *dW, void *damp, int dtype,
+                          void *workspace, size_t ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }

@@ -26,10 +26,14 @@ PREC_BF16X3 = 2
 
 PEEL_RESCORE = 1
 
+DTYPE_F32 = 0
+DTYPE_F64 = 1
+
 _p = ctypes.c_void_p
 _i = ctypes.c_int
 _f = ctypes.c_float
 _sz = ctypes.c_size_t
+_d = ctypes.c_double
 
 # name -> (restype, argtypes); must match include/ksm.h
 SIGNATURES = {
@@ -50,6 +54,11 @@ SIGNATURES = {
     "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
                       _i, _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
+    "ksm_residual_workspace": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
+    "ksm_residual": (_i, [_p, _i, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _i, _p, _i,
+                          _p, _sz, _p]),
+    "ksm_residual_backward": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _d,
+                                   _p, _p, _p, _i, _p, _sz, _p]),
     "ksm_profile_enable": (_i, [_i]),
     "ksm_profile_reset": (_i, []),
     "ksm_profile_query": (_i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
