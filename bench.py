@@ -152,6 +152,47 @@ def time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, st
                 streams=n_lanes)
 
 
+def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
+    """Template refinement on the workload: spikes and amplitudes from the FP32
+    matching of `nbatch` batches, then the residual loss and its gradients for
+    U, wPCA and the amplitudes (float32 kernels), timed with events."""
+    from kilosort_amd import refine
+    W = ops["wPCA"].to(device)
+    data = []
+    for i in range(nbatch):
+        st, amps, _, _ = tm.run_matching(dict(ops, matching_precision="fp32"),
+                                         batches[i % len(batches)], U, ctc, device=device)
+        data.append((batches[i % len(batches)], st, amps[:, 0].contiguous()))
+    nsp = sum(len(d[1]) for d in data)
+
+    def one_pass():
+        Ur = U.detach().requires_grad_(True)
+        Wr = W.detach().requires_grad_(True)
+        for X, st, a in data:
+            ar = a.detach().requires_grad_(True)
+            refine.residual_loss(X, Ur, Wr, ar, st, NTAP).backward()
+
+    one_pass()
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record()
+    for _ in range(reps):
+        for X, st, a in data:
+            refine.residual(X, U, W, a, st, NTAP)
+    e1.record()
+    for _ in range(reps):
+        one_pass()
+    e2.record()
+    torch.cuda.synchronize()
+    fwd = e0.elapsed_time(e1) / (reps * nbatch)
+    both = e1.elapsed_time(e2) / (reps * nbatch)
+    return {"batches": nbatch, "spikes_per_batch": nsp / nbatch, "templates": K,
+            "residual_ms_per_batch": fwd, "loss_and_gradients_ms_per_batch": both,
+            "note": "float32; loss = sum of squared residual over a padded batch, gradients "
+                    "for U, wPCA and amplitudes with spike times fixed (backward recomputes "
+                    "the residual)"}
+
+
 PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_update",
             "features"]
 
@@ -303,6 +344,11 @@ def main():
                            "traffic": measured_traffic(f"{gname}_mfma"),
                            "avg_launch_ms": g["avg_launch_ms"]}
 
+    log("template refinement (residual loss and gradients)")
+    refine_res = time_refine(tm, ops, batches, U, ctc, device)
+    log(f"refinement: residual {refine_res['residual_ms_per_batch']:.3f} ms, "
+        f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch")
+
     cpu = None
     agree = None
     if not args.no_cpu_baseline:
@@ -349,6 +395,7 @@ def main():
                                                for k, v in prof[f][0].items()})
                   for f in forms},
         "agreement_batch0": agree,
+        "refinement": refine_res,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

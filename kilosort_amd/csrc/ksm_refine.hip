@@ -14,7 +14,9 @@
 //     dW[p, k]    = -scale * sum_s a_s * sum_c U[id_s, p, c] R[c, x_s - h + k]
 //     da_s        = -scale * sum_c sum_k R[c, x_s - h + k] T_{id_s}[c, k]
 // The backward pass reads the residual windows of each spike again from R
-// (lag-window recomputation) instead of keeping per-spike windows.
+// (lag-window recomputation) instead of keeping per-spike windows: one wave
+// per spike forms its window's projections F (on W) and H (on U[id]), which
+// fixed-order reductions turn into dU, dW and the amplitude gradients.
 //
 // Both passes are deterministic: sums over spikes follow a fixed order (time
 // order for the residual, per template then time for the gradients) and
@@ -26,77 +28,85 @@ namespace {
 
 constexpr int kResT = 256;        // samples per residual block (one per thread)
 constexpr int kResC = 16;         // channels per residual block
-constexpr int kResSpk = 64;       // spikes staged in LDS at a time
-constexpr int kBwdC = 64;         // channels per backward block (one per lane)
+constexpr int kResSpk = 32;       // spikes staged in LDS at a time
+constexpr int kBwdC = 64;         // channels per step of a spike's wave (one per lane)
+constexpr int kBwdWaves = 4;      // spikes per backward block (one wave each)
 constexpr int kMaxNtR = 255;
+constexpr int kMaxJB = (kMaxNtR + 63) / 64;   // lag blocks of 64
 constexpr int kMaxPR = 8;
+constexpr int kSlices = 256;      // first-level slices of the dW reduction
 
-// Spike q of order[] (x, id) from st [nsp][2].
-__device__ __forceinline__ long long spike_x(const int64_t *st, const int32_t *order, int q) {
-    return st[2 * (size_t)order[q]];
-}
-
-// First index q in [0, nsp] with x(order[q]) >= v (times non-decreasing).
-__device__ int lower_bound_x(const int64_t *st, const int32_t *order, int nsp, long long v) {
-    int lo = 0, hi = nsp;
+// rng[2 b], rng[2 b + 1] = spikes (positions in `order`, times
+// non-decreasing) whose window [x - h, x + h] meets samples
+// [b kResT, (b + 1) kResT).
+__global__ void tile_ranges_kernel(const int64_t *__restrict__ st, const int32_t *__restrict__ order,
+                                   int nsp, int ntile, int h, int *__restrict__ rng) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= 2 * ntile) return;
+    const long long v = b % 2 == 0 ? (long long)(b / 2) * kResT - h
+                                   : (long long)(b / 2 + 1) * kResT + h;
+    int lo = 0, hi = nsp;                       // first q with x(order[q]) >= v
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (spike_x(st, order, mid) < v) lo = mid + 1;
+        if (st[2 * (size_t)order[mid]] < v) lo = mid + 1;
         else hi = mid;
     }
-    return lo;
+    rng[b] = lo;
 }
 
 // R[c, t] = X[c, t] - sum over the spikes covering t (time order) of
-// a_s * sum_p U[id, p, c] W[p, t - x_s + h]; per-block partial sums of R^2.
-template <typename real>
+// a_s sum_p U[id_s, p, c] W[p, t - x_s + h]; per-block partial sums of R^2.
+// The block's channel slice of each staged spike's U sits in LDS.
+template <typename real, int PT>
 __global__ __launch_bounds__(kResT) void residual_kernel(
     const real *__restrict__ X, int ldx, const real *__restrict__ U, const real *__restrict__ W,
     const int64_t *__restrict__ st, const real *__restrict__ amps,
-    const int32_t *__restrict__ order, int nsp, int P, int C, int T, int nt,
+    const int32_t *__restrict__ order, const int *__restrict__ rng, int P, int C, int T, int nt,
     real *__restrict__ R, int ldr, double *__restrict__ partial) {
-    __shared__ real s_w[kMaxPR * kMaxNtR];
+    __shared__ real s_w[PT * kMaxNtR];
+    __shared__ real s_u[kResSpk][PT][kResC];
     __shared__ long long s_x[kResSpk];
-    __shared__ int s_id[kResSpk];
     __shared__ real s_a[kResSpk];
-    __shared__ int s_rng[2];
     __shared__ double s_red[kResT / 64];
     const int h = nt / 2;
     const int t0 = blockIdx.x * kResT, c0 = blockIdx.y * kResC;
     const int tid = threadIdx.x, t = t0 + tid;
-    for (int e = tid; e < P * nt; e += kResT) s_w[e] = W[e];
-    if (tid == 0) {
-        // spikes whose window [x - h, x + h] meets [t0, t0 + kResT)
-        s_rng[0] = lower_bound_x(st, order, nsp, (long long)t0 - h);
-        s_rng[1] = lower_bound_x(st, order, nsp, (long long)t0 + kResT + h);
-    }
-    __syncthreads();
-    const int q0 = s_rng[0], q1 = s_rng[1];
-    const int c1 = min(C, c0 + kResC);
+    for (int e = tid; e < PT * nt; e += kResT) s_w[e] = e < P * nt ? W[e] : real(0);
+    const int q0 = rng[2 * blockIdx.x], q1 = rng[2 * blockIdx.x + 1];
+    const int nc = min(C - c0, kResC);
     real acc[kResC];
 #pragma unroll
     for (int i = 0; i < kResC; ++i) acc[i] = real(0);
     for (int qb = q0; qb < q1; qb += kResSpk) {
         const int n = min(kResSpk, q1 - qb);
         __syncthreads();
-        if (tid < n) {
-            const int s = order[qb + tid];
-            s_x[tid] = st[2 * (size_t)s];
-            s_id[tid] = (int)st[2 * (size_t)s + 1];
-            s_a[tid] = amps[s];
+        // thread: spike j = e / kResC, channel i = e % kResC of the slice
+        for (int e = tid; e < n * kResC; e += kResT) {
+            const int j = e / kResC, i = e % kResC;
+            const int sidx = order[qb + j];
+            const int id = (int)st[2 * (size_t)sidx + 1];
+            if (i == 0) {
+                s_x[j] = st[2 * (size_t)sidx];
+                s_a[j] = amps[sidx];
+            }
+            const int c = c0 + i;
+#pragma unroll
+            for (int p = 0; p < PT; ++p)
+                s_u[j][p][i] = (p < P && c < C) ? U[((size_t)id * P + p) * C + c] : real(0);
         }
         __syncthreads();
         for (int j = 0; j < n; ++j) {
             const long long k = (long long)t - s_x[j] + h;
             if (k < 0 || k >= nt) continue;
             const real a = s_a[j];
-            const real *u = U + (size_t)s_id[j] * P * C;
+            real w[PT];
+#pragma unroll
+            for (int p = 0; p < PT; ++p) w[p] = s_w[p * nt + (int)k];
 #pragma unroll
             for (int i = 0; i < kResC; ++i) {
-                const int c = c0 + i;
-                if (c >= c1) break;
                 real tv = real(0);
-                for (int p = 0; p < P; ++p) tv += u[(size_t)p * C + c] * s_w[p * nt + (int)k];
+#pragma unroll
+                for (int p = 0; p < PT; ++p) tv += s_u[j][p][i] * w[p];
                 acc[i] += a * tv;
             }
         }
@@ -105,8 +115,8 @@ __global__ __launch_bounds__(kResT) void residual_kernel(
     if (t < T) {
 #pragma unroll
         for (int i = 0; i < kResC; ++i) {
+            if (i >= nc) break;
             const int c = c0 + i;
-            if (c >= c1) break;
             const real r = X[(size_t)c * ldx + t] - acc[i];
             if (R) R[(size_t)c * ldr + t] = r;
             sq += (double)r * (double)r;
@@ -137,128 +147,150 @@ __global__ void sum_partials_kernel(const double *__restrict__ partial, int n,
     }
 }
 
-// Block (template k, channels [c0, c0 + 64)): over the template's spikes in
-// time order, with G the residual window of the spike on these channels,
-//   F[c, p] = sum_j G[c, j] W[p, j]     -> dU[k, p, c] and the spike's dot
-//   H[p, j] = sum_c U[k, p, c] G[c, j]  -> this block's share of dW.
-// Wave 0: lane = channel (F, dU, dot); wave 1: lane = lag (H).
-template <typename real>
-__global__ __launch_bounds__(128) void residual_bwd_kernel(
+// One wave per spike q (template order), over its residual window
+// G[c, j] = R[c, x - h + j] (zero outside the batch), 64 channels at a time
+// staged in the wave's own LDS rows (coalesced loads along the lags):
+//   F[q][p][c] = sum_j G[c, j] W[p, j]          (lane = channel)
+//   H[q][p][j] = sum_c U[id, p, c] G[c, j]      (lane = lag)
+//   dot[q]     = sum_c sum_p U[id, p, c] F[c, p]
+// Every wave runs the same channel steps (waves past the last spike idle
+// through them), so the block can synchronise between steps.
+template <typename real, int PT>
+__global__ __launch_bounds__(64 * kBwdWaves) void spike_windows_kernel(
     const real *__restrict__ R, int ldr, const real *__restrict__ U, const real *__restrict__ W,
-    const int64_t *__restrict__ st, const real *__restrict__ amps,
-    const int32_t *__restrict__ order_k, const int32_t *__restrict__ seg, int P, int C, int T,
-    int nt, real scale, real *__restrict__ dU, real *__restrict__ dWpart,
-    real *__restrict__ dotpart, int nchunk) {
-    __shared__ real s_w[kMaxPR * kMaxNtR];
-    __shared__ real s_u[kMaxPR * kBwdC];
-    extern __shared__ char s_dyn[];                     // G window [kBwdC][nt | 1]
-    real *s_g = reinterpret_cast<real *>(s_dyn);
-    const int k = blockIdx.x, chunk = blockIdx.y;
-    const int c0 = chunk * kBwdC;
+    const int64_t *__restrict__ st, const int32_t *__restrict__ order_k, int nsp, int P, int C,
+    int T, int nt, real *__restrict__ F, real *__restrict__ H, real *__restrict__ dot) {
+    __shared__ real s_w[PT * kMaxNtR];
+    extern __shared__ char s_dyn[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int h = nt / 2;
     const int ld = nt | 1;                              // odd row stride
-    for (int e = tid; e < P * nt; e += 128) s_w[e] = W[e];
-    for (int e = tid; e < P * kBwdC; e += 128) {
-        const int p = e / kBwdC, l = e % kBwdC;
-        s_u[e] = c0 + l < C ? U[((size_t)k * P + p) * C + c0 + l] : real(0);
-    }
-    real accU[kMaxPR], accW[kMaxPR];
+    real *s_g = reinterpret_cast<real *>(s_dyn) + (size_t)wave * (kBwdC * ld + kMaxPR * kBwdC);
+    real *s_u = s_g + kBwdC * ld;                       // [p][channel]
+    for (int e = tid; e < PT * nt; e += 64 * kBwdWaves) s_w[e] = e < P * nt ? W[e] : real(0);
+    const int q = blockIdx.x * kBwdWaves + wave;
+    const bool live = q < nsp;
+    const int h = nt / 2;
+    const int s = live ? order_k[q] : 0;
+    const long long x = live ? st[2 * (size_t)s] : 0;
+    const int id = live ? (int)st[2 * (size_t)s + 1] : 0;
+    const int njb = (nt + 63) / 64;
+    real hacc[kMaxJB][PT];
 #pragma unroll
-    for (int p = 0; p < kMaxPR; ++p) accU[p] = accW[p] = real(0);
-    const int q0 = seg[k], q1 = seg[k + 1];
-    for (int q = q0; q < q1; ++q) {
-        const int s = order_k[q];
-        const long long x = st[2 * (size_t)s];
-        const real a = amps[s];
-        __syncthreads();            // the previous window is no longer read
-        for (int e = tid; e < kBwdC * nt; e += 128) {
-            const int l = e / nt, j = e % nt;
-            const long long t = x - h + j;
-            const int c = c0 + l;
-            s_g[l * ld + j] = (c < C && t >= 0 && t < T) ? R[(size_t)c * ldr + t] : real(0);
+    for (int b = 0; b < kMaxJB; ++b)
+#pragma unroll
+        for (int p = 0; p < PT; ++p) hacc[b][p] = real(0);
+    real dot_l = real(0);
+    for (int c0 = 0; c0 < C; c0 += kBwdC) {
+        const int ncs = min(C - c0, kBwdC);
+        __syncthreads();                                // previous step's reads done
+        if (live) {
+            for (int row = 0; row < ncs; ++row) {
+                const real *rr = R + (size_t)(c0 + row) * ldr;
+                for (int j = lane; j < nt; j += 64) {
+                    const long long t = x - h + j;
+                    s_g[row * ld + j] = (t >= 0 && t < T) ? rr[t] : real(0);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < PT; ++p)
+                s_u[p * kBwdC + lane] =
+                    (p < P && lane < ncs) ? U[((size_t)id * P + p) * C + c0 + lane] : real(0);
         }
         __syncthreads();
-        if (wave == 0) {
-            real f[kMaxPR];
+        if (!live) continue;
+        // F: lane = channel
+        if (lane < ncs) {
+            real f[PT];
 #pragma unroll
-            for (int p = 0; p < kMaxPR; ++p) f[p] = real(0);
+            for (int p = 0; p < PT; ++p) f[p] = real(0);
             const real *g = s_g + lane * ld;
             for (int j = 0; j < nt; ++j) {
                 const real gv = g[j];
 #pragma unroll
-                for (int p = 0; p < kMaxPR; ++p)
-                    if (p < P) f[p] += gv * s_w[p * nt + j];
+                for (int p = 0; p < PT; ++p) f[p] += gv * s_w[p * nt + j];
             }
-            real dot = real(0);
+            const int c = c0 + lane;
 #pragma unroll
-            for (int p = 0; p < kMaxPR; ++p)
-                if (p < P) {
-                    accU[p] += a * f[p];
-                    dot += s_u[p * kBwdC + lane] * f[p];
-                }
-            for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
-            if (lane == 0) dotpart[(size_t)q * nchunk + chunk] = dot;
-        } else if (lane < nt || lane + 64 < nt) {
-            // lags lane, lane + 64, ... (nt <= 255): this wave covers them in turn
-            for (int j = lane; j < nt; j += 64) {
-                real hv[kMaxPR];
+            for (int p = 0; p < PT; ++p) {
+                dot_l += s_u[p * kBwdC + lane] * f[p];
+                if (p < P) F[((size_t)q * P + p) * C + c] = f[p];
+            }
+        }
+        // H: lane = lag, channels of this step in order
 #pragma unroll
-                for (int p = 0; p < kMaxPR; ++p) hv[p] = real(0);
-                for (int l = 0; l < kBwdC; ++l) {
-                    const real gv = s_g[l * ld + j];
+        for (int b = 0; b < kMaxJB; ++b) {
+            if (b >= njb) break;
+            const int j = b * 64 + lane;
+            if (j < nt) {
+                for (int cc = 0; cc < ncs; ++cc) {
+                    const real gv = s_g[cc * ld + j];
 #pragma unroll
-                    for (int p = 0; p < kMaxPR; ++p)
-                        if (p < P) hv[p] += s_u[p * kBwdC + l] * gv;
-                }
-                // lane-held accumulators cover lag `lane` only; further lags
-                // (nt > 64) go straight to their partial slot
-                if (j == lane) {
-#pragma unroll
-                    for (int p = 0; p < kMaxPR; ++p) accW[p] += a * hv[p];
-                } else {
-                    real *dw = dWpart + ((size_t)k * gridDim.y + chunk) * P * nt;
-                    for (int p = 0; p < P; ++p) dw[p * nt + j] += a * hv[p];
+                    for (int p = 0; p < PT; ++p) hacc[b][p] += s_u[p * kBwdC + cc] * gv;
                 }
             }
         }
     }
-    if (wave == 0) {
-        if (c0 + lane < C)
-            for (int p = 0; p < P; ++p) dU[((size_t)k * P + p) * C + c0 + lane] = -scale * accU[p];
-    } else if (lane < nt) {
-        real *dw = dWpart + ((size_t)k * gridDim.y + chunk) * P * nt;
-        for (int p = 0; p < P; ++p) dw[p * nt + lane] += accW[p];
+    for (int o = 32; o > 0; o >>= 1) dot_l += __shfl_xor(dot_l, o, 64);
+    if (!live) return;
+    if (lane == 0) dot[q] = dot_l;
+#pragma unroll
+    for (int b = 0; b < kMaxJB; ++b) {
+        const int j = b * 64 + lane;
+        if (b < njb && j < nt) {
+#pragma unroll
+            for (int p = 0; p < PT; ++p)
+                if (p < P) H[((size_t)q * P + p) * nt + j] = hacc[b][p];
+        }
     }
 }
 
-// damp[s] = -scale * sum over channel chunks of the dots (chunk order).
+// dU[k][p][c] = -scale * sum over template k's spikes (template order) of
+// a_q F[q][p][c].
 template <typename real>
-__global__ void dot_reduce_kernel(const real *__restrict__ dotpart, const int32_t *__restrict__ order_k,
-                                  int nsp, int nchunk, real scale, real *__restrict__ damp) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nsp) return;
+__global__ void du_reduce_kernel(const real *__restrict__ F, const real *__restrict__ amps,
+                                 const int32_t *__restrict__ order_k,
+                                 const int32_t *__restrict__ seg, int P, int C, real scale,
+                                 real *__restrict__ dU) {
+    const int k = blockIdx.y;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;        // p * C + c
+    if (e >= P * C) return;
     real a = real(0);
-    for (int c = 0; c < nchunk; ++c) a += dotpart[(size_t)q * nchunk + c];
-    damp[order_k[q]] = -scale * a;
+    for (int q = seg[k]; q < seg[k + 1]; ++q) a += amps[order_k[q]] * F[(size_t)q * P * C + e];
+    dU[(size_t)k * P * C + e] = -scale * a;
 }
 
-// dW[p, j] = -scale * sum over (template, chunk) partials (index order).
+// damp[s] = -scale * dot[q] for s = order_k[q].
 template <typename real>
-__global__ void dw_reduce_kernel(const real *__restrict__ dWpart, int nparts, int PN, real scale,
-                                 real *__restrict__ dW) {
+__global__ void damp_kernel(const real *__restrict__ dot, const int32_t *__restrict__ order_k,
+                            int nsp, real scale, real *__restrict__ damp) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nsp) damp[order_k[q]] = -scale * dot[q];
+}
+
+// First level of dW: slice b sums a_q H[q] over its contiguous range of q.
+template <typename real>
+__global__ void dw_slices_kernel(const real *__restrict__ H, const real *__restrict__ amps,
+                                 const int32_t *__restrict__ order_k, int nsp, int PN,
+                                 real *__restrict__ part) {
+    const int b = blockIdx.x;
+    const int per = (nsp + kSlices - 1) / kSlices;
+    const int q0 = b * per, q1 = min(nsp, q0 + per);
+    for (int e = threadIdx.x; e < PN; e += blockDim.x) {
+        real a = real(0);
+        for (int q = q0; q < q1; ++q) a += amps[order_k[q]] * H[(size_t)q * PN + e];
+        part[(size_t)b * PN + e] = a;
+    }
+}
+
+// dW[e] = -scale * sum of the slices, in slice order.
+template <typename real>
+__global__ void dw_final_kernel(const real *__restrict__ part, int PN, real scale,
+                                real *__restrict__ dW) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= PN) return;
     real a = real(0);
-    for (int i = 0; i < nparts; ++i) a += dWpart[(size_t)i * PN + e];
+    for (int b = 0; b < kSlices; ++b) a += part[(size_t)b * PN + e];
     dW[e] = -scale * a;
-}
-
-template <typename real>
-__global__ void zero_kernel(real *__restrict__ p, size_t n) {
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
-         e += (size_t)gridDim.x * blockDim.x)
-        p[e] = real(0);
 }
 
 bool bad_sizes(int nsp, int K, int P, int C, int T, int nt) {
@@ -268,16 +300,28 @@ bool bad_sizes(int nsp, int K, int P, int C, int T, int nt) {
 
 size_t elem_size(int dtype) { return dtype == KSM_DTYPE_F64 ? 8 : 4; }
 
+int n_tiles(int T) { return (T + kResT - 1) / kResT; }
+
 template <typename real>
 int residual_impl(const void *X, int ldx, const void *U, const void *W, const int64_t *st,
-                  const void *amps, const int32_t *order, int nsp, int P, int C, int T, int nt,
-                  void *R, int ldr, double *loss, void *workspace, hipStream_t s) {
-    dim3 grid((T + kResT - 1) / kResT, (C + kResC - 1) / kResC);
-    double *partial = static_cast<double *>(workspace);
-    residual_kernel<real><<<grid, kResT, 0, s>>>(
-        static_cast<const real *>(X), ldx, static_cast<const real *>(U),
-        static_cast<const real *>(W), st, static_cast<const real *>(amps), order, nsp, P, C, T,
-        nt, static_cast<real *>(R), ldr, partial);
+                  const void *amps, const int32_t *order, int nsp, int K, int P, int C, int T,
+                  int nt, void *R, int ldr, double *loss, void *workspace, hipStream_t s) {
+    dim3 grid(n_tiles(T), (C + kResC - 1) / kResC);
+    ksm::Carver cv(workspace);
+    double *partial = cv.take<double>((size_t)grid.x * grid.y);
+    int *rng = cv.take<int>(2 * (size_t)grid.x);
+    tile_ranges_kernel<<<(2 * grid.x + 255) / 256, 256, 0, s>>>(st, order, nsp, grid.x, nt / 2,
+                                                                rng);
+    KSM_LAUNCH_CHECK("tile_ranges_kernel");
+    // P = 6 (Kilosort's n_pcs) compiled exactly; other P padded to kMaxPR
+    auto launch = [&](auto kern) {
+        kern<<<grid, kResT, 0, s>>>(static_cast<const real *>(X), ldx,
+                                    static_cast<const real *>(U), static_cast<const real *>(W),
+                                    st, static_cast<const real *>(amps), order, rng, P, C, T, nt,
+                                    static_cast<real *>(R), ldr, partial);
+    };
+    if (P == 6) launch(residual_kernel<real, 6>);
+    else launch(residual_kernel<real, kMaxPR>);
     KSM_LAUNCH_CHECK("residual_kernel");
     if (loss) {
         sum_partials_kernel<<<1, 256, 0, s>>>(partial, (int)(grid.x * grid.y), loss);
@@ -291,27 +335,37 @@ int backward_impl(const void *R, int ldr, const void *U, const void *W, const in
                   const void *amps, const int32_t *order_k, const int32_t *seg, int nsp, int K,
                   int P, int C, int T, int nt, double scale, void *dU, void *dW, void *damp,
                   void *workspace, hipStream_t s) {
-    const int nchunk = (C + kBwdC - 1) / kBwdC;
     ksm::Carver cv(workspace);
-    real *dWpart = cv.take<real>((size_t)K * nchunk * P * nt);
-    real *dotpart = cv.take<real>((size_t)(nsp > 0 ? nsp : 1) * nchunk);
-    const size_t nparts = (size_t)K * nchunk * P * nt;
-    zero_kernel<real><<<256, 256, 0, s>>>(dWpart, nparts);
-    KSM_LAUNCH_CHECK("zero_kernel");
-    const size_t gdyn = (size_t)kBwdC * (nt | 1) * sizeof(real);
-    residual_bwd_kernel<real><<<dim3(K, nchunk), 128, gdyn, s>>>(
-        static_cast<const real *>(R), ldr, static_cast<const real *>(U),
-        static_cast<const real *>(W), st, static_cast<const real *>(amps), order_k, seg, P, C, T,
-        nt, (real)scale, static_cast<real *>(dU), dWpart, dotpart, nchunk);
-    KSM_LAUNCH_CHECK("residual_bwd_kernel");
+    const size_t n1 = nsp > 0 ? nsp : 1;
+    real *F = cv.take<real>(n1 * P * C);
+    real *H = cv.take<real>(n1 * P * nt);
+    real *dot = cv.take<real>(n1);
+    real *part = cv.take<real>((size_t)kSlices * P * nt);
+    const real sc = (real)scale;
+    const real *a = static_cast<const real *>(amps);
     if (nsp > 0) {
-        dot_reduce_kernel<real><<<(nsp + 255) / 256, 256, 0, s>>>(
-            dotpart, order_k, nsp, nchunk, (real)scale, static_cast<real *>(damp));
-        KSM_LAUNCH_CHECK("dot_reduce_kernel");
+        const size_t wdyn =
+            (size_t)kBwdWaves * (kBwdC * (nt | 1) + kMaxPR * kBwdC) * sizeof(real);
+        auto launch = [&](auto kern) {
+            kern<<<(nsp + kBwdWaves - 1) / kBwdWaves, 64 * kBwdWaves, wdyn, s>>>(
+                static_cast<const real *>(R), ldr, static_cast<const real *>(U),
+                static_cast<const real *>(W), st, order_k, nsp, P, C, T, nt, F, H, dot);
+        };
+        if (P == 6) launch(spike_windows_kernel<real, 6>);
+        else launch(spike_windows_kernel<real, kMaxPR>);
+        KSM_LAUNCH_CHECK("spike_windows_kernel");
+        damp_kernel<real><<<(nsp + 255) / 256, 256, 0, s>>>(dot, order_k, nsp, sc,
+                                                             static_cast<real *>(damp));
+        KSM_LAUNCH_CHECK("damp_kernel");
     }
-    dw_reduce_kernel<real><<<(P * nt + 255) / 256, 256, 0, s>>>(
-        dWpart, K * nchunk, P * nt, (real)scale, static_cast<real *>(dW));
-    KSM_LAUNCH_CHECK("dw_reduce_kernel");
+    du_reduce_kernel<real><<<dim3((P * C + 255) / 256, K), 256, 0, s>>>(
+        F, a, order_k, seg, P, C, sc, static_cast<real *>(dU));
+    KSM_LAUNCH_CHECK("du_reduce_kernel");
+    dw_slices_kernel<real><<<kSlices, 256, 0, s>>>(H, a, order_k, nsp, P * nt, part);
+    KSM_LAUNCH_CHECK("dw_slices_kernel");
+    dw_final_kernel<real><<<(P * nt + 255) / 256, 256, 0, s>>>(part, P * nt, sc,
+                                                                static_cast<real *>(dW));
+    KSM_LAUNCH_CHECK("dw_final_kernel");
     return KSM_OK;
 }
 
@@ -322,11 +376,14 @@ extern "C" {
 size_t ksm_residual_workspace(int nsp, int K, int P, int C, int T, int nt, int dtype) {
     if (bad_sizes(nsp, K, P, C, T, nt) || (dtype != KSM_DTYPE_F32 && dtype != KSM_DTYPE_F64))
         return 0;
-    const size_t blocks = (size_t)((T + kResT - 1) / kResT) * ((C + kResC - 1) / kResC);
-    const size_t nchunk = (C + kBwdC - 1) / kBwdC;
-    const size_t fwd = ksm::align_up(blocks * sizeof(double), 256);
-    const size_t bwd = ksm::align_up((size_t)K * nchunk * P * nt * elem_size(dtype), 256) +
-                       ksm::align_up((size_t)(nsp > 0 ? nsp : 1) * nchunk * elem_size(dtype), 256);
+    const size_t es = elem_size(dtype);
+    const size_t tiles = n_tiles(T);
+    const size_t blocks = tiles * ((C + kResC - 1) / kResC);
+    const size_t n1 = nsp > 0 ? nsp : 1;
+    const size_t fwd = ksm::align_up(blocks * sizeof(double), 256) +
+                       ksm::align_up(2 * tiles * sizeof(int), 256);
+    const size_t bwd = ksm::align_up(n1 * P * C * es, 256) + ksm::align_up(n1 * P * nt * es, 256) +
+                       ksm::align_up(n1 * es, 256) + ksm::align_up((size_t)kSlices * P * nt * es, 256);
     return (fwd > bwd ? fwd : bwd) + 256;
 }
 
@@ -342,9 +399,9 @@ int ksm_residual(const void *X, int ldx, const void *U, const void *W, const int
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_residual: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
     if (dtype == KSM_DTYPE_F64)
-        return residual_impl<double>(X, ldx, U, W, st, amps, order, nsp, P, C, T, nt, R, ldr,
+        return residual_impl<double>(X, ldx, U, W, st, amps, order, nsp, K, P, C, T, nt, R, ldr,
                                      loss, workspace, s);
-    return residual_impl<float>(X, ldx, U, W, st, amps, order, nsp, P, C, T, nt, R, ldr, loss,
+    return residual_impl<float>(X, ldx, U, W, st, amps, order, nsp, K, P, C, T, nt, R, ldr, loss,
                                 workspace, s);
 }
 
