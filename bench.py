@@ -193,6 +193,61 @@ def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
                     "the residual)"}
 
 
+def ragged_workload(n_rec, max_batches, seed, device):
+    """Recordings of uneven size: channels from {64, 128, 192, 256, 384},
+    templates uniform in [50, 1000], 1 .. max_batches batches of NT, the same
+    generator as the main workload with a seed per recording."""
+    from kilosort_amd import synth
+    from kilosort_amd.batches import ArrayBatches
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for r in range(n_rec):
+        Cr = int(rng.choice([64, 128, 192, 256, 384]))
+        Kr = int(rng.integers(50, 1001))
+        nb = int(rng.integers(1, max_batches + 1))
+        rec = synth.make_recording(seed * 1000 + r, Cr, Kr, nb * NT, rate_range=(1.0, 20.0),
+                                   amp_range=(8.0, 40.0), overlap_frac=0.2, with_data=False)
+        data = synth.synthesize_torch(rec.n_samples, Cr, rec.times, rec.ids, rec.amps, rec.U,
+                                      rec.wPCA, rec.nt0min, device, seed=seed * 1000 + r)
+        bf = ArrayBatches(data, NT=NT, nt=NTAP, device=device)
+        ops = dict(nt=NTAP, nt0min=rec.nt0min, Th_learned=8, max_peels=100, batch_size=NT,
+                   wPCA=torch.from_numpy(rec.wPCA).to(device), xc=rec.xc, yc=rec.yc,
+                   settings=dict(nearest_chans=10, position_limit=100.0, n_pcs=P))
+        jobs.append((ops, bf, torch.from_numpy(rec.U).to(device)))
+    return jobs
+
+
+def time_ragged(tm, jobs, device):
+    """The recordings one after another (extract, one stream each: the
+    per-recording sequential loop) against all of them through extract_many
+    with the stream count the autotuner picks; both include each recording's
+    preparation (prepare_extract + prepare_matching)."""
+    from kilosort_amd import autotune
+    work = sum(int(U.shape[2]) * NT * int(bf.n_batches) for _, bf, U in jobs)
+
+    def seq():
+        for ops, bf, U in jobs:
+            tm.extract(dict(ops, matching_streams=1), bf, U, device=device)
+
+    seq()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    best, tried = autotune.tune_streams(jobs, device=device)
+    t0 = time.perf_counter()
+    tm.extract_many([(dict(o), b, U) for o, b, U in jobs], device=device, streams=best)
+    torch.cuda.synchronize()
+    t_many = time.perf_counter() - t0
+    return {"recordings": len(jobs),
+            "shapes": [[int(U.shape[2]), int(U.shape[0]), int(bf.n_batches)] for _, bf, U in jobs],
+            "sequential_value": work / t_seq, "sequential_s": t_seq,
+            "batched_value": work / t_many, "batched_s": t_many, "batched_streams": best,
+            "autotune_s": {str(k): v for k, v in tried.items()},
+            "unit": "channel·samples/s", "precision": "fp32"}
+
+
 PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_update",
             "features"]
 
@@ -266,6 +321,8 @@ def main():
     ap.add_argument("--forms", default=",".join(FORMS),
                     help="comma list of forms to time beside `value` ('' for none)")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--ragged", type=int, default=8,
+                    help="recordings of the ragged (many uneven recordings) run; 0 skips it")
     ap.add_argument("--streams", type=int, default=None,
                     help="device streams of the batch pipeline (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true",
@@ -349,6 +406,16 @@ def main():
     log(f"refinement: residual {refine_res['residual_ms_per_batch']:.3f} ms, "
         f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch")
 
+    ragged = None
+    if args.ragged > 0:
+        log(f"ragged: {args.ragged} recordings, sequential extract vs extract_many")
+        jobs = ragged_workload(args.ragged, 2, args.seed + 100, device)
+        ragged = time_ragged(tm, jobs, device)
+        del jobs
+        log(f"ragged: sequential {ragged['sequential_value']:.3e}, batched "
+            f"{ragged['batched_value']:.3e} channel·samples/s on {ragged['batched_streams']} "
+            f"streams")
+
     cpu = None
     agree = None
     if not args.no_cpu_baseline:
@@ -396,6 +463,7 @@ def main():
                   for f in forms},
         "agreement_batch0": agree,
         "refinement": refine_res,
+        "ragged": ragged,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -110,7 +110,7 @@ _ws = _Workspace()
 class _Gram:
     """Peel-side form of one (templates, Gram) pair."""
     __slots__ = ("base", "ldl", "rows", "nrows", "perm", "rank", "lohi", "off", "cc",
-                 "ready")
+                 "ready", "src")
 
 
 class _GramCache:
@@ -119,18 +119,24 @@ class _GramCache:
     its nonzero-row lists, the peel order of the templates (sorted by peak
     channel) and the compact lag-major Gram over that order."""
 
+    MAX_ENTRIES = 16          # recordings interleaved by extract_many stay cached
+
     def __init__(self):
-        self.key = None
-        self.val = None
+        self.entries = {}         # key -> _Gram, in insertion (use) order
+        self.lock = threading.Lock()
 
     def get(self, ctc, U, nt, device):
         K, P, C = U.shape
         key = (ctc.data_ptr(), ctc._version, tuple(ctc.shape), tuple(ctc.stride()),
                U.data_ptr(), U._version, tuple(U.shape))
-        if self.key == key:
+        with self.lock:
+            g = self.entries.pop(key, None)
+            if g is not None:
+                self.entries[key] = g             # most recently used last
+        if g is not None:
             # built on another stream: order this stream after it
-            torch.cuda.current_stream(device).wait_event(self.val.ready)
-            return self.val
+            torch.cuda.current_stream(device).wait_event(g.ready)
+            return g
         L = _lags(nt)
         if tuple(ctc.shape) != (K, K, L):
             raise ValueError(f"ctc must be [{K}, {K}, {L}] (got {tuple(ctc.shape)})")
@@ -168,8 +174,13 @@ class _GramCache:
                                         g.cc.data_ptr(), stream), "ksm_gram_compact")
         g.ready = torch.cuda.Event()
         g.ready.record(torch.cuda.current_stream(device))
-        self.key = key
-        self.val = g
+        # the key holds addresses: keep the tensors alive while cached, so
+        # that no other tensor can take their memory and match the key
+        g.src = (ctc, U)
+        with self.lock:
+            self.entries[key] = g
+            while len(self.entries) > self.MAX_ENTRIES:
+                self.entries.pop(next(iter(self.entries)))
         return g
 
 
@@ -324,14 +335,47 @@ def _features(ops, Xres, st, amps, iCC, iU, Ucc, device):
     return tF
 
 
+class _Ctx:
+    """What one recording's batches are matched with."""
+    __slots__ = ("ops", "U", "ctc", "iCC", "iU", "Ucc")
+
+    def __init__(self, ops, U, ctc, iCC, iU, Ucc):
+        self.ops, self.U, self.ctc = ops, U, ctc
+        self.iCC, self.iU, self.Ucc = iCC, iU, Ucc
+
+
+# idle lanes (host thread + device stream + pinned buffers) per device, reused
+# by later pipelines: a new host thread pays for its own HIP set-up, pinned
+# progress words and pinned buffers, milliseconds per pipeline
+_idle_lanes = {}
+_idle_lock = threading.Lock()
+
+
+def _lane_take(device):
+    with _idle_lock:
+        free = _idle_lanes.get(str(device))
+        if free:
+            return free.pop()
+    return _Lane(device)
+
+
+def _lane_give(lane):
+    with _idle_lock:
+        free = _idle_lanes.setdefault(str(lane.device), [])
+        if len(free) < 16:
+            free.append(lane)
+            return
+    lane.jobs.put(None)
+
+
 class _Lane:
     """One device stream of a _BatchPipeline and the host thread that drives
     it: run_matching + features per job, then the host copies of (st,
     th_amps, tF) into one of two pinned buffer sets."""
 
-    def __init__(self, pipe):
-        self.pipe = pipe
-        self.stream = torch.cuda.Stream(device=pipe.device)
+    def __init__(self, device):
+        self.device = device
+        self.stream = torch.cuda.Stream(device=device)
         self.jobs = queue.Queue()
         self.results = queue.Queue()
         self.bufs = [None, None]
@@ -340,8 +384,8 @@ class _Lane:
 
     def _host(self, slot, k, nC, P):
         b = self.bufs[slot]
-        if b is None or b[0].shape[0] < k:
-            cap = max(2 * k, 4096)
+        if b is None or b[0].shape[0] < k or tuple(b[2].shape[1:]) != (nC, P):
+            cap = max(2 * k, 1 << 15)
             b = (torch.empty((cap, 2), dtype=torch.int64, pin_memory=True),
                  torch.empty((cap, 1), dtype=torch.float32, pin_memory=True),
                  torch.empty((cap, nC, P), dtype=torch.float32, pin_memory=True))
@@ -349,20 +393,20 @@ class _Lane:
         return b
 
     def _run(self):
-        p = self.pipe
-        with torch.cuda.device(p.device), torch.cuda.stream(self.stream):
+        dev = self.device
+        with torch.cuda.device(dev), torch.cuda.stream(self.stream):
             while True:
                 job = self.jobs.get()
                 if job is None:
                     return
-                slot, X, tag, ready = job
+                slot, X, tag, ready, c = job
                 try:
                     self.stream.wait_event(ready)
-                    stt, amps, th, Xres = run_matching(p.ops, X, p.U, p.ctc, device=p.device)
+                    stt, amps, th, Xres = run_matching(c.ops, X, c.U, c.ctc, device=dev)
                     iters = _iters.n
-                    xfeat = _features(p.ops, Xres, stt, amps, p.iCC, p.iU, p.Ucc, p.device)
+                    xfeat = _features(c.ops, Xres, stt, amps, c.iCC, c.iU, c.Ucc, dev)
                     k = stt.shape[0]
-                    nC, _, P = p.Ucc.shape
+                    nC, _, P = c.Ucc.shape
                     hb = self._host(slot, k, nC, P)
                     if k:
                         hb[0][:k].copy_(stt, non_blocking=True)
@@ -385,29 +429,35 @@ class _BatchPipeline:
     runs on lane i mod lanes; each lane copies its results to pinned host
     buffers, two sets per lane.
 
-    ``submit(X, tag)`` queues batch X (a device tensor made on the caller's
-    current stream) and returns the batches finished so far that the lanes
-    need back, in submission order, as a list of ``(tag, st int64 [k, 2],
-    th_amps float32 [k, 1], tF float32 [k, nC, P])`` -- views into pinned
-    buffers that stay valid until the next ``submit`` or ``flush`` returns;
-    ``flush()`` returns all remaining batches and ``close()`` ends the
-    threads.  ``peel_iterations`` sums the peel iterations of the batches
-    returned.
+    ``submit(X, tag, ctx)`` queues batch X (a device tensor made on the
+    caller's current stream) of the recording ``ctx`` (default: the one the
+    pipeline was made with) and returns the batches finished so far that the
+    lanes need back, in submission order, as a list of ``(tag, st int64
+    [k, 2], th_amps float32 [k, 1], tF float32 [k, nC, P])`` -- views into
+    pinned buffers that stay valid until the next ``submit`` or ``flush``
+    returns; ``flush()`` returns all remaining batches and ``close()`` hands
+    the lanes (threads, streams, pinned buffers) back to a per-device pool for
+    later pipelines.  ``peel_iterations`` sums the peel iterations of the
+    batches returned.
     """
 
     def __init__(self, ops, U, ctc, iCC, iU, Ucc, device, lanes=None):
-        self.ops, self.U, self.ctc = ops, U, ctc
-        self.iCC, self.iU, self.Ucc = iCC, iU, Ucc
+        self.ctx = _Ctx(ops, U, ctc, iCC, iU, Ucc) if ops is not None else None
         self.device = device
-        # the derived Gram once, on the caller's stream (lanes wait for it)
-        _grams.get(ctc, _check_U(U, device), int(ops['nt']), device)
+        if self.ctx is not None:
+            self.prepare(self.ctx)
         n = MATCHING_STREAMS if lanes is None else int(lanes)
         if n < 1:
             raise ValueError("lanes must be >= 1")
-        self.lanes = [_Lane(self) for _ in range(n)]
+        self.lanes = [_lane_take(device) for _ in range(n)]
         self.n_sub = 0
         self.n_done = 0
         self.peel_iterations = 0
+
+    def prepare(self, ctx):
+        """Builds the recording's derived Gram on the caller's stream (lanes
+        wait for it) before its batches are submitted."""
+        _grams.get(ctx.ctc, _check_U(ctx.U, self.device), int(ctx.ops['nt']), self.device)
 
     def _take(self):
         lane = self.lanes[self.n_done % len(self.lanes)]
@@ -419,7 +469,8 @@ class _BatchPipeline:
         self.peel_iterations += iters
         return tag, st, th, tf
 
-    def submit(self, X, tag=None):
+    def submit(self, X, tag=None, ctx=None):
+        ctx = self.ctx if ctx is None else ctx
         n = len(self.lanes)
         i = self.n_sub
         out = []
@@ -429,7 +480,7 @@ class _BatchPipeline:
             out.append(self._take())
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.device))
-        self.lanes[i % n].jobs.put(((i // n) % 2, X, tag, ready))
+        self.lanes[i % n].jobs.put(((i // n) % 2, X, tag, ready, ctx))
         self.n_sub += 1
         return out
 
@@ -440,10 +491,14 @@ class _BatchPipeline:
         return out
 
     def close(self):
+        """Waits for the batches still in flight (results dropped) and hands
+        the lanes back for reuse."""
+        while self.n_done < self.n_sub:
+            lane = self.lanes[self.n_done % len(self.lanes)]
+            lane.results.get()
+            self.n_done += 1
         for lane in self.lanes:
-            lane.jobs.put(None)
-        for lane in self.lanes:
-            lane.thread.join()
+            _lane_give(lane)
         self.lanes = []
 
     def __del__(self):
@@ -462,6 +517,69 @@ def _extract_batch(ops, X, U, ctc, iCC, iU, Ucc, device):
     return stt.cpu(), th_amps.cpu(), xfeat.cpu()
 
 
+class _Collector:
+    """A recording's outputs as ``extract`` assembles them (reference
+    ``template_matching.py:88-124``): batch-0 spikes before the recording
+    start dropped, float64 times shifted by the batch offset, and a final sort
+    by time with numpy's default argsort over the rows in batch order.  Rows
+    are kept per batch and joined at the end instead of written into the
+    reference's preallocated 10^6-row buffers (whose zero-filled pages cost
+    more than a small recording's matching); the returned arrays are the
+    same."""
+
+    def __init__(self, ops, bfile, nC, P, progress_bar=None):
+        self.ops, self.bfile = ops, bfile
+        self.nC, self.P = nC, P
+        self.st_parts, self.tf_parts = [], []
+        self.n_batches = int(bfile.n_batches)
+        self.progress_bar = progress_bar
+        self.last = None                  # host st of the last batch (error reports)
+
+    def add(self, jb, stt_h, th_h, xfeat_h):
+        ops = self.ops
+        nt = ops['nt']
+        self.last = stt_h
+        x = stt_h[:, 0].numpy()
+        if jb == 0:
+            # times before the recording start (trough alignment) are dropped
+            keep = (x - nt - nt // 2 + ops['nt0min']) >= 0
+            stt_h, th_h, xfeat_h = stt_h[keep], th_h[keep], xfeat_h[keep]
+        nsp = len(stt_h)
+        t_shift = jb * self.bfile.batch_downsampling * (ops['batch_size'])
+        sd = stt_h.numpy().astype(np.float64)
+        st = np.empty((nsp, 3), np.float64)
+        st[:, 0] = ((sd[:, 0] - nt) + t_shift) - nt // 2 + ops['nt0min']
+        st[:, 1] = sd[:, 1]
+        st[:, 2] = th_h.numpy().astype(np.float64).reshape(-1)
+        self.st_parts.append(st)
+        self.tf_parts.append(xfeat_h.clone())       # the pinned views are reused
+        if self.progress_bar is not None:
+            self.progress_bar.emit(int((jb + 1) / self.n_batches * 100))
+
+    def result(self):
+        st = (np.concatenate(self.st_parts, 0) if self.st_parts
+              else np.zeros((0, 3), np.float64))
+        tF = (torch.cat(self.tf_parts, 0) if self.tf_parts
+              else torch.zeros((0, self.nC, self.P)))
+        isort = np.argsort(st[:, 0])
+        return st[isort], tF[isort]
+
+
+def _prepare_recording(ops, U, device):
+    """prepare_extract + prepare_matching of one recording; writes
+    ops['iCC'], ops['iCC_mask'], ops['iU'] (reference :62-67)."""
+    nC = ops['settings']['nearest_chans']
+    position_limit = ops['settings']['position_limit']
+    U = _check_U(U, device)
+    iCC, iCC_mask, iU, Ucc = prepare_extract(ops['xc'], ops['yc'], U, nC, position_limit,
+                                             device=device)
+    ops['iCC'] = iCC
+    ops['iCC_mask'] = iCC_mask
+    ops['iU'] = iU
+    ctc = prepare_matching(ops, U)
+    return _Ctx(ops, U, ctc, iCC, iU, Ucc)
+
+
 def extract(ops, bfile, U, device=torch.device('cuda'), progress_bar=None):
     """Spike extraction over every batch of ``bfile`` (reference ``template_matching.py:56-126``).
 
@@ -471,63 +589,75 @@ def extract(ops, bfile, U, device=torch.device('cuda'), progress_bar=None):
     ``iU``.
     """
     device = _lib.require_gpu(device)
-    nC = ops['settings']['nearest_chans']
-    position_limit = ops['settings']['position_limit']
-    U = _check_U(U, device)
-    iCC, iCC_mask, iU, Ucc = prepare_extract(ops['xc'], ops['yc'], U, nC, position_limit,
-                                             device=device)
-    ops['iCC'] = iCC
-    ops['iCC_mask'] = iCC_mask
-    ops['iU'] = iU
-    nt = ops['nt']
-    ctc = prepare_matching(ops, U)
-    st = np.zeros((10 ** 6, 3), 'float64')
-    tF = torch.zeros((10 ** 6, iCC.shape[0], ops['settings']['n_pcs']))
-    k = 0
-    n_batches = int(bfile.n_batches)
-    ibatch, X, stt_h = -1, None, None
-    pipe = _BatchPipeline(ops, U, ctc, iCC, iU, Ucc, device,
-                          lanes=ops.get('matching_streams', MATCHING_STREAMS))
-
-    def consume(res):
-        nonlocal st, tF, k, stt_h
-        jb, stt_h, th_h, xfeat_h = res
-        if jb == 0:
-            # times before the recording start (trough alignment) are dropped
-            neg = (stt_h[:, 0] - nt - nt // 2 + ops['nt0min']) < 0
-            stt_h, th_h, xfeat_h = stt_h[~neg], th_h[~neg], xfeat_h[~neg]
-        nsp = len(stt_h)
-        if k + nsp > st.shape[0]:
-            st = np.concatenate((st, np.zeros_like(st)), 0)
-            tF = torch.cat((tF, torch.zeros_like(tF)), 0)
-        t_shift = jb * bfile.batch_downsampling * (ops['batch_size'])
-        sd = stt_h.double()
-        st[k:k + nsp, 0] = ((sd[:, 0] - nt) + t_shift).numpy() - nt // 2 + ops['nt0min']
-        st[k:k + nsp, 1] = sd[:, 1].numpy()
-        st[k:k + nsp, 2] = th_h.double().numpy().reshape(-1)
-        tF[k:k + nsp] = xfeat_h
-        k += nsp
-        if progress_bar is not None:
-            progress_bar.emit(int((jb + 1) / n_batches * 100))
-
+    ctx = _prepare_recording(ops, U, device)
+    col = _Collector(ops, bfile, ctx.iCC.shape[0], ops['settings']['n_pcs'], progress_bar)
+    ibatch, X = -1, None
+    pipe = _BatchPipeline(ctx.ops, ctx.U, ctx.ctc, ctx.iCC, ctx.iU, ctx.Ucc, device,
+                          lanes=_streams(ops))
     try:
-        for ibatch in range(n_batches):
+        for ibatch in range(col.n_batches):
             if ibatch % 100 == 0:
-                logger.debug("extract: batch %d of %d", ibatch, n_batches)
+                logger.debug("extract: batch %d of %d", ibatch, col.n_batches)
             X = bfile.padded_batch_to_torch(ibatch, ops)
             for res in pipe.submit(X, tag=ibatch):
-                consume(res)
+                col.add(*res)
             X = None
         for res in pipe.flush():
-            consume(res)
+            col.add(*res)
     except Exception:
         logger.exception(f'Error in template_matching.extract on batch {ibatch}')
         logger.debug(f'X shape: {None if X is None else tuple(X.shape)}')
-        logger.debug(f'stt shape: {None if stt_h is None else tuple(stt_h.shape)}')
+        logger.debug(f'stt shape: {None if col.last is None else tuple(col.last.shape)}')
         raise
     finally:
         pipe.close()
-    isort = np.argsort(st[:k, 0])
-    st = st[isort]
-    tF = tF[isort]
+    st, tF = col.result()
     return st, tF, ops
+
+
+def _streams(ops):
+    n = ops.get('matching_streams', MATCHING_STREAMS)
+    return MATCHING_STREAMS if n is None else int(n)
+
+
+def extract_many(jobs, device=torch.device('cuda'), streams=None):
+    """``extract`` over several recordings of any sizes at once.
+
+    ``jobs`` is a list of ``(ops, bfile, U)``; returns a list of ``(st, tF,
+    ops)``, each exactly what ``extract(ops, bfile, U)`` returns.  All
+    recordings' batches go through one batch pipeline (recording by
+    recording, batch by batch), so that small recordings -- whose batches
+    leave most of the GPU idle -- share the device streams with their
+    neighbours instead of running one after another.  ``streams`` defaults to
+    ``MATCHING_STREAMS``.
+    """
+    device = _lib.require_gpu(device)
+    cols = [_Collector(ops, bfile, ops['settings']['nearest_chans'], ops['settings']['n_pcs'])
+            for ops, bfile, _ in jobs]
+    pipe = _BatchPipeline(None, None, None, None, None, None, device,
+                          lanes=MATCHING_STREAMS if streams is None else streams)
+    r, ibatch = -1, -1
+    try:
+        for r, (ops, bfile, U) in enumerate(jobs):
+            # prepared when its turn comes: only the recordings in flight hold
+            # their Gram on the device
+            c = _prepare_recording(ops, U, device)
+            pipe.prepare(c)
+            for ibatch in range(cols[r].n_batches):
+                X = bfile.padded_batch_to_torch(ibatch, ops)
+                for tag, *res in pipe.submit(X, tag=(r, ibatch), ctx=c):
+                    cols[tag[0]].add(tag[1], *res)
+                X = None
+        for tag, *res in pipe.flush():
+            cols[tag[0]].add(tag[1], *res)
+    except Exception:
+        logger.exception(f'Error in template_matching.extract_many on recording {r}, '
+                         f'batch {ibatch}')
+        raise
+    finally:
+        pipe.close()
+    out = []
+    for (ops, _, _), col in zip(jobs, cols):
+        st, tF = col.result()
+        out.append((st, tF, ops))
+    return out

@@ -89,3 +89,27 @@ def test_extract_streams_identical(streams):
     assert len(st3) > 100
     np.testing.assert_array_equal(st, st3)
     np.testing.assert_array_equal(tF.numpy(), tF3.numpy())
+
+
+def test_extract_many_matches_extract():
+    """Recordings of different channel and template counts and lengths through
+    one pipeline give exactly what extract gives for each."""
+    say("extract_many: 3 recordings (C, K, batches) = (64, 40, 3), (32, 12, 1), (96, 70, 2)")
+    jobs, refs = [], []
+    for seed, C, K, nb in ((83, 64, 40, 3), (84, 32, 12, 1), (85, 96, 70, 2)):
+        # whole batches: a partial last batch is padded by repeating its last
+        # sample, and unfiltered data then gives long runs of tied scores
+        rec = synth.make_recording(seed, C, K, nb * NT, rate_range=(5.0, 30.0))
+        ops = dict(nt=61, nt0min=rec.nt0min, Th_learned=8, max_peels=100, batch_size=NT,
+                   wPCA=torch.from_numpy(rec.wPCA).to(DEV), xc=rec.xc, yc=rec.yc,
+                   settings=dict(nearest_chans=10, position_limit=100.0, n_pcs=6))
+        U = torch.from_numpy(rec.U).to(DEV)
+        bf = ArrayBatches(rec.data, NT=NT, nt=61, device=DEV)
+        refs.append(tm.extract(dict(ops), bf, U, device=DEV))
+        jobs.append((dict(ops), bf, U))
+    got = tm.extract_many(jobs, device=DEV)
+    for (st, tF, o), (st_r, tF_r, o_r) in zip(got, refs):
+        assert len(st_r) > 20
+        np.testing.assert_array_equal(st, st_r)
+        np.testing.assert_array_equal(tF.numpy(), tF_r.numpy())
+        np.testing.assert_array_equal(o["iCC"].cpu().numpy(), o_r["iCC"].cpu().numpy())
