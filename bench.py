@@ -229,18 +229,24 @@ def time_ragged(tm, jobs, device):
         for ops, bf, U in jobs:
             tm.extract(dict(ops, matching_streams=1), bf, U, device=device)
 
-    seq()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    seq()
-    torch.cuda.synchronize()
-    t_seq = time.perf_counter() - t0
-    best, tried = autotune.tune_streams(jobs, device=device)
-    t0 = time.perf_counter()
-    tm.extract_many([(dict(o), b, U) for o, b, U in jobs], device=device, streams=best)
-    torch.cuda.synchronize()
-    t_many = time.perf_counter() - t0
-    return {"recordings": len(jobs),
+    def many():
+        tm.extract_many([(dict(o), b, U) for o, b, U in jobs], device=device, streams=best)
+
+    def best_of(f, n=3):
+        f()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            f()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    t_seq = best_of(seq)
+    best, tried = autotune.tune_streams(jobs, device=device, reps=2)
+    t_many = best_of(many)
+    return {"recordings": len(jobs), "timing": "best of 3 runs after one untimed run",
             "shapes": [[int(U.shape[2]), int(U.shape[0]), int(bf.n_batches)] for _, bf, U in jobs],
             "sequential_value": work / t_seq, "sequential_s": t_seq,
             "batched_value": work / t_many, "batched_s": t_many, "batched_streams": best,
@@ -321,7 +327,7 @@ def main():
     ap.add_argument("--forms", default=",".join(FORMS),
                     help="comma list of forms to time beside `value` ('' for none)")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--ragged", type=int, default=8,
+    ap.add_argument("--ragged", type=int, default=16,
                     help="recordings of the ragged (many uneven recordings) run; 0 skips it")
     ap.add_argument("--streams", type=int, default=None,
                     help="device streams of the batch pipeline (default: the library's)")
@@ -409,7 +415,7 @@ def main():
     ragged = None
     if args.ragged > 0:
         log(f"ragged: {args.ragged} recordings, sequential extract vs extract_many")
-        jobs = ragged_workload(args.ragged, 2, args.seed + 100, device)
+        jobs = ragged_workload(args.ragged, 3, args.seed + 100, device)
         ragged = time_ragged(tm, jobs, device)
         del jobs
         log(f"ragged: sequential {ragged['sequential_value']:.3e}, batched "
