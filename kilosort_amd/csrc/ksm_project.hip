@@ -26,7 +26,7 @@ __host__ __device__ constexpr int skew(int e) { return e + (e >> 3); }
 template <int P, int NTAP>
 __global__ __launch_bounds__(kConvThreads) void conv_pc_kernel(
     const float *__restrict__ X, int ldx, const float *__restrict__ W, int C, int T,
-    int ldc, float *__restrict__ Bc) {
+    int ldc, float *__restrict__ Bc, int *__restrict__ nonfinite) {
     constexpr int H = NTAP / 2;
     constexpr int NWIN = kConvTT + NTAP - 1;
     constexpr int NLOAD = kConvSpan + NTAP - 1;
@@ -34,10 +34,15 @@ __global__ __launch_bounds__(kConvThreads) void conv_pc_kernel(
     const int c = blockIdx.y;
     const int t0 = blockIdx.x * kConvSpan;
     const float *xr = X + (size_t)c * ldx;
+    bool bad = false;
     for (int e = threadIdx.x; e < NLOAD; e += kConvThreads) {
         const int t = t0 - H + e;
-        xs[skew(e)] = (t >= 0 && t < T) ? xr[t] : 0.f;
+        const float v = (t >= 0 && t < T) ? xr[t] : 0.f;
+        bad |= !__builtin_isfinite(v);
+        xs[skew(e)] = v;
     }
+    if (__builtin_amdgcn_ballot_w64(bad) != 0ull && (threadIdx.x & 63) == 0)
+        atomicOr(nonfinite, 1);
     __syncthreads();
     const int base = threadIdx.x * kConvTT;
     float win[NWIN];
@@ -86,10 +91,12 @@ __global__ __launch_bounds__(kConvThreads) void conv_pc_kernel(
 __global__ __launch_bounds__(256) void conv_pc_generic(const float *__restrict__ X, int ldx,
                                                        const float *__restrict__ W, int P,
                                                        int nt, int C, int T, int ldc,
-                                                       float *__restrict__ Bc) {
+                                                       float *__restrict__ Bc,
+                                                       int *__restrict__ nonfinite) {
     const int c = blockIdx.y;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ldc) return;
+    if (t < T && !__builtin_isfinite(X[(size_t)c * ldx + t])) atomicOr(nonfinite, 1);
     const int H = nt / 2;
     const float *xr = X + (size_t)c * ldx;
     for (int p = 0; p < P; ++p) {
@@ -124,6 +131,56 @@ __global__ void zero_rows_kernel(float *__restrict__ p, size_t n) {
         p[e] = 0.f;
 }
 
+__global__ void set_flag_kernel(int *__restrict__ p) { *p = 0; }
+
+// The contraction steps (GBK rows of the reduction index r = p*C + c) that a
+// template tile needs: steps where the tile's 64 columns of At are not all
+// zero.  Templates are zero outside their channel footprint, and the tiles of
+// templates sorted by peak channel touch a band of channels, so most steps
+// are skipped; a skipped step would add only exact zeros.  With a non-finite
+// value in X, W or U every step is kept (0 * inf must give NaN as in the
+// dense product).  steps[tm][0 .. counts[tm]) in increasing order.
+constexpr int kStepsMax = 4096;        // reduction steps per tile handled sparsely
+__global__ __launch_bounds__(256) void tile_steps_kernel(
+    const float *__restrict__ At, int Kp, int nk, const float *__restrict__ W, int PW,
+    const int *__restrict__ nonfinite, int *__restrict__ steps, int *__restrict__ counts) {
+    __shared__ unsigned char s_need[kStepsMax];
+    __shared__ int s_dense;
+    const int tm = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_dense = *nonfinite != 0 || nk > kStepsMax;
+    __syncthreads();
+    bool wbad = false;
+    for (int e = tid; e < PW; e += 256) wbad |= !__builtin_isfinite(W[e]);
+    if (wbad) s_dense = 1;                 // benign race: any writer stores 1
+    __syncthreads();
+    const bool dense = s_dense != 0;
+    if (!dense) {
+        const float *a = At + (size_t)tm * 64 + lane;
+        for (int st = wave; st < nk; st += 4) {
+            bool nz = false;
+            for (int i = 0; i < 16; ++i) {
+                const float v = a[((size_t)st * 16 + i) * Kp];
+                nz |= v != 0.f;                    // NaN counts as nonzero
+            }
+            const bool any = __builtin_amdgcn_ballot_w64(nz) != 0ull;
+            if (lane == 0) s_need[st] = any;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int cnt = 0;
+        for (int c0 = 0; c0 < nk; c0 += 64) {
+            const int st = c0 + lane;
+            const bool f = st < nk && (dense || s_need[st]);
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(f);
+            const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (f) steps[(size_t)tm * nk + pos] = st;
+            cnt += __popcll(m);
+        }
+        if (lane == 0) counts[tm] = cnt;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // FP32 GEMM on v_mfma_f32_32x32x2f32: Bout[m][n] = sum_r At[r][m] * Bc[r][n].
 // Block tile 64 (templates) x 256 (time), depth step 16, 4 waves each owning
@@ -145,8 +202,9 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 }
 
 __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
-    const float *__restrict__ At, int Kp, const float *__restrict__ Bc, int ldc, int Rp,
-    float *__restrict__ Bout, int ldb, int K, int nmax, int nTm, int nTn) {
+    const float *__restrict__ At, int Kp, const float *__restrict__ Bc, int ldc, int nkt,
+    const int *__restrict__ steps, const int *__restrict__ counts, float *__restrict__ Bout,
+    int ldb, int K, int nmax, int nTm, int nTn) {
     __shared__ float As[2][GBK][GBM];
     __shared__ float Bs[2][GBK][GBN];
     const int logical = xcd_remap(blockIdx.x, nTm * nTn);
@@ -167,10 +225,15 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-    const int nk = Rp / GBK;
-    const float *ga = At + (size_t)la * Kp + m0 + lca;
-    const float *gb = Bc + (size_t)lb * ldc + n0 + lcb;
+    // the tile's reduction steps (rows GBK*step ..), in increasing order
+    const int nk = counts[tm];
+    const int *lst = steps + (size_t)tm * nkt;
+    const float *ga0 = At + (size_t)la * Kp + m0 + lca;
+    const float *gb0 = Bc + (size_t)lb * ldc + n0 + lcb;
     const size_t sb = (size_t)4 * ldc;
+    const int st0 = nk > 0 ? lst[0] : 0;
+    const float *ga = ga0 + (size_t)st0 * GBK * Kp;
+    const float *gb = gb0 + (size_t)st0 * GBK * ldc;
     float4 ra = *reinterpret_cast<const float4 *>(ga);
     float4 rb0 = *reinterpret_cast<const float4 *>(gb);
     float4 rb1 = *reinterpret_cast<const float4 *>(gb + sb);
@@ -187,8 +250,9 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
         const int buf = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            ga += (size_t)GBK * Kp;
-            gb += (size_t)GBK * ldc;
+            const int st = lst[kt + 1];
+            ga = ga0 + (size_t)st * GBK * Kp;
+            gb = gb0 + (size_t)st * GBK * ldc;
             ra = *reinterpret_cast<const float4 *>(ga);
             rb0 = *reinterpret_cast<const float4 *>(gb);
             rb1 = *reinterpret_cast<const float4 *>(gb + sb);
@@ -245,7 +309,7 @@ __global__ __launch_bounds__(256, 2) void gemm_f32_mfma(
 }
 
 struct ProjectLayout {
-    int R, Rp, Kp, ldc;
+    int R, Rp, Kp, ldc, nTm, nk;
     size_t bc_bytes, at_bytes;
 };
 
@@ -257,6 +321,8 @@ ProjectLayout project_layout(int C, int T, int K, int P) {
     L.ldc = (int)ksm::align_up(T, GBN);
     L.bc_bytes = (size_t)L.Rp * L.ldc * sizeof(float);
     L.at_bytes = (size_t)L.Rp * L.Kp * sizeof(float);
+    L.nTm = L.Kp / GBM;
+    L.nk = L.Rp / GBK;
     return L;
 }
 
@@ -270,7 +336,9 @@ size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) 
     if (precision == KSM_PREC_BF16 || precision == KSM_PREC_BF16X3)
         return ksm::project_bf16_workspace(C, T, K, P, precision);
     const ProjectLayout L = project_layout(C, T, K, P);
-    return ksm::align_up(L.bc_bytes, 256) + ksm::align_up(L.at_bytes, 256) + 512;
+    return ksm::align_up(L.bc_bytes, 256) + ksm::align_up(L.at_bytes, 256) +
+           ksm::align_up((size_t)L.nTm * L.nk * sizeof(int), 256) +
+           ksm::align_up((size_t)L.nTm * sizeof(int), 256) + 256 + 512;
 }
 
 int ksm_project(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
@@ -295,15 +363,22 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     ksm::Carver cv(workspace);
     float *Bc = cv.take<float>((size_t)L.Rp * L.ldc);
     float *At = cv.take<float>((size_t)L.Rp * L.Kp);
+    int *steps = cv.take<int>((size_t)L.nTm * L.nk);
+    int *counts = cv.take<int>(L.nTm);
+    int *nonfinite = cv.take<int>(1);
 
+    set_flag_kernel<<<1, 1, 0, s>>>(nonfinite);
+    KSM_LAUNCH_CHECK("set_flag_kernel");
     if (P == 6 && nt == 61) {
         ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
         dim3 grid((L.ldc + kConvSpan - 1) / kConvSpan, C);
-        prof.launch(conv_pc_kernel<6, 61>, grid, dim3(kConvThreads), 0u, X, ldx, W, C, T, L.ldc, Bc);
+        prof.launch(conv_pc_kernel<6, 61>, grid, dim3(kConvThreads), 0u, X, ldx, W, C, T, L.ldc, Bc,
+                    nonfinite);
     } else {
         ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
         dim3 grid((L.ldc + 255) / 256, C);
-        prof.launch(conv_pc_generic, grid, dim3(256), 0u, X, ldx, W, P, nt, C, T, L.ldc, Bc);
+        prof.launch(conv_pc_generic, grid, dim3(256), 0u, X, ldx, W, P, nt, C, T, L.ldc, Bc,
+                    nonfinite);
     }
     KSM_LAUNCH_CHECK("conv_pc_kernel");
     if (L.Rp > L.R) {
@@ -313,10 +388,12 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     }
     transpose_pad_kernel<<<512, 256, 0, s>>>(U, perm, K, L.R, L.Kp, L.Rp, At);
     KSM_LAUNCH_CHECK("transpose_pad_kernel");
-    const int nTm = L.Kp / GBM, nTn = L.ldc / GBN;
+    tile_steps_kernel<<<L.nTm, 256, 0, s>>>(At, L.Kp, L.nk, W, P * nt, nonfinite, steps, counts);
+    KSM_LAUNCH_CHECK("tile_steps_kernel");
+    const int nTm = L.nTm, nTn = L.ldc / GBN;
     ksm::ProfileScope prof("gemm_f32", ksm::kProfProject, s);
-    prof.launch(gemm_f32_mfma, dim3(nTm * nTn), dim3(256), 0u, At, L.Kp, Bc, L.ldc, L.Rp, Bt,
-                ldbt, K, (int)ksm::align_up(T, 128), nTm, nTn);
+    prof.launch(gemm_f32_mfma, dim3(nTm * nTn), dim3(256), 0u, At, L.Kp, Bc, L.ldc, L.nk, steps,
+                counts, Bt, ldbt, K, (int)ksm::align_up(T, 128), nTm, nTn);
     KSM_LAUNCH_CHECK("gemm_f32_mfma");
     return KSM_OK;
 }

@@ -41,18 +41,23 @@ constexpr int kCvT = 4 * kCvTT;  // 32 time samples per block (4 waves)
 template <int P, int NTAP, bool SPLIT>
 __global__ __launch_bounds__(256) void conv_pc_bf16_kernel(
     const float *__restrict__ X, int ldx, const float *__restrict__ W, int C, int T, int Tp,
-    int ldr, bf16_t *__restrict__ Bh, bf16_t *__restrict__ Bl) {
+    int ldr, bf16_t *__restrict__ Bh, bf16_t *__restrict__ Bl, int *__restrict__ nonfinite) {
     constexpr int H = NTAP / 2;
     constexpr int SPAN = kCvT + NTAP - 1;
     constexpr int LD = SPAN | 1;                 // odd row stride: no bank conflicts
     __shared__ float xs[kCvCh * LD];
     const int t0 = blockIdx.x * kCvT;
     const int c0 = blockIdx.y * kCvCh;
+    bool bad = false;
     for (int e = threadIdx.x; e < kCvCh * SPAN; e += 256) {
         const int cl = e / SPAN, j = e % SPAN;
         const int c = c0 + cl, t = t0 - H + j;
-        xs[cl * LD + j] = (c < C && t >= 0 && t < T) ? X[(size_t)c * ldx + t] : 0.f;
+        const float v = (c < C && t >= 0 && t < T) ? X[(size_t)c * ldx + t] : 0.f;
+        bad |= !__builtin_isfinite(v);
+        xs[cl * LD + j] = v;
     }
+    if (__builtin_amdgcn_ballot_w64(bad) != 0ull && (threadIdx.x & 63) == 0)
+        atomicOr(nonfinite, 1);
     __syncthreads();
     const int cl = threadIdx.x & 63, tg = threadIdx.x >> 6;
     const float *xr = xs + cl * LD + tg * kCvTT;
@@ -147,14 +152,69 @@ struct Segs {
     const bf16_t *b[3];
 };
 
+__global__ void set_flag_h_kernel(int *__restrict__ p) { *p = 0; }
+
+// The depth steps (HBK columns of r = c*P + p) where a tile of HBM_ template
+// rows of Uh is not all zero, in increasing order: steps[tm][0 .. counts[tm]).
+// A skipped step would add only exact zeros (templates are zero outside their
+// channel footprint); with a non-finite value in X, W or U every step is kept
+// (0 * inf must give NaN as in the dense product).  Ul is zero wherever Uh is.
+constexpr int kHStepsMax = 4096;
+__global__ __launch_bounds__(256) void tile_steps_bf16_kernel(
+    const bf16_t *__restrict__ Uh, int ldr, int nk, const float *__restrict__ W, int PW,
+    const int *__restrict__ nonfinite, int *__restrict__ steps, int *__restrict__ counts) {
+    __shared__ unsigned char s_need[kHStepsMax];
+    __shared__ int s_dense;
+    const int tm = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) s_dense = *nonfinite != 0 || nk > kHStepsMax;
+    __syncthreads();
+    bool wbad = false;
+    for (int e = tid; e < PW; e += 256) wbad |= !__builtin_isfinite(W[e]);
+    if (wbad) s_dense = 1;                 // benign race: any writer stores 1
+    __syncthreads();
+    const bool dense = s_dense != 0;
+    if (!dense) {
+        // lane: rows lane / 8 + 8 i of the tile, 16-byte chunk lane % 8 of the step
+        const int rsub = lane >> 3, chunk = lane & 7;
+        for (int st = wave; st < nk; st += 4) {
+            bool nz = false;
+            for (int i = 0; i < HBM_ / 8; ++i) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(
+                    Uh + (size_t)(tm * HBM_ + rsub + 8 * i) * ldr + st * HBK + chunk * 8);
+                // bf16 +0 and -0 are zero; any other pattern (NaN included) is not
+                const unsigned m = 0x7fff7fffu;
+                nz |= ((v.x & m) | (v.y & m) | (v.z & m) | (v.w & m)) != 0u;
+            }
+            const bool any = __builtin_amdgcn_ballot_w64(nz) != 0ull;
+            if (lane == 0) s_need[st] = any;
+        }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int cnt = 0;
+        for (int c0 = 0; c0 < nk; c0 += 64) {
+            const int st = c0 + lane;
+            const bool f = st < nk && (dense || s_need[st]);
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(f);
+            const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (f) steps[(size_t)tm * nk + pos] = st;
+            cnt += __popcll(m);
+        }
+        if (lane == 0) counts[tm] = cnt;
+    }
+}
+
 __device__ __forceinline__ int xcd_remap_h(int bid, int nb) {
     const int xcd = bid & 7, pos = bid >> 3;
     const int q = nb >> 3, r = nb & 7;
     return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
 }
 
-__global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int ldr, float *__restrict__ Bout,
-                                                      int ldb, int K, int nTm, int nTn) {
+__global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int ldr,
+                                                      const int *__restrict__ steps,
+                                                      const int *__restrict__ counts,
+                                                      float *__restrict__ Bout, int ldb, int K,
+                                                      int nTm, int nTn) {
     __shared__ bf16_t As[2][HBM_ * HLD];
     __shared__ bf16_t Bs[2][HBN * HLD];
     const int logical = xcd_remap_h(blockIdx.x, nTm * nTn);
@@ -173,11 +233,14 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-    const int nk_seg = ldr / HBK;
+    const int nk_all = ldr / HBK;
+    const int nk_seg = counts[tm];              // the tile's steps in each segment
+    const int *lst = steps + (size_t)tm * nk_all;
     const int nk = nk_seg * nseg;
-    // k step kt reads segment kt / nk_seg at depth (kt % nk_seg) * HBK
-    const bf16_t *ap = sg.a[0] + (size_t)(m0 + grow) * ldr + gcol;
-    const bf16_t *bp = sg.b[0] + (size_t)(n0 + grow) * ldr + gcol;
+    // k step kt reads segment kt / nk_seg at depth lst[kt % nk_seg] * HBK
+    const int k00 = nk_seg > 0 ? lst[0] * HBK : 0;
+    const bf16_t *ap = sg.a[0] + (size_t)(m0 + grow) * ldr + k00 + gcol;
+    const bf16_t *bp = sg.b[0] + (size_t)(n0 + grow) * ldr + k00 + gcol;
     uint4 ra0 = reinterpret_cast<const uint4 *>(ap)[0], ra1 = reinterpret_cast<const uint4 *>(ap)[1];
     uint4 ra2 = reinterpret_cast<const uint4 *>(ap)[2], ra3 = reinterpret_cast<const uint4 *>(ap)[3];
     uint4 rb0 = reinterpret_cast<const uint4 *>(bp)[0], rb1 = reinterpret_cast<const uint4 *>(bp)[1];
@@ -194,7 +257,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
         const int buf = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            const int s = (kt + 1) / nk_seg, k0 = ((kt + 1) % nk_seg) * HBK;
+            const int s = (kt + 1) / nk_seg, k0 = lst[(kt + 1) % nk_seg] * HBK;
             const bf16_t *sa = s == 0 ? sg.a[0] : (s == 1 ? sg.a[1] : sg.a[2]);
             const bf16_t *sb = s == 0 ? sg.b[0] : (s == 1 ? sg.b[1] : sg.b[2]);
             const uint4 *pa = reinterpret_cast<const uint4 *>(sa + (size_t)(m0 + grow) * ldr + k0 + gcol);
@@ -276,7 +339,9 @@ namespace ksm {
 size_t project_bf16_workspace(int C, int T, int K, int P, int precision) {
     const HLayout L = h_layout(C, T, K, P);
     const int terms = precision == KSM_PREC_BF16X3 ? 2 : 1;
-    return terms * (align_up(L.bc_elems * 2, 256) + align_up(L.u_elems * 2, 256)) + 512;
+    const size_t nTm = L.Kp / HBM_, nk = L.ldr / HBK;
+    return terms * (align_up(L.bc_elems * 2, 256) + align_up(L.u_elems * 2, 256)) +
+           align_up(nTm * nk * sizeof(int), 256) + align_up(nTm * sizeof(int), 256) + 256 + 512;
 }
 
 int project_bf16(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
@@ -291,19 +356,28 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     bf16_t *Uh = cv.take<bf16_t>(L.u_elems);
     bf16_t *Bl = split ? cv.take<bf16_t>(L.bc_elems) : nullptr;
     bf16_t *Ul = split ? cv.take<bf16_t>(L.u_elems) : nullptr;
+    const int nTm = L.Kp / HBM_, nTn = L.Tp / HBN, nk = L.ldr / HBK;
+    int *steps = cv.take<int>((size_t)nTm * nk);
+    int *counts = cv.take<int>(nTm);
+    int *nonfinite = cv.take<int>(1);
+    set_flag_h_kernel<<<1, 1, 0, s>>>(nonfinite);
+    KSM_LAUNCH_CHECK("set_flag_h_kernel");
     {
         ProfileScope prof("conv_pc", kProfProject, s);
         dim3 grid(L.Tp / kCvT, (L.ldr / P + kCvCh - 1) / kCvCh);
         if (split)
             prof.launch(conv_pc_bf16_kernel<6, 61, true>, grid, dim3(256), 0u, X, ldx, W, C, T,
-                        L.Tp, L.ldr, Bh, Bl);
+                        L.Tp, L.ldr, Bh, Bl, nonfinite);
         else
             prof.launch(conv_pc_bf16_kernel<6, 61, false>, grid, dim3(256), 0u, X, ldx, W, C, T,
-                        L.Tp, L.ldr, Bh, Bl);
+                        L.Tp, L.ldr, Bh, Bl, nonfinite);
     }
     KSM_LAUNCH_CHECK("conv_pc_bf16_kernel");
     u_bf16_kernel<<<512, 256, 0, s>>>(U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
     KSM_LAUNCH_CHECK("u_bf16_kernel");
+    tile_steps_bf16_kernel<<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite, steps,
+                                               counts);
+    KSM_LAUNCH_CHECK("tile_steps_bf16_kernel");
     Segs sg;
     sg.a[0] = Uh;
     sg.b[0] = Bh;
@@ -311,11 +385,10 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     sg.b[1] = split ? Bl : Bh;
     sg.a[2] = split ? Ul : Uh;
     sg.b[2] = Bh;
-    const int nTm = L.Kp / HBM_, nTn = L.Tp / HBN;
     {
         ProfileScope prof("gemm_bf16", kProfProject, s);
-        prof.launch(gemm_bf16_mfma, dim3(nTm * nTn), dim3(256), 0u, sg, split ? 3 : 1, L.ldr, Bt,
-                    ldbt, K, nTm, nTn);
+        prof.launch(gemm_bf16_mfma, dim3(nTm * nTn), dim3(256), 0u, sg, split ? 3 : 1, L.ldr,
+                    steps, counts, Bt, ldbt, K, nTm, nTn);
     }
     KSM_LAUNCH_CHECK("gemm_bf16_mfma");
     return KSM_OK;
