@@ -173,6 +173,52 @@ def test_projection_matches_float64():
     assert np.abs(got - ref).max() <= 1e-6 * np.abs(ref).max()
 
 
+def _project_gpu(X, U, W, precision):
+    lib = _lib.load()
+    C, T = X.shape
+    K = U.shape[0]
+    ldbt = (K + 63) // 64 * 64
+    Bt = torch.empty(((T + 127) // 128 * 128, ldbt), device=DEV)
+    nb = lib.ksm_project_workspace(C, T, K, 6, 61, precision)
+    ws = torch.empty(nb, dtype=torch.uint8, device=DEV)
+    Xd, Ud, Wd = (t.to(DEV).contiguous() for t in (X, U, W))
+    _lib.check(lib.ksm_project(Xd.data_ptr(), T, Wd.data_ptr(), Ud.data_ptr(), None, C, T, K, 6,
+                               61, Bt.data_ptr(), ldbt, precision, ws.data_ptr(), nb,
+                               _lib.stream_of(DEV)), "ksm_project")
+    return Bt[:T, :K].T.cpu().double().numpy()
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_projection_sparse_templates_and_nonfinite_input(precision):
+    """Templates zero outside a band of channels (the GEMMs skip their zero
+    reduction steps): scores match the dense float64 product; with an inf in X
+    the skipped products are taken again, so that every template reads NaN
+    where the dense product does (0 * inf)."""
+    say(f"projection sparse templates, precision {precision}")
+    rec = synth.make_recording(44, 192, 256, 6000 + 122, rate_range=(5.0, 20.0))
+    U = torch.from_numpy(rec.U).clone()
+    # footprints of 24 channels around each template's peak channel
+    pk = (U ** 2).sum(1).argmax(1)
+    ch = torch.arange(U.shape[2])
+    U[(ch[None, :] - pk[:, None]).abs()[:, None, :].expand_as(U) > 12] = 0
+    W = torch.from_numpy(rec.wPCA)
+    X = ArrayBatches(rec.data, NT=6000, nt=61, device=CPU).padded_batch_to_torch(0)
+    ref = O.project(dict(nt=61, wPCA=W.double()), X.double(), U.double()).numpy()
+    got = _project_gpu(X, U, W, precision)
+    tol = {0: 1e-6, 1: 2e-2, 2: 1e-4}[precision]
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+    X2 = X.clone()
+    X2[7, 3000] = float("inf")
+    ref2 = O.project(dict(nt=61, wPCA=W), X2, U).numpy()      # float32, as the reference
+    got2 = _project_gpu(X2, U, W, precision)
+    np.testing.assert_array_equal(np.isfinite(got2), np.isfinite(ref2))
+    if precision == 0:
+        np.testing.assert_array_equal(np.isnan(got2), np.isnan(ref2))
+    assert np.isnan(ref2[:, 3000]).all()
+    fin = np.isfinite(ref2)
+    assert np.abs(got2[fin] - ref2[fin]).max() <= tol * np.abs(ref2[fin]).max()
+
+
 def test_run_matching_matches_reference_golden(golden_dir):
     say("run_matching golden C=16 K=8")
     g = np.load(os.path.join(golden_dir, "run_matching_c16_k8.npz"))
