@@ -9,13 +9,13 @@ A step = one batch through the hot path exactly as ``extract`` runs it:
 run_matching (temporal-PC correlation, projection, peel) + PC features + the
 device-to-host copies of st/th_amps/tF.  Prints one JSON line.
 
-Every form (fp32: the parity form; bf16x3, bf16: projection on BF16 MFMA) is
-timed the same way; `value` is the --precision form.  A separate pass with
-events around each kernel gives per-kernel times for the roofline objects, and
-batch 0 is matched by the CPU oracle in FP32 (the timed CPU baseline) and
+Every form (fp32: the parity form; bf16x3, bf16: projection on BF16 MFMA;
+fp8: projection on FP8 e4m3 MFMA) is timed the same way; `value` is the
+--precision form.  A separate pass with events around each kernel gives
+per-kernel times for the roofline objects, and batch 0 is matched by the CPU oracle in FP32 (the timed CPU baseline) and
 FP64 (spike agreement and amplitude error of every form).
 
-    python bench.py [--steps K] [--warmup W] [--precision fp32|bf16x3|bf16]
+    python bench.py [--steps K] [--warmup W] [--precision fp32|bf16x3|bf16|fp8]
 """
 from __future__ import annotations
 
@@ -33,10 +33,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "matched channel·samples/s, FP32 vs BF16-MFMA, + spike agreement % at 383ch"
 C, K, NT, NTAP, P = 383, 700, 60000, 61, 6
-FORMS = ("fp32", "bf16x3", "bf16")
+FORMS = ("fp32", "bf16x3", "bf16", "fp8")
 T_PAD = NT + 2 * 61          # padded batch columns
-PEAK_MFMA = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15}   # dense MFMA peaks
-GEMM_KERNEL = {"fp32": "gemm_f32", "bf16x3": "gemm_bf16", "bf16": "gemm_bf16"}
+PEAK_MFMA = {"fp32": 157.3e12, "bf16x3": 2.5e15, "bf16": 2.5e15,    # dense MFMA peaks
+             "fp8": 5.0e15}
+GEMM_KERNEL = {"fp32": "gemm_f32", "bf16x3": "gemm_bf16", "bf16": "gemm_bf16",
+               "fp8": "gemm_fp8"}
 PEAK_HBM = 8.0e12
 
 
@@ -195,22 +197,33 @@ def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
 
 
 def executed_gemm_flops(U, perm, precision):
-    """MFMA flops the projection GEMM executes: the FP32 kernel skips, per
-    tile of 64 templates (peel order), every 16-row step of the reduction
-    index r = p*C + c where the tile's U is zero; 2 * 64 * 16 * (steps) * the
-    padded time columns.  The BF16 forms run every step."""
-    Un = U.detach().cpu().numpy()
+    """MFMA flops the projection GEMM executes and the fraction of reduction
+    steps it keeps.  Each form skips, per tile of templates (peel order), the
+    reduction steps where the tile's U is zero:
+      fp32:         64-template tiles, 16-row steps of r = p*C + c,
+                    time columns padded to 256;
+      bf16, bf16x3: 128-template tiles, 64-column steps of r = c*P + p
+                    (bf16x3 runs three products per step), columns padded
+                    to 128;
+      fp8:          128-template tiles, 128-column steps of r = c*P + p."""
+    Un = U.detach().cpu().numpy()[perm.cpu().numpy()]
     Kt, Pt, Ct = Un.shape
-    ldc = -(-T_PAD // 256) * 256
-    if precision != "fp32":
-        return None
-    A = Un[perm.cpu().numpy()].reshape(Kt, Pt * Ct)
-    Kp = -(-Kt // 64) * 64
-    Rp = -(-Pt * Ct // 16) * 16
-    Ap = np.zeros((Kp, Rp), bool)
+    rup = lambda a, b: -(-a // b) * b
+    if precision == "fp32":
+        tm_, step, cols, nseg = 64, 16, rup(T_PAD, 256), 1
+        A = Un.reshape(Kt, Pt * Ct)                              # r = p*C + c
+        R = rup(Pt * Ct, 16)
+    else:
+        tm_, cols = 128, rup(T_PAD, 128)
+        step, nseg = {"bf16": (64, 1), "bf16x3": (64, 3), "fp8": (128, 1)}[precision]
+        A = Un.transpose(0, 2, 1).reshape(Kt, Ct * Pt)          # r = c*P + p
+        R = rup(Pt * Ct, 192) if step == 64 else rup(Pt * Ct, Pt * step)
+    Kp = rup(Kt, tm_)
+    Ap = np.zeros((Kp, R), bool)
     Ap[:Kt, :Pt * Ct] = A != 0
-    steps = Ap.reshape(Kp // 64, 64, Rp // 16, 16).any(axis=(1, 3)).sum()
-    return 2.0 * 64 * 16 * float(steps) * ldc, float(steps) / ((Kp // 64) * (Rp // 16))
+    steps = Ap.reshape(Kp // tm_, tm_, R // step, step).any(axis=(1, 3)).sum()
+    return (2.0 * tm_ * step * float(steps) * cols * nseg,
+            float(steps) / ((Kp // tm_) * (R // step)))
 
 
 def ragged_workload(n_rec, max_batches, seed, device):
@@ -274,8 +287,8 @@ def time_ragged(tm, jobs, device):
             "unit": "channel·samples/s", "precision": "fp32"}
 
 
-PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "colmax_all", "peel_select", "peel_update",
-            "features"]
+PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "gemm_fp8", "colmax_all", "peel_select",
+            "peel_update", "features"]
 
 
 def profile_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, nbatch):
@@ -419,8 +432,8 @@ def main():
     gname = GEMM_KERNEL[args.precision]
     g = kern[gname]
     flops_gemm = 2.0 * K * P * C * T
-    ex = executed_gemm_flops(U, tm._grams.get(ctc, U, NTAP, device).perm, args.precision)
-    flops_run, kept = (ex if ex is not None else (flops_gemm, 1.0))
+    flops_run, kept = executed_gemm_flops(U, tm._grams.get(ctc, U, NTAP, device).perm,
+                                          args.precision)
     g_ach = flops_run / (g["avg_launch_ms"] / 1e3)
     peak = PEAK_MFMA[args.precision]
     roofline_projection = {"kernel": f"{gname} (projection GEMM)", "bound": "mfma",

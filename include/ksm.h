@@ -47,7 +47,8 @@ typedef enum ksm_dtype {
 typedef enum ksm_precision {
     KSM_PREC_FP32 = 0,          /* FP32 correlation + FP32 MFMA projection   */
     KSM_PREC_BF16 = 1,          /* BF16 operands on MFMA, FP32 accumulation  */
-    KSM_PREC_BF16X3 = 2         /* split BF16 (hi*hi + hi*lo + lo*hi) on MFMA */
+    KSM_PREC_BF16X3 = 2,        /* split BF16 (hi*hi + hi*lo + lo*hi) on MFMA */
+    KSM_PREC_FP8 = 3            /* scaled OCP e4m3 operands on MFMA, FP32 accumulation */
 } ksm_precision;
 
 /* Library version (major*10000 + minor*100 + patch). */

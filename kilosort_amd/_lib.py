@@ -23,6 +23,7 @@ KSM_ERR_TIES = 5
 PREC_FP32 = 0
 PREC_BF16 = 1
 PREC_BF16X3 = 2
+PREC_FP8 = 3
 
 PEEL_RESCORE = 1
 

@@ -335,6 +335,7 @@ size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) 
     if (C <= 0 || T <= 0 || K <= 0 || P <= 0) return 0;
     if (precision == KSM_PREC_BF16 || precision == KSM_PREC_BF16X3)
         return ksm::project_bf16_workspace(C, T, K, P, precision);
+    if (precision == KSM_PREC_FP8) return ksm::project_fp8_workspace(C, T, K, P);
     const ProjectLayout L = project_layout(C, T, K, P);
     return ksm::align_up(L.bc_bytes, 256) + ksm::align_up(L.at_bytes, 256) +
            ksm::align_up((size_t)L.nTm * L.nk * sizeof(int), 256) +
@@ -348,7 +349,7 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
         (nt % 2) == 0 || ldx < T)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: bad arguments");
     if (precision != KSM_PREC_FP32 && precision != KSM_PREC_BF16 &&
-        precision != KSM_PREC_BF16X3)
+        precision != KSM_PREC_BF16X3 && precision != KSM_PREC_FP8)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: unknown precision %d", precision);
     const ProjectLayout L = project_layout(C, T, K, P);
     if (ldbt < K || (ldbt % 4) != 0)
@@ -357,6 +358,8 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
     if (!workspace || ws_bytes < ksm_project_workspace(C, T, K, P, nt, precision))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_project: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
+    if (precision == KSM_PREC_FP8)
+        return ksm::project_fp8(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, workspace, s);
     if (precision != KSM_PREC_FP32)
         return ksm::project_bf16(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision,
                                  workspace, s);

@@ -24,12 +24,15 @@ Optional key ``ops['matching_precision']``:
 * ``'fp32'`` (default): FP32 correlation and FP32 MFMA projection -- the
   parity form;
 * ``'bf16'``: BF16 operands on MFMA with FP32 accumulation;
-* ``'bf16x3'``: split BF16 operands (hi*hi + hi*lo + lo*hi) on MFMA.
+* ``'bf16x3'``: split BF16 operands (hi*hi + hi*lo + lo*hi) on MFMA;
+* ``'fp8'``: OCP e4m3 operands on MFMA with FP32 accumulation, the
+  correlation scaled per (channel, PC) by a bound on its magnitude and U per
+  template.
 
-In the BF16 forms every accepted spike is re-scored in FP32 from the residual
-before it is recorded and subtracted, so amplitudes, th_amps and the
-subtraction use FP32 scores; the accept/reject decisions use the
-reduced-precision scores.
+In the reduced-precision forms every accepted spike is re-scored in FP32
+from the residual before it is recorded and subtracted, so amplitudes,
+th_amps and the subtraction use FP32 scores; the accept/reject decisions use
+the reduced-precision scores.
 
 Optional key ``ops['matching_streams']`` (default ``MATCHING_STREAMS``):
 device streams over which ``extract`` spreads its batches, one host thread
@@ -243,7 +246,8 @@ def prepare_matching(ops, U):
     return store[:, :, :L].permute(1, 0, 2)
 
 
-_PRECISIONS = {'fp32': _lib.PREC_FP32, 'bf16': _lib.PREC_BF16, 'bf16x3': _lib.PREC_BF16X3}
+_PRECISIONS = {'fp32': _lib.PREC_FP32, 'bf16': _lib.PREC_BF16, 'bf16x3': _lib.PREC_BF16X3,
+               'fp8': _lib.PREC_FP8}
 
 
 def _precision(ops):

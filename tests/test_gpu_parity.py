@@ -188,7 +188,7 @@ def _project_gpu(X, U, W, precision):
     return Bt[:T, :K].T.cpu().double().numpy()
 
 
-@pytest.mark.parametrize("precision", [0, 1, 2])
+@pytest.mark.parametrize("precision", [0, 1, 2, 3])
 def test_projection_sparse_templates_and_nonfinite_input(precision):
     """Templates zero outside a band of channels (the GEMMs skip their zero
     reduction steps): scores match the dense float64 product; with an inf in X
@@ -205,7 +205,7 @@ def test_projection_sparse_templates_and_nonfinite_input(precision):
     X = ArrayBatches(rec.data, NT=6000, nt=61, device=CPU).padded_batch_to_torch(0)
     ref = O.project(dict(nt=61, wPCA=W.double()), X.double(), U.double()).numpy()
     got = _project_gpu(X, U, W, precision)
-    tol = {0: 1e-6, 1: 2e-2, 2: 1e-4}[precision]
+    tol = {0: 1e-6, 1: 2e-2, 2: 1e-4, 3: 0.15}[precision]
     assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
     X2 = X.clone()
     X2[7, 3000] = float("inf")
