@@ -155,6 +155,106 @@ def time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, st
                 streams=n_lanes)
 
 
+def time_preprocess(device, seed, nbatch=3, reps=5, cpu_baseline=True):
+    """Batch preprocessing at 383 channels (int16 raw, NT=60000): channel means,
+    CAR, the default 300 Hz high-pass filter and a whitening matrix.
+    * device rate: raw batch resident in HBM, ksm_filter_batch alone;
+    * end-to-end rate: FilteredBatches.padded_batch_to_torch from a host
+      int16 array (slice, pinned staging, host-to-device copy, filter);
+    * per-kernel times (library events) and the high-pass product's MFMA
+      roofline; the oracle (torch CPU, FFT filter) on one batch as its CPU
+      baseline."""
+    from kilosort_amd import _lib, preprocess as PP
+    lib = _lib.load()
+    n = nbatch * NT + 1000
+    rng = np.random.default_rng(seed)
+    data = np.clip(np.round(rng.standard_normal((n, C), dtype=np.float32) * 40.0
+                            + rng.uniform(-200, 200, C).astype(np.float32)),
+                   -32768, 32767).astype(np.int16)
+    A = np.eye(C, dtype=np.float32) + 0.02 * rng.standard_normal((C, C), dtype=np.float32)
+    W = (A @ A.T / np.sqrt(C)).astype(np.float32)
+    hp = PP.get_highpass_filter(device=device)
+    fb = PP.FilteredBatches(n_chan_bin=C, NT=NT, file_object=data, device=device,
+                            hp_filter=hp, whiten_mat=W)
+    T = NT + 2 * NTAP
+    fb.padded_batch_to_torch(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for ib in range(1, nbatch):
+            fb.padded_batch_to_torch(ib)
+    torch.cuda.synchronize()
+    e2e_ms = (time.perf_counter() - t0) * 1e3 / (reps * (nbatch - 1))
+    # raw resident: one middle batch's window
+    raw = torch.from_numpy(np.ascontiguousarray(data[NT - NTAP:NT - NTAP + T])).to(device)
+    taps, J = fb._taps_for(T)
+    Wd = fb._whiten
+    X = torch.empty((C, T), device=device)
+    flags = _lib.FILTER_CAR | _lib.FILTER_WHITEN
+    nb = lib.ksm_filter_workspace(C, T, J, flags)
+    ws = torch.empty(nb, dtype=torch.uint8, device=device)
+    s = _lib.stream_of(device)
+
+    def one():
+        _lib.check(lib.ksm_filter_batch(raw.data_ptr(), _lib.RAW_I16, C, T, 0, None, C, T,
+                                        flags, 0.0, 0.0, taps.data_ptr(), J, float("inf"),
+                                        Wd.data_ptr(), X.data_ptr(), T, ws.data_ptr(), nb, s),
+                   "ksm_filter_batch")
+    for _ in range(3):
+        one()
+    torch.cuda.synchronize()
+    k = 20
+    t0 = time.perf_counter()
+    for _ in range(k):
+        one()
+    torch.cuda.synchronize()
+    dev_ms = (time.perf_counter() - t0) * 1e3 / k
+    lib.ksm_profile_reset()
+    lib.ksm_profile_enable(3)
+    for _ in range(5):
+        one()
+    torch.cuda.synchronize()
+    lib.ksm_profile_enable(0)
+    kern = {}
+    for name in ("filter_car", "filter_highpass", "filter_whiten"):
+        ms, cnt = _lib.profile_query(name)
+        if cnt:
+            kern[name] = ms / cnt
+    lib.ksm_profile_reset()
+    flops_hp = 2.0 * C * T * (2 * J + 1)
+    # tiles of 128 channels x 64 samples, depth steps of 32 over 2J + 64
+    nks = -(-(2 * J + 64) // 32)
+    flops_hp_exec = 2.0 * (-(-C // 128) * 128) * (-(-T // 64) * 64) * 32 * nks
+    flops_wh = 2.0 * C * C * T
+    out = dict(value=C * T / (dev_ms / 1e3), unit="channel·samples/s", ms_per_batch=dev_ms,
+               end_to_end_value=C * T / (e2e_ms / 1e3), end_to_end_ms_per_batch=e2e_ms,
+               taps=2 * J + 1, kernel_ms=kern,
+               config="383 ch int16, NT=60000 (+2x61), CAR, 300 Hz high-pass "
+                      f"({2 * J + 1} taps), whitening")
+    if "filter_highpass" in kern:
+        t_hp = kern["filter_highpass"] / 1e3
+        out["roofline_highpass"] = dict(
+            kernel="mm_mfma_kernel<Toeplitz> (high-pass filter)", bound="mfma",
+            achieved=flops_hp / t_hp / 1e12, peak=157.3, unit="TFLOP/s",
+            frac=flops_hp / t_hp / 157.3e12, flops_per_launch=flops_hp,
+            executed_flops_per_launch=flops_hp_exec, avg_launch_ms=kern["filter_highpass"])
+    if "filter_whiten" in kern:
+        t_w = kern["filter_whiten"] / 1e3
+        out["roofline_whiten"] = dict(achieved=flops_wh / t_w / 1e12, peak=157.3,
+                                      unit="TFLOP/s", frac=flops_wh / t_w / 157.3e12)
+    if cpu_baseline:
+        from oracle import filter_oracle as FO
+        Xc, _ = FO.padded_batch(data, 1, NT, NTAP, int(fb.n_batches), imax=int(fb.imax))
+        t0 = time.perf_counter()
+        FO.filter_batch(Xc, hp_filter=hp.cpu(), whiten_mat=torch.from_numpy(W))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=C * T / dt, unit="channel·samples/s",
+                                   cores=torch.get_num_threads(), kind="port",
+                                   sample="1 padded batch: oracle filter (torch CPU FFT)",
+                                   seconds=dt)
+    return out
+
+
 def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
     """Template refinement on the workload: spikes and amplitudes from the FP32
     matching of `nbatch` batches, then the residual loss and its gradients for
@@ -449,6 +549,12 @@ def main():
     log(f"refinement: residual {refine_res['residual_ms_per_batch']:.3f} ms, "
         f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch")
 
+    log("batch preprocessing (CAR, high-pass, whitening) at 383 channels")
+    pre = time_preprocess(device, args.seed + 200, cpu_baseline=not args.no_cpu_baseline)
+    log(f"preprocessing: {pre['ms_per_batch']:.3f} ms/batch resident, "
+        f"{pre['end_to_end_ms_per_batch']:.3f} ms/batch from host int16; kernels "
+        + ", ".join(f"{k}={v:.3f}" for k, v in pre["kernel_ms"].items()))
+
     ragged = None
     if args.ragged > 0:
         log(f"ragged: {args.ragged} recordings, sequential extract vs extract_many")
@@ -506,6 +612,7 @@ def main():
                   for f in forms},
         "agreement_batch0": agree,
         "refinement": refine_res,
+        "preprocess": pre,
         "ragged": ragged,
         "cpu_baseline": cpu,
     }

@@ -250,6 +250,61 @@ int ksm_residual_backward(const void *R, int ldr, const void *U, const void *W,
                           double scale, void *dU, void *dW, void *damp, int dtype,
                           void *workspace, size_t ws_bytes, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Batch preprocessing (the step before the matching path).
+ */
+
+/* Raw sample layouts of ksm_filter_batch. */
+typedef enum ksm_raw {
+    KSM_RAW_I16 = 0,            /* int16   [nsamp][ldraw], time-major (binary file)   */
+    KSM_RAW_U16 = 1,            /* uint16  [nsamp][ldraw], shifted by -2^15          */
+    KSM_RAW_F32 = 2,            /* float32 [nsamp][ldraw], time-major                */
+    KSM_RAW_F32_CM = 3          /* float32 [n_chan][ldraw], channel-major (a batch)  */
+} ksm_raw;
+
+#define KSM_FILTER_INVERT     1    /* invert_sign                                    */
+#define KSM_FILTER_CAR        2    /* do_CAR: subtract the median across channels    */
+#define KSM_FILTER_SKIP       4    /* skip_preproc: stop after the median stage      */
+#define KSM_FILTER_SCALE      8    /* multiply the raw values by `scale`             */
+#define KSM_FILTER_SHIFT      16   /* then add `shift`                               */
+#define KSM_FILTER_EDGE_FIRST 32   /* columns before `off` repeat raw row 0          */
+#define KSM_FILTER_EDGE_LAST  64   /* columns after the data repeat the last row     */
+#define KSM_FILTER_WHITEN     128  /* `whiten` is given (workspace for its input)    */
+
+/* Workspace bytes of ksm_filter_batch (J = -1: no high-pass filter). */
+size_t ksm_filter_workspace(int C, int T, int J, int flags);
+
+/*
+ * One padded batch through the reference's filter.  Replaces
+ * BinaryRWFile.padded_batch_to_torch (io.py:789-831) followed by
+ * BinaryFiltered.filter (io.py:990-1026):
+ *   column t of the batch holds raw row t - off (0 <= t - off < nsamp); other
+ *   columns repeat row 0 / the last row (EDGE_FIRST / EDGE_LAST) or are 0;
+ *   X[c, t] = converted raw[., chan_map[c]] (chan_map NULL: identity), times -1
+ *   with INVERT; minus the mean over t of each row; minus the lower median
+ *   over c of each column (CAR); high-pass
+ *   out[c, n] = sum_{j=-J..J} taps[J + j] * x[c, (n + j) mod T]
+ *   (taps NULL: none; the reference's Fourier-domain filter equals this
+ *   circular correlation with its impulse response centred at T/2); if any
+ *   |out| >= artifact_threshold the batch becomes 0 (INFINITY: no test);
+ *   X = whiten @ out (whiten [C][C] row-major, already multiplied by the
+ *   batch's drift matrix when there is one; NULL: none).
+ * X float32 [C][ldx] on the device.  Limits: C <= 1536, J <= 4096, 2J+1 <= T.
+ */
+int ksm_filter_batch(const void *raw, int fmt, int64_t ldraw, int nsamp, int off,
+                     const int32_t *chan_map, int C, int T, int flags, float scale,
+                     float shift, const float *taps, int J, float artifact_threshold,
+                     const float *whiten, float *X, int ldx, void *workspace,
+                     size_t ws_bytes, void *stream);
+
+/*
+ * Row-major FP32 product C[M][N] = A[M][K] @ B[K][N] on MFMA (FP32
+ * accumulation), for the drift-matrix products (Kyx @ iKxx, M @ Wrot) of
+ * preprocessing.py:67 and io.py:1021-1023.
+ */
+int ksm_gemm_f32(const float *A, int lda, const float *B, int ldb, float *Cm, int ldc,
+                 int M, int N, int K, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

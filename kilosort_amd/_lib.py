@@ -30,11 +30,26 @@ PEEL_RESCORE = 1
 DTYPE_F32 = 0
 DTYPE_F64 = 1
 
+RAW_I16 = 0
+RAW_U16 = 1
+RAW_F32 = 2
+RAW_F32_CM = 3
+
+FILTER_INVERT = 1
+FILTER_CAR = 2
+FILTER_SKIP = 4
+FILTER_SCALE = 8
+FILTER_SHIFT = 16
+FILTER_EDGE_FIRST = 32
+FILTER_EDGE_LAST = 64
+FILTER_WHITEN = 128
+
 _p = ctypes.c_void_p
 _i = ctypes.c_int
 _f = ctypes.c_float
 _sz = ctypes.c_size_t
 _d = ctypes.c_double
+_i64 = ctypes.c_int64
 
 # name -> (restype, argtypes); must match include/ksm.h
 SIGNATURES = {
@@ -60,6 +75,10 @@ SIGNATURES = {
                           _p, _sz, _p]),
     "ksm_residual_backward": (_i, [_p, _i, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _d,
                                    _p, _p, _p, _i, _p, _sz, _p]),
+    "ksm_filter_workspace": (_sz, [_i, _i, _i, _i]),
+    "ksm_filter_batch": (_i, [_p, _i, _i64, _i, _i, _p, _i, _i, _i, _f, _f, _p, _i, _f, _p, _p,
+                              _i, _p, _sz, _p]),
+    "ksm_gemm_f32": (_i, [_p, _i, _p, _i, _p, _i, _i, _i, _i, _p]),
     "ksm_profile_enable": (_i, [_i]),
     "ksm_profile_reset": (_i, []),
     "ksm_profile_query": (_i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
