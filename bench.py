@@ -466,6 +466,8 @@ def main():
                     help="device streams of the batch pipeline (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU oracle runs (cpu_baseline and FP64 agreement)")
+    ap.add_argument("--preprocess-only", action="store_true",
+                    help="time only the batch preprocessing and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the matching path is sharded by batch "
@@ -474,6 +476,13 @@ def main():
 
     from kilosort_amd import _lib
     from kilosort_amd import template_matching as tm
+
+    if args.preprocess_only:
+        device = torch.device("cuda:0")
+        _lib.load()
+        print(json.dumps(time_preprocess(device, args.seed + 200,
+                                         cpu_baseline=not args.no_cpu_baseline)), flush=True)
+        return
 
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)

@@ -131,13 +131,17 @@ __global__ __launch_bounds__(256) void chan_sums_kernel(Src s, int C, int T,
     }
 }
 
-__global__ void chan_means_kernel(const double *__restrict__ psum, int nb, int C, int T,
-                                  float *__restrict__ mean) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// mean[c] = (sum over blocks of psum[b][c]) / T: one wave per channel, lanes
+// over blocks, then a fixed-order tree over the lanes.
+__global__ __launch_bounds__(256) void chan_means_kernel(const double *__restrict__ psum, int nb,
+                                                         int C, int T, float *__restrict__ mean) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (c >= C) return;
     double acc = 0.0;
-    for (int b = 0; b < nb; ++b) acc += psum[(size_t)b * C + c];
-    mean[c] = (float)(acc / (double)T);
+    for (int b = lane; b < nb; b += 64) acc += psum[(size_t)b * C + c];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) mean[c] = (float)(acc / (double)T);
 }
 
 __device__ __forceinline__ unsigned order_key(float v) {
@@ -263,17 +267,31 @@ __global__ __launch_bounds__(256) void car_kernel(Src s, int C, int T, int ncol,
     if (__ballot(big) != 0ull && lane == 0) atomicOr(flag, 1);
 }
 
-// Zero columns [from, ldd) of rows [0, C): the window columns the filter
-// product reads beyond the halo (times 0 taps, so they must be finite).
-__global__ void zero_cols_kernel(float *__restrict__ y, int ldd, int from, int C) {
+// Zero columns [from, ldd) of rows [0, C) and all of rows [C, Cp): what the
+// filter product reads beyond the haloed batch (times 0 taps, or rows that
+// are not stored, so they must be finite).
+__global__ void zero_pad_kernel(float *__restrict__ y, int ldd, int from, int C, int Cp) {
     const int w = ldd - from;
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)C * w) return;
-    const int c = (int)(e / w), j = (int)(e - (long long)c * w);
-    y[(size_t)c * ldd + from + j] = 0.f;
+    const long long n1 = (long long)C * w;
+    if (e < n1) {
+        const int c = (int)(e / w), j = (int)(e - (long long)c * w);
+        y[(size_t)c * ldd + from + j] = 0.f;
+    } else if (e - n1 < (long long)(Cp - C) * ldd) {
+        y[(size_t)C * ldd + (e - n1)] = 0.f;
+    }
 }
 
 __global__ void clear_flag_kernel(int *__restrict__ f) { *f = 0; }
+
+// dst[r][c] = src[r][c] for r, c < n, else 0 (dst [rows][ld]).
+__global__ void pad_matrix_kernel(const float *__restrict__ src, int n, float *__restrict__ dst,
+                                  int rows, int ld) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * ld) return;
+    const int r = e / ld, c = e - r * ld;
+    dst[e] = (r < n && c < n) ? src[(size_t)r * n + c] : 0.f;
+}
 
 __global__ void zero_if_flag_kernel(float *__restrict__ X, int ldx, int C, int T,
                                     const int *__restrict__ flag) {
@@ -295,10 +313,14 @@ __device__ __forceinline__ int xcd_remap_f(int bid, int nb) {
 // launch at about two rounds of resident blocks (383 x 60122: 2820 tiles),
 // so that the last round is not a small tail.
 //   TOEPLITZ: A(m, k) = a[m][n0 + k] (the sliding window of row m; a holds
-//             the haloed batch), B(k, n) = taps[k - (n - n0)] for
+//             the haloed batch, its rows zero padded to a multiple of 128),
+//             B(k, n) = taps[k - (n - n0)] for
 //             0 <= k - (n - n0) <= 2J (from LDS), depth 2J + NB;
-//   else:     A(m, k) = a[m][k] (k < K), B(k, n) = b[k][n] (k < K); rows of b
-//             are read as float4 when `vecb` (b 16-byte aligned, ldb % 4 == 0).
+//   else:     A(m, k) = a[m][k] (k < K), B(k, n) = b[k][n] (k < K); with
+//             `veca`, a is zero padded to whole tiles (rows to a multiple of
+//             128, columns to a multiple of 32, lda % 4 == 0) and read as
+//             float4 without bounds tests; rows of b are read as float4 when
+//             `vecb` (b 16-byte aligned, ldb % 4 == 0).
 // Rows m < M and columns n < N are stored; with `flag` set (an artifact),
 // zeros are stored instead.  With `thr` finite, |out| >= thr raises *flag
 // (read by a later launch).
@@ -307,7 +329,7 @@ __global__ __launch_bounds__(256, 2) void mm_mfma_kernel(
     const float *__restrict__ a, int lda, const float *__restrict__ b, int ldb,
     const float *__restrict__ taps, int J, float *__restrict__ out, int ldo, int M, int N,
     int K, int nTm, int nTn, float thr, int *__restrict__ flag, const int *__restrict__ zflag,
-    int vecb) {
+    int veca, int vecb) {
     extern __shared__ float s_g[];               // Toeplitz taps, zero padded
     __shared__ float As[2][MB * ALD];
     __shared__ float Bs[TOEPLITZ ? 1 : 2][TOEPLITZ ? 1 : KB * BLD];
@@ -328,32 +350,37 @@ __global__ __launch_bounds__(256, 2) void mm_mfma_kernel(
     // A tile: 128 rows x 32 depth; thread -> row tid/2, 16 consecutive columns
     const int ar = tid >> 1, ac = (tid & 1) * 16;
     const int arow = m0 + ar;
-    const float *ap = a + (size_t)(arow < M ? arow : 0) * lda + (TOEPLITZ ? n0 : 0) + ac;
+    const float *ap = a + (size_t)(arow < M || veca || TOEPLITZ ? arow : 0) * lda +
+                      (TOEPLITZ ? n0 : 0) + ac;
     // B tile (general): 32 rows x 64 columns; thread -> rows tid/16 and +16
     const int br = tid >> 4, bc = (tid & 15) * 4;
     float4 ra0, ra1, ra2, ra3, rb0, rb1;
-    auto load = [&](int st) {
+    // (always inlined: captured registers must not be given stack addresses)
+    auto load = [&](int st) __attribute__((always_inline)) {
         const int k0 = st * KB;
-        if (TOEPLITZ) {
-            if (arow < M) {
+        if constexpr (TOEPLITZ) {              // rows padded with zeros to whole tiles
+            ra0 = *reinterpret_cast<const float4 *>(ap + k0);
+            ra1 = *reinterpret_cast<const float4 *>(ap + k0 + 4);
+            ra2 = *reinterpret_cast<const float4 *>(ap + k0 + 8);
+            ra3 = *reinterpret_cast<const float4 *>(ap + k0 + 12);
+        } else {
+            if (veca) {                      // zero-padded, aligned A
                 ra0 = *reinterpret_cast<const float4 *>(ap + k0);
                 ra1 = *reinterpret_cast<const float4 *>(ap + k0 + 4);
                 ra2 = *reinterpret_cast<const float4 *>(ap + k0 + 8);
                 ra3 = *reinterpret_cast<const float4 *>(ap + k0 + 12);
             } else {
-                ra0 = ra1 = ra2 = ra3 = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        } else {
-            float v[16];
+                float v[16];
 #pragma unroll
-            for (int e = 0; e < 16; ++e) {
-                const int k = k0 + ac + e;
-                v[e] = (arow < M && k < K) ? ap[k0 + e] : 0.f;
+                for (int e = 0; e < 16; ++e) {
+                    const int k = k0 + ac + e;
+                    v[e] = (arow < M && k < K) ? ap[k0 + e] : 0.f;
+                }
+                ra0 = make_float4(v[0], v[1], v[2], v[3]);
+                ra1 = make_float4(v[4], v[5], v[6], v[7]);
+                ra2 = make_float4(v[8], v[9], v[10], v[11]);
+                ra3 = make_float4(v[12], v[13], v[14], v[15]);
             }
-            ra0 = make_float4(v[0], v[1], v[2], v[3]);
-            ra1 = make_float4(v[4], v[5], v[6], v[7]);
-            ra2 = make_float4(v[8], v[9], v[10], v[11]);
-            ra3 = make_float4(v[12], v[13], v[14], v[15]);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = k0 + br + 16 * h;
@@ -375,13 +402,13 @@ __global__ __launch_bounds__(256, 2) void mm_mfma_kernel(
             }
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf) __attribute__((always_inline)) {
         float *d = &As[buf][ar * ALD + ac];
         d[0] = ra0.x; d[1] = ra0.y; d[2] = ra0.z; d[3] = ra0.w;
         d[4] = ra1.x; d[5] = ra1.y; d[6] = ra1.z; d[7] = ra1.w;
         d[8] = ra2.x; d[9] = ra2.y; d[10] = ra2.z; d[11] = ra2.w;
         d[12] = ra3.x; d[13] = ra3.y; d[14] = ra3.z; d[15] = ra3.w;
-        if (!TOEPLITZ) {
+        if constexpr (!TOEPLITZ) {
             *reinterpret_cast<float4 *>(&Bs[buf][br * BLD + bc]) = rb0;
             *reinterpret_cast<float4 *>(&Bs[buf][(br + 16) * BLD + bc]) = rb1;
         }
@@ -407,7 +434,7 @@ __global__ __launch_bounds__(256, 2) void mm_mfma_kernel(
             float bv[2];
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                if (TOEPLITZ)
+                if constexpr (TOEPLITZ)
                     bv[j] = s_g[st * KB + kk + kr - (j * 32 + kc) + NB - 1];
                 else
                     bv[j] = Bs[buf][(kk + kr) * BLD + j * 32 + kc];
@@ -461,17 +488,17 @@ int car_columns(int C) {
 
 int launch_mm(bool toeplitz, const float *a, int lda, const float *b, int ldb, const float *taps,
               int J, float *out, int ldo, int M, int N, int K, float thr, int *flag,
-              const int *zflag, hipStream_t s, const char *name) {
+              const int *zflag, hipStream_t s, const char *name, int veca = 0) {
     const int nTm = (M + MB - 1) / MB, nTn = (N + NB - 1) / NB;
     ksm::ProfileScope prof(name, ksm::kProfProject, s);
     if (toeplitz) {
         const unsigned shm = (unsigned)((2 * J + 2 * NB + KB) * sizeof(float));
         prof.launch(mm_mfma_kernel<true>, dim3(nTm * nTn), dim3(256), shm, a, lda, b, ldb, taps,
-                    J, out, ldo, M, N, K, nTm, nTn, thr, flag, zflag, 0);
+                    J, out, ldo, M, N, K, nTm, nTn, thr, flag, zflag, 0, 0);
     } else {
         const int vecb = (ldb % 4) == 0 && (reinterpret_cast<uintptr_t>(b) & 15u) == 0;
         prof.launch(mm_mfma_kernel<false>, dim3(nTm * nTn), dim3(256), 0u, a, lda, b, ldb, taps,
-                    J, out, ldo, M, N, K, nTm, nTn, thr, flag, zflag, vecb);
+                    J, out, ldo, M, N, K, nTm, nTn, thr, flag, zflag, veca, vecb);
     }
     KSM_LAUNCH_CHECK(name);
     return KSM_OK;
@@ -486,8 +513,10 @@ size_t ksm_filter_workspace(int C, int T, int J, int flags) {
     const FilterLayout L = filter_layout(T, J < 0 ? 0 : J);
     size_t n = ksm::align_up((size_t)L.nsb * C * sizeof(double), 256) +
                ksm::align_up((size_t)C * sizeof(float), 256) + 256;
-    if (J >= 0) n += ksm::align_up((size_t)C * L.ldy * sizeof(float), 256);
-    if (flags & KSM_FILTER_WHITEN) n += ksm::align_up((size_t)C * L.ldz * sizeof(float), 256);
+    if (J >= 0) n += ksm::align_up(ksm::align_up(C, MB) * L.ldy * sizeof(float), 256);
+    if (flags & KSM_FILTER_WHITEN)
+        n += ksm::align_up((size_t)C * L.ldz * sizeof(float), 256) +
+             ksm::align_up(ksm::align_up(C, MB) * ksm::align_up(C, KB) * sizeof(float), 256);
     return n + 256;
 }
 
@@ -520,8 +549,11 @@ int ksm_filter_batch(const void *raw, int fmt, int64_t ldraw, int nsamp, int off
     double *psum = cv.take<double>((size_t)L.nsb * C);
     float *mean = cv.take<float>(C);
     int *flag = cv.take<int>(1);
-    float *Y = hp ? cv.take<float>((size_t)C * L.ldy) : nullptr;
+    const int Cp = (int)ksm::align_up(C, MB);
+    float *Y = hp ? cv.take<float>((size_t)Cp * L.ldy) : nullptr;
     float *Z = (flags & KSM_FILTER_WHITEN) ? cv.take<float>((size_t)C * L.ldz) : nullptr;
+    const int wrows = (int)ksm::align_up(C, MB), wld = (int)ksm::align_up(C, KB);
+    float *Wp = (flags & KSM_FILTER_WHITEN) ? cv.take<float>((size_t)wrows * wld) : nullptr;
 
     Src src;
     src.raw = raw;
@@ -541,7 +573,7 @@ int ksm_filter_batch(const void *raw, int fmt, int64_t ldraw, int nsamp, int off
     KSM_LAUNCH_CHECK("clear_flag_kernel");
     chan_sums_kernel<<<dim3(L.nsb, (C + 63) / 64), 256, 0, s>>>(src, C, T, psum);
     KSM_LAUNCH_CHECK("chan_sums_kernel");
-    chan_means_kernel<<<(C + 255) / 256, 256, 0, s>>>(psum, L.nsb, C, T, mean);
+    chan_means_kernel<<<(C + 3) / 4, 256, 0, s>>>(psum, L.nsb, C, T, mean);
     KSM_LAUNCH_CHECK("chan_means_kernel");
 
     // destination of the median stage
@@ -580,12 +612,10 @@ int ksm_filter_batch(const void *raw, int fmt, int64_t ldraw, int nsamp, int off
     KSM_LAUNCH_CHECK("car_kernel");
     if (skip) return KSM_OK;
     if (hp) {
-        const int from = T + 2 * J;
-        if (L.ldy > from) {
-            const long long n = (long long)C * (L.ldy - from);
-            zero_cols_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(Y, L.ldy, from, C);
-            KSM_LAUNCH_CHECK("zero_cols_kernel");
-        }
+        const int from = T + 2 * J;          // < ldy (ldy >= T + 2J + 64)
+        const long long n = (long long)C * (L.ldy - from) + (long long)(Cp - C) * L.ldy;
+        zero_pad_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(Y, L.ldy, from, C, Cp);
+        KSM_LAUNCH_CHECK("zero_pad_kernel");
         float *d2 = wh ? Z : X;
         const int ld2 = wh ? L.ldz : ldx;
         const int rc = launch_mm(true, Y, L.ldy, nullptr, 0, taps, J, d2, ld2, C, T, 0, thr,
@@ -593,8 +623,10 @@ int ksm_filter_batch(const void *raw, int fmt, int64_t ldraw, int nsamp, int off
         if (rc != KSM_OK) return rc;
     }
     if (wh) {
-        const int rc = launch_mm(false, whiten, C, Z, L.ldz, nullptr, 0, X, ldx, C, T, C,
-                                 INFINITY, nullptr, check ? flag : nullptr, s, "filter_whiten");
+        pad_matrix_kernel<<<(wrows * wld + 255) / 256, 256, 0, s>>>(whiten, C, Wp, wrows, wld);
+        KSM_LAUNCH_CHECK("pad_matrix_kernel");
+        const int rc = launch_mm(false, Wp, wld, Z, L.ldz, nullptr, 0, X, ldx, C, T, C, INFINITY,
+                                 nullptr, check ? flag : nullptr, s, "filter_whiten", 1);
         if (rc != KSM_OK) return rc;
     } else if (check) {
         const long long n = (long long)C * T;

@@ -215,3 +215,46 @@ def test_gemm_f32_odd_shapes():
         ref = (A.double() @ B.double()).numpy()
         got = Cd.cpu().double().numpy()
         assert np.abs(got - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1.0), (M, N, K)
+
+
+class _OracleFiltered:
+    """A bfile whose batches are the oracle's padded + filtered batches."""
+
+    def __init__(self, data, fb, W):
+        self.data, self.fb, self.W = data, fb, W
+        self.n_batches = fb.n_batches
+        self.batch_downsampling = 1
+
+    def padded_batch_to_torch(self, ibatch, ops=None):
+        out, _ = oracle_batch(self.data, self.fb, ibatch, W=self.W)
+        return torch.from_numpy(out).to(DEV)
+
+
+def test_extract_reads_filtered_batches():
+    """FilteredBatches is the bfile of extract: spikes found from its batches
+    agree with those found from the oracle's filtered batches (the inputs
+    differ by float32 rounding only)."""
+    from kilosort_amd import template_matching as tm
+    C, NT = 64, 30000
+    rec = synth.config1_recording(seed=3, n_batches=2)
+    n = 2 * NT
+    rng = np.random.default_rng(3)
+    raw = 20.0 * rec.data[:n] + rng.uniform(-300, 300, C)[None, :] \
+        + 30.0 * rng.standard_normal((n, 1))
+    raw = np.clip(np.round(raw), -32768, 32767).astype(np.int16)
+    W = (np.eye(C) / 20.0).astype(np.float32)
+    fb = PP.FilteredBatches(n_chan_bin=C, NT=NT, file_object=raw, device=DEV, whiten_mat=W,
+                            hp_filter=PP.get_highpass_filter(device=DEV))
+    ops = dict(nt=61, nt0min=rec.nt0min, Th_learned=8, max_peels=100, batch_size=NT,
+               wPCA=torch.from_numpy(rec.wPCA).to(DEV), xc=rec.xc, yc=rec.yc,
+               settings=dict(nearest_chans=10, position_limit=100.0, n_pcs=6))
+    U = torch.from_numpy(rec.U).to(DEV)
+    say("extract from FilteredBatches vs oracle-filtered batches")
+    st, tF, _ = tm.extract(dict(ops), fb, U, device=DEV)
+    st_o, tF_o, _ = tm.extract(dict(ops), _OracleFiltered(raw, fb, W), U, device=DEV)
+    a = set(map(tuple, st[:, :2].astype(np.int64).tolist()))
+    b = set(map(tuple, st_o[:, :2].astype(np.int64).tolist()))
+    agree = len(a & b) / max(len(a), len(b))
+    say(f"{len(st)} spikes vs {len(st_o)}: agreement {100 * agree:.3f} %")
+    assert len(st_o) > 500
+    assert agree >= 0.995
