@@ -305,6 +305,37 @@ int ksm_filter_batch(const void *raw, int fmt, int64_t ldraw, int nsamp, int off
 int ksm_gemm_f32(const float *A, int lda, const float *B, int ldb, float *Cm, int ldc,
                  int M, int N, int K, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Universal-template spike detection (the pass before clustering).
+ */
+
+/* Workspace bytes of ksm_detect. */
+size_t ksm_detect_workspace(int C, int T, int nk, int Nfilt);
+
+/*
+ * Replaces spikedetect.template_match (spikedetect.py:133-178) and the
+ * per-batch features of spikedetect.run (:256-281, yweighted :190-197):
+ *   B[k][c][t] = sum_m wTEMP[k, m] X[c, t + m - nt/2];
+ *   A[i, k, f, t] = sum_j weigh[i, j, f] B[k][iC[j, f]][t];
+ *   As = max over q = i*nk + k of |A| (first q on ties), Im = (1 + q) sign(A_q);
+ *   spikes where As > th and As equals the maximum of As over the nC2
+ *   templates iC2[:, f] and over |dt| <= nt0 (columns outside [nt, T-nt)
+ *   count as 0), in (f, t) order.
+ * iC int32 [nC][Nfilt], iC2 int32 [nC2][Nfilt], weigh [nsizes][nC][Nfilt],
+ * yc [C], wPCA [P][nt].  Outputs for spikes s < cap: xy [cap][2] = (f, t),
+ * imax [cap] = q, amp [cap] = As, adist [nC][cap] = B[q % nk][iC[j, f]][t] *
+ * sign, yct [cap] (relu(adist)-weighted y of the channels), tF
+ * [cap][nC][P] (window t - nt/2 .. t + nt/2 of X on the channels, times wPCA);
+ * *count (device) = the number of spikes, also when it exceeds cap.
+ * Limits: nC <= 32, nsizes <= 16, nk <= 16, nC*P <= 64, nt0 <= 256.
+ */
+int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk, int nt,
+               const int32_t *iC, int nC, const int32_t *iC2, int nC2, const float *weigh,
+               int nsizes, int Nfilt, int nt0, float th, const float *yc, const float *wPCA,
+               int P, int cap, int32_t *xy, int32_t *imax, float *amp, float *adist,
+               float *yct, float *tF, int *count, void *workspace, size_t ws_bytes,
+               void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,339 @@
+// Universal-template spike detection: Kilosort4's first spike-finding pass.
+//
+// Reference: kilosort/spikedetect.py:133-178 (template_match), :190-197
+// (yweighted) and :256-281 (the per-batch features of run).
+//
+// Per padded batch X [C][T]:
+//   B[k][c][t] = sum_m wTEMP[k, m] X[c, t + m - nt/2]          (conv_pc, FP32)
+//   A[i, k, f, t] = sum_j weigh[i, j, f] B[k][iC[j, f]][t]       (score_kernel)
+//   As[f, t] = max_{q = i nk + k} |A|, first q on ties;
+//   Im[f, t] = (1 + q) sign(A[q])
+//   Amaxs[f, t] = max over the nC2 nearest templates f' = iC2[n, f] and over
+//     |dt| <= nt0 of As[f', t + dt], where columns t + dt outside
+//     [nt, T - nt) count as 0 (the reference zeroes them before its
+//     max_pool1d; the maxima are exact, so their order does not matter)
+//   spikes: As == Amaxs and As > Th, in (f, t) order (torch.nonzero).
+// Kernels: window_max_kernel (R[f, t] = the time-window maximum of row f),
+// mask_kernel (per (f, t): the maximum of R over the neighbours, stopping at
+// the first neighbour above As; one 64-bit mask word per 64 columns),
+// row_count_kernel + row_scan_kernel (offsets per row), emit_kernel (one wave per
+// template row: the spikes in time order with amp, imax, adist, the
+// channel-weighted y position and the PC features of the residual window).
+#include "ksm_common.h"
+#include "ksm_profile.h"
+#include "ksm_project_internal.h"
+
+namespace {
+
+constexpr int kScoreF = 8;        // templates per score block
+constexpr int kMaxSizes = 16;     // template sizes (nsizes)
+constexpr int kMaxNk = 16;        // universal waveforms (nk)
+constexpr int kMaxNC = 32;        // nearest channels (nC)
+constexpr int kWinT = 256;        // columns per window-max block
+constexpr int kMaxNt0 = 256;
+
+// As[f][t], Im[f][t] for templates [f0, f0 + kScoreF) and columns of the block.
+__global__ __launch_bounds__(256) void score_kernel(
+    const float *__restrict__ B, int C, int ldc, const int32_t *__restrict__ iC, int nC,
+    const float *__restrict__ weigh, int nsizes, int nk, int Nfilt, int T,
+    float *__restrict__ As, int16_t *__restrict__ Im) {
+    __shared__ float s_w[kScoreF][kMaxSizes * kMaxNC];
+    __shared__ int s_c[kScoreF][kMaxNC];
+    const int f0 = blockIdx.y * kScoreF;
+    for (int e = threadIdx.x; e < kScoreF * nsizes * nC; e += 256) {
+        const int g = e / (nsizes * nC), r = e - g * nsizes * nC;
+        const int i = r / nC, j = r - i * nC;
+        const int f = f0 + g;
+        s_w[g][i * nC + j] = f < Nfilt ? weigh[((size_t)i * nC + j) * Nfilt + f] : 0.f;
+    }
+    for (int e = threadIdx.x; e < kScoreF * nC; e += 256) {
+        const int g = e / nC, j = e - g * nC;
+        const int f = f0 + g;
+        s_c[g][j] = f < Nfilt ? iC[(size_t)j * Nfilt + f] : 0;
+    }
+    __syncthreads();
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    for (int g = 0; g < kScoreF; ++g) {
+        const int f = f0 + g;
+        if (f >= Nfilt) break;
+        float best = -1.f, bval = 0.f;
+        int bq = 0;
+        for (int k = 0; k < nk; ++k) {
+            float b[kMaxNC];
+#pragma unroll
+            for (int j = 0; j < kMaxNC; ++j)
+                if (j < nC) b[j] = B[((size_t)k * C + s_c[g][j]) * ldc + t];
+            for (int i = 0; i < nsizes; ++i) {
+                float a = 0.f;
+#pragma unroll
+                for (int j = 0; j < kMaxNC; ++j)
+                    if (j < nC) a = __builtin_fmaf(s_w[g][i * nC + j], b[j], a);
+                const float m = fabsf(a);
+                const int q = i * nk + k;
+                // the first q on ties (torch.max); NaN wins and stays
+                if (m > best || (m == best && q < bq) || (m != m && best == best)) {
+                    best = m;
+                    bq = q;
+                    bval = a;
+                }
+            }
+        }
+        const int sg = bval > 0.f ? 1 : (bval < 0.f ? -1 : 0);
+        As[(size_t)f * ldc + t] = best;
+        Im[(size_t)f * ldc + t] = (int16_t)((1 + bq) * sg);
+    }
+}
+
+// R[f][t] = max of As[f][s] over s in [t - nt0, t + nt0] and [nt, T - nt),
+// 0 if that range is empty (the reference's zeroed columns).  Block: row f,
+// kWinT columns, the window staged in LDS.
+__global__ __launch_bounds__(256) void window_max_kernel(const float *__restrict__ As, int ldc,
+                                                         int T, int nt, int nt0,
+                                                         float *__restrict__ R) {
+    __shared__ float s_a[kWinT + 2 * kMaxNt0];
+    const int f = blockIdx.y, t0 = blockIdx.x * kWinT;
+    const float *row = As + (size_t)f * ldc;
+    const int w = kWinT + 2 * nt0;
+    for (int e = threadIdx.x; e < w; e += 256) {
+        const int s = t0 - nt0 + e;
+        s_a[e] = (s >= nt && s < T - nt) ? row[s] : 0.f;
+    }
+    __syncthreads();
+    const int t = t0 + threadIdx.x;
+    if (t >= T) return;
+    float m = 0.f;
+    for (int d = 0; d <= 2 * nt0; ++d) m = fmaxf(m, s_a[threadIdx.x + d]);
+    R[(size_t)f * ldc + t] = m;
+}
+
+// mask[f][w] bit b: column t = 64 w + b is a spike of template f.
+__global__ __launch_bounds__(256) void mask_kernel(const float *__restrict__ As,
+                                                   const float *__restrict__ R, int ldc, int T,
+                                                   const int32_t *__restrict__ iC2, int nC2,
+                                                   int Nfilt, float th,
+                                                   unsigned long long *__restrict__ mask,
+                                                   int nw) {
+    const int f = blockIdx.y;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool hit = false;
+    if (t < T) {
+        const float a = As[(size_t)f * ldc + t];
+        if (a > th) {
+            float m = -1.f;
+            for (int n = 0; n < nC2; ++n) {
+                const float r = R[(size_t)iC2[(size_t)n * Nfilt + f] * ldc + t];
+                m = fmaxf(m, r);
+                if (m > a) break;            // already not the maximum
+            }
+            hit = m == a;
+        }
+    }
+    const unsigned long long bits = __ballot(hit);
+    const int w = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 6;
+    if (lane == 0 && w < nw) mask[(size_t)f * nw + w] = bits;
+}
+
+// cnt[f] = the spikes of row f (one wave per row).
+__global__ __launch_bounds__(256) void row_count_kernel(const unsigned long long *__restrict__ mask,
+                                                        int nw, int Nfilt, int *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= Nfilt) return;
+    int c = 0;
+    for (int w = lane; w < nw; w += 64) c += __popcll(mask[(size_t)f * nw + w]);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (lane == 0) cnt[f] = c;
+}
+
+// off[f] = exclusive prefix of cnt over rows; *count = the total.
+__global__ __launch_bounds__(1024) void row_scan_kernel(const int *__restrict__ cnt, int Nfilt,
+                                                        int *__restrict__ off,
+                                                        int *__restrict__ count) {
+    __shared__ int s_part[1024];
+    const int tid = threadIdx.x;
+    const int per = (Nfilt + 1023) / 1024;    // a contiguous run of rows per thread
+    const int r0 = tid * per, r1 = min(Nfilt, r0 + per);
+    int acc = 0;
+    for (int f = r0; f < r1; ++f) acc += cnt[f];
+    s_part[tid] = acc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {     // inclusive scan over threads
+        const int v = tid >= o ? s_part[tid - o] : 0;
+        __syncthreads();
+        s_part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid > 0 ? s_part[tid - 1] : 0;
+    for (int f = r0; f < r1; ++f) {
+        off[f] = run;
+        run += cnt[f];
+    }
+    if (tid == 1023) *count = s_part[1023];
+}
+
+// One wave per template row f: its spikes in time order at off[f] ...
+// Outputs (index s < cap): xy[2 s] = f, xy[2 s + 1] = t; imax[s] = q;
+// amp[s] = As; adist[j * cap + s] = B[q % nk][iC[j, f]][t] * sign;
+// yct[s] = sum_j (relu(adist_j) / sum relu(adist)) * yc[iC[j, f]];
+// tF[(s * nC + j) * P + p] = sum_m X[iC[j, f], t + m - nt/2] wPCA[p, m].
+__global__ __launch_bounds__(256) void emit_kernel(
+    const unsigned long long *__restrict__ mask, int nw, const int *__restrict__ off,
+    const float *__restrict__ As, const int16_t *__restrict__ Im, const float *__restrict__ B,
+    int C, int ldc, const float *__restrict__ X, int ldx, int T,
+    const int32_t *__restrict__ iC, int nC, int Nfilt, int nk, const float *__restrict__ yc,
+    const float *__restrict__ wPCA, int P, int nt, int cap, int32_t *__restrict__ xy,
+    int32_t *__restrict__ imax, float *__restrict__ amp, float *__restrict__ adist,
+    float *__restrict__ yct, float *__restrict__ tF) {
+    const int lane = threadIdx.x & 63;
+    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= Nfilt) return;
+    const int h = nt / 2;
+    int s = off[f];
+    for (int w = 0; w < nw; ++w) {
+        unsigned long long bits = mask[(size_t)f * nw + w];
+        while (bits) {
+            const int b = __builtin_ctzll(bits);
+            bits &= bits - 1ull;
+            const int t = 64 * w + b;
+            if (s < cap) {
+                const int im = Im[(size_t)f * ldc + t];
+                const int q = (im < 0 ? -im : im) - 1;
+                const float sg = im > 0 ? 1.f : (im < 0 ? -1.f : 0.f);
+                const int k = q % nk;
+                // lane j < nC: adist_j
+                float ad = 0.f, yj = 0.f;
+                if (lane < nC) {
+                    const int c = iC[(size_t)lane * Nfilt + f];
+                    ad = B[((size_t)k * C + c) * ldc + t] * sg;
+                    yj = yc[c];
+                    adist[(size_t)lane * cap + s] = ad;
+                }
+                const float rl = lane < nC ? (ad != ad ? ad : fmaxf(ad, 0.f)) : 0.f;
+                // sum_j relu in j order (lane 0 gathers)
+                float S = 0.f;
+                for (int j = 0; j < nC; ++j) S += __shfl(rl, j, 64);
+                float y = 0.f;
+                for (int j = 0; j < nC; ++j) y += (__shfl(rl, j, 64) / S) * __shfl(yj, j, 64);
+                // features: lane = j * P + p
+                if (lane < nC * P) {
+                    const int j = lane / P, p = lane - j * P;
+                    const int c = iC[(size_t)j * Nfilt + f];
+                    const float *xr = X + (size_t)c * ldx;
+                    float acc = 0.f;
+                    for (int m = 0; m < nt; ++m) {
+                        const int u = t + m - h;        // inside [0, T) for nt0 <= nt - nt/2 - 1
+                        const float x = (u >= 0 && u < T) ? xr[u] : 0.f;
+                        acc = __builtin_fmaf(x, wPCA[p * nt + m], acc);
+                    }
+                    tF[((size_t)s * nC + j) * P + p] = acc;
+                }
+                if (lane == 0) {
+                    xy[2 * (size_t)s] = f;
+                    xy[2 * (size_t)s + 1] = t;
+                    imax[s] = q;
+                    amp[s] = As[(size_t)f * ldc + t];
+                    yct[s] = y;
+                }
+            }
+            ++s;
+        }
+    }
+}
+
+__global__ void clear_int_kernel(int *__restrict__ p) { *p = 0; }
+
+struct DetectLayout {
+    int ldc, nw;
+};
+
+DetectLayout detect_layout(int T) {
+    DetectLayout L;
+    L.ldc = (int)ksm::align_up(T, 256);
+    L.nw = (T + 63) / 64;
+    return L;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ksm_detect_workspace(int C, int T, int nk, int Nfilt) {
+    if (C <= 0 || T <= 0 || nk <= 0 || Nfilt <= 0) return 0;
+    const DetectLayout L = detect_layout(T);
+    return ksm::align_up((size_t)nk * C * L.ldc * sizeof(float), 256) +
+           2 * ksm::align_up((size_t)Nfilt * L.ldc * sizeof(float), 256) +
+           ksm::align_up((size_t)Nfilt * L.ldc * sizeof(int16_t), 256) +
+           ksm::align_up((size_t)Nfilt * L.nw * sizeof(unsigned long long), 256) +
+           2 * ksm::align_up((size_t)Nfilt * sizeof(int), 256) + 512;
+}
+
+int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk, int nt,
+               const int32_t *iC, int nC, const int32_t *iC2, int nC2, const float *weigh,
+               int nsizes, int Nfilt, int nt0, float th, const float *yc, const float *wPCA,
+               int P, int cap, int32_t *xy, int32_t *imax, float *amp, float *adist, float *yct,
+               float *tF, int *count, void *workspace, size_t ws_bytes, void *stream) {
+    if (!X || !wTEMP || !iC || !iC2 || !weigh || !yc || !wPCA || !count || C <= 0 || T <= 0 ||
+        ldx < T || nk <= 0 || nt <= 0 || (nt % 2) == 0 || Nfilt <= 0 || nC2 <= 0 || P <= 0 ||
+        cap < 0 || (cap > 0 && (!xy || !imax || !amp || !adist || !yct || !tF)))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_detect: bad arguments");
+    if (nC <= 0 || nC > kMaxNC || nsizes <= 0 || nsizes > kMaxSizes || nk > kMaxNk ||
+        nC * P > 64 || nt0 < 0 || nt0 > kMaxNt0 || 2 * nt > T)
+        return ksm::fail(KSM_ERR_INVALID,
+                         "ksm_detect: limits nC <= %d, nsizes <= %d, nk <= %d, nC*P <= 64, "
+                         "nt0 <= %d, 2 nt <= T",
+                         kMaxNC, kMaxSizes, kMaxNk, kMaxNt0);
+    if (!workspace || ws_bytes < ksm_detect_workspace(C, T, nk, Nfilt))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_detect: workspace too small");
+    hipStream_t s = ksm::as_stream(stream);
+    const DetectLayout L = detect_layout(T);
+    ksm::Carver cv(workspace);
+    float *B = cv.take<float>((size_t)nk * C * L.ldc);
+    float *As = cv.take<float>((size_t)Nfilt * L.ldc);
+    float *R = cv.take<float>((size_t)Nfilt * L.ldc);
+    int16_t *Im = cv.take<int16_t>((size_t)Nfilt * L.ldc);
+    unsigned long long *mask = cv.take<unsigned long long>((size_t)Nfilt * L.nw);
+    int *off = cv.take<int>(Nfilt);
+    int *cnt = cv.take<int>(Nfilt);
+    int *nonfinite = cv.take<int>(1);
+
+    clear_int_kernel<<<1, 1, 0, s>>>(nonfinite);
+    KSM_LAUNCH_CHECK("clear_int_kernel");
+    int rc = ksm::conv_pc(X, ldx, wTEMP, C, T, nk, nt, B, L.ldc, nonfinite, s);
+    if (rc != KSM_OK) return rc;
+    {
+        ksm::ProfileScope prof("detect_score", ksm::kProfProject, s);
+        prof.launch(score_kernel, dim3((T + 255) / 256, (Nfilt + kScoreF - 1) / kScoreF),
+                    dim3(256), 0u, static_cast<const float *>(B), C, L.ldc, iC, nC, weigh,
+                    nsizes, nk, Nfilt, T, As, Im);
+    }
+    KSM_LAUNCH_CHECK("score_kernel");
+    window_max_kernel<<<dim3((T + kWinT - 1) / kWinT, Nfilt), 256, 0, s>>>(As, L.ldc, T, nt, nt0,
+                                                                         R);
+    KSM_LAUNCH_CHECK("window_max_kernel");
+    {
+        ksm::ProfileScope prof("detect_mask", ksm::kProfProject, s);
+        prof.launch(mask_kernel, dim3((T + 255) / 256, Nfilt), dim3(256), 0u,
+                    static_cast<const float *>(As), static_cast<const float *>(R), L.ldc, T, iC2,
+                    nC2, Nfilt, th, mask, L.nw);
+    }
+    KSM_LAUNCH_CHECK("mask_kernel");
+    row_count_kernel<<<(Nfilt + 3) / 4, 256, 0, s>>>(mask, L.nw, Nfilt, cnt);
+    KSM_LAUNCH_CHECK("row_count_kernel");
+    row_scan_kernel<<<1, 1024, 0, s>>>(cnt, Nfilt, off, count);
+    KSM_LAUNCH_CHECK("row_scan_kernel");
+    {
+        ksm::ProfileScope prof("detect_emit", ksm::kProfProject, s);
+        prof.launch(emit_kernel, dim3((Nfilt + 3) / 4), dim3(256), 0u,
+                    static_cast<const unsigned long long *>(mask), L.nw,
+                    static_cast<const int *>(off), static_cast<const float *>(As),
+                    static_cast<const int16_t *>(Im), static_cast<const float *>(B), C, L.ldc, X,
+                    ldx, T, iC, nC, Nfilt, nk, yc, wPCA, P, nt, cap, xy, imax, amp, adist, yct,
+                    tF);
+    }
+    KSM_LAUNCH_CHECK("emit_kernel");
+    return KSM_OK;
+}
+
+}  // extern "C"

@@ -328,6 +328,27 @@ ProjectLayout project_layout(int C, int T, int K, int P) {
 
 }  // namespace
 
+namespace ksm {
+
+int conv_pc(const float *X, int ldx, const float *W, int C, int T, int P, int nt, float *Bc,
+            int ldc, int *nonfinite, hipStream_t s) {
+    if (P == 6 && nt == 61) {
+        ProfileScope prof("conv_pc", kProfProject, s);
+        dim3 grid((ldc + kConvSpan - 1) / kConvSpan, C);
+        prof.launch(conv_pc_kernel<6, 61>, grid, dim3(kConvThreads), 0u, X, ldx, W, C, T, ldc, Bc,
+                    nonfinite);
+    } else {
+        ProfileScope prof("conv_pc", kProfProject, s);
+        dim3 grid((ldc + 255) / 256, C);
+        prof.launch(conv_pc_generic, grid, dim3(256), 0u, X, ldx, W, P, nt, C, T, ldc, Bc,
+                    nonfinite);
+    }
+    KSM_LAUNCH_CHECK("conv_pc_kernel");
+    return KSM_OK;
+}
+
+}  // namespace ksm
+
 extern "C" {
 
 size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) {
@@ -372,18 +393,10 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
 
     set_flag_kernel<<<1, 1, 0, s>>>(nonfinite);
     KSM_LAUNCH_CHECK("set_flag_kernel");
-    if (P == 6 && nt == 61) {
-        ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
-        dim3 grid((L.ldc + kConvSpan - 1) / kConvSpan, C);
-        prof.launch(conv_pc_kernel<6, 61>, grid, dim3(kConvThreads), 0u, X, ldx, W, C, T, L.ldc, Bc,
-                    nonfinite);
-    } else {
-        ksm::ProfileScope prof("conv_pc", ksm::kProfProject, s);
-        dim3 grid((L.ldc + 255) / 256, C);
-        prof.launch(conv_pc_generic, grid, dim3(256), 0u, X, ldx, W, P, nt, C, T, L.ldc, Bc,
-                    nonfinite);
+    {
+        const int rc = ksm::conv_pc(X, ldx, W, C, T, P, nt, Bc, L.ldc, nonfinite, s);
+        if (rc != KSM_OK) return rc;
     }
-    KSM_LAUNCH_CHECK("conv_pc_kernel");
     if (L.Rp > L.R) {
         zero_rows_kernel<<<64, 256, 0, s>>>(Bc + (size_t)L.R * L.ldc,
                                             (size_t)(L.Rp - L.R) * L.ldc);

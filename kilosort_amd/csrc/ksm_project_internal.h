@@ -5,6 +5,12 @@
 
 namespace ksm {
 
+// Temporal correlation Bc[(p*C + c)*ldc + t] = sum_k W[p,k] X[c, t+k-nt/2]
+// (zero padding; columns T <= t < ldc hold 0), FP32 as the projection's;
+// raises *nonfinite on a non-finite sample.  ldc: a multiple of 256.
+int conv_pc(const float *X, int ldx, const float *W, int C, int T, int P, int nt, float *Bc,
+            int ldc, int *nonfinite, hipStream_t s);
+
 // Workspace bytes of the BF16 projection forms.
 size_t project_bf16_workspace(int C, int T, int K, int P, int precision);
 
