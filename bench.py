@@ -255,6 +255,68 @@ def time_preprocess(device, seed, nbatch=3, reps=5, cpu_baseline=True):
     return out
 
 
+def time_detect(ops, batches, device, reps=10, cpu_baseline=True):
+    """Universal-template detection (spikedetect.template_match + the
+    per-batch features of run) on the bench's 383-channel batches: device time
+    per batch (the spike count is read back each batch, as run does),
+    per-kernel times and the oracle on one batch as its CPU baseline."""
+    from kilosort_amd import _lib, spikedetect as SD
+    lib = _lib.load()
+    C_ = len(ops["xc"])
+    dops = dict(xc=ops["xc"], yc=ops["yc"], kcoords=np.zeros(C_), nt=NTAP, Th_universal=9.0,
+                max_channel_distance=32.0, fs=30000.0, batch_size=NT,
+                wTEMP=ops["wPCA"], wPCA=ops["wPCA"],
+                settings=dict(dmin=None, dminx=32.0, nearest_chans=10, nearest_templates=100,
+                              min_template_size=10.0, template_sizes=5, nt0min=20,
+                              n_templates=6, n_pcs=6))
+    iC, iC2, weigh = SD.prepare(dict(dops), device=device)
+    det = SD._Detector(dops, iC, iC2, weigh, device)
+    for i in range(2):
+        det.run(batches[i % len(batches)])
+    torch.cuda.synchronize()
+    nsp = 0
+    t0 = time.perf_counter()
+    for r in range(reps):
+        nsp += len(det.run(batches[r % len(batches)])[0])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    lib.ksm_profile_reset()
+    lib.ksm_profile_enable(3)
+    for r in range(3):
+        det.run(batches[r % len(batches)])
+    torch.cuda.synchronize()
+    lib.ksm_profile_enable(0)
+    kern = {}
+    for name in ("conv_pc", "detect_score", "detect_mask", "detect_emit"):
+        t_ms, cnt = _lib.profile_query(name)
+        if cnt:
+            kern[name] = t_ms / 3
+    lib.ksm_profile_reset()
+    T = NT + 2 * NTAP
+    nf = int(iC.shape[1])
+    out = dict(value=C_ * T / (ms / 1e3), unit="channel·samples/s", ms_per_batch=ms,
+               spikes_per_batch=nsp / reps, templates=nf, kernel_ms=kern,
+               config=f"{C_} ch, {nf} universal-template positions x 5 sizes x 6 waveforms, "
+                      "nC=10, nC2=100, Th=9")
+    if "detect_score" in kern:
+        # algorithmic work: 2 * nsizes * nk * nC flops per (template, column)
+        flops = 2.0 * 5 * 6 * 10 * nf * T
+        out["detect_score_tflops"] = flops / (kern["detect_score"] / 1e3) / 1e12
+    if cpu_baseline:
+        from oracle import detect_oracle as DO
+        Xc = batches[0].cpu()
+        cops = dict(dops, wTEMP=ops["wPCA"].cpu(), wPCA=ops["wPCA"].cpu())
+        ciC, ciC2, cweigh = DO.setup(dict(cops))
+        t0 = time.perf_counter()
+        DO.detect_batch(Xc, cops, ciC, ciC2, cweigh)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=C_ * T / dt, unit="channel·samples/s",
+                                   cores=torch.get_num_threads(), kind="port",
+                                   sample="1 padded batch: oracle template_match + features "
+                                          "(torch CPU)", seconds=dt)
+    return out
+
+
 def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
     """Template refinement on the workload: spikes and amplitudes from the FP32
     matching of `nbatch` batches, then the residual loss and its gradients for
@@ -468,6 +530,8 @@ def main():
                     help="skip the CPU oracle runs (cpu_baseline and FP64 agreement)")
     ap.add_argument("--preprocess-only", action="store_true",
                     help="time only the batch preprocessing and print its JSON object")
+    ap.add_argument("--detect-only", action="store_true",
+                    help="time only the universal-template detection and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the matching path is sharded by batch "
@@ -490,6 +554,10 @@ def main():
     n_distinct = min(args.steps + args.warmup, 12)
     log(f"generating {n_distinct} synthetic batches")
     rec, ops, batches = make_workload(n_distinct, args.seed, device)
+    if args.detect_only:
+        print(json.dumps(time_detect(ops, batches, device,
+                                     cpu_baseline=not args.no_cpu_baseline)), flush=True)
+        return
     U = torch.from_numpy(rec.U).to(device)
     iCC, _, iU, Ucc = tm.prepare_extract(ops["xc"], ops["yc"], U, 10, 100.0, device=device)
     ctc = tm.prepare_matching(ops, U)
@@ -558,6 +626,11 @@ def main():
     log(f"refinement: residual {refine_res['residual_ms_per_batch']:.3f} ms, "
         f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch")
 
+    log("universal-template detection at 383 channels")
+    det = time_detect(ops, batches, device, cpu_baseline=not args.no_cpu_baseline)
+    log(f"detection: {det['ms_per_batch']:.3f} ms/batch, {det['spikes_per_batch']:.0f} spikes; "
+        "kernels " + ", ".join(f"{k}={v:.3f}" for k, v in det["kernel_ms"].items()))
+
     log("batch preprocessing (CAR, high-pass, whitening) at 383 channels")
     pre = time_preprocess(device, args.seed + 200, cpu_baseline=not args.no_cpu_baseline)
     log(f"preprocessing: {pre['ms_per_batch']:.3f} ms/batch resident, "
@@ -622,6 +695,7 @@ def main():
         "agreement_batch0": agree,
         "refinement": refine_res,
         "preprocess": pre,
+        "detect": det,
         "ragged": ragged,
         "cpu_baseline": cpu,
     }

@@ -327,7 +327,7 @@ size_t ksm_detect_workspace(int C, int T, int nk, int Nfilt);
  * sign, yct [cap] (relu(adist)-weighted y of the channels), tF
  * [cap][nC][P] (window t - nt/2 .. t + nt/2 of X on the channels, times wPCA);
  * *count (device) = the number of spikes, also when it exceeds cap.
- * Limits: nC <= 32, nsizes <= 16, nk <= 16, nC*P <= 64, nt0 <= 256.
+ * Limits: nC <= 16, nsizes <= 8, nsizes*nC <= 64, nk <= 8, nC*P <= 64, nt0 <= 256.
  */
 int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk, int nt,
                const int32_t *iC, int nC, const int32_t *iC2, int nC2, const float *weigh,

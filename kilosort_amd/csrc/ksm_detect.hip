@@ -13,7 +13,8 @@
 //     [nt, T - nt) count as 0 (the reference zeroes them before its
 //     max_pool1d; the maxima are exact, so their order does not matter)
 //   spikes: As == Amaxs and As > Th, in (f, t) order (torch.nonzero).
-// Kernels: window_max_kernel (R[f, t] = the time-window maximum of row f),
+// Kernels: score_kernel (per template group and column tile, the group's
+// channel range of B staged in LDS), window_max_kernel (R[f, t] = the time-window maximum of row f),
 // mask_kernel (per (f, t): the maximum of R over the neighbours, stopping at
 // the first neighbour above As; one 64-bit mask word per 64 columns),
 // row_count_kernel + row_scan_kernel (offsets per row), emit_kernel (one wave per
@@ -25,50 +26,175 @@
 
 namespace {
 
-constexpr int kScoreF = 8;        // templates per score block
-constexpr int kMaxSizes = 16;     // template sizes (nsizes)
-constexpr int kMaxNk = 16;        // universal waveforms (nk)
-constexpr int kMaxNC = 32;        // nearest channels (nC)
-constexpr int kWinT = 256;        // columns per window-max block
+constexpr int kScoreG = 32;       // templates per score block
+constexpr int kScoreT = 32;       // columns per score block
+constexpr int kScoreLds = 40960;  // bytes of LDS for the staged channel range
+constexpr int kMaxW = 64;         // nsizes * nC
+constexpr int kMaxSizes = 8;      // template sizes (nsizes)
+constexpr int kMaxNk = 8;         // universal waveforms (nk)
+constexpr int kMaxNC = 16;        // nearest channels (nC)
+constexpr int kWinT = 1024;       // columns per window-max block
 constexpr int kMaxNt0 = 256;
 
-// As[f][t], Im[f][t] for templates [f0, f0 + kScoreF) and columns of the block.
+// As[f][t], Im[f][t] for a group of kScoreG consecutive templates and
+// kScoreT columns.  Consecutive template positions are neighbours on the
+// probe, so the group's channels lie in a short index range [cmin, cmax]:
+// B[k][cmin..cmax][tile] is staged once in LDS and every template of the
+// group reads its nC channels from there (a group whose range exceeds
+// max_span channels reads B from global memory instead).  Thread: one column, the
+// group's templates slot, slot + 8, ...
+// NS, NCH, NK > 0: nsizes, nC and nk fixed at compile time (the template's
+// weights and its NS x NK sums live in registers); 0: the sizes passed at
+// run time.
+template <int NS, int NCH, int NK>
 __global__ __launch_bounds__(256) void score_kernel(
-    const float *__restrict__ B, int C, int ldc, const int32_t *__restrict__ iC, int nC,
-    const float *__restrict__ weigh, int nsizes, int nk, int Nfilt, int T,
+    const float *__restrict__ B, int C, int ldc, const int32_t *__restrict__ iC, int nC_,
+    const float *__restrict__ weigh, int nsizes_, int nk, int Nfilt, int T, int max_span,
     float *__restrict__ As, int16_t *__restrict__ Im) {
-    __shared__ float s_w[kScoreF][kMaxSizes * kMaxNC];
-    __shared__ int s_c[kScoreF][kMaxNC];
-    const int f0 = blockIdx.y * kScoreF;
-    for (int e = threadIdx.x; e < kScoreF * nsizes * nC; e += 256) {
+    const int nC = NCH > 0 ? NCH : nC_;
+    const int nsizes = NS > 0 ? NS : nsizes_;
+    constexpr int MC = NCH > 0 ? NCH : kMaxNC;
+    constexpr int MS = NS > 0 ? NS : kMaxSizes;
+    extern __shared__ float s_b[];                  // [nk][span][kScoreT], span <= max_span
+    __shared__ float s_w[kScoreG][kMaxW];
+    __shared__ int s_c[kScoreG][kMaxNC];
+    __shared__ int s_lo, s_hi;
+    // blocks of neighbouring groups (sharing channels) are launched together
+    const int f0 = blockIdx.x * kScoreG, t0 = blockIdx.y * kScoreT;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        s_lo = 0x7fffffff;
+        s_hi = -1;
+    }
+    __syncthreads();
+    int lo = 0x7fffffff, hi = -1;
+    for (int e = tid; e < kScoreG * nC; e += 256) {
+        const int g = e / nC, j = e - g * nC;
+        const int f = f0 + g;
+        const int c = f < Nfilt ? iC[(size_t)j * Nfilt + f] : -1;
+        s_c[g][j] = c;
+        if (c >= 0) {
+            lo = min(lo, c);
+            hi = max(hi, c);
+        }
+    }
+    for (int e = tid; e < kScoreG * nsizes * nC; e += 256) {
         const int g = e / (nsizes * nC), r = e - g * nsizes * nC;
         const int i = r / nC, j = r - i * nC;
         const int f = f0 + g;
         s_w[g][i * nC + j] = f < Nfilt ? weigh[((size_t)i * nC + j) * Nfilt + f] : 0.f;
     }
-    for (int e = threadIdx.x; e < kScoreF * nC; e += 256) {
-        const int g = e / nC, j = e - g * nC;
-        const int f = f0 + g;
-        s_c[g][j] = f < Nfilt ? iC[(size_t)j * Nfilt + f] : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((tid & 63) == 0) {
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
     }
     __syncthreads();
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int cmin = s_lo, span = s_hi - s_lo + 1;
+    const bool staged = span <= max_span;
+    if (staged) {
+        for (int e = tid; e < nk * span * kScoreT; e += 256) {
+            const int k = e / (span * kScoreT), r = e - k * span * kScoreT;
+            const int c = r / kScoreT, u = r - c * kScoreT;
+            const int t = t0 + u;
+            s_b[e] = t < ldc ? B[((size_t)k * C + cmin + c) * ldc + t] : 0.f;
+        }
+    }
+    __syncthreads();
+    const int u = tid & (kScoreT - 1), slot = tid / kScoreT;
+    const int t = t0 + u;
     if (t >= T) return;
-    for (int g = 0; g < kScoreF; ++g) {
+    for (int g = slot; g < kScoreG; g += 256 / kScoreT) {
         const int f = f0 + g;
         if (f >= Nfilt) break;
         float best = -1.f, bval = 0.f;
         int bq = 0;
-        for (int k = 0; k < nk; ++k) {
-            float b[kMaxNC];
+        float wr[NS > 0 ? NS : 1][NCH > 0 ? NCH : 1];
+        int cr[NCH > 0 ? NCH : 1];
+        if constexpr (NS > 0 && NCH > 0) {
 #pragma unroll
-            for (int j = 0; j < kMaxNC; ++j)
-                if (j < nC) b[j] = B[((size_t)k * C + s_c[g][j]) * ldc + t];
-            for (int i = 0; i < nsizes; ++i) {
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int j = 0; j < NCH; ++j) wr[i][j] = s_w[g][i * NCH + j];
+#pragma unroll
+            for (int j = 0; j < NCH; ++j) cr[j] = s_c[g][j] - cmin;
+        }
+        if constexpr (NS > 0 && NCH > 0 && NK > 0) {
+            // all NS x NK sums in registers, then the maximum in q order
+            float A[NS][NK];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                float b[NCH];
+                if (staged) {
+#pragma unroll
+                    for (int j = 0; j < NCH; ++j) b[j] = s_b[(k * span + cr[j]) * kScoreT + u];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < NCH; ++j)
+                        b[j] = B[((size_t)k * C + cr[j] + cmin) * ldc + t];
+                }
+#pragma unroll
+                for (int i = 0; i < NS; ++i) {
+                    float a = 0.f;
+#pragma unroll
+                    for (int j = 0; j < NCH; ++j) a = __builtin_fmaf(wr[i][j], b[j], a);
+                    A[i][k] = a;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NS; ++i)
+#pragma unroll
+                for (int k = 0; k < NK; ++k) {
+                    const float m = fabsf(A[i][k]);
+                    // strictly larger (the first q on ties); the first NaN wins
+                    const bool take = m > best || (m != m && best == best);
+                    best = take ? m : best;
+                    bq = take ? i * NK + k : bq;
+                    bval = take ? A[i][k] : bval;
+                }
+            const int sg = bval > 0.f ? 1 : (bval < 0.f ? -1 : 0);
+            As[(size_t)f * ldc + t] = best;
+            Im[(size_t)f * ldc + t] = (int16_t)((1 + bq) * sg);
+            continue;
+        }
+        for (int k = 0; k < nk; ++k) {
+            float b[MC];
+            // the two sources in separate loops: a select between an LDS and a
+            // global address would become generic (flat) loads
+            if (staged) {
+#pragma unroll
+                for (int j = 0; j < MC; ++j)
+                    if (j < nC) {
+                        int c;
+                        if constexpr (NS > 0 && NCH > 0) c = cr[j];
+                        else c = s_c[g][j] - cmin;
+                        b[j] = s_b[(k * span + c) * kScoreT + u];
+                    }
+            } else {
+#pragma unroll
+                for (int j = 0; j < MC; ++j)
+                    if (j < nC) {
+                        int c;
+                        if constexpr (NS > 0 && NCH > 0) c = cr[j] + cmin;
+                        else c = s_c[g][j];
+                        b[j] = B[((size_t)k * C + c) * ldc + t];
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < MS; ++i) {
+                if (i >= nsizes) break;
                 float a = 0.f;
 #pragma unroll
-                for (int j = 0; j < kMaxNC; ++j)
-                    if (j < nC) a = __builtin_fmaf(s_w[g][i * nC + j], b[j], a);
+                for (int j = 0; j < MC; ++j)
+                    if (j < nC) {
+                        float wv;
+                        if constexpr (NS > 0 && NCH > 0) wv = wr[i][j];
+                        else wv = s_w[g][i * nC + j];
+                        a = __builtin_fmaf(wv, b[j], a);
+                    }
                 const float m = fabsf(a);
                 const int q = i * nk + k;
                 // the first q on ties (torch.max); NaN wins and stays
@@ -87,7 +213,7 @@ __global__ __launch_bounds__(256) void score_kernel(
 
 // R[f][t] = max of As[f][s] over s in [t - nt0, t + nt0] and [nt, T - nt),
 // 0 if that range is empty (the reference's zeroed columns).  Block: row f,
-// kWinT columns, the window staged in LDS.
+// kWinT columns (four per thread), the window staged in LDS.
 __global__ __launch_bounds__(256) void window_max_kernel(const float *__restrict__ As, int ldc,
                                                          int T, int nt, int nt0,
                                                          float *__restrict__ R) {
@@ -100,14 +226,17 @@ __global__ __launch_bounds__(256) void window_max_kernel(const float *__restrict
         s_a[e] = (s >= nt && s < T - nt) ? row[s] : 0.f;
     }
     __syncthreads();
-    const int t = t0 + threadIdx.x;
-    if (t >= T) return;
-    float m = 0.f;
-    for (int d = 0; d <= 2 * nt0; ++d) m = fmaxf(m, s_a[threadIdx.x + d]);
-    R[(size_t)f * ldc + t] = m;
+    for (int r = 0; r < kWinT / 256; ++r) {
+        const int u = threadIdx.x + 256 * r, t = t0 + u;
+        if (t >= T) return;
+        float m = 0.f;
+        for (int d = 0; d <= 2 * nt0; ++d) m = fmaxf(m, s_a[u + d]);
+        R[(size_t)f * ldc + t] = m;
+    }
 }
 
-// mask[f][w] bit b: column t = 64 w + b is a spike of template f.
+// mask[f][w] bit b: column t = 64 w + b is a spike of template f.  Block:
+// row f, 1024 columns; each wave writes four mask words.
 __global__ __launch_bounds__(256) void mask_kernel(const float *__restrict__ As,
                                                    const float *__restrict__ R, int ldc, int T,
                                                    const int32_t *__restrict__ iC2, int nC2,
@@ -115,24 +244,26 @@ __global__ __launch_bounds__(256) void mask_kernel(const float *__restrict__ As,
                                                    unsigned long long *__restrict__ mask,
                                                    int nw) {
     const int f = blockIdx.y;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    bool hit = false;
-    if (t < T) {
-        const float a = As[(size_t)f * ldc + t];
-        if (a > th) {
-            float m = -1.f;
-            for (int n = 0; n < nC2; ++n) {
-                const float r = R[(size_t)iC2[(size_t)n * Nfilt + f] * ldc + t];
-                m = fmaxf(m, r);
-                if (m > a) break;            // already not the maximum
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = 0; r < 4; ++r) {
+        const int w = blockIdx.x * 16 + wave * 4 + r;       // mask word
+        const int t = 64 * w + lane;
+        bool hit = false;
+        if (t < T) {
+            const float a = As[(size_t)f * ldc + t];
+            if (a > th) {
+                float m = -1.f;
+                for (int n = 0; n < nC2; ++n) {
+                    const float v = R[(size_t)iC2[(size_t)n * Nfilt + f] * ldc + t];
+                    m = fmaxf(m, v);
+                    if (m > a) break;        // already not the maximum
+                }
+                hit = m == a;
             }
-            hit = m == a;
         }
+        const unsigned long long bits = __ballot(hit);
+        if (lane == 0 && w < nw) mask[(size_t)f * nw + w] = bits;
     }
-    const unsigned long long bits = __ballot(hit);
-    const int w = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 6;
-    if (lane == 0 && w < nw) mask[(size_t)f * nw + w] = bits;
 }
 
 // cnt[f] = the spikes of row f (one wave per row).
@@ -173,6 +304,52 @@ __global__ __launch_bounds__(1024) void row_scan_kernel(const int *__restrict__ 
     if (tid == 1023) *count = s_part[1023];
 }
 
+// Spike s = (f, t) of a row, written by the whole wave (lane j < nC: adist
+// and y; lane j * P + p: feature p of channel j).
+__device__ __forceinline__ void emit_one(
+    int f, int t, int s, const float *__restrict__ As, const int16_t *__restrict__ Im,
+    const float *__restrict__ B, int C, int ldc, const float *__restrict__ X, int ldx, int T,
+    const int32_t *__restrict__ iC, int nC, int Nfilt, int nk, const float *__restrict__ yc,
+    const float *__restrict__ wPCA, int P, int nt, int h, int cap, int32_t *__restrict__ xy,
+    int32_t *__restrict__ imax, float *__restrict__ amp, float *__restrict__ adist,
+    float *__restrict__ yct, float *__restrict__ tF, int lane) {
+    const int im = Im[(size_t)f * ldc + t];
+    const int q = (im < 0 ? -im : im) - 1;
+    const float sg = im > 0 ? 1.f : (im < 0 ? -1.f : 0.f);
+    const int k = q % nk;
+    float ad = 0.f, yj = 0.f;
+    if (lane < nC) {
+        const int c = iC[(size_t)lane * Nfilt + f];
+        ad = B[((size_t)k * C + c) * ldc + t] * sg;
+        yj = yc[c];
+        adist[(size_t)lane * cap + s] = ad;
+    }
+    const float rl = lane < nC ? (ad != ad ? ad : fmaxf(ad, 0.f)) : 0.f;   // relu keeps NaN
+    float S = 0.f;                                  // sum of relu(adist) in channel order
+    for (int j = 0; j < nC; ++j) S += __shfl(rl, j, 64);
+    float y = 0.f;
+    for (int j = 0; j < nC; ++j) y += (__shfl(rl, j, 64) / S) * __shfl(yj, j, 64);
+    if (lane < nC * P) {
+        const int j = lane / P, p = lane - j * P;
+        const int c = iC[(size_t)j * Nfilt + f];
+        const float *xr = X + (size_t)c * ldx;
+        float acc = 0.f;
+        for (int m = 0; m < nt; ++m) {
+            const int u = t + m - h;                // inside [0, T) for nt0 <= nt - nt/2 - 1
+            const float x = (u >= 0 && u < T) ? xr[u] : 0.f;
+            acc = __builtin_fmaf(x, wPCA[p * nt + m], acc);
+        }
+        tF[((size_t)s * nC + j) * P + p] = acc;
+    }
+    if (lane == 0) {
+        xy[2 * (size_t)s] = f;
+        xy[2 * (size_t)s + 1] = t;
+        imax[s] = q;
+        amp[s] = As[(size_t)f * ldc + t];
+        yct[s] = y;
+    }
+}
+
 // One wave per template row f: its spikes in time order at off[f] ...
 // Outputs (index s < cap): xy[2 s] = f, xy[2 s + 1] = t; imax[s] = q;
 // amp[s] = As; adist[j * cap + s] = B[q % nk][iC[j, f]][t] * sign;
@@ -190,55 +367,36 @@ __global__ __launch_bounds__(256) void emit_kernel(
     const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (f >= Nfilt) return;
     const int h = nt / 2;
-    int s = off[f];
-    for (int w = 0; w < nw; ++w) {
-        unsigned long long bits = mask[(size_t)f * nw + w];
-        while (bits) {
-            const int b = __builtin_ctzll(bits);
-            bits &= bits - 1ull;
-            const int t = 64 * w + b;
-            if (s < cap) {
-                const int im = Im[(size_t)f * ldc + t];
-                const int q = (im < 0 ? -im : im) - 1;
-                const float sg = im > 0 ? 1.f : (im < 0 ? -1.f : 0.f);
-                const int k = q % nk;
-                // lane j < nC: adist_j
-                float ad = 0.f, yj = 0.f;
-                if (lane < nC) {
-                    const int c = iC[(size_t)lane * Nfilt + f];
-                    ad = B[((size_t)k * C + c) * ldc + t] * sg;
-                    yj = yc[c];
-                    adist[(size_t)lane * cap + s] = ad;
-                }
-                const float rl = lane < nC ? (ad != ad ? ad : fmaxf(ad, 0.f)) : 0.f;
-                // sum_j relu in j order (lane 0 gathers)
-                float S = 0.f;
-                for (int j = 0; j < nC; ++j) S += __shfl(rl, j, 64);
-                float y = 0.f;
-                for (int j = 0; j < nC; ++j) y += (__shfl(rl, j, 64) / S) * __shfl(yj, j, 64);
-                // features: lane = j * P + p
-                if (lane < nC * P) {
-                    const int j = lane / P, p = lane - j * P;
-                    const int c = iC[(size_t)j * Nfilt + f];
-                    const float *xr = X + (size_t)c * ldx;
-                    float acc = 0.f;
-                    for (int m = 0; m < nt; ++m) {
-                        const int u = t + m - h;        // inside [0, T) for nt0 <= nt - nt/2 - 1
-                        const float x = (u >= 0 && u < T) ? xr[u] : 0.f;
-                        acc = __builtin_fmaf(x, wPCA[p * nt + m], acc);
-                    }
-                    tF[((size_t)s * nC + j) * P + p] = acc;
-                }
-                if (lane == 0) {
-                    xy[2 * (size_t)s] = f;
-                    xy[2 * (size_t)s + 1] = t;
-                    imax[s] = q;
-                    amp[s] = As[(size_t)f * ldc + t];
-                    yct[s] = y;
-                }
-            }
-            ++s;
+    int base = off[f];
+    for (int w0 = 0; w0 < nw; w0 += 64) {
+        // lanes load 64 mask words; exclusive prefix of their popcounts
+        const int wl = w0 + lane;
+        const unsigned long long myw = wl < nw ? mask[(size_t)f * nw + wl] : 0ull;
+        const int pc = __popcll(myw);
+        int incl = pc;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
         }
+        const int excl = incl - pc;
+        const int total = __shfl(incl, 63, 64);
+        unsigned long long busy = __ballot(myw != 0ull);
+        while (busy) {
+            const int src = __builtin_ctzll(busy);
+            busy &= busy - 1ull;
+            unsigned long long bits = __shfl(myw, src, 64);
+            int s = base + __shfl(excl, src, 64);
+            while (bits) {
+                const int b = __builtin_ctzll(bits);
+                bits &= bits - 1ull;
+                const int t = 64 * (w0 + src) + b;
+                if (s < cap) emit_one(f, t, s, As, Im, B, C, ldc, X, ldx, T, iC, nC, Nfilt, nk,
+                                      yc, wPCA, P, nt, h, cap, xy, imax, amp, adist, yct, tF,
+                                      lane);
+                ++s;
+            }
+        }
+        base += total;
     }
 }
 
@@ -279,11 +437,12 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
         cap < 0 || (cap > 0 && (!xy || !imax || !amp || !adist || !yct || !tF)))
         return ksm::fail(KSM_ERR_INVALID, "ksm_detect: bad arguments");
     if (nC <= 0 || nC > kMaxNC || nsizes <= 0 || nsizes > kMaxSizes || nk > kMaxNk ||
+        nsizes * nC > kMaxW ||
         nC * P > 64 || nt0 < 0 || nt0 > kMaxNt0 || 2 * nt > T)
         return ksm::fail(KSM_ERR_INVALID,
-                         "ksm_detect: limits nC <= %d, nsizes <= %d, nk <= %d, nC*P <= 64, "
-                         "nt0 <= %d, 2 nt <= T",
-                         kMaxNC, kMaxSizes, kMaxNk, kMaxNt0);
+                         "ksm_detect: limits nC <= %d, nsizes <= %d, nsizes*nC <= %d, nk <= %d, "
+                         "nC*P <= 64, nt0 <= %d, 2 nt <= T",
+                         kMaxNC, kMaxSizes, kMaxW, kMaxNk, kMaxNt0);
     if (!workspace || ws_bytes < ksm_detect_workspace(C, T, nk, Nfilt))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_detect: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
@@ -304,9 +463,17 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
     if (rc != KSM_OK) return rc;
     {
         ksm::ProfileScope prof("detect_score", ksm::kProfProject, s);
-        prof.launch(score_kernel, dim3((T + 255) / 256, (Nfilt + kScoreF - 1) / kScoreF),
-                    dim3(256), 0u, static_cast<const float *>(B), C, L.ldc, iC, nC, weigh,
-                    nsizes, nk, Nfilt, T, As, Im);
+        const int max_span = kScoreLds / (nk * kScoreT * (int)sizeof(float));
+        const unsigned shm = (unsigned)(nk * max_span * kScoreT * sizeof(float));
+        const dim3 grid((Nfilt + kScoreG - 1) / kScoreG, (T + kScoreT - 1) / kScoreT);
+        if (nsizes == 5 && nC == 10 && nk == 6)          // the reference's defaults
+            prof.launch(score_kernel<5, 10, 6>, grid, dim3(256), shm,
+                        static_cast<const float *>(B), C, L.ldc, iC, nC, weigh, nsizes, nk, Nfilt,
+                        T, max_span, As, Im);
+        else
+            prof.launch(score_kernel<0, 0, 0>, grid, dim3(256), shm,
+                        static_cast<const float *>(B), C, L.ldc, iC, nC, weigh, nsizes, nk, Nfilt,
+                        T, max_span, As, Im);
     }
     KSM_LAUNCH_CHECK("score_kernel");
     window_max_kernel<<<dim3((T + kWinT - 1) / kWinT, Nfilt), 256, 0, s>>>(As, L.ldc, T, nt, nt0,
@@ -314,7 +481,7 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
     KSM_LAUNCH_CHECK("window_max_kernel");
     {
         ksm::ProfileScope prof("detect_mask", ksm::kProfProject, s);
-        prof.launch(mask_kernel, dim3((T + 255) / 256, Nfilt), dim3(256), 0u,
+        prof.launch(mask_kernel, dim3((T + 1023) / 1024, Nfilt), dim3(256), 0u,
                     static_cast<const float *>(As), static_cast<const float *>(R), L.ldc, T, iC2,
                     nC2, Nfilt, th, mask, L.nw);
     }
