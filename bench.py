@@ -287,7 +287,7 @@ def time_detect(ops, batches, device, reps=10, cpu_baseline=True):
     torch.cuda.synchronize()
     lib.ksm_profile_enable(0)
     kern = {}
-    for name in ("conv_pc", "detect_score", "detect_mask", "detect_emit"):
+    for name in ("conv_pc", "detect_score", "detect_mask", "detect_emit", "detect_features"):
         t_ms, cnt = _lib.profile_query(name)
         if cnt:
             kern[name] = t_ms / 3

@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kScoreG = 32;       // templates per score block
 constexpr int kScoreT = 32;       // columns per score block
-constexpr int kScoreLds = 40960;  // bytes of LDS for the staged channel range
+constexpr int kScoreLds = 24576;  // bytes of LDS for the staged channel range
 constexpr int kMaxW = 64;         // nsizes * nC
 constexpr int kMaxSizes = 8;      // template sizes (nsizes)
 constexpr int kMaxNk = 8;         // universal waveforms (nk)
@@ -44,8 +44,7 @@ constexpr int kMaxNt0 = 256;
 // max_span channels reads B from global memory instead).  Thread: one column, the
 // group's templates slot, slot + 8, ...
 // NS, NCH, NK > 0: nsizes, nC and nk fixed at compile time (the template's
-// weights and its NS x NK sums live in registers); 0: the sizes passed at
-// run time.
+// NK x NCH correlations live in registers); 0: the sizes passed at run time.
 template <int NS, int NCH, int NK>
 __global__ __launch_bounds__(256) void score_kernel(
     const float *__restrict__ B, int C, int ldc, const int32_t *__restrict__ iC, int nC_,
@@ -95,13 +94,13 @@ __global__ __launch_bounds__(256) void score_kernel(
     __syncthreads();
     const int cmin = s_lo, span = s_hi - s_lo + 1;
     const bool staged = span <= max_span;
-    if (staged) {
-        for (int e = tid; e < nk * span * kScoreT; e += 256) {
-            const int k = e / (span * kScoreT), r = e - k * span * kScoreT;
-            const int c = r / kScoreT, u = r - c * kScoreT;
-            const int t = t0 + u;
-            s_b[e] = t < ldc ? B[((size_t)k * C + cmin + c) * ldc + t] : 0.f;
-        }
+    if (staged) {                                   // thread: column tid % 32, rows tid / 32 + 8 n
+        const int uu = tid & (kScoreT - 1), r0 = tid / kScoreT;
+        const int t = t0 + uu;
+        for (int k = 0; k < nk; ++k)
+            for (int c = r0; c < span; c += 256 / kScoreT)
+                s_b[(k * span + c) * kScoreT + uu] =
+                    t < ldc ? B[((size_t)k * C + cmin + c) * ldc + t] : 0.f;
     }
     __syncthreads();
     const int u = tid & (kScoreT - 1), slot = tid / kScoreT;
@@ -112,49 +111,44 @@ __global__ __launch_bounds__(256) void score_kernel(
         if (f >= Nfilt) break;
         float best = -1.f, bval = 0.f;
         int bq = 0;
-        float wr[NS > 0 ? NS : 1][NCH > 0 ? NCH : 1];
         int cr[NCH > 0 ? NCH : 1];
         if constexpr (NS > 0 && NCH > 0) {
-#pragma unroll
-            for (int i = 0; i < NS; ++i)
-#pragma unroll
-                for (int j = 0; j < NCH; ++j) wr[i][j] = s_w[g][i * NCH + j];
 #pragma unroll
             for (int j = 0; j < NCH; ++j) cr[j] = s_c[g][j] - cmin;
         }
         if constexpr (NS > 0 && NCH > 0 && NK > 0) {
-            // all NS x NK sums in registers, then the maximum in q order
-            float A[NS][NK];
+            // the NK x NCH correlations of the template's channels in
+            // registers; sizes outer, waveforms inner (q = i NK + k
+            // increasing, so a strictly larger |A| keeps the first maximum);
+            // one size's weights at a time from LDS
+            float b[NK][NCH];
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
-                float b[NCH];
                 if (staged) {
 #pragma unroll
-                    for (int j = 0; j < NCH; ++j) b[j] = s_b[(k * span + cr[j]) * kScoreT + u];
+                    for (int j = 0; j < NCH; ++j) b[k][j] = s_b[(k * span + cr[j]) * kScoreT + u];
                 } else {
 #pragma unroll
                     for (int j = 0; j < NCH; ++j)
-                        b[j] = B[((size_t)k * C + cr[j] + cmin) * ldc + t];
-                }
-#pragma unroll
-                for (int i = 0; i < NS; ++i) {
-                    float a = 0.f;
-#pragma unroll
-                    for (int j = 0; j < NCH; ++j) a = __builtin_fmaf(wr[i][j], b[j], a);
-                    A[i][k] = a;
+                        b[k][j] = B[((size_t)k * C + cr[j] + cmin) * ldc + t];
                 }
             }
 #pragma unroll
-            for (int i = 0; i < NS; ++i)
+            for (int i = 0; i < NS; ++i) {
+                float w[NCH];
+#pragma unroll
+                for (int j = 0; j < NCH; ++j) w[j] = s_w[g][i * NCH + j];
 #pragma unroll
                 for (int k = 0; k < NK; ++k) {
-                    const float m = fabsf(A[i][k]);
-                    // strictly larger (the first q on ties); the first NaN wins
-                    const bool take = m > best || (m != m && best == best);
-                    best = take ? m : best;
+                    float a = 0.f;
+#pragma unroll
+                    for (int j = 0; j < NCH; ++j) a = __builtin_fmaf(w[j], b[k][j], a);
+                    const bool take = fabsf(a) > best;
+                    best = take ? fabsf(a) : best;
                     bq = take ? i * NK + k : bq;
-                    bval = take ? A[i][k] : bval;
+                    bval = take ? a : bval;
                 }
+            }
             const int sg = bval > 0.f ? 1 : (bval < 0.f ? -1 : 0);
             As[(size_t)f * ldc + t] = best;
             Im[(size_t)f * ldc + t] = (int16_t)((1 + bq) * sg);
@@ -189,12 +183,7 @@ __global__ __launch_bounds__(256) void score_kernel(
                 float a = 0.f;
 #pragma unroll
                 for (int j = 0; j < MC; ++j)
-                    if (j < nC) {
-                        float wv;
-                        if constexpr (NS > 0 && NCH > 0) wv = wr[i][j];
-                        else wv = s_w[g][i * nC + j];
-                        a = __builtin_fmaf(wv, b[j], a);
-                    }
+                    if (j < nC) a = __builtin_fmaf(s_w[g][i * nC + j], b[j], a);
                 const float m = fabsf(a);
                 const int q = i * nk + k;
                 // the first q on ties (torch.max); NaN wins and stays
@@ -304,15 +293,15 @@ __global__ __launch_bounds__(1024) void row_scan_kernel(const int *__restrict__ 
     if (tid == 1023) *count = s_part[1023];
 }
 
-// Spike s = (f, t) of a row, written by the whole wave (lane j < nC: adist
-// and y; lane j * P + p: feature p of channel j).
+// The outputs of spike s = (f, t) except its position, written by the whole
+// wave (lane j < nC: adist and y; lane j * P + p: feature p of channel j).
 __device__ __forceinline__ void emit_one(
     int f, int t, int s, const float *__restrict__ As, const int16_t *__restrict__ Im,
     const float *__restrict__ B, int C, int ldc, const float *__restrict__ X, int ldx, int T,
     const int32_t *__restrict__ iC, int nC, int Nfilt, int nk, const float *__restrict__ yc,
-    const float *__restrict__ wPCA, int P, int nt, int h, int cap, int32_t *__restrict__ xy,
-    int32_t *__restrict__ imax, float *__restrict__ amp, float *__restrict__ adist,
-    float *__restrict__ yct, float *__restrict__ tF, int lane) {
+    const float *__restrict__ wPCA, int P, int nt, int h, int cap, int32_t *__restrict__ imax,
+    float *__restrict__ amp, float *__restrict__ adist, float *__restrict__ yct,
+    float *__restrict__ tF, int lane) {
     const int im = Im[(size_t)f * ldc + t];
     const int q = (im < 0 ? -im : im) - 1;
     const float sg = im > 0 ? 1.f : (im < 0 ? -1.f : 0.f);
@@ -342,15 +331,33 @@ __device__ __forceinline__ void emit_one(
         tF[((size_t)s * nC + j) * P + p] = acc;
     }
     if (lane == 0) {
-        xy[2 * (size_t)s] = f;
-        xy[2 * (size_t)s + 1] = t;
         imax[s] = q;
         amp[s] = As[(size_t)f * ldc + t];
         yct[s] = y;
     }
 }
 
-// One wave per template row f: its spikes in time order at off[f] ...
+// One wave per spike s < min(*count, cap) (grid-stride): its outputs from
+// the position xy[s] written by emit_kernel.
+__global__ __launch_bounds__(256) void spike_features_kernel(
+    const int *__restrict__ count, const float *__restrict__ As, const int16_t *__restrict__ Im,
+    const float *__restrict__ B, int C, int ldc, const float *__restrict__ X, int ldx, int T,
+    const int32_t *__restrict__ iC, int nC, int Nfilt, int nk, const float *__restrict__ yc,
+    const float *__restrict__ wPCA, int P, int nt, int cap, const int32_t *__restrict__ xy,
+    int32_t *__restrict__ imax, float *__restrict__ amp, float *__restrict__ adist,
+    float *__restrict__ yct, float *__restrict__ tF) {
+    const int lane = threadIdx.x & 63;
+    const int n = min(*count, cap);
+    for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < n; s += gridDim.x * 4) {
+        const int f = xy[2 * (size_t)s], t = xy[2 * (size_t)s + 1];
+        emit_one(f, t, s, As, Im, B, C, ldc, X, ldx, T, iC, nC, Nfilt, nk, yc, wPCA, P, nt,
+                 nt / 2, cap, imax, amp, adist, yct, tF, lane);
+    }
+}
+
+// One wave per template row f: the positions of its spikes, in time order,
+// at off[f] ... (the mask words of the row are scanned 64 at a time).  The
+// other outputs (spike_features_kernel):
 // Outputs (index s < cap): xy[2 s] = f, xy[2 s + 1] = t; imax[s] = q;
 // amp[s] = As; adist[j * cap + s] = B[q % nk][iC[j, f]][t] * sign;
 // yct[s] = sum_j (relu(adist_j) / sum relu(adist)) * yc[iC[j, f]];
@@ -390,9 +397,10 @@ __global__ __launch_bounds__(256) void emit_kernel(
                 const int b = __builtin_ctzll(bits);
                 bits &= bits - 1ull;
                 const int t = 64 * (w0 + src) + b;
-                if (s < cap) emit_one(f, t, s, As, Im, B, C, ldc, X, ldx, T, iC, nC, Nfilt, nk,
-                                      yc, wPCA, P, nt, h, cap, xy, imax, amp, adist, yct, tF,
-                                      lane);
+                if (s < cap && lane == 0) {
+                    xy[2 * (size_t)s] = f;
+                    xy[2 * (size_t)s + 1] = t;
+                }
                 ++s;
             }
         }
@@ -500,6 +508,16 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
                     tF);
     }
     KSM_LAUNCH_CHECK("emit_kernel");
+    if (cap > 0) {
+        ksm::ProfileScope prof("detect_features", ksm::kProfProject, s);
+        const int nblk = (min(cap, 16384) + 3) / 4;
+        prof.launch(spike_features_kernel, dim3(nblk), dim3(256), 0u,
+                    static_cast<const int *>(count), static_cast<const float *>(As),
+                    static_cast<const int16_t *>(Im), static_cast<const float *>(B), C, L.ldc, X,
+                    ldx, T, iC, nC, Nfilt, nk, yc, wPCA, P, nt, cap,
+                    static_cast<const int32_t *>(xy), imax, amp, adist, yct, tF);
+        KSM_LAUNCH_CHECK("spike_features_kernel");
+    }
     return KSM_OK;
 }
 
