@@ -336,6 +336,41 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
                float *yct, float *tF, int *count, void *workspace, size_t ws_bytes,
                void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Template Gram consumers (post-processing of learned templates).
+ */
+
+/* Workspace bytes of ksm_similarity. */
+size_t ksm_similarity_workspace(int K, int C, int P, int nt);
+
+/*
+ * Replaces CCG.similarity (CCG.py:94-102) and the per-step Gram rows of
+ * merging_function (template_matching.py:285-293):
+ *   Wnorm[k] = Wall[k] / (1e-6 + ||Wall[k]||),  Wall float32 [K][C][P];
+ *   S[i][j] = max over lags l of sum_{p,q} UtU[i,j,p,q] WtW[p,q,l]
+ *           (the ctc of ksm_prepare_matching of Wnorm, float64 sums);
+ *   lag[i][j] = the first lag index (0 .. 2nt) attaining it (NULL: not written).
+ */
+int ksm_similarity(const float *Wall, const float *W, int K, int C, int P, int nt, float *S,
+                   int32_t *lag, void *workspace, size_t ws_bytes, void *stream);
+
+/* Workspace bytes of ksm_align_templates. */
+size_t ksm_align_workspace(int K, int C, int P, int nt, int nk);
+
+/*
+ * Replaces template_matching.align_U (template_matching.py:128-141):
+ * Wall float32 [K][C][P], wPCA [P][nt], wTEMP [nk][nt];
+ *   Y[k][c][t] = sum_p Wall[k][c][p] wPCA[p][t];
+ *   imax[k] = the first t maximising max over waveforms q and channels c of
+ *     |sum_m wTEMP[q][m] Y'[c][k nt + t + m - nt/2]|, Y' the channel rows of Y
+ *     laid end to end over templates (zero outside);
+ *   Unew[k][p][c] = sum_t wPCA[p][t] Y[k][c][(t - nt/2 + imax[k]) mod nt].
+ * nt odd, <= 256.
+ */
+int ksm_align_templates(const float *Wall, const float *wPCA, const float *wTEMP, int K, int C,
+                        int P, int nt, int nk, float *Unew, int32_t *imax, void *workspace,
+                        size_t ws_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,10 @@ SIGNATURES = {
     "ksm_filter_batch": (_i, [_p, _i, _i64, _i, _i, _p, _i, _i, _i, _f, _f, _p, _i, _f, _p, _p,
                               _i, _p, _sz, _p]),
     "ksm_gemm_f32": (_i, [_p, _i, _p, _i, _p, _i, _i, _i, _i, _p]),
+    "ksm_similarity_workspace": (_sz, [_i, _i, _i, _i]),
+    "ksm_similarity": (_i, [_p, _p, _i, _i, _i, _i, _p, _p, _p, _sz, _p]),
+    "ksm_align_workspace": (_sz, [_i, _i, _i, _i, _i]),
+    "ksm_align_templates": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _sz, _p]),
     "ksm_detect_workspace": (_sz, [_i, _i, _i, _i]),
     "ksm_detect": (_i, [_p, _i, _i, _i, _p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _i, _f, _p, _p,
                         _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
