@@ -255,6 +255,37 @@ def time_preprocess(device, seed, nbatch=3, reps=5, cpu_baseline=True):
     return out
 
 
+def time_templates(ops, U, device, reps=5, cpu_baseline=True):
+    """Template Gram consumers on the bench's 700 templates x 383 channels:
+    the all-pairs similarity over lags (CCG.similarity) and align_U, with the
+    oracle's similarity as its CPU baseline."""
+    from kilosort_amd import templates as TM
+    Wall = U.transpose(1, 2).contiguous()
+    W = ops["wPCA"]
+    out = {}
+    for name, fn in (("similarity", lambda: TM.similarity_matrix(Wall, W, NTAP, device)),
+                     ("align_U", lambda: TM.align_U(Wall, dict(nt=NTAP, wPCA=W, wTEMP=W),
+                                                    device=device))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        out[name + "_ms"] = (time.perf_counter() - t0) * 1e3 / reps
+    out["templates"] = int(U.shape[0])
+    out["note"] = "wall time per call including the host copy of the [K, K] result"
+    if cpu_baseline:
+        from oracle import templates_oracle as TO
+        Wc = Wall.cpu()
+        t0 = time.perf_counter()
+        TO.similarity(Wc, W.cpu(), NTAP)
+        out["cpu_baseline"] = dict(similarity_s=time.perf_counter() - t0,
+                                   cores=torch.get_num_threads(), kind="port",
+                                   sample="oracle similarity of the 700 templates (torch CPU)")
+    return out
+
+
 def time_detect(ops, batches, device, reps=10, cpu_baseline=True):
     """Universal-template detection (spikedetect.template_match + the
     per-batch features of run) on the bench's 383-channel batches: device time
@@ -626,6 +657,10 @@ def main():
     log(f"refinement: residual {refine_res['residual_ms_per_batch']:.3f} ms, "
         f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch")
 
+    log("template similarity and alignment (700 templates)")
+    tmpl = time_templates(ops, U, device, cpu_baseline=not args.no_cpu_baseline)
+    log(f"templates: similarity {tmpl['similarity_ms']:.2f} ms, align_U {tmpl['align_U_ms']:.2f} ms")
+
     log("universal-template detection at 383 channels")
     det = time_detect(ops, batches, device, cpu_baseline=not args.no_cpu_baseline)
     log(f"detection: {det['ms_per_batch']:.3f} ms/batch, {det['spikes_per_batch']:.0f} spikes; "
@@ -696,6 +731,7 @@ def main():
         "refinement": refine_res,
         "preprocess": pre,
         "detect": det,
+        "templates": tmpl,
         "ragged": ragged,
         "cpu_baseline": cpu,
     }
