@@ -546,8 +546,8 @@ def peel_update_bytes(ids, nnzK, nnzC, nbatch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--precision", default="fp32", choices=FORMS,
                     help="form timed for `value`; every form is also timed under `forms`")
     ap.add_argument("--forms", default=",".join(FORMS),

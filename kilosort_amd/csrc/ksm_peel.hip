@@ -566,6 +566,7 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
     }
     const int n = count < kSlotCap ? count : kSlotCap;
     __syncthreads();
+    if constexpr (NTH != kRsThreads) return;  // re-scoring runs in the 1024-thread form only
     if (!rescore || n == 0) return;          // block-uniform
     // tasks: (spike q, channel block b); spike q's nonzero channel range is
     // cut into s_tb[q+1] - s_tb[q] <= kRsTasks blocks of s_cw[q] channels
