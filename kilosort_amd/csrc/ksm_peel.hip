@@ -53,7 +53,8 @@ constexpr int kMaxChunks = 4096;    // T <= kChunk * kMaxChunks
 constexpr int kUpdThreads = 256;    // update / first-pass block (4 waves)
 constexpr int kXresItems = 8;       // residual work items per spike
 constexpr int kRsTasks = 32;        // re-scoring tasks (channel blocks) per spike
-constexpr int kRsThreads = 1024;    // select block when re-scoring (16 waves)
+constexpr int kRsThreads = 256;     // select block when re-scoring: small blocks
+                                    // co-schedule with the other streams' updates
 constexpr int kXresCh = 16;         // channels per residual step (loads first)
 
 enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
@@ -476,7 +477,7 @@ __device__ __forceinline__ float nan_max(float a, float b) {
     return a != a ? a : (b != b ? b : fmaxf(a, b));
 }
 
-template <int NTH>
+template <int NTH, bool RS>
 __global__ __launch_bounds__(NTH) void peel_select_kernel(
     const float *__restrict__ cfmax, const int *__restrict__ imax, float *__restrict__ Bt,
     int ldbt, const GramRef *__restrict__ gref, int T, int nt, float th2, int rescore,
@@ -566,7 +567,7 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
     }
     const int n = count < kSlotCap ? count : kSlotCap;
     __syncthreads();
-    if constexpr (NTH != kRsThreads) return;  // re-scoring runs in the 1024-thread form only
+    if constexpr (!RS) return;               // the re-scoring code only in its own form
     if (!rescore || n == 0) return;          // block-uniform
     // tasks: (spike q, channel block b); spike q's nonzero channel range is
     // cut into s_tb[q+1] - s_tb[q] <= kRsTasks blocks of s_cw[q] channels
@@ -1615,11 +1616,12 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         {
             ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
             if (flags & KSM_PEEL_RESCORE)
-                prof.launch(peel_select_kernel<kRsThreads>, dim3(L.nchunks), dim3(kRsThreads),
+                prof.launch(peel_select_kernel<kRsThreads, true>, dim3(L.nchunks),
+                            dim3(kRsThreads),
                             (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, 1, Xres,
                             ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
             else
-                prof.launch(peel_select_kernel<kChunk>, dim3(L.nchunks), dim3(kChunk),
+                prof.launch(peel_select_kernel<kChunk, false>, dim3(L.nchunks), dim3(kChunk),
                             (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, 0, Xres,
                             ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
         }
