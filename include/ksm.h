@@ -336,6 +336,22 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
                float *yct, float *tF, int *count, void *workspace, size_t ws_bytes,
                void *stream);
 
+/* Workspace bytes of ksm_snippets. */
+size_t ksm_snippets_workspace(int C, int T);
+
+/*
+ * Replaces spikedetect.extract_snippets (spikedetect.py:34-51): isolated
+ * single-channel peaks -- |X[c,t]| equal to the maximum of |X| over channels
+ * c +- lc, samples t +- lt (outside: -inf) and above th, with no other such
+ * peak over c +- ic, t +- it, and nt <= t < T - nt -- in (channel, time)
+ * order; clips [cap][nt] = X[c][t - twav_min + k], xy [cap][2] = (c, t);
+ * *count (device) = the number of peaks, also beyond cap.
+ * Limits: lc <= 16, lt <= 16, 2 nt <= T.
+ */
+int ksm_snippets(const float *X, int ldx, int C, int T, int nt, int twav_min, float th, int lc,
+                 int lt, int ic, int it, int cap, float *clips, int32_t *xy, int *count,
+                 void *workspace, size_t ws_bytes, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Template Gram consumers (post-processing of learned templates).
  */

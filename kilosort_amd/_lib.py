@@ -83,6 +83,9 @@ SIGNATURES = {
     "ksm_similarity": (_i, [_p, _p, _i, _i, _i, _i, _p, _p, _p, _sz, _p]),
     "ksm_align_workspace": (_sz, [_i, _i, _i, _i, _i]),
     "ksm_align_templates": (_i, [_p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _sz, _p]),
+    "ksm_snippets_workspace": (_sz, [_i, _i]),
+    "ksm_snippets": (_i, [_p, _i, _i, _i, _i, _i, _f, _i, _i, _i, _i, _i, _p, _p, _p, _p, _sz,
+                          _p]),
     "ksm_detect_workspace": (_sz, [_i, _i, _i, _i]),
     "ksm_detect": (_i, [_p, _i, _i, _i, _p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _i, _f, _p, _p,
                         _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),

@@ -408,6 +408,110 @@ __global__ __launch_bounds__(256) void emit_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Isolated single-channel peaks and their snippets (extract_snippets,
+// spikedetect.py:34-51): peak[c][t] = |X| equals the maximum of |X| over
+// channels c +- lc and samples t +- lt and exceeds th (max_pool2d); isolated
+// = a peak with no other peak over c +- ic, t +- it (avg_pool2d x window
+// size == 1, computed as the reference's float32 division and product) and
+// nt <= t < T - nt.
+
+constexpr int kSnT = 64;          // columns per peak block
+constexpr int kSnC = 64;          // channels per peak block
+constexpr int kMaxLc = 16, kMaxLt = 16;
+
+__global__ __launch_bounds__(256) void snippet_peak_kernel(const float *__restrict__ X, int ldx,
+                                                           int C, int T, int lc, int lt,
+                                                           float th,
+                                                           uint8_t *__restrict__ peak) {
+    __shared__ float s_a[kSnC + 2 * kMaxLc][kSnT + 2 * kMaxLt + 1];
+    __shared__ float s_r[kSnC + 2 * kMaxLc][kSnT + 1];
+    const int c0 = blockIdx.y * kSnC, t0 = blockIdx.x * kSnT;
+    const int rows = kSnC + 2 * lc, cols = kSnT + 2 * lt;
+    for (int e = threadIdx.x; e < rows * cols; e += 256) {
+        const int r = e / cols, j = e - r * cols;
+        const int c = c0 - lc + r, t = t0 - lt + j;
+        s_a[r][j] = (c >= 0 && c < C && t >= 0 && t < T) ? fabsf(X[(size_t)c * ldx + t])
+                                                         : -INFINITY;
+    }
+    __syncthreads();
+    // maximum over t +- lt per row, then over c +- lc (separable)
+    for (int e = threadIdx.x; e < rows * kSnT; e += 256) {
+        const int r = e / kSnT, j = e - r * kSnT;
+        float m = -INFINITY;
+        for (int d = 0; d <= 2 * lt; ++d) m = fmaxf(m, s_a[r][j + d]);
+        s_r[r][j] = m;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kSnC * kSnT; e += 256) {
+        const int r = e / kSnT, j = e - r * kSnT;
+        const int c = c0 + r, t = t0 + j;
+        if (c >= C || t >= T) continue;
+        float m = -INFINITY;
+        for (int d = 0; d <= 2 * lc; ++d) m = fmaxf(m, s_r[r + d][j]);
+        const float a = s_a[r + lc][j + lt];
+        peak[(size_t)c * T + t] = (a == m && a > th) ? 1 : 0;
+    }
+}
+
+// iso[c][w] bit b: t = 64 w + b is an isolated peak of channel c.
+__global__ __launch_bounds__(256) void snippet_iso_kernel(const uint8_t *__restrict__ peak,
+                                                          int C, int T, int ic, int it, int nt,
+                                                          unsigned long long *__restrict__ iso,
+                                                          int nw) {
+    const int c = blockIdx.y;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool hit = false;
+    if (t < T && peak[(size_t)c * T + t]) {
+        int cnt = 0;
+        for (int dc = -ic; dc <= ic; ++dc) {
+            const int cc = c + dc;
+            if (cc < 0 || cc >= C) continue;
+            const uint8_t *row = peak + (size_t)cc * T;
+            for (int dt = -it; dt <= it; ++dt) {
+                const int tt = t + dt;
+                if (tt >= 0 && tt < T) cnt += row[tt];
+            }
+        }
+        const float size = (float)((2 * ic + 1) * (2 * it + 1));
+        const float v = ((float)cnt / size) * size;
+        hit = v == 1.f && t >= nt && t < T - nt;
+    }
+    const unsigned long long bits = __ballot(hit);
+    const int w = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 6;
+    if (lane == 0 && w < nw) iso[(size_t)c * nw + w] = bits;
+}
+
+// One wave per channel: snippets X[c][t - twav_min + k], k < nt, of its
+// isolated peaks in time order at off[c] ...
+__global__ __launch_bounds__(256) void snippet_emit_kernel(
+    const unsigned long long *__restrict__ iso, int nw, const int *__restrict__ off,
+    const float *__restrict__ X, int ldx, int C, int nt, int twav_min, int cap,
+    float *__restrict__ clips, int32_t *__restrict__ xy) {
+    const int lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= C) return;
+    int s = off[c];
+    for (int w = 0; w < nw; ++w) {
+        unsigned long long bits = iso[(size_t)c * nw + w];
+        while (bits) {
+            const int b = __builtin_ctzll(bits);
+            bits &= bits - 1ull;
+            const int t = 64 * w + b;
+            if (s < cap) {
+                for (int k = lane; k < nt; k += 64)
+                    clips[(size_t)s * nt + k] = X[(size_t)c * ldx + t - twav_min + k];
+                if (lane == 0) {
+                    xy[2 * (size_t)s] = c;
+                    xy[2 * (size_t)s + 1] = t;
+                }
+            }
+            ++s;
+        }
+    }
+}
+
 __global__ void clear_int_kernel(int *__restrict__ p) { *p = 0; }
 
 struct DetectLayout {
@@ -518,6 +622,46 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
                     static_cast<const int32_t *>(xy), imax, amp, adist, yct, tF);
         KSM_LAUNCH_CHECK("spike_features_kernel");
     }
+    return KSM_OK;
+}
+
+size_t ksm_snippets_workspace(int C, int T) {
+    if (C <= 0 || T <= 0) return 0;
+    const int nw = (T + 63) / 64;
+    return ksm::align_up((size_t)C * T, 256) +
+           ksm::align_up((size_t)C * nw * sizeof(unsigned long long), 256) +
+           2 * ksm::align_up((size_t)C * sizeof(int), 256) + 256;
+}
+
+int ksm_snippets(const float *X, int ldx, int C, int T, int nt, int twav_min, float th, int lc,
+                 int lt, int ic, int it, int cap, float *clips, int32_t *xy, int *count,
+                 void *workspace, size_t ws_bytes, void *stream) {
+    if (!X || !count || C <= 0 || T <= 0 || ldx < T || nt <= 0 || twav_min < 0 ||
+        twav_min > nt || cap < 0 || (cap > 0 && (!clips || !xy)) || lc < 0 || lt < 0 ||
+        lc > kMaxLc || lt > kMaxLt || ic < 0 || it < 0 || 2 * nt > T)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_snippets: bad arguments (lc <= %d, lt <= %d)",
+                         kMaxLc, kMaxLt);
+    if (!workspace || ws_bytes < ksm_snippets_workspace(C, T))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_snippets: workspace too small");
+    hipStream_t s = ksm::as_stream(stream);
+    const int nw = (T + 63) / 64;
+    ksm::Carver cv(workspace);
+    uint8_t *peak = cv.take<uint8_t>((size_t)C * T);
+    unsigned long long *iso = cv.take<unsigned long long>((size_t)C * nw);
+    int *off = cv.take<int>(C);
+    int *cnt = cv.take<int>(C);
+    snippet_peak_kernel<<<dim3((T + kSnT - 1) / kSnT, (C + kSnC - 1) / kSnC), 256, 0, s>>>(
+        X, ldx, C, T, lc, lt, th, peak);
+    KSM_LAUNCH_CHECK("snippet_peak_kernel");
+    snippet_iso_kernel<<<dim3((T + 255) / 256, C), 256, 0, s>>>(peak, C, T, ic, it, nt, iso, nw);
+    KSM_LAUNCH_CHECK("snippet_iso_kernel");
+    row_count_kernel<<<(C + 3) / 4, 256, 0, s>>>(iso, nw, C, cnt);
+    KSM_LAUNCH_CHECK("row_count_kernel");
+    row_scan_kernel<<<1, 1024, 0, s>>>(cnt, C, off, count);
+    KSM_LAUNCH_CHECK("row_scan_kernel");
+    snippet_emit_kernel<<<(C + 3) / 4, 256, 0, s>>>(iso, nw, off, X, ldx, C, nt, twav_min, cap,
+                                                    clips, xy);
+    KSM_LAUNCH_CHECK("snippet_emit_kernel");
     return KSM_OK;
 }
 

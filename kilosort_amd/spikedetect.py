@@ -13,9 +13,11 @@ size-weighted channel sums and their maxima, the spatio-temporal
 neighbourhood maxima, the ordered spike list, the channel-weighted y
 positions and the PC features.  The set-up of ``run`` -- the template grid,
 nearest channels and templates, the size weights -- is host numpy as in the
-reference.  ``run`` takes the universal waveforms ``ops['wTEMP']`` and the
-PCs ``ops['wPCA']`` from ``ops`` (the reference's ``extract_wPCA_wTEMP``
-k-means / SVD or ``get_waves`` computes them once per recording).
+reference.  The universal waveforms and PCs come from the data as in the
+reference (``extract_wPCA_wTEMP``: isolated-peak snippets found on the GPU by
+``ksm_snippets``, then scikit-learn's TruncatedSVD and KMeans on the host),
+or from ``ops['wTEMP']`` / ``ops['wPCA']`` when ``templates_from_data`` is
+off (the reference then loads its built-in file).
 """
 from __future__ import annotations
 
@@ -140,6 +142,76 @@ class _Detector:
         return (xy[:n], imax[:n], amp[:n], adist[:, :n], yct[:n], tF[:n])
 
 
+def extract_snippets(X, nt, twav_min, Th_single_ch, loc_range=(4, 5), long_range=(6, 30),
+                     device=torch.device('cuda')):
+    """``spikedetect.extract_snippets`` (``spikedetect.py:34-51``) on the GPU:
+    the [n, nt] waveforms around isolated single-channel peaks of X, in
+    (channel, time) order (``ksm_snippets``)."""
+    device = _lib.require_gpu(device)
+    lib = _lib.load()
+    X = X.to(device, torch.float32)
+    if X.stride(1) != 1:
+        X = X.contiguous()
+    C, T = X.shape
+    nb = lib.ksm_snippets_workspace(C, T)
+    ws = torch.empty(nb, dtype=torch.uint8, device=device)
+    count = torch.zeros(1, dtype=torch.int32, device=device)
+    cap = 1024
+    while True:
+        clips = torch.empty((cap, nt), dtype=torch.float32, device=device)
+        xy = torch.empty((cap, 2), dtype=torch.int32, device=device)
+        _lib.check(lib.ksm_snippets(X.data_ptr(), X.stride(0), C, T, nt, twav_min,
+                                    float(Th_single_ch), loc_range[0], loc_range[1],
+                                    long_range[0], long_range[1], cap, clips.data_ptr(),
+                                    xy.data_ptr(), count.data_ptr(), ws.data_ptr(), nb,
+                                    _lib.stream_of(device)), "ksm_snippets")
+        n = int(count.item())
+        if n <= cap:
+            return clips[:n]
+        cap = 1 << int(np.ceil(np.log2(n)))
+
+
+def extract_wPCA_wTEMP(ops, bfile, nt=61, twav_min=20, Th_single_ch=6, nskip=25,
+                       device=torch.device('cuda')):
+    """``spikedetect.extract_wPCA_wTEMP`` (``spikedetect.py:53-92``): snippets
+    of every nskip-th batch (GPU), normalised, then scikit-learn's
+    ``TruncatedSVD`` (the PCs) and ``KMeans`` (the universal waveforms) on the
+    host, as the reference."""
+    import os
+    import warnings
+    from sklearn.cluster import KMeans
+    from sklearn.decomposition import TruncatedSVD
+    clips = np.zeros((500000, nt), 'float32')
+    i = 0
+    for j in range(0, int(bfile.n_batches), nskip):
+        X = bfile.padded_batch_to_torch(j, ops)
+        clips_new = extract_snippets(X, nt=nt, twav_min=twav_min, Th_single_ch=Th_single_ch,
+                                     device=device)
+        nnew = len(clips_new)
+        if i + nnew > clips.shape[0]:
+            break
+        clips[i:i + nnew] = clips_new.cpu().numpy()
+        i += nnew
+    clips = clips[:i]
+    clips /= (clips ** 2).sum(1, keepdims=True) ** .5
+    model = TruncatedSVD(n_components=ops['settings']['n_pcs']).fit(clips)
+    wPCA = torch.from_numpy(model.components_).to(device).float()
+    with warnings.catch_warnings():
+        warnings.filterwarnings("ignore", message='KMeans is known to have a memory leak')
+        nthread = os.environ.get('OMP_NUM_THREADS')
+        os.environ['OMP_NUM_THREADS'] = str(min(int(nthread), 7) if nthread else 7)
+        try:
+            model = KMeans(n_clusters=ops['settings']['n_templates'], n_init=10).fit(clips)
+        finally:
+            if nthread is not None:
+                os.environ['OMP_NUM_THREADS'] = nthread
+            else:
+                os.environ.pop('OMP_NUM_THREADS')
+    wTEMP = torch.from_numpy(model.cluster_centers_).to(device).float()
+    wTEMP = wTEMP / (wTEMP ** 2).sum(1).unsqueeze(1) ** .5
+    return wPCA, wTEMP
+
+
 def template_match(X, ops, iC, iC2, weigh, device=torch.device('cuda')):
     """``spikedetect.template_match`` (``spikedetect.py:133-178``) on the GPU:
     ``xy`` int64 [nsp, 2] (template, column) in (template, column) order,
@@ -153,16 +225,21 @@ def template_match(X, ops, iC, iC2, weigh, device=torch.device('cuda')):
 
 def run(ops, bfile, device=torch.device('cuda'), progress_bar=None, clear_cache=False,
         verbose=False):
-    """``spikedetect.run`` (``spikedetect.py:199-307``) with ``ops['wTEMP']``
-    and ``ops['wPCA']`` given.  Returns ``(st, tF, ops)``: st float64
+    """``spikedetect.run`` (``spikedetect.py:199-307``): universal waveforms
+    from the data (``settings['templates_from_data']``, the default) or from
+    ``ops['wTEMP']`` / ``ops['wPCA']``.  Returns ``(st, tF, ops)``: st float64
     [nsp, 6] = (time in s, y position, amplitude, template, batch, template
     position index), tF float32 [nsp, nC, n_pcs] (numpy, as the reference),
     and ops with ``iC``, ``iC2``, ``weigh``, ``ycup``, ``xcup``."""
     device = _lib.require_gpu(device)
-    if 'wTEMP' not in ops or 'wPCA' not in ops:
-        raise ValueError("ops['wTEMP'] and ops['wPCA'] are required (the universal "
-                         "waveforms and PCs of kilosort.spikedetect.extract_wPCA_wTEMP "
-                         "or get_waves)")
+    if ops['settings'].get('templates_from_data', True):
+        logger.info('Re-computing universal templates from data.')
+        ops['wPCA'], ops['wTEMP'] = extract_wPCA_wTEMP(
+            ops, bfile, nt=ops['nt'], twav_min=ops['nt0min'],
+            Th_single_ch=ops['settings']['Th_single_ch'], nskip=25, device=device)
+    elif 'wTEMP' not in ops or 'wPCA' not in ops:
+        raise ValueError("templates_from_data=False needs ops['wTEMP'] and ops['wPCA'] "
+                         "(the reference's built-in waveforms, kilosort.spikedetect.get_waves)")
     iC, iC2, weigh = prepare(ops, device=device)
     logger.info(f'Number of universal templates: {iC.shape[1]}')
     det = _Detector(ops, iC, iC2, weigh, device)

@@ -7,6 +7,7 @@ TEST INFRASTRUCTURE ONLY: imported by ``tests/`` and ``bench.py``'s
 Follows ``kilosort/spikedetect.py`` with the same torch / numpy operations in
 the same order:
 
+* ``extract_snippets`` (``spikedetect.py:20-51``);
 * ``template_centers`` (``spikedetect.py:101-130``), ``nearest_chans``
   (``:181-187``), the template weights of ``run`` (``:217-242``);
 * ``template_match`` (``:133-178``);
@@ -17,7 +18,36 @@ from __future__ import annotations
 
 import numpy as np
 import torch
-from torch.nn.functional import conv1d, max_pool1d
+from torch.nn.functional import avg_pool2d, conv1d, max_pool1d, max_pool2d
+
+
+def my_max2d(X, dt):
+    """``spikedetect.py:20-25``."""
+    Xmax = max_pool2d(X.unsqueeze(0), [2 * dt[0] + 1, 2 * dt[1] + 1], stride=[1, 1],
+                      padding=[dt[0], dt[1]])
+    return Xmax[0]
+
+
+def my_sum2d(X, dt):
+    """``spikedetect.py:27-32``."""
+    Xsum = avg_pool2d(X.unsqueeze(0), [2 * dt[0] + 1, 2 * dt[1] + 1], stride=[1, 1],
+                      padding=[dt[0], dt[1]])
+    Xsum *= (2 * dt[0] + 1) * (2 * dt[1] + 1)
+    return Xsum[0]
+
+
+def extract_snippets(X, nt, twav_min, Th_single_ch, loc_range=(4, 5), long_range=(6, 30)):
+    """``spikedetect.py:34-51``."""
+    Xabs = X.abs()
+    Xmax = my_max2d(Xabs, loc_range)
+    ispeak = torch.logical_and(Xmax == Xabs, Xabs > Th_single_ch).float()
+    ispeak_sum = my_sum2d(ispeak, long_range)
+    is_peak_iso = ((ispeak_sum == 1) * (ispeak == 1))
+    is_peak_iso[:, :nt] = 0
+    is_peak_iso[:, -nt:] = 0
+    xy = is_peak_iso.nonzero()
+    clips = X[xy[:, :1], xy[:, 1:2] - twav_min + torch.arange(nt)]
+    return clips
 
 
 def template_centers(ops):

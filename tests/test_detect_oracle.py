@@ -123,3 +123,22 @@ def test_template_grid():
     assert ops['dmin'] == 10.0                    # staggered 20 um rows: depths 10 um apart
     np.testing.assert_allclose(ops['yup'], np.arange(0, 150 + .01, 5.0))
     np.testing.assert_allclose(ops['xup'], [16.0, 32.0, 48.0])
+
+
+def test_snippets_isolated_peaks_closed_form():
+    """One isolated peak above threshold gives one clip starting twav_min
+    samples before it; a second peak within 6 channels and 30 samples makes
+    both non-isolated; a peak below threshold or within nt of the edges is
+    not taken."""
+    C, T, nt, tw = 20, 400, 61, 20
+    X = torch.zeros(C, T, dtype=torch.float64)
+    X[5, 200] = -10.0                     # isolated (|X| is used)
+    X[15, 150] = 12.0                     # pair within the isolation window
+    X[18, 170] = 11.0
+    X[2, 300] = 5.0                       # below threshold 6
+    X[9, 30] = 20.0                       # within nt of the start
+    clips = DO.extract_snippets(X, nt, tw, 6.0)
+    assert clips.shape == (1, nt)
+    ref = X[5, 200 - tw:200 - tw + nt]
+    np.testing.assert_array_equal(clips[0].numpy(), ref.numpy())
+    assert clips[0, tw] == -10.0

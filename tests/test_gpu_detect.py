@@ -39,7 +39,8 @@ def make_ops(rec, NT, nC2=100):
                 wTEMP=torch.from_numpy(rec.wPCA.copy()), wPCA=torch.from_numpy(rec.wPCA.copy()),
                 settings=dict(dmin=None, dminx=32.0, nearest_chans=10, nearest_templates=nC2,
                               min_template_size=10.0, template_sizes=5, nt0min=20,
-                              n_templates=6, n_pcs=6))
+                              n_templates=6, n_pcs=6, templates_from_data=False,
+                              Th_single_ch=6.0))
 
 
 def rel(a, b):
@@ -119,3 +120,32 @@ def test_capacity_growth():
     assert len(xy) > 8 and det.cap >= len(xy)
     ref = DO.template_match(X.cpu(), ops, iC, iC2, weigh)
     np.testing.assert_array_equal(xy.cpu().numpy(), ref[0].numpy())
+
+
+@pytest.mark.parametrize("C", [64, 383])
+def test_snippets_match_oracle(C):
+    """Isolated-peak snippets: the same (channel, time) peaks in the same order
+    and the same waveforms (copied samples, so bit-identical)."""
+    rec = synth.make_recording(65 + C, C, 32, 60000 + 122, rate_range=(5.0, 20.0))
+    X = ArrayBatches(rec.data, NT=60000, nt=61, device=CPU).padded_batch_to_torch(0)
+    ref = DO.extract_snippets(X, 61, 20, 6.0)
+    got = SD.extract_snippets(X.to(DEV), 61, 20, 6.0, device=DEV).cpu()
+    say(f"snippets C={C}: {len(got)} (oracle {len(ref)})")
+    assert len(ref) > 50
+    np.testing.assert_array_equal(got.numpy(), ref.numpy())
+
+
+def test_universal_waveforms_from_data():
+    """extract_wPCA_wTEMP: snippets of every 25th batch, then scikit-learn's
+    SVD and k-means as the reference: orthonormal PCs, unit-norm waveforms."""
+    NT = 20000
+    rec = synth.make_recording(66, 48, 32, 3 * NT + 122, rate_range=(10.0, 30.0))
+    ops = make_ops(rec, NT)
+    bf = ArrayBatches(rec.data, NT=NT, nt=61, device=DEV)
+    wPCA, wTEMP = SD.extract_wPCA_wTEMP(ops, bf, nt=61, twav_min=20, Th_single_ch=6.0,
+                                        nskip=1, device=DEV)
+    assert wPCA.shape == (6, 61) and wTEMP.shape == (6, 61)
+    G = (wPCA @ wPCA.T).cpu().numpy()
+    np.testing.assert_allclose(G, np.eye(6), atol=1e-4)
+    np.testing.assert_allclose((wTEMP ** 2).sum(1).cpu().numpy(), 1.0, atol=1e-5)
+
