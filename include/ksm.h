@@ -387,6 +387,82 @@ int ksm_align_templates(const float *Wall, const float *wPCA, const float *wTEMP
                         int P, int nt, int nk, float *Unew, int32_t *imax, void *workspace,
                         size_t ws_bytes, void *stream);
 
+/*
+ * Replaces postprocessing.compute_spike_positions (postprocessing.py:33-62):
+ * per spike s with template ids[s] and peak channel u = iU[ids[s]]
+ * (iU as from ksm_prepare_extract, in [0, C)), weights ||tF[s, j, :]||_2 *
+ * mask[j][u] normalised over j, and (xs, ys)[s] = the weighted mean of
+ * (xc, yc)[iCC[j][u]].  iCC int64 and mask uint8 are [nC][C] (nearest
+ * channels of every channel and their position mask); tF float32
+ * [n][nC][P]; all on the device.  The caller keeps ids in range.  nC <= 32.
+ * A spike with no unmasked feature mass gets NaN, as the reference's 0/0.
+ * amp (may be NULL) gets ||tF[s]||_F over channels and PCs, the amplitudes
+ * save_to_phy writes (io.py:330), from the same pass over tF.
+ */
+int ksm_spike_positions(const float *tF, const int64_t *ids, long long n, int nC, int P,
+                        const int64_t *iU, const int64_t *iCC, const uint8_t *mask, int C,
+                        const float *xc, const float *yc, float *xs, float *ys, float *amp,
+                        void *stream);
+
+/*
+ * Output formats (the per-spike work of io.save_to_phy, io.py:248-469).
+ *
+ * ksm_group: stable grouping of spikes by label (cluster id in
+ * [0, n_groups), n_groups <= 16384, n < 2^31): offsets [n_groups + 1] and
+ * order [n], with order[offsets[g] .. offsets[g+1]) the spikes of label g in
+ * input order.  The grouping that postprocessing.remove_duplicates
+ * (postprocessing.py:9-30, a dict of labels) and CCG.refract (CCG.py:105-120,
+ * one boolean mask per label) make on the host.  Labels out of range are
+ * left out.  Workspace from ksm_group_workspace (0 = invalid sizes).
+ */
+size_t ksm_group_workspace(long long n, int n_groups);
+int ksm_group(const int32_t *labels, long long n, int n_groups, int64_t *offsets, int64_t *order,
+              void *workspace, size_t ws_bytes, void *stream);
+
+/*
+ * Replaces postprocessing.remove_duplicates (postprocessing.py:9-30): with
+ * the grouping of ksm_group, keep[i] = 1 when spike i is the first of its
+ * label or at least dt samples after the last kept spike of its label,
+ * walking each label's spikes in input order.
+ */
+int ksm_remove_duplicates(const int64_t *times, long long n, int n_groups, const int64_t *offsets,
+                          const int64_t *order, long long dt, uint8_t *keep, void *stream);
+
+/*
+ * Replaces CCG.compute_CCG (CCG.py:8-36) for the autocorrelograms of
+ * CCG.refract (:105-120): for every label g with active[g], counts[g][b]
+ * (b < 2 nbins + 1, uint32) = pairs (j, k) of its spikes with
+ * t_j - dt < t_k <= t_j + dt, dt = nbins * tbin, in bin
+ * rint((t_j - t_k) / tbin) + nbins.  times float64 seconds, sorted within
+ * each label (the reference sorts each label's times); grouping from
+ * ksm_group.  nbins <= 1023.  Exact integer counts.
+ */
+int ksm_acg(const double *times, long long n, int n_groups, const int64_t *offsets,
+            const int64_t *order, const uint8_t *active, int nbins, double tbin, uint32_t *counts,
+            void *stream);
+
+/*
+ * The spike-level part of postprocessing.make_pc_features
+ * (postprocessing.py:65-130, with clustering_qr.get_data_cpu,
+ * clustering_qr.py:560-600): per detection template t < K, sums[t][j][p]
+ * (float64) = the sum of tF[s][j][p] over its spikes and counts[t] their
+ * number; presence (may be NULL) [n_groups][K] = 1 where label g holds a
+ * spike of template t.  tF float32 [n][nC][P]; template ids int64 in
+ * [0, K) and labels in [0, n_groups), kept in range by the caller.
+ */
+int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *labels, long long n,
+                      int nC, int P, int K, int n_groups, double *sums, uint64_t *counts,
+                      uint8_t *presence, void *stream);
+
+/*
+ * The feature rewrite of make_pc_features (postprocessing.py:120-127):
+ * out[s][k][p] (layout 0) or out[s][p][k] (layout 1, the Phy pc_features
+ * layout) = tF[s][sigma[t][k]][p], or 0 where sigma[t][k] < 0, t = the
+ * spike's template; sigma int32 [K][nC].  out must not alias tF.
+ */
+int ksm_gather_features(const float *tF, const int64_t *tmpl, long long n, int nC, int P,
+                        const int32_t *sigma, float *out, int layout, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,15 @@ SIGNATURES = {
     "ksm_snippets_workspace": (_sz, [_i, _i]),
     "ksm_snippets": (_i, [_p, _i, _i, _i, _i, _i, _f, _i, _i, _i, _i, _i, _p, _p, _p, _p, _sz,
                           _p]),
+    "ksm_spike_positions": (_i, [_p, _p, ctypes.c_longlong, _i, _i, _p, _p, _p, _i, _p, _p, _p,
+                                 _p, _p, _p]),
+    "ksm_group_workspace": (_sz, [ctypes.c_longlong, _i]),
+    "ksm_group": (_i, [_p, ctypes.c_longlong, _i, _p, _p, _p, _sz, _p]),
+    "ksm_remove_duplicates": (_i, [_p, ctypes.c_longlong, _i, _p, _p, ctypes.c_longlong, _p,
+                                   _p]),
+    "ksm_acg": (_i, [_p, ctypes.c_longlong, _i, _p, _p, _p, _i, ctypes.c_double, _p, _p]),
+    "ksm_template_sums": (_i, [_p, _p, _p, ctypes.c_longlong, _i, _i, _i, _i, _p, _p, _p, _p]),
+    "ksm_gather_features": (_i, [_p, _p, ctypes.c_longlong, _i, _i, _p, _p, _i, _p]),
     "ksm_detect_workspace": (_sz, [_i, _i, _i, _i]),
     "ksm_detect": (_i, [_p, _i, _i, _i, _p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _i, _f, _p, _p,
                         _i, _i, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),

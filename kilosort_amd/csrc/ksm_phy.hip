@@ -1,0 +1,390 @@
+// Output-format kernels: the per-spike work of the Phy export.
+//
+// Reference: kilosort/postprocessing.py:9-30 (remove_duplicates),
+// :65-130 (make_pc_features, with clustering_qr.py:560-600 get_data_cpu),
+// kilosort/CCG.py:8-36 (compute_CCG) and :105-120 (refract), as called by
+// kilosort/io.py:248-469 (save_to_phy).
+//
+// * ksm_group: a stable grouping of spike indices by label (cluster id):
+//   order[offsets[g] .. offsets[g+1]) lists the spikes of label g in input
+//   order.  Three passes: per-chunk label counts in LDS, a per-label scan
+//   over chunks, and an in-order placement where each wave ranks equal labels
+//   among its 64 lanes by bitwise ballots.
+// * ksm_remove_duplicates: one thread per label walks its spikes in input
+//   order and keeps a spike when it is at least dt samples after the last
+//   kept spike of its label (the first spike of a label is kept).
+// * ksm_acg: autocorrelogram counts of every active label: for every spike j
+//   of a label, the spikes k of the same label with t_j - dt < t_k <=
+//   t_j + dt (dt = nbins * tbin) land in bin rint((t_j - t_k) / tbin) +
+//   nbins (times in seconds, float64, sorted within each label), as the
+//   reference's two-pointer loop.  Integer counts, so the result is exact.
+// * ksm_template_sums: per detection template, the double sums of the PC
+//   features of its spikes and their count, and which templates each label
+//   holds (the inputs of the per-cluster channel ranking).
+// * ksm_gather_features: each spike's features re-indexed by its template's
+//   channel map (sigma[t][k] = source channel row, or -1 for zeros).
+#include "ksm_common.h"
+
+namespace {
+
+constexpr int kGrpThreads = 64;            // one wave per grouping block
+constexpr int kMaxGroups = 16384;          // label counters in 64 KB of LDS
+constexpr long long kMaxSpikes = 0x7fffffffLL;
+constexpr int kMaxBins = 1023;             // 2 nbins + 1 <= 2047 ACG bins
+
+long long group_chunk(long long n, int G) {
+    // chunks of at least 4096 spikes, with at most 2^24 chunk x label counters
+    long long chunk = 4096;
+    while (((n + chunk - 1) / chunk) * (long long)G > (1LL << 24)) chunk *= 2;
+    return chunk;
+}
+
+__global__ void zero_bytes_kernel(uint8_t *__restrict__ p, size_t nbytes) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nbytes; i += stride)
+        p[i] = 0;
+}
+
+int zero_async(void *p, size_t nbytes, hipStream_t s) {
+    if (!nbytes) return KSM_OK;
+    const size_t blocks = std::min<size_t>((nbytes + 255) / 256, 4096);
+    zero_bytes_kernel<<<(unsigned)blocks, 256, 0, s>>>(static_cast<uint8_t *>(p), nbytes);
+    KSM_LAUNCH_CHECK("zero_bytes_kernel");
+    return KSM_OK;
+}
+
+// counts[b][g] = spikes of label g in chunk b.
+__global__ __launch_bounds__(kGrpThreads) void group_count_kernel(
+    const int32_t *__restrict__ lab, long long n, int G, long long chunk,
+    uint32_t *__restrict__ counts) {
+    extern __shared__ uint32_t s_hist[];
+    const int lane = threadIdx.x;
+    for (int g = lane; g < G; g += kGrpThreads) s_hist[g] = 0;
+    __syncthreads();
+    const long long b0 = (long long)blockIdx.x * chunk;
+    const long long b1 = b0 + chunk < n ? b0 + chunk : n;
+    for (long long i = b0 + lane; i < b1; i += kGrpThreads) {
+        const int c = lab[i];
+        if ((unsigned)c < (unsigned)G) atomicAdd(&s_hist[c], 1u);
+    }
+    __syncthreads();
+    uint32_t *row = counts + (size_t)blockIdx.x * G;
+    for (int g = lane; g < G; g += kGrpThreads) row[g] = s_hist[g];
+}
+
+// counts[b][g] -> spikes of label g in chunks before b; total[g].
+__global__ void group_scan_kernel(uint32_t *__restrict__ counts, long long nb, int G,
+                                  int64_t *__restrict__ total) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    uint32_t run = 0;
+    for (long long b = 0; b < nb; ++b) {
+        uint32_t *c = counts + (size_t)b * G + g;
+        const uint32_t v = *c;
+        *c = run;
+        run += v;
+    }
+    total[g] = run;
+}
+
+// offsets[0] = 0, offsets[g + 1] = total[0] + ... + total[g]; one block.
+__global__ __launch_bounds__(1024) void group_offsets_kernel(const int64_t *__restrict__ total,
+                                                             int G, int64_t *__restrict__ off) {
+    __shared__ int64_t s[1024];
+    const int t = threadIdx.x;
+    const int per = (G + 1023) / 1024;
+    const int g0 = t * per, g1 = g0 + per < G ? g0 + per : G;
+    int64_t local = 0;
+    for (int g = g0; g < g1; ++g) local += total[g];
+    s[t] = local;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t v = t >= o ? s[t - o] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int64_t run = s[t] - local;
+    if (t == 0) off[0] = 0;
+    for (int g = g0; g < g1; ++g) {
+        run += total[g];
+        off[g + 1] = run;
+    }
+}
+
+// order[offsets[g] + rank] = i, rank = spikes of label g before i.
+__global__ __launch_bounds__(kGrpThreads) void group_place_kernel(
+    const int32_t *__restrict__ lab, long long n, int G, int bits, long long chunk,
+    const uint32_t *__restrict__ before, const int64_t *__restrict__ off,
+    int64_t *__restrict__ order) {
+    extern __shared__ uint32_t s_base[];
+    const int lane = threadIdx.x;
+    const uint32_t *row = before + (size_t)blockIdx.x * G;
+    for (int g = lane; g < G; g += kGrpThreads) s_base[g] = row[g];
+    __syncthreads();
+    const long long b0 = (long long)blockIdx.x * chunk;
+    const long long b1 = b0 + chunk < n ? b0 + chunk : n;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (long long i0 = b0; i0 < b1; i0 += kGrpThreads) {
+        const long long i = i0 + lane;
+        const int c = i < b1 ? lab[i] : -1;
+        const bool ok = (unsigned)c < (unsigned)G;
+        uint64_t eq = __ballot(ok);
+        for (int bit = 0; bit < bits; ++bit) {
+            const bool on = (c >> bit) & 1;
+            const uint64_t m = __ballot(ok && on);
+            eq &= on ? m : ~m;
+        }
+        if (ok) order[off[c] + s_base[c] + __popcll(eq & lt)] = i;
+        __syncthreads();
+        if (ok && (eq & lt) == 0) s_base[c] += (uint32_t)__popcll(eq);
+        __syncthreads();
+    }
+}
+
+__global__ void dedup_kernel(const int64_t *__restrict__ times, const int64_t *__restrict__ off,
+                             const int64_t *__restrict__ order, int G, long long dt,
+                             uint8_t *__restrict__ keep) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const long long a = off[g], b = off[g + 1];
+    if (a == b) return;
+    long long t0 = times[order[a]] - dt;
+    for (long long p = a; p < b; ++p) {
+        const long long i = order[p];
+        const long long t = times[i];
+        const bool k = t >= t0 + dt;
+        if (k) t0 = t;
+        keep[i] = k ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ int group_of(const int64_t *__restrict__ off, int G, long long p) {
+    int lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid + 1] > p) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double time_at(const double *__restrict__ times,
+                                          const int64_t *__restrict__ order, long long p) {
+    return times[order[p]];
+}
+
+// First position in [a, b) whose time exceeds x (times sorted within a group).
+__device__ __forceinline__ long long first_above(const double *__restrict__ times,
+                                                 const int64_t *__restrict__ order, long long a,
+                                                 long long b, double x) {
+    while (a < b) {
+        const long long mid = (a + b) >> 1;
+        if (time_at(times, order, mid) > x) b = mid;
+        else a = mid + 1;
+    }
+    return a;
+}
+
+// One thread per grouped position j; a block whose 256 positions share one
+// label counts in LDS and adds its nonzero bins once.
+__global__ __launch_bounds__(256) void acg_kernel(const double *__restrict__ times,
+                                                  const int64_t *__restrict__ order, long long n,
+                                                  const int64_t *__restrict__ off, int G,
+                                                  const uint8_t *__restrict__ active, int nbins,
+                                                  double tbin, uint32_t *__restrict__ counts) {
+    __shared__ uint32_t s_h[2 * kMaxBins + 1];
+    const int L = 2 * nbins + 1;
+    const long long p0 = (long long)blockIdx.x * 256;
+    const long long plast = (p0 + 256 < n ? p0 + 256 : n) - 1;
+    const int g0 = group_of(off, G, p0), g1 = group_of(off, G, plast);
+    const bool single = g0 == g1;
+    if (single) {
+        if (!active[g0]) return;
+        for (int b = threadIdx.x; b < L; b += 256) s_h[b] = 0;
+        __syncthreads();
+    }
+    const long long p = p0 + threadIdx.x;
+    if (p < n) {
+        const int g = single ? g0 : group_of(off, G, p);
+        if (active[g]) {
+            const long long a = off[g], b = off[g + 1];
+            const double dt = nbins * tbin;
+            const double tj = time_at(times, order, p);
+            const long long kl = first_above(times, order, a, b, tj - dt);
+            const long long kh = first_above(times, order, kl, b, tj + dt);
+            uint32_t *row = counts + (size_t)g * L;
+            for (long long k = kl; k < kh; ++k) {
+                const double d = (tj - time_at(times, order, k)) / tbin;
+                int ib = (int)rint(d) + nbins;
+                ib = ib < 0 ? 0 : (ib >= L ? L - 1 : ib);
+                if (single) atomicAdd(&s_h[ib], 1u);
+                else atomicAdd(&row[ib], 1u);
+            }
+        }
+    }
+    if (single) {
+        __syncthreads();
+        uint32_t *row = counts + (size_t)g0 * L;
+        for (int b = threadIdx.x; b < L; b += 256)
+            if (s_h[b]) atomicAdd(&row[b], s_h[b]);
+    }
+}
+
+__global__ void template_sums_kernel(const float *__restrict__ tF, const int64_t *__restrict__ tmpl,
+                                     const int32_t *__restrict__ lab, long long n, int R, int K,
+                                     double *__restrict__ sums,
+                                     unsigned long long *__restrict__ cnt,
+                                     uint8_t *__restrict__ presence) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * R) return;
+    const long long s = e / R;
+    const int r = (int)(e - s * R);
+    const long long t = tmpl[s];
+    atomicAdd(&sums[(size_t)t * R + r], (double)tF[e]);
+    if (r == 0) {
+        atomicAdd(&cnt[t], 1ull);
+        if (presence) presence[(size_t)lab[s] * K + t] = 1;
+    }
+}
+
+__global__ void gather_features_kernel(const float *__restrict__ tF,
+                                       const int64_t *__restrict__ tmpl, long long n, int nC,
+                                       int P, const int32_t *__restrict__ sigma,
+                                       float *__restrict__ out, int layout) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n * P) return;
+    const long long s = e / P;
+    const int p = (int)(e - s * P);
+    const int32_t *sg = sigma + (size_t)tmpl[s] * nC;
+    const float *src = tF + (size_t)s * nC * P + p;
+    for (int k = 0; k < nC; ++k) {
+        const int q = sg[k];
+        const float v = q >= 0 ? src[(size_t)q * P] : 0.f;
+        if (layout == 0) out[((size_t)s * nC + k) * P + p] = v;
+        else out[((size_t)s * P + p) * nC + k] = v;
+    }
+}
+
+struct GroupWs {
+    uint32_t *counts;
+    int64_t *total;
+    long long nb, chunk;
+    size_t bytes;
+};
+
+GroupWs group_layout(void *base, long long n, int G) {
+    GroupWs w{};
+    w.chunk = group_chunk(n, G);
+    w.nb = n > 0 ? (n + w.chunk - 1) / w.chunk : 0;
+    const size_t c_bytes = ((size_t)w.nb * G * sizeof(uint32_t) + 255) & ~(size_t)255;
+    w.bytes = c_bytes + (size_t)G * sizeof(int64_t);
+    uint8_t *p = static_cast<uint8_t *>(base);
+    w.counts = reinterpret_cast<uint32_t *>(p);
+    w.total = reinterpret_cast<int64_t *>(p + c_bytes);
+    return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ksm_group_workspace(long long n, int n_groups) {
+    if (n < 0 || n > kMaxSpikes || n_groups <= 0 || n_groups > kMaxGroups) return 0;
+    return group_layout(nullptr, n, n_groups).bytes;
+}
+
+int ksm_group(const int32_t *labels, long long n, int n_groups, int64_t *offsets, int64_t *order,
+              void *workspace, size_t ws_bytes, void *stream) {
+    if (n < 0 || n > kMaxSpikes || n_groups <= 0 || n_groups > kMaxGroups || !offsets ||
+        (n > 0 && (!labels || !order)))
+        return ksm::fail(KSM_ERR_INVALID,
+                         "ksm_group: bad arguments (0 <= n < 2^31, 1 <= n_groups <= %d)",
+                         kMaxGroups);
+    const GroupWs w = group_layout(workspace, n, n_groups);
+    if (!workspace || ws_bytes < w.bytes)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_group: workspace of %zu bytes needed", w.bytes);
+    hipStream_t s = ksm::as_stream(stream);
+    const size_t lds = (size_t)n_groups * sizeof(uint32_t);
+    if (w.nb > 0) {
+        group_count_kernel<<<(unsigned)w.nb, kGrpThreads, lds, s>>>(labels, n, n_groups, w.chunk,
+                                                                   w.counts);
+        KSM_LAUNCH_CHECK("group_count_kernel");
+    }
+    group_scan_kernel<<<(n_groups + 255) / 256, 256, 0, s>>>(w.counts, w.nb, n_groups, w.total);
+    KSM_LAUNCH_CHECK("group_scan_kernel");
+    group_offsets_kernel<<<1, 1024, 0, s>>>(w.total, n_groups, offsets);
+    KSM_LAUNCH_CHECK("group_offsets_kernel");
+    if (w.nb > 0) {
+        int bits = 0;
+        while ((1 << bits) < n_groups) ++bits;
+        group_place_kernel<<<(unsigned)w.nb, kGrpThreads, lds, s>>>(
+            labels, n, n_groups, bits, w.chunk, w.counts, offsets, order);
+        KSM_LAUNCH_CHECK("group_place_kernel");
+    }
+    return KSM_OK;
+}
+
+int ksm_remove_duplicates(const int64_t *times, long long n, int n_groups, const int64_t *offsets,
+                          const int64_t *order, long long dt, uint8_t *keep, void *stream) {
+    if (n < 0 || n_groups <= 0 || n_groups > kMaxGroups || !offsets ||
+        (n > 0 && (!times || !order || !keep)))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_remove_duplicates: bad arguments");
+    if (n == 0) return KSM_OK;
+    dedup_kernel<<<(n_groups + 255) / 256, 256, 0, ksm::as_stream(stream)>>>(
+        times, offsets, order, n_groups, dt, keep);
+    KSM_LAUNCH_CHECK("dedup_kernel");
+    return KSM_OK;
+}
+
+int ksm_acg(const double *times, long long n, int n_groups, const int64_t *offsets,
+            const int64_t *order, const uint8_t *active, int nbins, double tbin, uint32_t *counts,
+            void *stream) {
+    if (n < 0 || n_groups <= 0 || n_groups > kMaxGroups || !offsets || !active || !counts ||
+        (n > 0 && (!times || !order)) || nbins <= 0 || nbins > kMaxBins || !(tbin > 0.0))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_acg: bad arguments (1 <= nbins <= %d)", kMaxBins);
+    hipStream_t s = ksm::as_stream(stream);
+    const int rc = zero_async(counts, (size_t)n_groups * (2 * nbins + 1) * sizeof(uint32_t), s);
+    if (rc != KSM_OK || n == 0) return rc;
+    acg_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(times, order, n, offsets, n_groups,
+                                                          active, nbins, tbin, counts);
+    KSM_LAUNCH_CHECK("acg_kernel");
+    return KSM_OK;
+}
+
+int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *labels, long long n,
+                      int nC, int P, int K, int n_groups, double *sums, uint64_t *counts,
+                      uint8_t *presence, void *stream) {
+    if (n < 0 || nC <= 0 || P <= 0 || K <= 0 || !sums || !counts ||
+        (presence && (n_groups <= 0 || (double)n_groups * K > (double)(1LL << 30))) ||
+        (n > 0 && (!tF || !tmpl || (presence && !labels))))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_template_sums: bad arguments");
+    hipStream_t s = ksm::as_stream(stream);
+    const int R = nC * P;
+    int rc = zero_async(sums, (size_t)K * R * sizeof(double), s);
+    if (rc == KSM_OK) rc = zero_async(counts, (size_t)K * sizeof(uint64_t), s);
+    if (rc == KSM_OK && presence) rc = zero_async(presence, (size_t)n_groups * K, s);
+    if (rc != KSM_OK || n == 0) return rc;
+    const long long total = n * R;
+    template_sums_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
+        tF, tmpl, labels, n, R, K, sums, reinterpret_cast<unsigned long long *>(counts), presence);
+    KSM_LAUNCH_CHECK("template_sums_kernel");
+    return KSM_OK;
+}
+
+int ksm_gather_features(const float *tF, const int64_t *tmpl, long long n, int nC, int P,
+                        const int32_t *sigma, float *out, int layout, void *stream) {
+    if (n < 0 || nC <= 0 || P <= 0 || (layout != 0 && layout != 1) || !sigma ||
+        (n > 0 && (!tF || !tmpl || !out)))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_gather_features: bad arguments");
+    if (n > 0 && (const void *)out == (const void *)tF)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_gather_features: out must not alias tF");
+    if (n == 0) return KSM_OK;
+    const long long total = n * P;
+    gather_features_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ksm::as_stream(stream)>>>(
+        tF, tmpl, n, nC, P, sigma, out, layout);
+    KSM_LAUNCH_CHECK("gather_features_kernel");
+    return KSM_OK;
+}
+
+}  // extern "C"

@@ -128,6 +128,43 @@ __global__ void roll_project_kernel(const float *__restrict__ Y, const float *__
     Unew[e] = acc;
 }
 
+// Spike positions (postprocessing.py:33-62): per spike s of template id with
+// peak channel u = iU[id], m_j = ||tF[s, j, :]||_2 * mask[j, u],
+// w_j = m_j / sum_j m_j, (xs, ys) = sum_j w_j (xc, yc)[iCC[j, u]]
+// (iCC, mask [nC][C] over the C channels); amp[s] = ||tF[s]||_F when amp is
+// given (the Phy amplitudes, io.py:330).
+__global__ void positions_kernel(const float *__restrict__ tF, const int64_t *__restrict__ ids,
+                                 long long n, int nC, int P, const int64_t *__restrict__ iU,
+                                 const int64_t *__restrict__ iCC, const uint8_t *__restrict__ mask,
+                                 int C, const float *__restrict__ xc, const float *__restrict__ yc,
+                                 float *__restrict__ xs, float *__restrict__ ys,
+                                 float *__restrict__ amp) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int u = (int)iU[ids[s]];
+    float m[32];
+    float tot = 0.f, all = 0.f;
+    for (int j = 0; j < nC; ++j) {
+        const float *f = tF + ((size_t)s * nC + j) * P;
+        float a = 0.f;
+        for (int p = 0; p < P; ++p) a = __builtin_fmaf(f[p], f[p], a);
+        all += a;
+        const float v = sqrtf(a) * (mask[(size_t)j * C + u] ? 1.f : 0.f);
+        m[j] = v;
+        tot += v;
+    }
+    if (amp) amp[s] = sqrtf(all);
+    float x = 0.f, y = 0.f;
+    for (int j = 0; j < nC; ++j) {
+        const float w = m[j] / tot;
+        const int c = (int)iCC[(size_t)j * C + u];
+        x += xc[c] * w;
+        y += yc[c] * w;
+    }
+    xs[s] = x;
+    ys[s] = y;
+}
+
 __global__ void clear_flag_t_kernel(int *__restrict__ f) { *f = 0; }
 
 }  // namespace
@@ -203,6 +240,20 @@ int ksm_align_templates(const float *Wall, const float *wPCA, const float *wTEMP
     roll_project_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(Y, wPCA, imax, K, C, P, nt,
                                                                   Unew);
     KSM_LAUNCH_CHECK("roll_project_kernel");
+    return KSM_OK;
+}
+
+int ksm_spike_positions(const float *tF, const int64_t *ids, long long n, int nC, int P,
+                        const int64_t *iU, const int64_t *iCC, const uint8_t *mask, int C,
+                        const float *xc, const float *yc, float *xs, float *ys, float *amp,
+                        void *stream) {
+    if (n < 0 || (n > 0 && (!tF || !ids || !iU || !iCC || !mask || !xc || !yc || !xs || !ys)) ||
+        nC <= 0 || nC > 32 || P <= 0 || C <= 0)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_spike_positions: bad arguments (nC <= 32)");
+    if (n == 0) return KSM_OK;
+    positions_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ksm::as_stream(stream)>>>(
+        tF, ids, n, nC, P, iU, iCC, mask, C, xc, yc, xs, ys, amp);
+    KSM_LAUNCH_CHECK("positions_kernel");
     return KSM_OK;
 }
 
