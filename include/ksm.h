@@ -442,6 +442,26 @@ int ksm_acg(const double *times, long long n, int n_groups, const int64_t *offse
             void *stream);
 
 /*
+ * Replaces CCG.compute_CCG (CCG.py:8-36) for two trains, as called by
+ * check_CCG from merging_function (template_matching.py:300-304): counts
+ * [2 nbins + 1] (uint32) of pairs (j of t2, k of t1) with
+ * t2_j - dt < t1_k <= t2_j + dt, dt = nbins * tbin, in bin
+ * rint((t2_j - t1_k) / tbin) + nbins.  t1, t2 float64 seconds, each sorted
+ * (the reference sorts both).  nbins <= 1023.
+ */
+int ksm_ccg(const double *t1, long long n1, const double *t2, long long n2, int nbins,
+            double tbin, uint32_t *counts, void *stream);
+
+/*
+ * The feature shift of template_matching.roll_features (:361-379) in PC
+ * space: X [n][R][P] float32 on the device; for every spike rows[i],
+ * X[rows[i]][r][:] <- X[rows[i]][r][:] @ M, M [P][P] float32 (the caller's
+ * wPCA . S_dt . wPCA^T, S_dt the clamped time shift).  In place.  P <= 16.
+ */
+int ksm_pc_map(float *X, const int64_t *rows, long long nrows, int R, int P, const float *M,
+               void *stream);
+
+/*
  * The spike-level part of postprocessing.make_pc_features
  * (postprocessing.py:65-130, with clustering_qr.get_data_cpu,
  * clustering_qr.py:560-600): per detection template t < K, sums[t][j][p]

@@ -1,6 +1,7 @@
 // Output-format kernels: the per-spike work of the Phy export.
 //
 // Reference: kilosort/postprocessing.py:9-30 (remove_duplicates),
+// kilosort/template_matching.py:361-379 (roll_features),
 // :65-130 (make_pc_features, with clustering_qr.py:560-600 get_data_cpu),
 // kilosort/CCG.py:8-36 (compute_CCG) and :105-120 (refract), as called by
 // kilosort/io.py:248-469 (save_to_phy).
@@ -18,6 +19,9 @@
 //   t_j + dt (dt = nbins * tbin) land in bin rint((t_j - t_k) / tbin) +
 //   nbins (times in seconds, float64, sorted within each label), as the
 //   reference's two-pointer loop.  Integer counts, so the result is exact.
+// * ksm_ccg: the same counts for two sorted trains (t2 against t1).
+// * ksm_pc_map: the PC features of chosen spikes times a P x P matrix (the
+//   time shift of merging_function's roll_features, expressed in PC space).
 // * ksm_template_sums: per detection template, the double sums of the PC
 //   features of its spikes and their count, and which templates each label
 //   holds (the inputs of the per-cluster channel ranking).
@@ -231,6 +235,66 @@ __global__ __launch_bounds__(256) void acg_kernel(const double *__restrict__ tim
     }
 }
 
+// Cross-correlogram of two sorted trains: one thread per spike j of t2; the
+// block counts in LDS and adds its nonzero bins once.
+__global__ __launch_bounds__(256) void ccg_kernel(const double *__restrict__ t1, long long n1,
+                                                  const double *__restrict__ t2, long long n2,
+                                                  int nbins, double tbin,
+                                                  uint32_t *__restrict__ counts) {
+    __shared__ uint32_t s_h[2 * kMaxBins + 1];
+    const int L = 2 * nbins + 1;
+    for (int b = threadIdx.x; b < L; b += 256) s_h[b] = 0;
+    __syncthreads();
+    const long long j = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (j < n2) {
+        const double dt = nbins * tbin;
+        const double tj = t2[j];
+        long long a = 0, b = n1;
+        const double lo_x = tj - dt, hi_x = tj + dt;
+        while (a < b) {                       // first t1 > tj - dt
+            const long long mid = (a + b) >> 1;
+            if (t1[mid] > lo_x) b = mid;
+            else a = mid + 1;
+        }
+        const long long kl = a;
+        b = n1;
+        while (a < b) {                       // first t1 > tj + dt
+            const long long mid = (a + b) >> 1;
+            if (t1[mid] > hi_x) b = mid;
+            else a = mid + 1;
+        }
+        for (long long k = kl; k < a; ++k) {
+            int ib = (int)rint((tj - t1[k]) / tbin) + nbins;
+            ib = ib < 0 ? 0 : (ib >= L ? L - 1 : ib);
+            atomicAdd(&s_h[ib], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < L; b += 256)
+        if (s_h[b]) atomicAdd(&counts[b], s_h[b]);
+}
+
+// X[rows[i]][r][:] = X[rows[i]][r][:] @ M for every row r < R (M [P][P],
+// float32, products summed in p order); one thread per (spike, row).
+__global__ void pc_map_kernel(float *__restrict__ X, const int64_t *__restrict__ rows,
+                              long long nrows, int R, int P, const float *__restrict__ M) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nrows * R) return;
+    const long long i = e / R;
+    const int r = (int)(e - i * R);
+    float *x = X + ((size_t)rows[i] * R + r) * P;
+    float v[16];
+#pragma unroll
+    for (int p = 0; p < 16; ++p) v[p] = p < P ? x[p] : 0.f;
+    for (int q = 0; q < P; ++q) {
+        float acc = 0.f;
+#pragma unroll
+        for (int p = 0; p < 16; ++p)
+            if (p < P) acc = __builtin_fmaf(v[p], M[p * P + q], acc);
+        x[q] = acc;
+    }
+}
+
 __global__ void template_sums_kernel(const float *__restrict__ tF, const int64_t *__restrict__ tmpl,
                                      const int32_t *__restrict__ lab, long long n, int R, int K,
                                      double *__restrict__ sums,
@@ -349,6 +413,31 @@ int ksm_acg(const double *times, long long n, int n_groups, const int64_t *offse
     acg_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(times, order, n, offsets, n_groups,
                                                           active, nbins, tbin, counts);
     KSM_LAUNCH_CHECK("acg_kernel");
+    return KSM_OK;
+}
+
+int ksm_ccg(const double *t1, long long n1, const double *t2, long long n2, int nbins,
+            double tbin, uint32_t *counts, void *stream) {
+    if (n1 < 0 || n2 < 0 || (n1 > 0 && !t1) || (n2 > 0 && !t2) || !counts || nbins <= 0 ||
+        nbins > kMaxBins || !(tbin > 0.0))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_ccg: bad arguments (1 <= nbins <= %d)", kMaxBins);
+    hipStream_t s = ksm::as_stream(stream);
+    const int rc = zero_async(counts, (size_t)(2 * nbins + 1) * sizeof(uint32_t), s);
+    if (rc != KSM_OK || n1 == 0 || n2 == 0) return rc;
+    ccg_kernel<<<(unsigned)((n2 + 255) / 256), 256, 0, s>>>(t1, n1, t2, n2, nbins, tbin, counts);
+    KSM_LAUNCH_CHECK("ccg_kernel");
+    return KSM_OK;
+}
+
+int ksm_pc_map(float *X, const int64_t *rows, long long nrows, int R, int P, const float *M,
+               void *stream) {
+    if (nrows < 0 || R <= 0 || P <= 0 || P > 16 || !M || (nrows > 0 && (!X || !rows)))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_pc_map: bad arguments (P <= 16)");
+    if (nrows == 0) return KSM_OK;
+    const long long total = nrows * R;
+    pc_map_kernel<<<(unsigned)((total + 255) / 256), 256, 0, ksm::as_stream(stream)>>>(
+        X, rows, nrows, R, P, M);
+    KSM_LAUNCH_CHECK("pc_map_kernel");
     return KSM_OK;
 }
 

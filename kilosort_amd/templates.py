@@ -11,17 +11,17 @@ Gram of the matching path:
   (``template_matching.py:128-141``): each template rolled in time so that
   its peak correlation with the universal waveforms sits at the centre
   (``ksm_align_templates``);
-* ``merging_function(ops, Wall, clu, st, tF, r_thresh, mode='mu',
-  check_dt, device)`` -- ``template_matching.merging_function``
-  (``template_matching.py:247-358``) in its amplitude ('mu') mode: the
-  greedy merge of the most similar templates, largest clusters first; the
-  similarity row of each step comes from the GPU Gram of the current
-  templates (recomputed after each merge);
+* ``merging_function(ops, Wall, clu, st, tF, r_thresh, mode, check_dt,
+  device)`` -- ``template_matching.merging_function``
+  (``template_matching.py:247-358``): the greedy merge of the most similar
+  templates, largest clusters first; the similarity row and lag of each step
+  come from the GPU Gram of the current templates (recomputed after each
+  merge); mode 'ccg' tests cross-refractoriness with GPU correlogram counts
+  (``ksm_acg`` for the clusters' refractoriness, ``ksm_ccg`` per candidate
+  pair), mode 'mu' the template norms; ``check_dt`` shifts the absorbed
+  cluster's features in PC space (``roll_features``, ``ksm_pc_map``);
 * ``postprocess_templates(Wall, ops, clu, st, tF, device)``
   (``template_matching.py:144-152``).
-
-The 'ccg' mode of ``merging_function`` also needs the spike-train
-cross-correlograms of ``kilosort.CCG`` (numba) and is not provided here.
 """
 from __future__ import annotations
 
@@ -29,6 +29,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from . import ccg as CG
 
 
 def _f32(t, device):
@@ -81,14 +82,79 @@ def align_U(U, ops, device=torch.device('cuda')):
     return Unew, imax.long()
 
 
-def merging_function(ops, Wall, clu, st, tF, r_thresh=0.5, mode='mu', check_dt=True,
+def shift_operator(W, dt):
+    """[P, P] float64: x -> (x @ W) shifted by dt samples with the edge value
+    repeated, projected back on W (``roll_features``'s torch.roll by dt and
+    its edge fill, ``template_matching.py:363-375``, as one linear map)."""
+    W = np.asarray(W, np.float64)
+    nt = W.shape[1]
+    Sh = np.zeros((nt, nt))
+    Sh[np.clip(np.arange(nt) - dt, 0, nt - 1), np.arange(nt)] = 1.0
+    return W @ Sh @ W.T
+
+
+def roll_features(wPCA, tF, Wall, spike_idx, clust_idx, dt, device=None):
+    """``template_matching.roll_features``: tF rows ``spike_idx`` (bool mask or
+    indices) and template ``clust_idx`` of Wall shifted by dt samples in PC
+    space, in place (``ksm_pc_map``); returns (tF, Wall)."""
+    device = _lib.require_gpu(device if device is not None else
+                              (tF.device if tF.is_cuda else torch.device('cuda')))
+    lib = _lib.load()
+    M = torch.from_numpy(shift_operator(torch.as_tensor(wPCA).double().cpu().numpy(), dt)
+                         .astype(np.float32)).to(device)
+    rows = np.nonzero(np.asarray(spike_idx))[0] if np.asarray(spike_idx).dtype == bool \
+        else np.asarray(spike_idx, np.int64)
+    X = tF if (tF.is_cuda and tF.is_contiguous() and tF.dtype == torch.float32) else \
+        tF.to(device, torch.float32).contiguous()
+    r = torch.from_numpy(rows.astype(np.int64)).to(device)
+    n, R, P = X.shape
+    _lib.check(lib.ksm_pc_map(X.data_ptr(), r.data_ptr(), len(rows), R, P, M.data_ptr(),
+                              _lib.stream_of(device)), "ksm_pc_map")
+    Wk = torch.as_tensor(Wall[clust_idx]).to(device, torch.float32).contiguous().unsqueeze(0)
+    one = torch.zeros(1, dtype=torch.int64, device=device)
+    _lib.check(lib.ksm_pc_map(Wk.data_ptr(), one.data_ptr(), 1, Wk.shape[1], P, M.data_ptr(),
+                              _lib.stream_of(device)), "ksm_pc_map")
+    if X is not tF:
+        tF.copy_(X.to(tF.device, tF.dtype))
+    if torch.is_tensor(Wall):
+        Wall[clust_idx] = Wk[0].to(Wall.device, Wall.dtype)
+    else:
+        Wall[clust_idx] = Wk[0].cpu().numpy()
+    return tF, Wall
+
+
+def cross_refractory(st0, st1, ccg_threshold=0.25, device=None):
+    """``CCG.check_CCG(st0, st1)``'s cross-refractoriness (``CCG.py:86-101``):
+    the cross-correlogram counts of the two trains (seconds) from
+    ``ksm_ccg``, the metrics of ``CCG_metrics`` on the host."""
+    device = _lib.require_gpu(device if device is not None else torch.device('cuda'))
+    lib = _lib.load()
+    nbins, tbin = 500, 1 / 1000
+    a = np.sort(np.asarray(st0, np.float64))
+    b = np.sort(np.asarray(st1, np.float64))
+    T = np.maximum(a.max(), b.max()) - np.minimum(a.min(), b.min())
+    ta = torch.from_numpy(a).to(device)
+    tb = torch.from_numpy(b).to(device)
+    counts = torch.empty(2 * nbins + 1, dtype=torch.int32, device=device)
+    _lib.check(lib.ksm_ccg(ta.data_ptr(), a.size, tb.data_ptr(), b.size, nbins, tbin,
+                           counts.data_ptr(), _lib.stream_of(device)), "ksm_ccg")
+    K = counts.cpu().numpy().view(np.uint32).astype(np.float64)
+    R12, Q12, _ = CG.ccg_metrics(a.size, b.size, K, T, nbins, tbin)
+    return bool(R12 < ccg_threshold and (Q12 < .05))
+
+
+def merging_function(ops, Wall, clu, st, tF, r_thresh=0.5, mode='ccg', check_dt=True,
                      device=torch.device('cuda')):
-    """``template_matching.merging_function`` in mode 'mu' (merge when the
-    template norms differ by less than 20 %).  Returns (Ww [K', C, P] CPU,
-    clu2, None, st, tF) sorted by spike time, as the reference."""
-    if mode != 'mu':
-        raise NotImplementedError("merging_function: only mode='mu' is provided here; "
-                                  "mode='ccg' needs kilosort.CCG's cross-correlograms")
+    """``template_matching.merging_function``: the greedy merge of the most
+    similar templates, largest clusters first, in mode 'ccg' (merge when the
+    spike trains are cross-refractory; only refractory clusters absorb) or
+    'mu' (template norms within 20 %); with check_dt the absorbed cluster's
+    features and template are shifted to the lag of the similarity maximum
+    and its spike times in ``st`` are shifted in place, as in the reference.
+    Returns (Ww [K', C, P] CPU, clu2, is_ref or None, st, tF) sorted by spike
+    time."""
+    if mode not in ('ccg', 'mu'):
+        raise ValueError(f"merging_function: unknown mode {mode!r}")
     device = _lib.require_gpu(device)
     nt = int(ops['nt'])
     clu2 = clu.copy()
@@ -97,31 +163,52 @@ def merging_function(ops, Wall, clu, st, tF, r_thresh=0.5, mode='mu', check_dt=T
     NN = len(Ww)
     isort = np.argsort(ns)[::-1]
     is_merged = np.zeros(NN, 'bool')
+    acg_threshold = ops['settings']['acg_threshold']
+    ccg_threshold = ops['settings']['ccg_threshold']
+    if mode == 'ccg':
+        is_ref, est_contam_rate = CG.refract(clu, st[:, 0] / ops['fs'],
+                                             acg_threshold=acg_threshold,
+                                             ccg_threshold=ccg_threshold, device=device)
     W = ops['wPCA']
-    if check_dt:
-        raise NotImplementedError("merging_function: check_dt=True (feature rolling) is not "
-                                  "provided here")
+    tF_dev = None
     S, lag = similarity_matrix(Ww, W, nt, device, lags=True)
     t = 0
     nmerge = 0
     while t < NN:
         kk = clu_unq[isort[t]]
+        if (mode == 'ccg') and is_ref[kk] == 0:
+            t += 1
+            continue
         if is_merged[kk]:
             t += 1
             continue
         mu = np.sqrt((Ww.astype(np.float32) ** 2).sum((1, 2), dtype=np.float32))
         cmax = S[kk].copy()
+        imax = lag[kk]
         cmax[kk] = 0
         jsort = np.argsort(cmax)[::-1]
+        if mode == 'ccg':
+            st0 = st[:, 0][clu2 == kk] / ops['fs']
         is_ccg = 0
         for j in range(NN):
             jj = jsort[j]
             if cmax[jj] < r_thresh:
                 break
-            dmu = 2 * (mu[kk] - mu[jj]) / (mu[kk] + mu[jj])
-            is_ccg = abs(dmu) < 0.2
+            if mode == 'ccg':
+                st1 = st[:, 0][clu2 == jj] / ops['fs']
+                is_ccg = cross_refractory(st0, st1, ccg_threshold, device)
+            else:
+                dmu = 2 * (mu[kk] - mu[jj]) / (mu[kk] + mu[jj])
+                is_ccg = abs(dmu) < 0.2
             if is_ccg:
                 is_merged[jj] = 1
+                dt = int(imax[kk]) - int(imax[jj])
+                if dt != 0 and check_dt:
+                    idx = (clu2 == jj)
+                    if tF_dev is None:
+                        tF_dev = torch.as_tensor(tF).to(device, torch.float32).contiguous()
+                    tF_dev, Ww = roll_features(W, tF_dev, Ww, idx, jj, dt, device=device)
+                    st[idx, 0] -= dt
                 a = np.float32(ns[kk] / (ns[kk] + ns[jj]))
                 b = np.float32(ns[jj] / (ns[kk] + ns[jj]))
                 Ww[kk] = a * Ww[kk] + b * Ww[jj]
@@ -135,15 +222,18 @@ def merging_function(ops, Wall, clu, st, tF, r_thresh=0.5, mode='mu', check_dt=T
             t += 1
         else:
             nmerge += 1
+    if tF_dev is not None:
+        tF.copy_(tF_dev.to(tF.device, tF.dtype))             # in place, as the reference
     imap = np.cumsum((~is_merged).astype('int32')) - 1
     if imap.size > 0:
         clu2 = imap[clu2]
     Ww = torch.from_numpy(Ww[~is_merged])
+    is_ref = is_ref[~is_merged] if mode == 'ccg' else None
     sorted_idx = np.argsort(st[:, 0])
     st = np.take_along_axis(st, sorted_idx[..., np.newaxis], axis=0)
     clu2 = clu2[sorted_idx]
-    tF = tF[torch.from_numpy(sorted_idx)]
-    return Ww, clu2, None, st, tF
+    tF = tF[torch.from_numpy(sorted_idx).to(tF.device)]
+    return Ww, clu2, is_ref, st, tF
 
 
 def postprocess_templates(Wall, ops, clu, st, tF, device=torch.device('cuda')):

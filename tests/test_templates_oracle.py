@@ -70,7 +70,7 @@ def test_mu_merge_of_planted_duplicates():
     clu = np.array([0] * 5 + [1] * 3 + [2] * 4)
     st = np.stack([rng.permutation(12).astype(float), np.zeros(12)], 1)
     tF = torch.arange(12 * 2, dtype=torch.float64).reshape(12, 2)
-    ops = dict(nt=nt, wPCA=W)
+    ops = dict(nt=nt, wPCA=W, settings=dict(acg_threshold=0.2, ccg_threshold=0.25))
     Ww, clu2, _, st2, tF2 = TO.merging_function(ops, Wall, clu, st, tF, 0.9)
     assert Ww.shape[0] == 2
     np.testing.assert_allclose(Ww[0].numpy(), (5 * w0 + 3 * 1.1 * w0) / 8, atol=1e-12)
@@ -78,3 +78,49 @@ def test_mu_merge_of_planted_duplicates():
     np.testing.assert_array_equal(st2[:, 0], np.sort(st[:, 0]))
     np.testing.assert_array_equal(clu2, np.where(clu == 2, 1, 0)[order])
     np.testing.assert_array_equal(tF2.numpy(), tF.numpy()[order])
+
+
+def test_roll_features_moves_the_peak_by_dt():
+    """A waveform in the span of orthonormal PCs, peaked at t0: after the
+    roll, the waveform of the new features peaks at t0 + dt (both signs)."""
+    nt, P = 61, 12
+    t = np.arange(nt)
+    w = np.exp(-0.5 * ((t - 30) / 3.0) ** 2)
+    rng = np.random.default_rng(8)
+    q, _ = np.linalg.qr(np.column_stack([w, np.roll(w, 4), np.roll(w, -4),
+                                         rng.standard_normal((nt, P - 3))]))
+    W = torch.from_numpy(q.T.copy())                          # [P, nt], w in the span
+    x = torch.from_numpy((w @ q)[None, None, :].copy())      # one spike, one channel
+    Wall = torch.from_numpy(np.repeat((w @ q)[None, None, :], 2, 0).copy())
+    for dt in (4, -4):
+        tF, Wl = TO.roll_features(W, x.clone(), Wall.clone(), np.array([True]), 1, dt)
+        assert int(np.argmax((tF[0, 0] @ W).numpy())) == 30 + dt
+        assert int(np.argmax((Wl[1, 0] @ W).numpy())) == 30 + dt
+        np.testing.assert_array_equal(Wl[0].numpy(), Wall[0].numpy())   # other rows untouched
+
+
+def test_ccg_mode_merges_a_split_refractory_unit():
+    """Two clusters with one template, made by splitting one refractory train,
+    are cross-refractory and merge; a third cluster with the same template but
+    an independent Poisson train stays apart."""
+    rng = np.random.default_rng(9)
+    C, P, nt, fs = 8, 6, 61, 30000.0
+    W = basis(nt, P, 10)
+    w0 = rng.standard_normal((C, P))
+    Wall = torch.from_numpy(np.stack([w0, w0, w0]))
+    unit = np.cumsum(0.004 + rng.exponential(0.03, 3000))    # refractory, ~30 Hz
+    half = rng.uniform(size=unit.size) < 0.5
+    other = np.sort(rng.uniform(0, unit.max(), 1500))
+    times = np.concatenate([unit, other])
+    clu = np.concatenate([np.where(half, 0, 1), np.full(other.size, 2)])
+    o = np.argsort(times)
+    st = np.stack([np.round(times[o] * fs), np.zeros(o.size)], 1)
+    clu = clu[o]
+    tF = torch.zeros((o.size, 2))
+    ops = dict(nt=nt, wPCA=W, fs=fs, settings=dict(acg_threshold=0.2, ccg_threshold=0.25))
+    Ww, clu2, is_ref, st2, _ = TO.merging_function(ops, Wall, clu, st, tF, 0.5, 'ccg',
+                                                   check_dt=False)
+    assert Ww.shape[0] == 2
+    assert len(np.unique(clu2)) == 2
+    merged = clu2[np.isin(clu[np.argsort(st[:, 0], kind='stable')], [0, 1])]
+    assert len(np.unique(merged)) == 1
