@@ -286,6 +286,81 @@ def time_templates(ops, U, device, reps=5, cpu_baseline=True):
     return out
 
 
+def phy_workload(ops, iCC, iU, iCC_mask, n, seed, n_clu=500, hours=1.0):
+    """A sorted recording's export inputs: n spikes over `hours` from the
+    bench's 700 detection templates at 383 channels, n_clu clusters (each
+    template owned by one cluster, 10 % of spikes reassigned at random), 3 %
+    artefact duplicates 4 samples apart, features [n, 10, 6]."""
+    rng = np.random.default_rng(seed)
+    K = int(iU.shape[0])
+    t = np.sort(rng.integers(0, int(hours * 3600 * 30000), n)).astype(np.float64)
+    dup = rng.uniform(size=n) < 0.03
+    t[1:][dup[1:]] = t[:-1][dup[1:]] + 4
+    t = np.sort(t)
+    tmpl = rng.integers(0, K, n)
+    owner = rng.permutation(np.arange(K) % n_clu)
+    clu = owner[tmpl]
+    split = rng.uniform(size=n) < 0.1
+    clu[split] = rng.integers(0, n_clu, split.sum())
+    st = np.stack([t, tmpl, rng.uniform(8, 30, n)], 1)
+    tF = torch.from_numpy(rng.standard_normal((n, 10, 6)).astype(np.float32))
+    pops = dict(iU=iU, iCC=iCC, iCC_mask=iCC_mask, xc=ops["xc"].astype(np.float32),
+                yc=ops["yc"].astype(np.float32), nearest_chans=10, dmin=20.0, dminx=32.0,
+                fs=30000.0, duplicate_spike_bins=7)
+    return st, clu.astype(np.int32), tF, pops
+
+
+def time_phy(ops, iCC, iU, iCC_mask, device, n=2_000_000, n_sample=200_000, seed=9,
+             cpu_baseline=True):
+    """The Phy export's per-spike work on n spikes (positions and amplitudes,
+    duplicate removal, refractoriness, PC features): wall time per function
+    with features resident on the device; the oracle on the first n_sample
+    spikes as the CPU baseline."""
+    from kilosort_amd import ccg as CG, postprocessing as PP
+    st, clu, tF, pops = phy_workload(ops, iCC, iU, iCC_mask, n, seed)
+    tFd = tF.to(device)
+    t_s = st[:, 0].astype(np.int64)
+    steps = {
+        "positions_amplitudes": lambda: PP.positions_and_amplitudes(st, tFd, pops, device),
+        "remove_duplicates": lambda: PP.remove_duplicates(t_s, clu, 7, device=device),
+        "refract": lambda: CG.refract(clu, t_s / 30000.0, device=device),
+        "make_pc_features": lambda: PP.make_pc_features(pops, st[:, 1].astype(np.int32), clu,
+                                                        tFd.clone(), device=device),
+    }
+    out = {"spikes": n, "clusters": int(clu.max()) + 1, "templates": int(iU.shape[0]),
+           "ms": {}}
+    for name, fn in steps.items():
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        out["ms"][name] = (time.perf_counter() - t0) * 1e3
+    total = sum(out["ms"].values())
+    out["value"] = n / (total / 1e3)
+    out["unit"] = "spikes/s"
+    out["note"] = ("wall time of one call each, features resident on the device, results "
+                   "copied to host numpy as the reference returns them")
+    if cpu_baseline:
+        from oracle import phy_oracle as PH, positions_oracle as PO
+        m = n_sample
+        sts, clus, tFs = st[:m], clu[:m], tF[:m].clone()
+        cops = dict(pops, iU=iU.cpu(), iCC=iCC.cpu(), iCC_mask=iCC_mask.cpu())
+        t0 = time.perf_counter()
+        PO.compute_spike_positions(sts, tFs, cops)
+        torch.norm(tFs, dim=[-2, -1])
+        PH.remove_duplicates(sts[:, 0].astype(np.int64), clus, dt=7)
+        PH.refract(clus, sts[:, 0].astype(np.int64) / 30000.0)
+        PH.make_pc_features(cops, sts[:, 1].astype(np.int32), clus, tFs)
+        sec = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=m / sec, unit="spikes/s", seconds=sec,
+                                   cores=torch.get_num_threads(), kind="port",
+                                   sample=f"first {m} spikes of the same workload: oracle "
+                                          "positions, amplitudes, remove_duplicates, refract, "
+                                          "make_pc_features")
+    return out
+
+
 def time_detect(ops, batches, device, reps=10, cpu_baseline=True):
     """Universal-template detection (spikedetect.template_match + the
     per-batch features of run) on the bench's 383-channel batches: device time
@@ -563,6 +638,8 @@ def main():
                     help="time only the batch preprocessing and print its JSON object")
     ap.add_argument("--detect-only", action="store_true",
                     help="time only the universal-template detection and print its JSON object")
+    ap.add_argument("--phy-only", action="store_true",
+                    help="time only the Phy export's per-spike work and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the matching path is sharded by batch "
@@ -590,7 +667,12 @@ def main():
                                      cpu_baseline=not args.no_cpu_baseline)), flush=True)
         return
     U = torch.from_numpy(rec.U).to(device)
-    iCC, _, iU, Ucc = tm.prepare_extract(ops["xc"], ops["yc"], U, 10, 100.0, device=device)
+    iCC, iCC_mask, iU, Ucc = tm.prepare_extract(ops["xc"], ops["yc"], U, 10, 100.0,
+                                                device=device)
+    if args.phy_only:
+        print(json.dumps(time_phy(ops, iCC, iU, iCC_mask, device,
+                                  cpu_baseline=not args.no_cpu_baseline)), flush=True)
+        return
     ctc = tm.prepare_matching(ops, U)
     nnzK = (ctc != 0).any(dim=2).sum(dim=0).cpu().numpy().astype(np.int64)
     nnzC = (U != 0).any(dim=1).sum(dim=1).cpu().numpy().astype(np.int64)
@@ -660,6 +742,10 @@ def main():
     log("template similarity and alignment (700 templates)")
     tmpl = time_templates(ops, U, device, cpu_baseline=not args.no_cpu_baseline)
     log(f"templates: similarity {tmpl['similarity_ms']:.2f} ms, align_U {tmpl['align_U_ms']:.2f} ms")
+
+    log("Phy export per-spike work (2 M spikes)")
+    phy = time_phy(ops, iCC, iU, iCC_mask, device, cpu_baseline=not args.no_cpu_baseline)
+    log("phy: " + ", ".join(f"{k}={v:.1f} ms" for k, v in phy["ms"].items()))
 
     log("universal-template detection at 383 channels")
     det = time_detect(ops, batches, device, cpu_baseline=not args.no_cpu_baseline)
@@ -732,6 +818,7 @@ def main():
         "preprocess": pre,
         "detect": det,
         "templates": tmpl,
+        "phy": phy,
         "ragged": ragged,
         "cpu_baseline": cpu,
     }

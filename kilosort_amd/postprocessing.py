@@ -131,28 +131,26 @@ def _channel_maps(iC, sums, counts, presence, clusters, n_chans):
     cluster re-reads rows that an earlier cluster holding the same template
     has already rewritten, as the reference does)."""
     nC, K = iC.shape
+    P = sums.shape[-1]
     sigma = np.tile(np.arange(nC, dtype=np.int32), (K, 1))
     feature_ind = np.zeros((clusters.size, n_chans), dtype=np.uint32)
-    zero = np.zeros(sums.shape[-1])
+    rows = np.arange(nC)
     for i in clusters:
         S = np.nonzero(presence[i])[0]
         ichan, imap = np.unique(iC[:, S], return_inverse=True)
-        imap = imap.reshape(nC, S.size)
-        acc = np.zeros((ichan.size, sums.shape[-1]))
-        for col, t in enumerate(S):
-            for j in range(nC):
-                q = sigma[t, j]
-                acc[imap[j, col]] += sums[t, q] if q >= 0 else zero
+        pos = imap.reshape(nC, S.size).T                       # [|S|, nC] union positions
+        sig = sigma[S]                                         # current source rows
+        vals = np.where((sig >= 0)[..., None], sums[S[:, None], np.maximum(sig, 0)], 0.0)
+        acc = np.zeros((ichan.size, P))
+        np.add.at(acc, pos.ravel(), vals.reshape(-1, P))       # template-major, as a loop
         mean = acc / counts[S].sum()
         norm = np.sqrt((mean ** 2).sum(1))
         ind = np.argsort(-norm, kind='stable')[:n_chans]
         feature_ind[i, :] = ichan[ind]
-        for col, t in enumerate(S):
-            row_of = np.full(ichan.size, -1, np.int64)
-            row_of[imap[:, col]] = np.arange(nC)
-            src = row_of[ind]
-            old = sigma[t].copy()
-            sigma[t] = np.where(src >= 0, old[np.maximum(src, 0)], -1)
+        row_of = np.full((S.size, ichan.size), -1, np.int64)
+        row_of[np.arange(S.size)[:, None], pos] = rows[None, :]
+        src = row_of[:, ind]                                   # [|S|, n_chans]
+        sigma[S] = np.where(src >= 0, np.take_along_axis(sig, np.maximum(src, 0), 1), -1)
     return sigma, feature_ind
 
 
