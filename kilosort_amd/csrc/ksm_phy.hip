@@ -11,9 +11,10 @@
 //   order.  Three passes: per-chunk label counts in LDS, a per-label scan
 //   over chunks, and an in-order placement where each wave ranks equal labels
 //   among its 64 lanes by bitwise ballots.
-// * ksm_remove_duplicates: one thread per label walks its spikes in input
-//   order and keeps a spike when it is at least dt samples after the last
-//   kept spike of its label (the first spike of a label is kept).
+// * ksm_remove_duplicates: one wave per label walks its spikes in input
+//   order, 64 at a time, and keeps a spike when it is at least dt samples
+//   after the last kept spike of its label (the first spike of a label is
+//   kept).
 // * ksm_acg: autocorrelogram counts of every active label: for every spike j
 //   of a label, the spikes k of the same label with t_j - dt < t_k <=
 //   t_j + dt (dt = nbins * tbin) land in bin rint((t_j - t_k) / tbin) +
@@ -146,20 +147,34 @@ __global__ __launch_bounds__(kGrpThreads) void group_place_kernel(
     }
 }
 
-__global__ void dedup_kernel(const int64_t *__restrict__ times, const int64_t *__restrict__ off,
-                             const int64_t *__restrict__ order, int G, long long dt,
-                             uint8_t *__restrict__ keep) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= G) return;
+// One wave per label.  Per chunk of 64 consecutive spikes of the label, the
+// kept ones are found in order by ballots: the next kept spike is the first
+// lane after the previous kept one with t >= t0 + dt, and t0 becomes its time.
+__global__ __launch_bounds__(64) void dedup_kernel(const int64_t *__restrict__ times,
+                                                   const int64_t *__restrict__ off,
+                                                   const int64_t *__restrict__ order, int G,
+                                                   long long dt, uint8_t *__restrict__ keep) {
+    const int g = blockIdx.x;
+    const int lane = threadIdx.x;
     const long long a = off[g], b = off[g + 1];
     if (a == b) return;
     long long t0 = times[order[a]] - dt;
-    for (long long p = a; p < b; ++p) {
-        const long long i = order[p];
-        const long long t = times[i];
-        const bool k = t >= t0 + dt;
-        if (k) t0 = t;
-        keep[i] = k ? 1 : 0;
+    for (long long p0 = a; p0 < b; p0 += 64) {
+        const long long p = p0 + lane;
+        const bool valid = p < b;
+        const long long i = valid ? order[p] : 0;
+        const long long t = valid ? times[i] : 0;
+        uint64_t rest = __ballot(valid);
+        uint64_t kept = 0;
+        while (true) {
+            const uint64_t cand = __ballot(valid && t >= t0 + dt) & rest;
+            if (!cand) break;
+            const int l = __ffsll((unsigned long long)cand) - 1;
+            kept |= 1ull << l;
+            t0 = __shfl(t, l, 64);
+            rest = l == 63 ? 0ull : rest & ~((2ull << l) - 1ull);
+        }
+        if (valid) keep[i] = (kept >> lane) & 1ull ? 1 : 0;
     }
 }
 
@@ -395,8 +410,8 @@ int ksm_remove_duplicates(const int64_t *times, long long n, int n_groups, const
         (n > 0 && (!times || !order || !keep)))
         return ksm::fail(KSM_ERR_INVALID, "ksm_remove_duplicates: bad arguments");
     if (n == 0) return KSM_OK;
-    dedup_kernel<<<(n_groups + 255) / 256, 256, 0, ksm::as_stream(stream)>>>(
-        times, offsets, order, n_groups, dt, keep);
+    dedup_kernel<<<n_groups, 64, 0, ksm::as_stream(stream)>>>(times, offsets, order, n_groups,
+                                                              dt, keep);
     KSM_LAUNCH_CHECK("dedup_kernel");
     return KSM_OK;
 }

@@ -133,19 +133,19 @@ __global__ void roll_project_kernel(const float *__restrict__ Y, const float *__
 // w_j = m_j / sum_j m_j, (xs, ys) = sum_j w_j (xc, yc)[iCC[j, u]]
 // (iCC, mask [nC][C] over the C channels); amp[s] = ||tF[s]||_F when amp is
 // given (the Phy amplitudes, io.py:330).
-__global__ void positions_kernel(const float *__restrict__ tF, const int64_t *__restrict__ ids,
-                                 long long n, int nC, int P, const int64_t *__restrict__ iU,
-                                 const int64_t *__restrict__ iCC, const uint8_t *__restrict__ mask,
-                                 int C, const float *__restrict__ xc, const float *__restrict__ yc,
-                                 float *__restrict__ xs, float *__restrict__ ys,
-                                 float *__restrict__ amp) {
-    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
+template <typename Row>
+__device__ __forceinline__ void position_of(Row f0, long long s, const int64_t *__restrict__ ids,
+                                            int nC, int P, const int64_t *__restrict__ iU,
+                                            const int64_t *__restrict__ iCC,
+                                            const uint8_t *__restrict__ mask, int C,
+                                            const float *__restrict__ xc,
+                                            const float *__restrict__ yc, float *__restrict__ xs,
+                                            float *__restrict__ ys, float *__restrict__ amp) {
     const int u = (int)iU[ids[s]];
     float m[32];
     float tot = 0.f, all = 0.f;
     for (int j = 0; j < nC; ++j) {
-        const float *f = tF + ((size_t)s * nC + j) * P;
+        Row f = f0 + j * P;
         float a = 0.f;
         for (int p = 0; p < P; ++p) a = __builtin_fmaf(f[p], f[p], a);
         all += a;
@@ -163,6 +163,44 @@ __global__ void positions_kernel(const float *__restrict__ tF, const int64_t *__
     }
     xs[s] = x;
     ys[s] = y;
+}
+
+// One thread per spike; the block's rows (contiguous in tF) are first copied
+// to LDS with coalesced loads, each row padded to an odd stride R + 1.
+__global__ void positions_staged_kernel(const float *__restrict__ tF,
+                                        const int64_t *__restrict__ ids, long long n, int nC,
+                                        int P, const int64_t *__restrict__ iU,
+                                        const int64_t *__restrict__ iCC,
+                                        const uint8_t *__restrict__ mask, int C,
+                                        const float *__restrict__ xc,
+                                        const float *__restrict__ yc, float *__restrict__ xs,
+                                        float *__restrict__ ys, float *__restrict__ amp) {
+    extern __shared__ float s_rows[];
+    const int R = nC * P;
+    const long long s0 = (long long)blockIdx.x * blockDim.x;
+    const long long nb = n - s0 < (long long)blockDim.x ? n - s0 : (long long)blockDim.x;
+    const float *src = tF + (size_t)s0 * R;
+    const int total = (int)nb * R;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        const int q = e / R;
+        s_rows[q * (R + 1) + (e - q * R)] = src[e];
+    }
+    __syncthreads();
+    if (threadIdx.x >= nb) return;
+    position_of(s_rows + threadIdx.x * (R + 1), s0 + threadIdx.x, ids, nC, P, iU, iCC, mask, C,
+                xc, yc, xs, ys, amp);
+}
+
+// The same from global memory, for rows too long to stage (nC P > 255).
+__global__ void positions_kernel(const float *__restrict__ tF, const int64_t *__restrict__ ids,
+                                 long long n, int nC, int P, const int64_t *__restrict__ iU,
+                                 const int64_t *__restrict__ iCC, const uint8_t *__restrict__ mask,
+                                 int C, const float *__restrict__ xc, const float *__restrict__ yc,
+                                 float *__restrict__ xs, float *__restrict__ ys,
+                                 float *__restrict__ amp) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    position_of(tF + (size_t)s * nC * P, s, ids, nC, P, iU, iCC, mask, C, xc, yc, xs, ys, amp);
 }
 
 __global__ void clear_flag_t_kernel(int *__restrict__ f) { *f = 0; }
@@ -251,9 +289,18 @@ int ksm_spike_positions(const float *tF, const int64_t *ids, long long n, int nC
         nC <= 0 || nC > 32 || P <= 0 || C <= 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_spike_positions: bad arguments (nC <= 32)");
     if (n == 0) return KSM_OK;
-    positions_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ksm::as_stream(stream)>>>(
-        tF, ids, n, nC, P, iU, iCC, mask, C, xc, yc, xs, ys, amp);
-    KSM_LAUNCH_CHECK("positions_kernel");
+    const int R = nC * P;
+    if (R <= 255) {
+        const int spb = (R + 1) * 256 * 4 <= 65536 ? 256 : ((R + 1) * 128 * 4 <= 65536 ? 128 : 64);
+        positions_staged_kernel<<<(unsigned)((n + spb - 1) / spb), spb,
+                                  (size_t)spb * (R + 1) * sizeof(float), ksm::as_stream(stream)>>>(
+            tF, ids, n, nC, P, iU, iCC, mask, C, xc, yc, xs, ys, amp);
+        KSM_LAUNCH_CHECK("positions_staged_kernel");
+    } else {
+        positions_kernel<<<(unsigned)((n + 255) / 256), 256, 0, ksm::as_stream(stream)>>>(
+            tF, ids, n, nC, P, iU, iCC, mask, C, xc, yc, xs, ys, amp);
+        KSM_LAUNCH_CHECK("positions_kernel");
+    }
     return KSM_OK;
 }
 
