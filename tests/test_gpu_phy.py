@@ -216,3 +216,13 @@ def test_save_to_phy_matches_oracle(tmp_path):
     amp_rows = (tmp_path / 'cluster_Amplitude.tsv').read_text().splitlines()[1:]
     assert amp_rows == [f'{i}\t{p:.1f}' for i, p in enumerate(want['_template_amplitudes'])]
     assert (tmp_path / 'params.py').read_text().splitlines()[0] == f"n_channels_dat = {Wall.shape[1]}"
+
+
+def test_empty_inputs():
+    ts, cs, keep = PP.remove_duplicates(np.zeros(0, np.int64), np.zeros(0, np.int32), 7,
+                                        device=DEV)
+    assert ts.size == 0 and cs.size == 0 and keep.size == 0
+    ops, _, _, _ = pc_case(1, 100, 5, 16, 4, 2)
+    pcf, fi = PP.make_pc_features(ops, np.zeros(0, np.int64), np.zeros(0, np.int32),
+                                  torch.zeros((0, 4, 6)), device=DEV)
+    assert pcf.shape == (0, 6, 4) and fi.shape == (0, 4)

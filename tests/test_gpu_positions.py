@@ -28,8 +28,12 @@ def _gpu():
     yield
 
 
-@pytest.mark.parametrize("C,K,n,nC", [(64, 64, 5000, 10), (383, 700, 200000, 10), (96, 40, 3000, 32)])
-def test_positions_match_oracle(C, K, n, nC):
+@pytest.mark.parametrize("C,K,n,nC,P", [(64, 64, 5000, 10, 6), (383, 700, 200000, 10, 6),
+                                        (64, 30, 2000, 16, 6), (96, 40, 3000, 32, 6),
+                                        (96, 40, 3001, 32, 9)])
+def test_positions_match_oracle(C, K, n, nC, P):
+    """nC P <= 255 stages the rows through LDS (blocks of 256, 128 or 64
+    spikes); 32 x 9 = 288 takes the unstaged kernel."""
     rec = synth.make_recording(80 + C, C, K, 1000, with_data=False)
     U = torch.from_numpy(rec.U).to(DEV)
     iCC, iCC_mask, iU, _ = TMg.prepare_extract(rec.xc, rec.yc, U, nC, 60.0, device=DEV)
@@ -38,9 +42,9 @@ def test_positions_match_oracle(C, K, n, nC):
     rng = np.random.default_rng(C)
     st = np.zeros((n, 6))
     st[:, 1] = rng.integers(0, K, n)
-    tF = torch.from_numpy(rng.standard_normal((n, nC, 6)).astype(np.float32))
+    tF = torch.from_numpy(rng.standard_normal((n, nC, P)).astype(np.float32))
     tF[:3] = 0                                              # no mass: 0/0
-    print(f"[gpu-positions] C={C} K={K} n={n} nC={nC}", flush=True)
+    print(f"[gpu-positions] C={C} K={K} n={n} nC={nC} P={P}", flush=True)
     sys.stdout.flush()
     xs, ys = PP.compute_spike_positions(st, tF.to(DEV), ops, device=DEV)
     rx, ry = PO.compute_spike_positions(st, tF, ops)
