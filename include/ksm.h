@@ -475,6 +475,25 @@ int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *label
                       uint8_t *presence, void *stream);
 
 /*
+ * The per-cluster channel ranking of make_pc_features (postprocessing.py:
+ * 104-127), replayed in ascending cluster order on the output of
+ * ksm_template_sums: for each label g holding spikes, the channels of its
+ * templates (iC [nC][K] int64: channel of feature row j of template t, < C),
+ * the norm of the mean current feature row per channel, the nC channels of
+ * largest norm (ties: lowest channel) into feature_ind[g][0..nC) (uint32,
+ * [n_groups][nC]; rows of labels without spikes untouched), and sigma
+ * [K][nC] (int32; row k of template t's spikes <- original row sigma[t][k],
+ * or zeros for -1) composed with each choice, as the reference's in-place
+ * rewrites of tF do.  sigma_tmp [K][nC] is scratch.  One block; limits
+ * nC <= 32 and C (8 P + 12) + 2 K (nC + 1) <= 65536 bytes of LDS (e.g. 383
+ * channels, 6 PCs and up to 1850 templates of 10 channels).
+ */
+int ksm_pc_channel_rank(const double *sums, const uint64_t *counts, const uint8_t *presence,
+                        const int64_t *iC, int K, int nC, int P, int C, int n_groups,
+                        int32_t *sigma, int32_t *sigma_tmp, uint32_t *feature_ind,
+                        void *stream);
+
+/*
  * The feature rewrite of make_pc_features (postprocessing.py:120-127):
  * out[s][k][p] (layout 0) or out[s][p][k] (layout 1, the Phy pc_features
  * layout) = tF[s][sigma[t][k]][p], or 0 where sigma[t][k] < 0, t = the

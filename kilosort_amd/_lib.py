@@ -97,6 +97,7 @@ SIGNATURES = {
                      _p]),
     "ksm_pc_map": (_i, [_p, _p, ctypes.c_longlong, _i, _i, _p, _p]),
     "ksm_template_sums": (_i, [_p, _p, _p, ctypes.c_longlong, _i, _i, _i, _i, _p, _p, _p, _p]),
+    "ksm_pc_channel_rank": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p]),
     "ksm_gather_features": (_i, [_p, _p, ctypes.c_longlong, _i, _i, _p, _p, _i, _p]),
     "ksm_detect_workspace": (_sz, [_i, _i, _i, _i]),
     "ksm_detect": (_i, [_p, _i, _i, _i, _p, _i, _i, _p, _i, _p, _i, _p, _i, _i, _i, _f, _p, _p,

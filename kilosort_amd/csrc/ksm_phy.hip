@@ -26,6 +26,8 @@
 // * ksm_template_sums: per detection template, the double sums of the PC
 //   features of its spikes and their count, and which templates each label
 //   holds (the inputs of the per-cluster channel ranking).
+// * ksm_pc_channel_rank: the per-cluster channel ranking of make_pc_features
+//   replayed in cluster order on those sums (one block).
 // * ksm_gather_features: each spike's features re-indexed by its template's
 //   channel map (sigma[t][k] = source channel row, or -1 for zeros).
 #include "ksm_common.h"
@@ -345,6 +347,141 @@ __global__ void gather_features_kernel(const float *__restrict__ tF,
     }
 }
 
+// The cluster-by-cluster channel ranking of make_pc_features, one block
+// walking the clusters in ascending order (each depends on the earlier ones
+// through the shared templates' maps).  Per cluster g: the templates it holds
+// (presence), the sum over their spikes of each current feature row placed at
+// its channel (acc[c][p], float64, from the per-template sums and each
+// template's map sigma), the norm of the mean per channel, the n_chans
+// channels of largest norm (ties: lowest channel), and each held template's
+// map composed with the choice.
+constexpr int kRankThreads = 1024;
+
+__device__ __forceinline__ void better(double &v, int &c, double v2, int c2) {
+    if (v2 > v || (v2 == v && c2 < c)) {
+        v = v2;
+        c = c2;
+    }
+}
+
+__global__ __launch_bounds__(kRankThreads) void pc_rank_kernel(
+    const double *__restrict__ sums, const unsigned long long *__restrict__ counts,
+    const uint8_t *__restrict__ presence, const int64_t *__restrict__ iC, int K, int nC, int P,
+    int C, int G, int32_t *__restrict__ sigma, int32_t *__restrict__ sigma_new,
+    uint32_t *__restrict__ feature_ind) {
+    extern __shared__ double s_dyn[];
+    double *s_acc = s_dyn;                                        // [C][P]
+    double *s_norm = s_acc + (size_t)C * P;                       // [C]
+    int *s_mark = reinterpret_cast<int *>(s_norm + C);            // [C]
+    short *s_ic = reinterpret_cast<short *>(s_mark + C);          // [K][nC] channels
+    short *s_held = s_ic + (size_t)K * nC;                        // [K] held templates
+    __shared__ double s_wv[kRankThreads / 64];
+    __shared__ int s_wc[kRankThreads / 64];
+    __shared__ int s_top[32];
+    __shared__ int s_nheld;
+    __shared__ unsigned long long s_N;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int KN = K * nC;
+    for (int e = tid; e < KN; e += kRankThreads) {
+        const int t = e / nC, j = e - t * nC;
+        sigma[e] = j;
+        s_ic[e] = (short)iC[(size_t)j * K + t];
+    }
+    __syncthreads();
+    for (int g = 0; g < G; ++g) {
+        const uint8_t *pres = presence + (size_t)g * K;
+        for (int e = tid; e < C * P; e += kRankThreads) s_acc[e] = 0.0;
+        for (int c = tid; c < C; c += kRankThreads) s_mark[c] = 0;
+        if (tid == 0) {
+            s_N = 0;
+            s_nheld = 0;
+        }
+        __syncthreads();
+        // the held templates, compacted in ascending order wave by wave
+        for (int t0 = 0; t0 < K; t0 += kRankThreads) {
+            const int t = t0 + tid;
+            const bool h = t < K && pres[t];
+            const uint64_t m = __ballot(h);
+            if (lane == 0) s_wc[wv] = __popcll(m);
+            __syncthreads();
+            int base = s_nheld;
+            for (int w = 0; w < wv; ++w) base += s_wc[w];
+            if (h) s_held[base + __popcll(m & ((1ull << lane) - 1ull))] = (short)t;
+            __syncthreads();
+            if (tid == 0) {
+                int tot = 0;
+                for (int w = 0; w < kRankThreads / 64; ++w) tot += s_wc[w];
+                s_nheld += tot;
+            }
+            __syncthreads();
+        }
+        const int nh = s_nheld;
+        if (nh == 0) continue;                   // label without spikes (block-uniform)
+        for (int e = tid; e < nh * nC; e += kRankThreads) {
+            const int i = e / nC, j = e - i * nC;
+            const int t = s_held[i];
+            const int c = s_ic[t * nC + j];
+            s_mark[c] = 1;
+            const int q = sigma[t * nC + j];
+            if (q >= 0)
+                for (int p = 0; p < P; ++p)
+                    atomicAdd(&s_acc[c * P + p], sums[((size_t)t * nC + q) * P + p]);
+            if (j == 0) atomicAdd(&s_N, counts[t]);
+        }
+        __syncthreads();
+        const double dN = (double)s_N;
+        for (int c = tid; c < C; c += kRankThreads) {
+            double a = 0.0;
+            for (int p = 0; p < P; ++p) {
+                const double m = s_acc[c * P + p] / dN;
+                a += m * m;
+            }
+            s_norm[c] = s_mark[c] ? sqrt(a) : -1.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < nC; ++k) {
+            double v = -3.0;
+            int bc = 0x7fffffff;
+            for (int c = tid; c < C; c += kRankThreads) better(v, bc, s_norm[c], c);
+            for (int o = 32; o > 0; o >>= 1) {
+                const double v2 = __shfl_xor(v, o, 64);
+                const int c2 = __shfl_xor(bc, o, 64);
+                better(v, bc, v2, c2);
+            }
+            if (lane == 0) {
+                s_wv[wv] = v;
+                s_wc[wv] = bc;
+            }
+            __syncthreads();
+            if (tid == 0) {
+                double bv = s_wv[0];
+                int bcc = s_wc[0];
+                for (int w = 1; w < kRankThreads / 64; ++w) better(bv, bcc, s_wv[w], s_wc[w]);
+                s_top[k] = bcc;
+                s_norm[bcc] = -2.0;              // taken
+                feature_ind[(size_t)g * nC + k] = (uint32_t)bcc;
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < nh * nC; e += kRankThreads) {
+            const int i = e / nC, k = e - i * nC;
+            const int t = s_held[i];
+            const int c = s_top[k];
+            int src = -1;
+            for (int j = 0; j < nC; ++j)
+                if (s_ic[t * nC + j] == c) src = j;
+            sigma_new[t * nC + k] = src >= 0 ? sigma[t * nC + src] : -1;
+        }
+        __syncthreads();
+        for (int e = tid; e < nh * nC; e += kRankThreads) {
+            const int t = s_held[e / nC];
+            const int k = e - (e / nC) * nC;
+            sigma[t * nC + k] = sigma_new[t * nC + k];
+        }
+        __syncthreads();
+    }
+}
+
 struct GroupWs {
     uint32_t *counts;
     int64_t *total;
@@ -473,6 +610,25 @@ int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *label
     template_sums_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
         tF, tmpl, labels, n, R, K, sums, reinterpret_cast<unsigned long long *>(counts), presence);
     KSM_LAUNCH_CHECK("template_sums_kernel");
+    return KSM_OK;
+}
+
+int ksm_pc_channel_rank(const double *sums, const uint64_t *counts, const uint8_t *presence,
+                        const int64_t *iC, int K, int nC, int P, int C, int n_groups,
+                        int32_t *sigma, int32_t *sigma_tmp, uint32_t *feature_ind,
+                        void *stream) {
+    const size_t lds = (size_t)C * P * sizeof(double) + (size_t)C * sizeof(double) +
+                       (size_t)C * sizeof(int) + ((size_t)K * nC + K) * sizeof(short);
+    if (!sums || !counts || !presence || !iC || !sigma || !sigma_tmp || !feature_ind || K <= 0 ||
+        nC <= 0 || nC > 32 || P <= 0 || C <= 0 || C > 32767 || K > 32767 || n_groups <= 0 ||
+        lds > 65536)
+        return ksm::fail(KSM_ERR_INVALID,
+                         "ksm_pc_channel_rank: bad arguments (nC <= 32, "
+                         "C (8 P + 12) + 2 K (nC + 1) <= 65536)");
+    pc_rank_kernel<<<1, kRankThreads, lds, ksm::as_stream(stream)>>>(
+        sums, reinterpret_cast<const unsigned long long *>(counts), presence, iC, K, nC, P, C,
+        n_groups, sigma, sigma_tmp, feature_ind);
+    KSM_LAUNCH_CHECK("pc_rank_kernel");
     return KSM_OK;
 }
 

@@ -14,8 +14,8 @@ outputs:
 * ``make_pc_features(ops, spike_templates, spike_clusters, tF)``
   (``:65-130``): per cluster, the channels of its templates ranked by the
   norm of the mean feature, and each spike's features re-ordered to the
-  top ``nearest_chans`` of them (``ksm_template_sums`` +
-  ``ksm_gather_features``).
+  top ``nearest_chans`` of them (``ksm_template_sums``,
+  ``ksm_pc_channel_rank``, ``ksm_gather_features``).
 """
 from __future__ import annotations
 
@@ -123,37 +123,6 @@ def remove_duplicates(spike_times, spike_clusters, dt=15, device=torch.device('c
     return spike_times[keep], spike_clusters[keep], keep
 
 
-def _channel_maps(iC, sums, counts, presence, clusters, n_chans):
-    """The per-cluster channel ranking of ``make_pc_features`` from the
-    per-template feature sums, replaying its in-place updates: sigma[t][k]
-    = the original channel row of template t's spikes now in row k (or -1),
-    composed over the clusters that hold t in ascending cluster order (a
-    cluster re-reads rows that an earlier cluster holding the same template
-    has already rewritten, as the reference does)."""
-    nC, K = iC.shape
-    P = sums.shape[-1]
-    sigma = np.tile(np.arange(nC, dtype=np.int32), (K, 1))
-    feature_ind = np.zeros((clusters.size, n_chans), dtype=np.uint32)
-    rows = np.arange(nC)
-    for i in clusters:
-        S = np.nonzero(presence[i])[0]
-        ichan, imap = np.unique(iC[:, S], return_inverse=True)
-        pos = imap.reshape(nC, S.size).T                       # [|S|, nC] union positions
-        sig = sigma[S]                                         # current source rows
-        vals = np.where((sig >= 0)[..., None], sums[S[:, None], np.maximum(sig, 0)], 0.0)
-        acc = np.zeros((ichan.size, P))
-        np.add.at(acc, pos.ravel(), vals.reshape(-1, P))       # template-major, as a loop
-        mean = acc / counts[S].sum()
-        norm = np.sqrt((mean ** 2).sum(1))
-        ind = np.argsort(-norm, kind='stable')[:n_chans]
-        feature_ind[i, :] = ichan[ind]
-        row_of = np.full((S.size, ichan.size), -1, np.int64)
-        row_of[np.arange(S.size)[:, None], pos] = rows[None, :]
-        src = row_of[:, ind]                                   # [|S|, n_chans]
-        sigma[S] = np.where(src >= 0, np.take_along_axis(sig, np.maximum(src, 0), 1), -1)
-    return sigma, feature_ind
-
-
 def make_pc_features(ops, spike_templates, spike_clusters, tF, device=None):
     """``postprocessing.make_pc_features``: (tF permuted to [n, P, nC],
     feature_ind [n_clusters, nearest_chans] uint32).  tF is updated in place
@@ -162,9 +131,11 @@ def make_pc_features(ops, spike_templates, spike_clusters, tF, device=None):
     lib = _lib.load()
     spike_templates = np.asarray(spike_templates)
     spike_clusters = np.asarray(spike_clusters)
-    iU = torch.as_tensor(ops['iU']).cpu().numpy()
-    iC = torch.as_tensor(ops['iCC']).cpu().numpy()[:, iU]
+    iU = torch.as_tensor(ops['iU']).to(device, torch.int64)
+    iCC = torch.as_tensor(ops['iCC']).to(device, torch.int64)
+    iC = iCC[:, iU].contiguous()                             # [nC, K]: xy_templates' iC
     nC, K = iC.shape
+    C = iCC.shape[1]
     n_chans = ops['nearest_chans']
     tFd = torch.as_tensor(tF).to(device, torch.float32).contiguous()
     n, nCf, P = tFd.shape
@@ -172,6 +143,11 @@ def make_pc_features(ops, spike_templates, spike_clusters, tF, device=None):
         raise ValueError(f"tF has {nCf} channel rows, iCC {nC}, nearest_chans {n_chans}")
     if spike_templates.size and (spike_templates.min() < 0 or spike_templates.max() >= K):
         raise ValueError("template ids out of range")
+    clusters = np.unique(spike_clusters)
+    if clusters.size and clusters.max() >= clusters.size:
+        # the reference writes feature_ind[i] into n_clusters rows
+        raise IndexError(f"index {clusters.max()} is out of bounds for axis 0 with size "
+                         f"{clusters.size}")
     lab, G = _labels(spike_clusters, device)
     tmpl = torch.as_tensor(spike_templates.astype(np.int64)).to(device)
     sums = torch.empty((K, nC, P), dtype=torch.float64, device=device)
@@ -181,13 +157,19 @@ def make_pc_features(ops, spike_templates, spike_clusters, tF, device=None):
     _lib.check(lib.ksm_template_sums(tFd.data_ptr(), tmpl.data_ptr(), lab.data_ptr(), n, nC, P,
                                      K, G, sums.data_ptr(), counts.data_ptr(),
                                      presence.data_ptr(), st), "ksm_template_sums")
-    clusters = np.unique(spike_clusters)
-    sigma, feature_ind = _channel_maps(iC, sums.cpu().numpy(), counts.cpu().numpy(),
-                                       presence.cpu().numpy().astype(bool), clusters, n_chans)
-    sig = torch.from_numpy(sigma).to(device)
+    sigma = torch.empty((K, nC), dtype=torch.int32, device=device)
+    sig_tmp = torch.empty_like(sigma)
+    fi = torch.zeros((max(clusters.size, 1), nC), dtype=torch.int32, device=device)
+    if n:
+        _lib.check(lib.ksm_pc_channel_rank(sums.data_ptr(), counts.data_ptr(),
+                                           presence.data_ptr(), iC.data_ptr(), K, nC, P, C, G,
+                                           sigma.data_ptr(), sig_tmp.data_ptr(), fi.data_ptr(),
+                                           st), "ksm_pc_channel_rank")
     out = torch.empty_like(tFd)
-    _lib.check(lib.ksm_gather_features(tFd.data_ptr(), tmpl.data_ptr(), n, nC, P, sig.data_ptr(),
-                                       out.data_ptr(), 0, st), "ksm_gather_features")
+    _lib.check(lib.ksm_gather_features(tFd.data_ptr(), tmpl.data_ptr(), n, nC, P,
+                                       sigma.data_ptr(), out.data_ptr(), 0, st),
+               "ksm_gather_features")
+    feature_ind = fi[:clusters.size].cpu().numpy().view(np.uint32)
     if torch.is_tensor(tF):
         tF.copy_(out)
         res = tF

@@ -87,7 +87,7 @@ def trains(seed):
 def test_acg_counts_match_oracle():
     st0, clu = trains(3)
     say(f"acg n={len(st0)}")
-    K, n, T = CG.acg_counts(clu, st0, device=DEV)
+    K, n, T, _ = CG.acg_counts(clu, st0, device=DEV)
     for kk in range(clu.max() + 1):
         st1 = st0[clu == kk]
         Kr, Tr = PH.compute_CCG(st1, st1.copy())

@@ -107,3 +107,21 @@ def test_make_pc_features_matches_replay():
         np.testing.assert_array_equal(fi, fi_want)
         np.testing.assert_allclose(got.numpy(), want, rtol=0, atol=0)
         assert got.shape == (len(stt), 6, ops['nearest_chans'])
+
+
+def test_vectorised_ccg_metrics_equal_the_per_pair_formula():
+    """The refractoriness metrics evaluated for many correlograms at once
+    (host part of the GPU refract) equal CCG_metrics row by row, bit for bit."""
+    from kilosort_amd.ccg import ccg_metrics_rows
+    rng = np.random.default_rng(3)
+    G = 40
+    K = rng.poisson(rng.uniform(0.1, 30, (G, 1)), (G, 1001)).astype(np.float64)
+    K[::7, 490:511] = 0                                       # refractory dips
+    n1 = rng.integers(11, 5000, G)
+    n2 = n1.copy()
+    T = rng.uniform(10, 3600, G)
+    R12, Q12, Q00 = ccg_metrics_rows(n1, n2, K, T, 500, 1 / 1000)
+    for g in range(G):
+        st1 = np.zeros(int(n1[g]))
+        r, q, q0 = PH.CCG_metrics(st1, st1, K[g].copy(), T[g], nbins=500, tbin=1 / 1000)
+        assert (R12[g], Q12[g], Q00[g]) == (r, q, q0)
