@@ -286,6 +286,42 @@ def time_templates(ops, U, device, reps=5, cpu_baseline=True):
     return out
 
 
+def time_extract(tm, n_batches, seed, device, U, ops, runs=5):
+    """The whole ``extract`` call as a caller sees it: the recording in pinned
+    host memory (float32 [samples, 383]), batches copied to the device one by
+    one, prepare_extract + prepare_matching inside, st / tF returned to the
+    host.  One untimed call, then the median of `runs` calls."""
+    from kilosort_amd import synth
+    from kilosort_amd.batches import ArrayBatches
+    # exactly n_batches batches: a short last batch would be padded with its
+    # last sample (io.py:816-818), a constant run that only filtering makes
+    # harmless, and this recording is not filtered
+    rec = synth.make_recording(seed, C, K, n_batches * NT,
+                               rate_range=(1.0, 20.0), amp_range=(8.0, 40.0),
+                               overlap_frac=0.2, with_data=False)
+    data = synth.synthesize_torch(rec.n_samples, C, rec.times, rec.ids, rec.amps, rec.U,
+                                  rec.wPCA, rec.nt0min, device, seed=seed)
+    host = torch.empty(data.shape, dtype=data.dtype, pin_memory=True)
+    host.copy_(data)
+    del data
+    bf = ArrayBatches(host, NT=NT, nt=NTAP, device=device)
+    nb = int(bf.n_batches)
+    tm.extract(dict(ops), bf, U, device=device)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st, tF, _ = tm.extract(dict(ops), bf, U, device=device)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return dict(value=C * NT * nb / med, unit="channel·samples/s", batches=nb,
+                ms_per_batch=med * 1e3 / nb, median_s=med, runs=runs,
+                spikes=int(len(st)),
+                note="wall time of extract(): prepare_extract + prepare_matching, per-batch "
+                     "host-to-device copies of float32 batches from pinned memory, matching, "
+                     "features, st / tF on the host; median of the runs after one untimed call")
+
+
 def phy_workload(ops, iCC, iU, iCC_mask, n, seed, n_clu=500, hours=1.0):
     """A sorted recording's export inputs: n spikes over `hours` from the
     bench's 700 detection templates at 383 channels, n_clu clusters (each
@@ -743,6 +779,11 @@ def main():
     tmpl = time_templates(ops, U, device, cpu_baseline=not args.no_cpu_baseline)
     log(f"templates: similarity {tmpl['similarity_ms']:.2f} ms, align_U {tmpl['align_U_ms']:.2f} ms")
 
+    log("extract() wall time, recording in host memory (12 batches)")
+    ext = time_extract(tm, 12, args.seed + 300, device, U, ops)
+    log(f"extract: {ext['value']:.3e} channel·samples/s, {ext['ms_per_batch']:.3f} ms/batch "
+        f"({ext['batches']} batches, median of {ext['runs']})")
+
     log("Phy export per-spike work (2 M spikes)")
     phy = time_phy(ops, iCC, iU, iCC_mask, device, cpu_baseline=not args.no_cpu_baseline)
     log("phy: " + ", ".join(f"{k}={v:.1f} ms" for k, v in phy["ms"].items()))
@@ -818,6 +859,7 @@ def main():
         "preprocess": pre,
         "detect": det,
         "templates": tmpl,
+        "extract_wall": ext,
         "phy": phy,
         "ragged": ragged,
         "cpu_baseline": cpu,

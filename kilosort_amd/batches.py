@@ -67,7 +67,11 @@ class ArrayBatches:
     def padded_batch_to_torch(self, ibatch, ops=None, return_inds=False):
         ibatch = int(ibatch) * self.batch_downsampling
         bstart, bend = self._get_batch_edges(ibatch)
-        seg = self.data[bstart:bend].to(self.device, torch.float32).T
+        # from pinned host memory the copy is queued on the caller's stream and
+        # the host goes on (the recording outlives the copy)
+        src = self.data[bstart:bend]
+        seg = src.to(self.device, torch.float32,
+                     non_blocking=bool(src.device.type == 'cpu' and src.is_pinned())).T
         nsamp = seg.shape[-1]
         NT, nt = int(self.NT), self.nt
         X = torch.zeros((self.n_chan_bin, NT + 2 * nt), device=self.device)

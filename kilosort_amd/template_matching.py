@@ -372,27 +372,30 @@ def _lane_give(lane):
     lane.jobs.put(None)
 
 
+_LANE_SLOTS = 3                 # pinned result buffer sets per lane
+
+
 class _Lane:
     """One device stream of a _BatchPipeline and the host thread that drives
     it: run_matching + features per job, then the host copies of (st,
-    th_amps, tF) into one of two pinned buffer sets."""
+    th_amps) into one of its pinned buffer sets; the features stay on the
+    device."""
 
     def __init__(self, device):
         self.device = device
         self.stream = torch.cuda.Stream(device=device)
         self.jobs = queue.Queue()
         self.results = queue.Queue()
-        self.bufs = [None, None]
+        self.bufs = [None] * _LANE_SLOTS
         self.thread = threading.Thread(target=self._run, daemon=True)
         self.thread.start()
 
-    def _host(self, slot, k, nC, P):
+    def _host(self, slot, k):
         b = self.bufs[slot]
-        if b is None or b[0].shape[0] < k or tuple(b[2].shape[1:]) != (nC, P):
+        if b is None or b[0].shape[0] < k:
             cap = max(2 * k, 1 << 15)
             b = (torch.empty((cap, 2), dtype=torch.int64, pin_memory=True),
-                 torch.empty((cap, 1), dtype=torch.float32, pin_memory=True),
-                 torch.empty((cap, nC, P), dtype=torch.float32, pin_memory=True))
+                 torch.empty((cap, 1), dtype=torch.float32, pin_memory=True))
             self.bufs[slot] = b
         return b
 
@@ -410,17 +413,15 @@ class _Lane:
                     iters = _iters.n
                     xfeat = _features(c.ops, Xres, stt, amps, c.iCC, c.iU, c.Ucc, dev)
                     k = stt.shape[0]
-                    nC, _, P = c.Ucc.shape
-                    hb = self._host(slot, k, nC, P)
+                    hb = self._host(slot, k)
                     if k:
                         hb[0][:k].copy_(stt, non_blocking=True)
                         hb[1][:k].copy_(th, non_blocking=True)
-                        hb[2][:k].copy_(xfeat, non_blocking=True)
                     done = torch.cuda.Event()
                     done.record(self.stream)
                     done.synchronize()
-                    del X, stt, amps, th, Xres, xfeat
-                    self.results.put((tag, hb[0][:k], hb[1][:k], hb[2][:k], iters))
+                    del X, stt, amps, th, Xres
+                    self.results.put((tag, hb[0][:k], hb[1][:k], xfeat, iters))
                 except BaseException as e:      # handed to the submitting thread
                     self.results.put(e)
 
@@ -430,16 +431,16 @@ class _BatchPipeline:
     device streams at once with one host thread each (the peel's host loop
     follows its stream's progress), so that one batch's latency-bound late
     peel iterations and small kernels overlap other batches' work.  Batch i
-    runs on lane i mod lanes; each lane copies its results to pinned host
-    buffers, two sets per lane.
+    runs on lane i mod lanes, two batches deep; each lane copies its results
+    to pinned host buffers, three sets per lane.
 
     ``submit(X, tag, ctx)`` queues batch X (a device tensor made on the
     caller's current stream) of the recording ``ctx`` (default: the one the
     pipeline was made with) and returns the batches finished so far that the
     lanes need back, in submission order, as a list of ``(tag, st int64
-    [k, 2], th_amps float32 [k, 1], tF float32 [k, nC, P])`` -- views into
-    pinned buffers that stay valid until the next ``submit`` or ``flush``
-    returns; ``flush()`` returns all remaining batches and ``close()`` hands
+    [k, 2], th_amps float32 [k, 1], tF float32 [k, nC, P])`` -- st and
+    th_amps views into pinned buffers that stay valid until the next
+    ``submit`` or ``flush`` returns, tF a device tensor (completed); ``flush()`` returns all remaining batches and ``close()`` hands
     the lanes (threads, streams, pinned buffers) back to a per-device pool for
     later pipelines.  ``peel_iterations`` sums the peel iterations of the
     batches returned.
@@ -478,13 +479,15 @@ class _BatchPipeline:
         n = len(self.lanes)
         i = self.n_sub
         out = []
-        # the lane's previous batch (i - n) must be back before it starts
-        # batch i, whose host buffers alternate with it
-        while self.n_done <= i - n:
+        # each lane holds up to _LANE_SLOTS - 1 batches (one running, the next
+        # queued so that the lane never waits for this thread); batch i takes
+        # its lane's host buffer set (i // n) % _LANE_SLOTS, last used by batch
+        # i - _LANE_SLOTS n, which was handed out at least one submit ago
+        while self.n_done <= i - (_LANE_SLOTS - 1) * n:
             out.append(self._take())
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.device))
-        self.lanes[i % n].jobs.put(((i // n) % 2, X, tag, ready, ctx))
+        self.lanes[i % n].jobs.put(((i // n) % _LANE_SLOTS, X, tag, ready, ctx))
         self.n_sub += 1
         return out
 
@@ -528,8 +531,9 @@ class _Collector:
     by time with numpy's default argsort over the rows in batch order.  Rows
     are kept per batch and joined at the end instead of written into the
     reference's preallocated 10^6-row buffers (whose zero-filled pages cost
-    more than a small recording's matching); the returned arrays are the
-    same."""
+    more than a small recording's matching); the features stay on the device
+    and come back in one copy, in the final order; the returned arrays are
+    the same."""
 
     def __init__(self, ops, bfile, nC, P, progress_bar=None):
         self.ops, self.bfile = ops, bfile
@@ -539,15 +543,20 @@ class _Collector:
         self.progress_bar = progress_bar
         self.last = None                  # host st of the last batch (error reports)
 
-    def add(self, jb, stt_h, th_h, xfeat_h):
+    def add(self, jb, stt_h, th_h, xfeat_d):
         ops = self.ops
         nt = ops['nt']
         self.last = stt_h
         x = stt_h[:, 0].numpy()
+        # the features stay on the device until result(); the caller's stream
+        # reads them there
+        if xfeat_d.numel():
+            xfeat_d.record_stream(torch.cuda.current_stream(xfeat_d.device))
         if jb == 0:
             # times before the recording start (trough alignment) are dropped
             keep = (x - nt - nt // 2 + ops['nt0min']) >= 0
-            stt_h, th_h, xfeat_h = stt_h[keep], th_h[keep], xfeat_h[keep]
+            stt_h, th_h = stt_h[keep], th_h[keep]
+            xfeat_d = xfeat_d[torch.from_numpy(keep).to(xfeat_d.device)]
         nsp = len(stt_h)
         t_shift = jb * self.bfile.batch_downsampling * (ops['batch_size'])
         sd = stt_h.numpy().astype(np.float64)
@@ -556,17 +565,21 @@ class _Collector:
         st[:, 1] = sd[:, 1]
         st[:, 2] = th_h.numpy().astype(np.float64).reshape(-1)
         self.st_parts.append(st)
-        self.tf_parts.append(xfeat_h.clone())       # the pinned views are reused
+        self.tf_parts.append(xfeat_d)
         if self.progress_bar is not None:
             self.progress_bar.emit(int((jb + 1) / self.n_batches * 100))
 
     def result(self):
         st = (np.concatenate(self.st_parts, 0) if self.st_parts
               else np.zeros((0, 3), np.float64))
-        tF = (torch.cat(self.tf_parts, 0) if self.tf_parts
-              else torch.zeros((0, self.nC, self.P)))
         isort = np.argsort(st[:, 0])
-        return st[isort], tF[isort]
+        if self.tf_parts:
+            dev = self.tf_parts[0].device
+            tF = torch.cat(self.tf_parts, 0)[torch.from_numpy(isort).to(dev)].cpu()
+        else:
+            tF = torch.zeros((0, self.nC, self.P))
+        self.tf_parts = []
+        return st[isort], tF
 
 
 def _prepare_recording(ops, U, device):
