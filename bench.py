@@ -322,6 +322,40 @@ def time_extract(tm, n_batches, seed, device, U, ops, runs=5):
                      "features, st / tF on the host; median of the runs after one untimed call")
 
 
+def time_extract_int16(tm, n_batches, seed, device, U, ops, runs=5):
+    """``extract`` over a raw int16 recording in host memory through the device
+    preprocessing (FilteredBatches: int16 host-to-device copy, channel means,
+    CAR, the 300 Hz high-pass filter, whitening), as a sorting run streams
+    it.  The recording is the bench's synthetic whitened data scaled by 20 and
+    rounded to int16, with whitening matrix I / 20.  One untimed call, then the
+    median of `runs` calls."""
+    from kilosort_amd import synth, preprocess as PPr
+    rec = synth.make_recording(seed, C, K, n_batches * NT,
+                               rate_range=(1.0, 20.0), amp_range=(8.0, 40.0),
+                               overlap_frac=0.2, with_data=False)
+    data = synth.synthesize_torch(rec.n_samples, C, rec.times, rec.ids, rec.amps, rec.U,
+                                  rec.wPCA, rec.nt0min, device, seed=seed)
+    raw = (data * 20.0).round_().clamp_(-32768, 32767).to(torch.int16).cpu().numpy()
+    del data
+    fb = PPr.FilteredBatches(n_chan_bin=C, NT=NT, file_object=raw, device=device,
+                             hp_filter=PPr.get_highpass_filter(device=device),
+                             whiten_mat=np.eye(C, dtype=np.float32) / 20.0)
+    nb = int(fb.n_batches)
+    tm.extract(dict(ops), fb, U, device=device)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        st, tF, _ = tm.extract(dict(ops), fb, U, device=device)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return dict(value=C * NT * nb / med, unit="channel·samples/s", batches=nb,
+                ms_per_batch=med * 1e3 / nb, median_s=med, runs=runs, spikes=int(len(st)),
+                note="wall time of extract() over FilteredBatches of a host int16 recording: "
+                     "int16 host-to-device copies, CAR, high-pass, whitening, matching, "
+                     "features, st / tF on the host; median of the runs after one untimed call")
+
+
 def phy_workload(ops, iCC, iU, iCC_mask, n, seed, n_clu=500, hours=1.0):
     """A sorted recording's export inputs: n spikes over `hours` from the
     bench's 700 detection templates at 383 channels, n_clu clusters (each
@@ -780,9 +814,14 @@ def main():
     log(f"templates: similarity {tmpl['similarity_ms']:.2f} ms, align_U {tmpl['align_U_ms']:.2f} ms")
 
     log("extract() wall time, recording in host memory (12 batches)")
-    ext = time_extract(tm, 12, args.seed + 300, device, U, ops)
+    ext = time_extract(tm, 12, args.seed, device, U, ops)
     log(f"extract: {ext['value']:.3e} channel·samples/s, {ext['ms_per_batch']:.3f} ms/batch "
         f"({ext['batches']} batches, median of {ext['runs']})")
+    log("extract() over a raw int16 recording through the device preprocessing")
+    ext16 = time_extract_int16(tm, 12, args.seed, device, U, ops)
+    ext["int16_preprocessed"] = ext16
+    log(f"extract int16: {ext16['value']:.3e} channel·samples/s, {ext16['ms_per_batch']:.3f} "
+        f"ms/batch ({ext16['spikes']} spikes)")
 
     log("Phy export per-spike work (2 M spikes)")
     phy = time_phy(ops, iCC, iU, iCC_mask, device, cpu_baseline=not args.no_cpu_baseline)
