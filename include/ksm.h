@@ -485,8 +485,8 @@ int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *label
  * [K][nC] (int32; row k of template t's spikes <- original row sigma[t][k],
  * or zeros for -1) composed with each choice, as the reference's in-place
  * rewrites of tF do.  sigma_tmp [K][nC] is scratch.  One block; limits
- * nC <= 32 and C (8 P + 12) + 2 K (nC + 1) <= 65536 bytes of LDS (e.g. 383
- * channels, 6 PCs and up to 1850 templates of 10 channels).
+ * nC <= 32, C <= 2048 and C (8 P + 12) + 2 K (nC + 1) <= 57344 bytes of LDS
+ * (e.g. 383 channels, 6 PCs and up to 1560 templates of 10 channels).
  */
 int ksm_pc_channel_rank(const double *sums, const uint64_t *counts, const uint8_t *presence,
                         const int64_t *iC, int K, int nC, int P, int C, int n_groups,

@@ -356,6 +356,7 @@ __global__ void gather_features_kernel(const float *__restrict__ tF,
 // channels of largest norm (ties: lowest channel), and each held template's
 // map composed with the choice.
 constexpr int kRankThreads = 1024;
+constexpr int kMaxRankC = 2048;                 // channels a thread's candidates cover
 
 __device__ __forceinline__ void better(double &v, int &c, double v2, int c2) {
     if (v2 > v || (v2 == v && c2 < c)) {
@@ -375,8 +376,9 @@ __global__ __launch_bounds__(kRankThreads) void pc_rank_kernel(
     int *s_mark = reinterpret_cast<int *>(s_norm + C);            // [C]
     short *s_ic = reinterpret_cast<short *>(s_mark + C);          // [K][nC] channels
     short *s_held = s_ic + (size_t)K * nC;                        // [K] held templates
-    __shared__ double s_wv[kRankThreads / 64];
     __shared__ int s_wc[kRankThreads / 64];
+    __shared__ double s_cv[(kRankThreads / 64) * 32];      // per-wave top channels
+    __shared__ int s_cc[(kRankThreads / 64) * 32];
     __shared__ int s_top[32];
     __shared__ int s_nheld;
     __shared__ unsigned long long s_N;
@@ -439,27 +441,67 @@ __global__ __launch_bounds__(kRankThreads) void pc_rank_kernel(
             s_norm[c] = s_mark[c] ? sqrt(a) : -1.0;
         }
         __syncthreads();
-        for (int k = 0; k < nC; ++k) {
-            double v = -3.0;
-            int bc = 0x7fffffff;
-            for (int c = tid; c < C; c += kRankThreads) better(v, bc, s_norm[c], c);
-            for (int o = 32; o > 0; o >>= 1) {
-                const double v2 = __shfl_xor(v, o, 64);
-                const int c2 = __shfl_xor(bc, o, 64);
-                better(v, bc, v2, c2);
+        // top nC channels by (norm descending, channel ascending), a strict
+        // order: each wave takes its own top nC by shuffles, then wave 0 takes
+        // the top nC of those candidates
+        {
+            constexpr int kPer = (kMaxRankC + kRankThreads - 1) / kRankThreads;
+            double lv[kPer];
+            int lc[kPer];
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int c = tid + u * kRankThreads;
+                lv[u] = c < C ? s_norm[c] : -3.0;
+                lc[u] = c < C ? c : 0x7fffffff;
             }
-            if (lane == 0) {
-                s_wv[wv] = v;
-                s_wc[wv] = bc;
+            for (int k = 0; k < nC; ++k) {
+                double v = -4.0;
+                int bc = 0x7fffffff;
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) better(v, bc, lv[u], lc[u]);
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double v2 = __shfl_xor(v, o, 64);
+                    const int c2 = __shfl_xor(bc, o, 64);
+                    better(v, bc, v2, c2);
+                }
+#pragma unroll
+                for (int u = 0; u < kPer; ++u)
+                    if (lc[u] == bc) lv[u] = -5.0;           // taken
+                if (lane == 0) {
+                    s_cv[wv * 32 + k] = v;
+                    s_cc[wv * 32 + k] = bc;
+                }
             }
             __syncthreads();
-            if (tid == 0) {
-                double bv = s_wv[0];
-                int bcc = s_wc[0];
-                for (int w = 1; w < kRankThreads / 64; ++w) better(bv, bcc, s_wv[w], s_wc[w]);
-                s_top[k] = bcc;
-                s_norm[bcc] = -2.0;              // taken
-                feature_ind[(size_t)g * nC + k] = (uint32_t)bcc;
+            if (wv == 0) {
+                constexpr int kCand = (kRankThreads / 64) * 32 / 64;
+                double cv[kCand];
+                int cc[kCand];
+#pragma unroll
+                for (int u = 0; u < kCand; ++u) {
+                    const int e = lane + u * 64, w = e / 32, r = e % 32;
+                    const bool ok = r < nC;
+                    cv[u] = ok ? s_cv[w * 32 + r] : -6.0;
+                    cc[u] = ok ? s_cc[w * 32 + r] : 0x7fffffff;
+                }
+                for (int k = 0; k < nC; ++k) {
+                    double v = -7.0;
+                    int bc = 0x7fffffff;
+#pragma unroll
+                    for (int u = 0; u < kCand; ++u) better(v, bc, cv[u], cc[u]);
+                    for (int o = 32; o > 0; o >>= 1) {
+                        const double v2 = __shfl_xor(v, o, 64);
+                        const int c2 = __shfl_xor(bc, o, 64);
+                        better(v, bc, v2, c2);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kCand; ++u)
+                        if (cc[u] == bc) cv[u] = -8.0;
+                    if (lane == 0) {
+                        s_top[k] = bc;
+                        feature_ind[(size_t)g * nC + k] = (uint32_t)bc;
+                    }
+                }
             }
             __syncthreads();
         }
@@ -620,11 +662,11 @@ int ksm_pc_channel_rank(const double *sums, const uint64_t *counts, const uint8_
     const size_t lds = (size_t)C * P * sizeof(double) + (size_t)C * sizeof(double) +
                        (size_t)C * sizeof(int) + ((size_t)K * nC + K) * sizeof(short);
     if (!sums || !counts || !presence || !iC || !sigma || !sigma_tmp || !feature_ind || K <= 0 ||
-        nC <= 0 || nC > 32 || P <= 0 || C <= 0 || C > 32767 || K > 32767 || n_groups <= 0 ||
-        lds > 65536)
+        nC <= 0 || nC > 32 || P <= 0 || C <= 0 || C > kMaxRankC || K > 32767 ||
+        n_groups <= 0 || lds > 65536 - 8192)
         return ksm::fail(KSM_ERR_INVALID,
-                         "ksm_pc_channel_rank: bad arguments (nC <= 32, "
-                         "C (8 P + 12) + 2 K (nC + 1) <= 65536)");
+                         "ksm_pc_channel_rank: bad arguments (nC <= 32, C <= 2048, "
+                         "C (8 P + 12) + 2 K (nC + 1) <= 57344)");
     pc_rank_kernel<<<1, kRankThreads, lds, ksm::as_stream(stream)>>>(
         sums, reinterpret_cast<const unsigned long long *>(counts), presence, iC, K, nC, P, C,
         n_groups, sigma, sigma_tmp, feature_ind);
