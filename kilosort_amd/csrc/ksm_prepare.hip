@@ -90,22 +90,24 @@ __global__ void transpose_u_kernel(const float *__restrict__ U, int K, int PC,
 // One block per (firing template j, 64 templates i): UtU[i,j,:,:] in float64
 // registers, then ctc[j][i][l] = sum_pq UtU[pq] * WtW[pq][l] staged in LDS and
 // written as 64 contiguous rows.
+constexpr int kCtcChunk = 32;      // lags staged per output step
+
+// ctc[j][i][l] for the 64 templates i of the block: utu[p][q] = the float64
+// sum over channels of U[i][p][c] U[j][q][c] in registers, then per lag the
+// sum over (p, q) of utu * WtW (the table read with wave-uniform loads), the
+// rows staged through LDS 32 lags at a time for coalesced stores.
 template <int P>
 __global__ __launch_bounds__(64) void ctc_kernel(const float *__restrict__ Ut,
                                                  const float *__restrict__ U,
                                                  const double *__restrict__ wtw,
                                                  int K, int C, int nt, float *__restrict__ ctc,
                                                  int ldl) {
-    extern __shared__ unsigned char smem[];
+    __shared__ float s_out[64][kCtcChunk + 1];
     const int L = 2 * nt + 1;
-    double *s_wtw = reinterpret_cast<double *>(smem);           // [P*P][L]
-    float *s_out = reinterpret_cast<float *>(s_wtw + P * P * L);  // [64][ldl]
     const int j = blockIdx.x;
     const int i0 = blockIdx.y * 64;
     const int lane = threadIdx.x;
     const int i = i0 + lane;
-
-    for (int e = lane; e < P * P * L; e += 64) s_wtw[e] = wtw[e];
 
     double utu[P * P];
 #pragma unroll
@@ -125,34 +127,38 @@ __global__ __launch_bounds__(64) void ctc_kernel(const float *__restrict__ Ut,
                 for (int q = 0; q < P; ++q) utu[p * P + q] = fma(ui[p], uq[q], utu[p * P + q]);
         }
     }
-    __syncthreads();
-    for (int l = 0; l < ldl; ++l) {
-        double acc = 0.0;
-        if (l < L) {
-#pragma unroll
-            for (int e = 0; e < P * P; ++e) acc = fma(utu[e], s_wtw[e * L + l], acc);
-        }
-        s_out[lane * ldl + l] = (float)acc;
-    }
-    __syncthreads();
     const int nrow = min(64, K - i0);
     float *dst = ctc + ((size_t)j * K + i0) * ldl;
-    for (int e = lane; e < nrow * ldl; e += 64) dst[e] = s_out[e];
+    for (int l0 = 0; l0 < ldl; l0 += kCtcChunk) {
+        const int nl = min(kCtcChunk, ldl - l0);
+        for (int u = 0; u < nl; ++u) {
+            const int l = l0 + u;
+            double acc = 0.0;
+            if (l < L) {
+#pragma unroll
+                for (int e = 0; e < P * P; ++e) acc = fma(utu[e], wtw[e * L + l], acc);
+            }
+            s_out[lane][u] = (float)acc;
+        }
+        __syncthreads();
+        for (int e = lane; e < nrow * nl; e += 64) {
+            const int r = e / nl, u = e - r * nl;
+            dst[(size_t)r * ldl + l0 + u] = s_out[r][u];
+        }
+        __syncthreads();
+    }
 }
 
 // Generic-P fallback of ctc_kernel (runtime P <= 8).
 __global__ __launch_bounds__(64) void ctc_kernel_generic(
     const float *__restrict__ Ut, const float *__restrict__ U, const double *__restrict__ wtw,
     int K, int P, int C, int nt, float *__restrict__ ctc, int ldl) {
-    extern __shared__ unsigned char smem[];
+    __shared__ float s_out[64][kCtcChunk + 1];
     const int L = 2 * nt + 1;
-    double *s_wtw = reinterpret_cast<double *>(smem);
-    float *s_out = reinterpret_cast<float *>(s_wtw + P * P * L);
     const int j = blockIdx.x;
     const int i0 = blockIdx.y * 64;
     const int lane = threadIdx.x;
     const int i = i0 + lane;
-    for (int e = lane; e < P * P * L; e += 64) s_wtw[e] = wtw[e];
     double utu[64];
     for (int e = 0; e < 64; ++e) utu[e] = 0.0;
     const float *uj = U + (size_t)j * P * C;
@@ -164,17 +170,24 @@ __global__ __launch_bounds__(64) void ctc_kernel_generic(
                     utu[p * P + q] = fma(a, (double)uj[(size_t)q * C + c], utu[p * P + q]);
             }
     }
-    __syncthreads();
-    for (int l = 0; l < ldl; ++l) {
-        double acc = 0.0;
-        if (l < L)
-            for (int e = 0; e < P * P; ++e) acc = fma(utu[e], s_wtw[e * L + l], acc);
-        s_out[lane * ldl + l] = (float)acc;
-    }
-    __syncthreads();
     const int nrow = min(64, K - i0);
     float *dst = ctc + ((size_t)j * K + i0) * ldl;
-    for (int e = lane; e < nrow * ldl; e += 64) dst[e] = s_out[e];
+    for (int l0 = 0; l0 < ldl; l0 += kCtcChunk) {
+        const int nl = min(kCtcChunk, ldl - l0);
+        for (int u = 0; u < nl; ++u) {
+            const int l = l0 + u;
+            double acc = 0.0;
+            if (l < L)
+                for (int e = 0; e < P * P; ++e) acc = fma(utu[e], wtw[e * L + l], acc);
+            s_out[lane][u] = (float)acc;
+        }
+        __syncthreads();
+        for (int e = lane; e < nrow * nl; e += 64) {
+            const int r = e / nl, u = e - r * nl;
+            dst[(size_t)r * ldl + l0 + u] = s_out[r][u];
+        }
+        __syncthreads();
+    }
 }
 
 // Nonzero rows of ctc[j][:, :L]: one block of 256 per firing template j.
@@ -265,14 +278,11 @@ int ksm_prepare_matching(const float *U, const float *W, int K, int P, int C, in
     KSM_LAUNCH_CHECK("wtw_kernel");
     transpose_u_kernel<<<1024, 256, 0, s>>>(U, K, P * C, Ut);
     KSM_LAUNCH_CHECK("transpose_u_kernel");
-    const size_t shm = (size_t)P * P * L * sizeof(double) + (size_t)64 * ldl * sizeof(float);
-    if (shm > 160 * 1024)
-        return ksm::fail(KSM_ERR_INVALID, "ksm_prepare_matching: nt too large (%d)", nt);
     dim3 grid(K, (K + 63) / 64);
     if (P == 6) {
-        ctc_kernel<6><<<grid, 64, shm, s>>>(Ut, U, wtw, K, C, nt, ctc, ldl);
+        ctc_kernel<6><<<grid, 64, 0, s>>>(Ut, U, wtw, K, C, nt, ctc, ldl);
     } else {
-        ctc_kernel_generic<<<grid, 64, shm, s>>>(Ut, U, wtw, K, P, C, nt, ctc, ldl);
+        ctc_kernel_generic<<<grid, 64, 0, s>>>(Ut, U, wtw, K, P, C, nt, ctc, ldl);
     }
     KSM_LAUNCH_CHECK("ctc_kernel");
     return KSM_OK;
