@@ -710,6 +710,10 @@ def main():
                     help="time only the universal-template detection and print its JSON object")
     ap.add_argument("--phy-only", action="store_true",
                     help="time only the Phy export's per-spike work and print its JSON object")
+    ap.add_argument("--templates-only", action="store_true",
+                    help="time only template similarity and alignment and print their JSON object")
+    ap.add_argument("--refine-only", action="store_true",
+                    help="time only the template refinement and print its JSON object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the matching path is sharded by batch "
@@ -744,6 +748,13 @@ def main():
                                   cpu_baseline=not args.no_cpu_baseline)), flush=True)
         return
     ctc = tm.prepare_matching(ops, U)
+    if args.templates_only:
+        print(json.dumps(time_templates(ops, U, device, cpu_baseline=not args.no_cpu_baseline)),
+              flush=True)
+        return
+    if args.refine_only:
+        print(json.dumps(time_refine(tm, ops, batches, U, ctc, device)), flush=True)
+        return
     nnzK = (ctc != 0).any(dim=2).sum(dim=0).cpu().numpy().astype(np.int64)
     nnzC = (U != 0).any(dim=1).sum(dim=1).cpu().numpy().astype(np.int64)
     torch.cuda.synchronize()

@@ -30,30 +30,58 @@ __global__ __launch_bounds__(256) void norms_kernel(const float *__restrict__ U,
 
 // iU[i] = argmax_c sum_p U[i,p,c]^2 (float32 sum over p in order, first index
 // on ties) and Ucc[n,i,p] = U[i,p,iCC[n,iU[i]]].
-__global__ void peak_channel_kernel(const float *__restrict__ U, int K, int P, int C,
-                                    const int64_t *__restrict__ iCC, int nC,
-                                    int64_t *__restrict__ iU, float *__restrict__ Ucc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K) return;
+struct PeakKey {
+    float best;
+    int arg;                              // 0x7fffffff: no channel seen
+    bool nan;
+};
+
+// The sequential rule (first NaN if any, else the first maximum) as an
+// associative choice between two partial results.
+__device__ __forceinline__ PeakKey peak_pick(PeakKey a, PeakKey b) {
+    if (b.arg == 0x7fffffff) return a;
+    if (a.arg == 0x7fffffff) return b;
+    if (a.nan || b.nan) {
+        if (a.nan && b.nan) return a.arg < b.arg ? a : b;
+        return a.nan ? a : b;
+    }
+    if (b.best > a.best || (b.best == a.best && b.arg < a.arg)) return b;
+    return a;
+}
+
+// One wave per template: lanes take channels (coalesced loads), each keeps the
+// sequential rule over its channels, then a wave reduction.
+__global__ __launch_bounds__(64) void peak_channel_kernel(const float *__restrict__ U, int K,
+                                                         int P, int C,
+                                                         const int64_t *__restrict__ iCC, int nC,
+                                                         int64_t *__restrict__ iU,
+                                                         float *__restrict__ Ucc) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
     const float *u = U + (size_t)i * P * C;
-    float best = 0.f;
-    int arg = 0;
-    for (int c = 0; c < C; ++c) {
+    PeakKey k{0.f, 0x7fffffff, false};
+    for (int c = lane; c < C; c += 64) {
         float s = 0.f;
         for (int p = 0; p < P; ++p) {
-            float v = u[(size_t)p * C + c];
+            const float v = u[(size_t)p * C + c];
             s = s + v * v;
         }
-        if (c == 0 || s > best || (s != s && best == best)) {
-            best = s;
-            arg = c;
-        }
+        const bool sn = s != s;
+        if (k.arg == 0x7fffffff || (!k.nan && (sn || s > k.best))) k = PeakKey{s, c, sn};
     }
-    iU[i] = arg;
-    for (int n = 0; n < nC; ++n) {
+    for (int o = 32; o > 0; o >>= 1) {
+        PeakKey o2;
+        o2.best = __shfl_xor(k.best, o, 64);
+        o2.arg = __shfl_xor(k.arg, o, 64);
+        o2.nan = __shfl_xor((int)k.nan, o, 64) != 0;
+        k = peak_pick(k, o2);
+    }
+    const int arg = k.arg;
+    if (lane == 0) iU[i] = arg;
+    for (int e = lane; e < nC * P; e += 64) {
+        const int n = e / P, p = e - n * P;
         const int64_t ch = iCC[(size_t)n * C + arg];
-        for (int p = 0; p < P; ++p)
-            Ucc[((size_t)n * K + i) * P + p] = u[(size_t)p * C + ch];
+        Ucc[((size_t)n * K + i) * P + p] = u[(size_t)p * C + ch];
     }
 }
 
@@ -250,8 +278,7 @@ int ksm_prepare_extract(const float *U, int K, int P, int C, const int64_t *iCC,
                         int64_t *iU, float *Ucc, void *stream) {
     if (!U || !iCC || !iU || !Ucc || K <= 0 || P <= 0 || C <= 0 || nC <= 0 || nC > C)
         return ksm::fail(KSM_ERR_INVALID, "ksm_prepare_extract: bad arguments");
-    peak_channel_kernel<<<(K + 63) / 64, 64, 0, ksm::as_stream(stream)>>>(U, K, P, C, iCC, nC,
-                                                                          iU, Ucc);
+    peak_channel_kernel<<<K, 64, 0, ksm::as_stream(stream)>>>(U, K, P, C, iCC, nC, iU, Ucc);
     KSM_LAUNCH_CHECK("peak_channel_kernel");
     return KSM_OK;
 }

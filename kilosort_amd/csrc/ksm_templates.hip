@@ -43,23 +43,43 @@ __global__ __launch_bounds__(256) void normalize_kernel(const float *__restrict_
 
 // S[i][j] = max over l < L of store[j][i][l] (the Gram of ksm_prepare_matching
 // is stored firing-template major); lag[i][j] = the first l attaining it.
-__global__ void maxlag_kernel(const float *__restrict__ store, int K, int L, int ldl,
-                              float *__restrict__ S, int32_t *__restrict__ lag) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)K * K) return;
-    const int i = (int)(e / K), j = (int)(e - (long long)i * K);
-    const float *r = store + ((size_t)j * K + i) * ldl;
-    float m = r[0];
-    int bl = 0;
-    for (int l = 1; l < L; ++l) {
-        const float v = r[l];
-        if (v > m) {
-            m = v;
-            bl = l;
+// S[i][j] = max over l < L of store[j][i][l] (the Gram of ksm_prepare_matching
+// is stored firing-template major); lag[i][j] = the first l attaining it.  One
+// wave per (j, i) row: lanes read consecutive lags, then a wave reduction
+// (larger value first, then smaller lag).
+__global__ __launch_bounds__(256) void maxlag_kernel(const float *__restrict__ store, int K,
+                                                     int L, int ldl, float *__restrict__ S,
+                                                     int32_t *__restrict__ lag) {
+    const int lane = threadIdx.x & 63;
+    const long long nrow = (long long)K * K;
+    const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long row = w0; row < nrow; row += nw) {
+        const float *r = store + (size_t)row * ldl;      // row = j K + i
+        float m = -INFINITY;
+        int bl = 0x7fffffff;
+        for (int l = lane; l < L; l += 64) {
+            const float v = r[l];
+            if (v > m || (v == m && l < bl)) {
+                m = v;
+                bl = l;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const float v2 = __shfl_xor(m, o, 64);
+            const int l2 = __shfl_xor(bl, o, 64);
+            if (v2 > m || (v2 == m && l2 < bl)) {
+                m = v2;
+                bl = l2;
+            }
+        }
+        if (lane == 0) {
+            const int j = (int)(row / K), i = (int)(row - (long long)j * K);
+            const long long e = (long long)i * K + j;
+            S[e] = m;
+            if (lag) lag[e] = bl;
         }
     }
-    S[e] = m;
-    if (lag) lag[e] = bl;
 }
 
 // X[c][k nt + t] = Y[(k C + c) nt + t].
@@ -108,24 +128,40 @@ __global__ __launch_bounds__(256) void peak_kernel(const float *__restrict__ Bc,
 }
 
 // Unew[k][p][c] = sum_t wPCA[p][t] Y[(k C + c) nt + (t - s_k) mod nt],
-// s_k = nt/2 - imax[k] (torch.roll by s_k along time).
-__global__ void roll_project_kernel(const float *__restrict__ Y, const float *__restrict__ wPCA,
-                                    const int32_t *__restrict__ imax, int K, int C, int P,
-                                    int nt, float *__restrict__ Unew) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)K * P * C) return;
-    const int c = (int)(e % C);
-    const long long r = e / C;
-    const int p = (int)(r % P), k = (int)(r / P);
-    const int s = nt / 2 - imax[k];
-    const float *y = Y + ((size_t)k * C + c) * nt;
-    float acc = 0.f;
-    for (int t = 0; t < nt; ++t) {
-        int u = (t - s) % nt;
-        if (u < 0) u += nt;
-        acc = __builtin_fmaf(wPCA[p * nt + t], y[u], acc);
+// s_k = nt/2 - imax[k] (torch.roll by s_k along time).  One block per
+// (template, 64 channels): the 64 waveforms are copied to LDS with coalesced
+// loads (rows padded to nt + 1), then thread (p, c) sums over t in order.
+constexpr int kRollCh = 64;
+constexpr int kMaxRollNt = 128;
+
+__global__ __launch_bounds__(256) void roll_project_kernel(const float *__restrict__ Y,
+                                                           const float *__restrict__ wPCA,
+                                                           const int32_t *__restrict__ imax,
+                                                           int K, int C, int P, int nt,
+                                                           float *__restrict__ Unew) {
+    __shared__ float s_y[kRollCh * (kMaxRollNt + 1)];
+    const int k = blockIdx.y;
+    const int c0 = blockIdx.x * kRollCh;
+    const int nc = min(kRollCh, C - c0);
+    const float *y0 = Y + ((size_t)k * C + c0) * nt;
+    for (int e = threadIdx.x; e < nc * nt; e += blockDim.x) {
+        const int q = e / nt;
+        s_y[q * (nt + 1) + (e - q * nt)] = y0[e];
     }
-    Unew[e] = acc;
+    __syncthreads();
+    const int s = nt / 2 - imax[k];
+    for (int e = threadIdx.x; e < P * kRollCh; e += blockDim.x) {
+        const int p = e / kRollCh, cl = e - p * kRollCh;
+        if (cl >= nc) continue;
+        const float *y = s_y + cl * (nt + 1);
+        float acc = 0.f;
+        for (int t = 0; t < nt; ++t) {
+            int u = (t - s) % nt;
+            if (u < 0) u += nt;
+            acc = __builtin_fmaf(wPCA[p * nt + t], y[u], acc);
+        }
+        Unew[((size_t)k * P + p) * C + c0 + cl] = acc;
+    }
 }
 
 // Spike positions (postprocessing.py:33-62): per spike s of template id with
@@ -235,7 +271,8 @@ int ksm_similarity(const float *Wall, const float *W, int K, int C, int P, int n
     int rc = ksm_prepare_matching(U, W, K, P, C, nt, store, ldl, ws2, nb, stream);
     if (rc != KSM_OK) return rc;
     const long long n = (long long)K * K;
-    maxlag_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(store, K, L, ldl, S, lag);
+    maxlag_kernel<<<(unsigned)std::min<long long>((n + 3) / 4, 65536), 256, 0, s>>>(
+        store, K, L, ldl, S, lag);
     KSM_LAUNCH_CHECK("maxlag_kernel");
     return KSM_OK;
 }
@@ -274,9 +311,10 @@ int ksm_align_templates(const float *Wall, const float *wPCA, const float *wTEMP
     if (rc != KSM_OK) return rc;
     peak_kernel<<<K, 256, 0, s>>>(Bc, nk, C, ldc, nt, imax);
     KSM_LAUNCH_CHECK("peak_kernel");
-    const long long m = (long long)K * P * C;
-    roll_project_kernel<<<(unsigned)((m + 255) / 256), 256, 0, s>>>(Y, wPCA, imax, K, C, P, nt,
-                                                                  Unew);
+    if (nt > kMaxRollNt)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_align_templates: nt <= %d", kMaxRollNt);
+    roll_project_kernel<<<dim3((C + kRollCh - 1) / kRollCh, K), 256, 0, s>>>(
+        Y, wPCA, imax, K, C, P, nt, Unew);
     KSM_LAUNCH_CHECK("roll_project_kernel");
     return KSM_OK;
 }
