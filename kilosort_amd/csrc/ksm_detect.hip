@@ -234,12 +234,20 @@ __global__ __launch_bounds__(256) void mask_kernel(const float *__restrict__ As,
                                                    int nw) {
     const int f = blockIdx.y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave's four score loads are issued together
+    float av[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int t = 64 * (blockIdx.x * 16 + wave * 4 + r) + lane;
+        av[r] = t < T ? As[(size_t)f * ldc + t] : 0.f;
+    }
+#pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int w = blockIdx.x * 16 + wave * 4 + r;       // mask word
         const int t = 64 * w + lane;
         bool hit = false;
         if (t < T) {
-            const float a = As[(size_t)f * ldc + t];
+            const float a = av[r];
             if (a > th) {
                 float m = -1.f;
                 for (int n = 0; n < nC2; ++n) {
