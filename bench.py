@@ -710,6 +710,9 @@ def main():
                     help="time only the universal-template detection and print its JSON object")
     ap.add_argument("--phy-only", action="store_true",
                     help="time only the Phy export's per-spike work and print its JSON object")
+    ap.add_argument("--matching-only", action="store_true",
+                    help="only the matching forms (timing, per-kernel times, rooflines): the "
+                         "run a kernel profile of the matching path uses")
     ap.add_argument("--templates-only", action="store_true",
                     help="time only template similarity and alignment and print their JSON object")
     ap.add_argument("--refine-only", action="store_true",
@@ -814,6 +817,13 @@ def main():
                            "dense_equivalent_tflops": flops_gemm / (g["avg_launch_ms"] / 1e3) / 1e12,
                            "traffic": measured_traffic(f"{gname}_mfma"),
                            "avg_launch_ms": g["avg_launch_ms"]}
+
+    if args.matching_only:
+        print(json.dumps({"metric": METRIC, "forms": {f: {k: v for k, v in r.items()}
+                                                      for f, r in results.items()},
+                          "roofline": roofline, "roofline_projection": roofline_projection,
+                          "kernels_per_batch": kern}), flush=True)
+        return
 
     log("template refinement (residual loss and gradients)")
     refine_res = time_refine(tm, ops, batches, U, ctc, device)
