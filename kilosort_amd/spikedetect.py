@@ -171,44 +171,43 @@ def extract_snippets(X, nt, twav_min, Th_single_ch, loc_range=(4, 5), long_range
         cap = 1 << int(np.ceil(np.log2(n)))
 
 
-def extract_wPCA_wTEMP(ops, bfile, nt=61, twav_min=20, Th_single_ch=6, nskip=25,
-                       device=torch.device('cuda')):
-    """``spikedetect.extract_wPCA_wTEMP`` (``spikedetect.py:53-92``): snippets
-    of every nskip-th batch (GPU), normalised, then scikit-learn's
-    ``TruncatedSVD`` (the PCs) and ``KMeans`` (the universal waveforms) on the
-    host, as the reference."""
-    import os
-    import warnings
-    from sklearn.cluster import KMeans
-    from sklearn.decomposition import TruncatedSVD
-    clips = np.zeros((500000, nt), 'float32')
-    i = 0
+def _snippet_sample(ops, bfile, nt, twav_min, Th_single_ch, nskip, max_clips, device):
+    """Isolated single-channel peaks of every ``nskip``-th batch, as one
+    device tensor [n, nt].  The sample stops before the first batch whose
+    clips would take it past ``max_clips`` (the reference keeps a buffer of
+    500 000 clips and stops there, ``spikedetect.py:56-69``)."""
+    parts, n = [], 0
     for j in range(0, int(bfile.n_batches), nskip):
         X = bfile.padded_batch_to_torch(j, ops)
-        clips_new = extract_snippets(X, nt=nt, twav_min=twav_min, Th_single_ch=Th_single_ch,
-                                     device=device)
-        nnew = len(clips_new)
-        if i + nnew > clips.shape[0]:
+        c = extract_snippets(X, nt=nt, twav_min=twav_min, Th_single_ch=Th_single_ch,
+                             device=device)
+        if n + len(c) > max_clips:
             break
-        clips[i:i + nnew] = clips_new.cpu().numpy()
-        i += nnew
-    clips = clips[:i]
-    clips /= (clips ** 2).sum(1, keepdims=True) ** .5
-    model = TruncatedSVD(n_components=ops['settings']['n_pcs']).fit(clips)
-    wPCA = torch.from_numpy(model.components_).to(device).float()
-    with warnings.catch_warnings():
-        warnings.filterwarnings("ignore", message='KMeans is known to have a memory leak')
-        nthread = os.environ.get('OMP_NUM_THREADS')
-        os.environ['OMP_NUM_THREADS'] = str(min(int(nthread), 7) if nthread else 7)
-        try:
-            model = KMeans(n_clusters=ops['settings']['n_templates'], n_init=10).fit(clips)
-        finally:
-            if nthread is not None:
-                os.environ['OMP_NUM_THREADS'] = nthread
-            else:
-                os.environ.pop('OMP_NUM_THREADS')
-    wTEMP = torch.from_numpy(model.cluster_centers_).to(device).float()
-    wTEMP = wTEMP / (wTEMP ** 2).sum(1).unsqueeze(1) ** .5
+        parts.append(c)
+        n += len(c)
+    return torch.cat(parts, 0) if parts else torch.zeros((0, nt), device=device)
+
+
+def extract_wPCA_wTEMP(ops, bfile, nt=61, twav_min=20, Th_single_ch=6, nskip=25,
+                       device=torch.device('cuda'), max_clips=500000):
+    """Universal PCs and waveforms from the data (``spikedetect.py:53-92``).
+
+    The snippet search runs on the GPU (``ksm_snippets``); the clips are
+    scaled to unit norm there and copied to the host once.  The basis and
+    the waveform clusters come from scikit-learn as in the reference (both
+    are randomised there too): ``TruncatedSVD(n_components=n_pcs)`` gives
+    ``wPCA`` = its components, ``KMeans(n_clusters=n_templates, n_init=10)``
+    gives ``wTEMP`` = its centres scaled to unit norm."""
+    from sklearn.cluster import KMeans
+    from sklearn.decomposition import TruncatedSVD
+    clips = _snippet_sample(ops, bfile, nt, twav_min, Th_single_ch, nskip, max_clips, device)
+    unit = (clips / clips.square().sum(1, keepdim=True).sqrt()).cpu().numpy()
+    settings = ops['settings']
+    basis = TruncatedSVD(n_components=settings['n_pcs']).fit(unit).components_
+    centres = KMeans(n_clusters=settings['n_templates'], n_init=10).fit(unit).cluster_centers_
+    wPCA = torch.from_numpy(basis).to(device).float()
+    wTEMP = torch.from_numpy(centres).to(device).float()
+    wTEMP = wTEMP / wTEMP.norm(dim=1, keepdim=True)
     return wPCA, wTEMP
 
 
