@@ -182,6 +182,30 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
              int capacity, int32_t *info, void *workspace, size_t ws_bytes,
              void *stream);
 
+/*
+ * ksm_peel without a workspace, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as ksm_peel; the peel state lives in one static device array of
+ * ksm_peel_static_capacity() bytes (32 MiB) per GPU, held until the process
+ * ends, whose device address is looked up once per device on the first call.
+ * Sizes whose ksm_peel_workspace exceeds it return KSM_ERR_WORKSPACE (e.g.
+ * T = 60122 holds K <= 2048).  The call allocates nothing (no device or pinned
+ * host memory, no timing events), so every iteration up to max_peels is
+ * enqueued (the device skips them once the peel has ended).  Every call on
+ * one GPU shares its array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be serialised by the caller.
+ */
+int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
+                    const float *W, const float *nm, const int32_t *perm,
+                    const int32_t *rank, const float *cc, const int32_t *lohi,
+                    const int64_t *off, int K, int P, int C, int T, int nt, float Th,
+                    int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
+                    int capacity, int32_t *info, void *stream);
+
+/* Bytes of the static peel state of ksm_peel_static. */
+size_t ksm_peel_static_capacity(void);
+
 /* ksm_peel flags. */
 #define KSM_PEEL_RESCORE 1  /* re-score every accepted spike in FP32 from the
                                residual (for scores from a BF16 projection):

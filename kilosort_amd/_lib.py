@@ -69,6 +69,9 @@ SIGNATURES = {
     "ksm_peel_workspace": (_sz, [_i, _i, _i]),
     "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
                       _i, _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
+    "ksm_peel_static": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i,
+                             _f, _i, _i, _p, _p, _p, _i, _p, _p]),
+    "ksm_peel_static_capacity": (_sz, []),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "ksm_residual_workspace": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
     "ksm_residual": (_i, [_p, _i, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _i, _p, _i,

@@ -36,6 +36,7 @@
 // Once an iteration accepts nothing the done flag makes every later launch
 // return at once, so the host never waits inside the loop.
 #include <chrono>
+#include <mutex>
 
 #include "ksm_common.h"
 #include "ksm_profile.h"
@@ -1538,6 +1539,37 @@ bool wait_progress(const HostFlags &f, unsigned seq, int want, bool *gave_up) {
     }
 }
 
+// Scratch of ksm_peel_static: the peel state a workspace holds for ksm_peel,
+// in one device array per GPU (held until the process ends).  Host code never
+// passes the array's name to a kernel: on the host it is a stand-in symbol,
+// not a device address.  The device address is looked up once per device
+// (with that device current) and kept.
+constexpr size_t kPeelScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kPeelScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+int peel_scratch_address(void **out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return ksm::hip_fail(e, "ksm_peel_static: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_peel_static: device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return ksm::hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                                 "ksm_peel_static: hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return KSM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1553,20 +1585,32 @@ size_t ksm_peel_workspace(int K, int T, int max_peels) {
            256;
 }
 
-int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
-             const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
-             const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
-             float Th, int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
-             int capacity, int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
-    if (!Bt || !Xres || !U || !W || !nm || !perm || !rank || !cc || !lohi || !off || !st ||
-        !amps || !th_amps || !info || K <= 0 || K > kMaxK || ldbt < K || (ldbt % 4) != 0 ||
-        P <= 0 || P > 8 || C <= 0 || C > kMaxC || nt <= 0 || nt > kMaxNt || (nt % 2) == 0 ||
-        T < 2 * nt + 1 || T > kChunk * kMaxChunks || ldx < T || max_peels < 0 ||
-        capacity < 0 || (flags & ~KSM_PEEL_RESCORE) != 0)
-        return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
-    if (!workspace || ws_bytes < ksm_peel_workspace(K, T, max_peels))
-        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
-    hipStream_t s = ksm::as_stream(stream);
+// Validation shared by ksm_peel and ksm_peel_static.
+static bool peel_args_ok(const float *Bt, int ldbt, const float *Xres, int ldx, const float *U,
+                         const float *W, const float *nm, const int32_t *perm,
+                         const int32_t *rank, const float *cc, const int32_t *lohi,
+                         const int64_t *off, int K, int P, int C, int T, int nt, int max_peels,
+                         int flags, const int64_t *st, const float *amps, const float *th_amps,
+                         int capacity, const int32_t *info) {
+    return Bt && Xres && U && W && nm && perm && rank && cc && lohi && off && st && amps &&
+           th_amps && info && K > 0 && K <= kMaxK && ldbt >= K && (ldbt % 4) == 0 && P > 0 &&
+           P <= 8 && C > 0 && C <= kMaxC && nt > 0 && nt <= kMaxNt && (nt % 2) == 1 &&
+           T >= 2 * nt + 1 && T <= kChunk * kMaxChunks && ldx >= T && max_peels >= 0 &&
+           capacity >= 0 && (flags & ~KSM_PEEL_RESCORE) == 0;
+}
+
+// The peel on a validated argument set with its scratch state at `workspace`
+// (at least ksm_peel_workspace bytes).  may_poll: the host may follow the
+// device's progress through its pinned progress words (allocated on a host
+// thread's first call); otherwise every iteration up to max_peels is enqueued.
+static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
+                    const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
+                    const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
+                    float Th, int max_peels, int flags, int64_t *st, float *amps,
+                    float *th_amps, int capacity, int32_t *info, void *workspace,
+                    hipStream_t s, bool may_poll) {
+    // the kernels are timed only for the entry point that may allocate (events)
+    const unsigned prof_group = may_poll ? ksm::kProfPeel : 0u;
     const PeelLayout L = peel_layout(K, T, max_peels);
     ksm::Carver cv(workspace);
     u64 *gkey = cv.take<u64>((size_t)T * L.G);
@@ -1596,13 +1640,14 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, crange, K, gref);
     KSM_LAUNCH_CHECK("gram_refs_kernel");
     {
-        ksm::ProfileScope prof("colmax_all", ksm::kProfPeel, s);
+        ksm::ProfileScope prof("colmax_all", prof_group, s);
         prof.launch(colmax_all_kernel, dim3((T + kFirstCols - 1) / kFirstCols), dim3(kUpdThreads),
                     (unsigned)tab_dyn, Bt, ldbt, pnm, ppid, K, L.G, L.Kg, T, nt, cfmax, imax, gkey);
     }
     KSM_LAUNCH_CHECK("colmax_all_kernel");
     // stop enqueuing once the device reports the end (not under stream capture)
-    HostFlags &hf = host_flags();
+    HostFlags none_flags;
+    HostFlags &hf = may_poll ? host_flags() : none_flags;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     const bool capturing =
         hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
@@ -1614,7 +1659,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     for (int it = 0; it < max_peels; ++it) {
         if (poll && !gave_up && it >= kLag && wait_progress(hf, seq, it - kLag, &gave_up)) break;
         {
-            ksm::ProfileScope prof("peel_select", ksm::kProfPeel, s);
+            ksm::ProfileScope prof("peel_select", prof_group, s);
             if (flags & KSM_PEEL_RESCORE)
                 prof.launch(peel_select_kernel<kRsThreads, true>, dim3(L.nchunks),
                             dim3(kRsThreads),
@@ -1626,7 +1671,7 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
                             ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
         }
         {
-            ksm::ProfileScope prof("peel_update", ksm::kProfPeel, s);
+            ksm::ProfileScope prof("peel_update", prof_group, s);
             prof.launch(peel_update_kernel, dim3(grid), dim3(kUpdThreads), (unsigned)upd_dyn, it,
                         Bt, ldbt, Xres, ldx, U, W, crange, P, C, pnm, ppid, cc, K, L.G, L.Kg, T,
                         nt, L.nchunks, chunk_count, slots, state, st, amps, th_amps, capacity,
@@ -1637,6 +1682,45 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
     peel_finish_kernel<<<1, 1, 0, s>>>(state, info);
     KSM_LAUNCH_CHECK("peel_finish_kernel");
     return KSM_OK;
+}
+
+int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
+             const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
+             const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
+             float Th, int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
+             int capacity, int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
+    if (!peel_args_ok(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt,
+                      max_peels, flags, st, amps, th_amps, capacity, info))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
+    if (!workspace || ws_bytes < ksm_peel_workspace(K, T, max_peels))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
+    return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
+                    max_peels, flags, st, amps, th_amps, capacity, info, workspace,
+                    ksm::as_stream(stream), true);
+}
+
+size_t ksm_peel_static_capacity(void) { return kPeelScratchBytes; }
+
+int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
+                    const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
+                    const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
+                    float Th, int max_peels, int flags, int64_t *st, float *amps,
+                    float *th_amps, int capacity, int32_t *info, void *stream) {
+    if (!peel_args_ok(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt,
+                      max_peels, flags, st, amps, th_amps, capacity, info))
+        return ksm::fail(KSM_ERR_INVALID, "ksm_peel_static: bad arguments");
+    const size_t need = ksm_peel_workspace(K, T, max_peels);
+    if (need > kPeelScratchBytes)
+        return ksm::fail(KSM_ERR_WORKSPACE,
+                         "ksm_peel_static: K=%d, T=%d, max_peels=%d need %zu bytes of scratch, "
+                         "more than the library's static %zu (use ksm_peel with a workspace)",
+                         K, T, max_peels, need, kPeelScratchBytes);
+    void *ws = nullptr;
+    const int rc = peel_scratch_address(&ws);
+    if (rc != KSM_OK) return rc;
+    return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
+                    max_peels, flags, st, amps, th_amps, capacity, info, ws,
+                    ksm::as_stream(stream), false);
 }
 
 int ksm_features(const float *Xres, int ldx, const float *W, const int64_t *st,
