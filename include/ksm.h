@@ -151,6 +151,39 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U,
                 int ldbt, int precision, void *workspace, size_t ws_bytes,
                 void *stream);
 
+/* Bytes of the bound buffer of ksm_project_bounded. */
+size_t ksm_bound_bytes(int C, int T, int K, int P, int nt);
+
+/*
+ * ksm_project for the reduced-precision forms (BF16, BF16X3, FP8), plus the
+ * error bound of their scores, for the peel's decisions (no reference
+ * counterpart: the reference scores templates in FP32 only).  bound: a
+ * device buffer of ksm_bound_bytes(C, T, K, P, nt) bytes, handed to ksm_peel
+ * with the same sizes.  Its sections, each 256-byte aligned: a 256-byte
+ * header; colD float [T]; colA float [T]; S float [nsb][T]; Wt float
+ * [nsb][K]; srng int32 [K][2]; Ttab float [K][C][nt] (nsb = ceil(C / 32)
+ * sub-blocks of 32 channels; template positions r in perm order).  For every
+ * template position r and column t, with D_r(t) = sum_b Wt[b][r] S[b][t]
+ * (BF16 forms) or sqrt(sum_b Wt[b][r] S[b][t]) (FP8):
+ *     |B~[r,t] - B[r,t]| / sqrt(nm_r) <= D_r(t) + colA[t] <= colD[t],
+ * B~ the reduced-precision score, B the exact one, nm_r = sum U[perm[r]]^2.
+ *   BF16 / BF16X3: a proof -- kappa sum |U||Bc| by Cauchy-Schwarz over each
+ *     sub-block (kappa = 2u + u^2 + gamma_n, u = 2^-8; 3u^2(1+u)^3 + gamma_n
+ *     for the split form), plus 2^-14 ||Bc[., t]|| for the FP32 rounding of
+ *     later subtractions;
+ *   FP8: a statistical margin -- 6 standard deviations of the modelled
+ *     independent e4m3 rounding errors (variance <= (2/3) 2^-8 sum_b E_rb
+ *     max_b Bc^2), plus worst-case accumulation and subnormal terms.
+ * srng / Ttab: each template's significant channels [lo, hi) (energy at
+ * least 2^-70 of its largest channel's) and its waveforms there, Ttab[r][c -
+ * lo][k] = sum_p U[perm[r],p,c] W[p,k], from which the peel re-scores in FP32.
+ * Same workspace as ksm_project.  KSM_PREC_FP32 is rejected; C <= 4096.
+ */
+int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
+                        const int32_t *perm, int C, int T, int K, int P, int nt, float *Bt,
+                        int ldbt, int precision, void *bound, size_t bound_bytes,
+                        void *workspace, size_t ws_bytes, void *stream);
+
 /* Workspace bytes of ksm_peel. */
 size_t ksm_peel_workspace(int K, int T, int max_peels);
 
@@ -166,11 +199,14 @@ size_t ksm_peel_workspace(int K, int T, int max_peels);
  * Bt (from ksm_project with the same perm) and Xres (a copy of X) are updated
  * in place; nm float32 [K] by template id (ksm_template_norms); perm/rank from
  * ksm_template_order; cc/lohi/off from ksm_gram_ranges + ksm_gram_compact.
- * flags: KSM_PEEL_RESCORE (below) or 0.
+ * flags: KSM_PEEL_RESCORE (below) with the bound buffer of
+ * ksm_project_bounded, or 0 with bound NULL (FP32 scores, decided as they
+ * are).
  * Outputs: st int64 [capacity][2] (time column, template id), amps and
  * th_amps float32 [capacity], and info int32 [4] on the device:
  *   info[0] = spike count k, info[1] = ksm_status of the device run,
- *   info[2] = iterations that accepted spikes.
+ *   info[2] = iterations that accepted spikes, info[3] = (column, template)
+ *   scores re-scored in FP32 (KSM_PEEL_RESCORE).
  * Spikes are ordered as the reference orders them: by iteration, then time.
  * Limits: K <= 8192, P <= 8, C <= 65535, odd nt <= 255.
  */
@@ -178,9 +214,9 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
              const float *W, const float *nm, const int32_t *perm,
              const int32_t *rank, const float *cc, const int32_t *lohi,
              const int64_t *off, int K, int P, int C, int T, int nt, float Th,
-             int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
-             int capacity, int32_t *info, void *workspace, size_t ws_bytes,
-             void *stream);
+             int max_peels, int flags, const void *bound, int64_t *st, float *amps,
+             float *th_amps, int capacity, int32_t *info, void *workspace,
+             size_t ws_bytes, void *stream);
 
 /*
  * ksm_peel without a workspace, for callers that cannot allocate device
@@ -200,16 +236,20 @@ int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
                     const float *W, const float *nm, const int32_t *perm,
                     const int32_t *rank, const float *cc, const int32_t *lohi,
                     const int64_t *off, int K, int P, int C, int T, int nt, float Th,
-                    int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
-                    int capacity, int32_t *info, void *stream);
+                    int max_peels, int flags, const void *bound, int64_t *st,
+                    float *amps, float *th_amps, int capacity, int32_t *info, void *stream);
 
 /* Bytes of the static peel state of ksm_peel_static. */
 size_t ksm_peel_static_capacity(void);
 
 /* ksm_peel flags. */
-#define KSM_PEEL_RESCORE 1  /* re-score every accepted spike in FP32 from the
-                               residual (for scores from a BF16 projection):
-                               amps, th_amps and the score entry become FP32 */
+#define KSM_PEEL_RESCORE 1  /* the scores come from a reduced-precision
+                               projection (ksm_project_bounded, same perm) and
+                               bound is its bound buffer: every decision the
+                               bound leaves open is taken on scores re-scored
+                               in FP32 from the residual, and accepted spikes
+                               take their FP32 score for amps and th_amps.
+                               Without it bound must be NULL. */
 
 /*
  * PC features of each spike on the nC channels nearest its template's peak

@@ -66,11 +66,14 @@ SIGNATURES = {
     "ksm_gram_compact": (_i, [_p, _i, _p, _p, _p, _i, _i, _p, _p]),
     "ksm_project_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
     "ksm_project": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
+    "ksm_bound_bytes": (_sz, [_i, _i, _i, _i, _i]),
+    "ksm_project_bounded": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz,
+                                 _p, _sz, _p]),
     "ksm_peel_workspace": (_sz, [_i, _i, _i]),
     "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
-                      _i, _i, _p, _p, _p, _i, _p, _p, _sz, _p]),
+                      _i, _i, _p, _p, _p, _p, _i, _p, _p, _sz, _p]),
     "ksm_peel_static": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i,
-                             _f, _i, _i, _p, _p, _p, _i, _p, _p]),
+                             _f, _i, _i, _p, _p, _p, _p, _i, _p, _p]),
     "ksm_peel_static_capacity": (_sz, []),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "ksm_residual_workspace": (_sz, [_i, _i, _i, _i, _i, _i, _i]),

@@ -1,0 +1,198 @@
+// Error bound of the reduced-precision projection forms, per template and
+// column, for the peel's decisions (no reference counterpart: the reference
+// scores templates in FP32 only, template_matching.py:180-181).
+//
+// Scores per unit template norm: beta_r(t) = B[r, t] / sqrt(nm_r), so that
+// sqrt(Cf) = relu(beta).  With the channels cut into sub-blocks b of 32 and
+// E_rb = sum over the sub-block's (channel, PC) rows of U[r]^2:
+//   BF16 forms (a proof): |B~ - B| <= kappa sum_rows |U||Bc|
+//       <= kappa sum_b sqrt(E_rb) ||Bc_b(t)||   (Cauchy-Schwarz per sub-block),
+//   FP8 (a model): the rounding error has variance <= (2/3) 2^-8 sum U^2 Bc^2
+//       <= (2/3) 2^-8 sum_b E_rb max_b Bc(t)^2, and the margin is zc times
+//       its standard deviation.
+// The buffer also carries each template's significant channels and its
+// waveforms on them, T_r[c][k] = sum_p U[r,p,c] W[p,k], from which the peel
+// re-scores columns in FP32.
+#include <cmath>
+
+#include "ksm_bound.h"
+
+namespace {
+
+using ksm::BoundLayout;
+using ksm::kBoundSub;
+
+__global__ void bound_header_kernel(int *__restrict__ hdr, int mode, int nsb) {
+    hdr[0] = mode;
+    hdr[1] = nsb;
+}
+
+// Per template position r: Wt[b][r], the significant channel range and the
+// waveform rows.  Significant channels: every channel whose energy
+// sum_p U[r,p,c]^2 is at least 2^-70 of the template's largest, closed into
+// one range; the channels left out carry at most C 2^-70 of the template's
+// energy, so a re-score without them differs by less than 2^-30 ||X window||.
+__global__ __launch_bounds__(256) void bound_weights_kernel(
+    const float *__restrict__ U, const float *__restrict__ W, const int32_t *__restrict__ perm,
+    int C, int K, int P, int nt, int nsb, int mode, double kappa, double zc,
+    float *__restrict__ Wt, int *__restrict__ srng, float *__restrict__ Ttab) {
+    __shared__ double s_e[ksm::kBoundMaxC];
+    __shared__ double s_red[4];
+    __shared__ int s_lo[4], s_hi[4];
+    const int r = blockIdx.x;
+    const int id = perm ? perm[r] : r;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float *u = U + (size_t)id * P * C;
+    double mx = 0.0;
+    for (int c = tid; c < C; c += 256) {
+        double e = 0.0;
+        for (int p = 0; p < P; ++p) {
+            const double v = u[(size_t)p * C + c];
+            e += v * v;
+        }
+        s_e[c] = e;
+        mx = e > mx ? e : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(mx, o, 64);
+        mx = w > mx ? w : mx;
+    }
+    if (lane == 0) s_red[wave] = mx;
+    __syncthreads();
+    mx = s_red[0];
+    for (int w = 1; w < 4; ++w) mx = s_red[w] > mx ? s_red[w] : mx;
+    const double thr = mx * 0x1p-70;
+    int lo = C, hi = 0;
+    for (int c = tid; c < C; c += 256)
+        if (mx > 0.0 && s_e[c] >= thr) {
+            lo = c < lo ? c : lo;
+            hi = c + 1 > hi ? c + 1 : hi;
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o, 64));
+        hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    __syncthreads();
+    if (lane == 0) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    __syncthreads();
+    lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+    hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
+    if (hi <= lo) lo = hi = 0;
+    // total energy in a fixed order (sub-block sums, then their sum)
+    double nm = 0.0;
+    for (int b = 0; b < nsb; ++b) {
+        double eb = 0.0;
+        for (int c = b * kBoundSub; c < min(C, (b + 1) * kBoundSub); ++c) eb += s_e[c];
+        nm += eb;
+    }
+    for (int b = tid; b < nsb; b += 256) {
+        double eb = 0.0;
+        for (int c = b * kBoundSub; c < min(C, (b + 1) * kBoundSub); ++c) eb += s_e[c];
+        double w = 0.0;
+        if (nm > 0.0)
+            w = mode == ksm::kBoundLinear ? kappa * sqrt(eb / nm) : zc * zc * (eb / nm);
+        Wt[(size_t)b * K + r] = (float)(w * (1.0 + 0x1p-20));
+    }
+    if (tid == 0) {
+        srng[2 * r] = lo;
+        srng[2 * r + 1] = hi;
+    }
+    float *tr = Ttab + (size_t)r * C * nt;
+    const int n = (hi - lo) * nt;
+    for (int e = tid; e < n; e += 256) {
+        const int c = lo + e / nt, k = e % nt;
+        float v = 0.f;
+        for (int p = 0; p < P; ++p) v = __builtin_fmaf(u[(size_t)p * C + c], W[p * nt + k], v);
+        tr[e] = v;
+    }
+}
+
+// Per column t: S[b][t], colA[t] and colD[t] = max_r D_r(t) + colA[t].  One
+// wave of 64 columns per block: the column statistics sit in LDS, the
+// template loop is uniform (weights through scalar loads).
+__global__ __launch_bounds__(64) void bound_columns_kernel(
+    const float *__restrict__ stats, int nsb2, int ldst, int T, int K, int nsb, int mode,
+    double acc, const float *__restrict__ abs_terms, const float *__restrict__ Wt,
+    float *__restrict__ S, float *__restrict__ colA, float *__restrict__ colD) {
+    constexpr int kMaxSub = ksm::kBoundMaxC / kBoundSub;
+    __shared__ float s_v[kMaxSub][64];
+    const int lane = threadIdx.x;
+    const int t = blockIdx.x * 64 + lane;
+    const bool live = t < T;
+    const int tt = live ? t : T - 1;
+    double s0 = 0.0, s2 = 0.0;
+    bool bad = false;
+    for (int b = 0; b < nsb; ++b) {
+        const float q = stats[(size_t)b * ldst + tt];
+        const float m = stats[((size_t)nsb2 + b) * ldst + tt];
+        bad |= !__builtin_isfinite(q) || !__builtin_isfinite(m);
+        s0 += q;
+        if (mode == ksm::kBoundSqrt) {
+            const float q2 = stats[((size_t)2 * nsb2 + b) * ldst + tt];
+            bad |= !__builtin_isfinite(q2);
+            s2 += q2;
+        }
+        // sqrt rounded up by one part in 2^20 (q carries its own float rounding)
+        const float v = mode == ksm::kBoundLinear ? sqrtf(q) * (1.f + 0x1p-20f) : m;
+        s_v[b][lane] = v;
+        if (live) S[(size_t)b * T + t] = bad ? __builtin_inff() : v;
+    }
+    const double up = 1.0 + 0x1p-18;
+    double a = (0x1p-14 + acc) * sqrt(s0 * up);
+    if (mode == ksm::kBoundSqrt && abs_terms)
+        a += (double)abs_terms[0] + (double)abs_terms[1] * sqrt(s2 * up);
+    a *= up;
+    float dmax = 0.f;
+    for (int r = 0; r < K; ++r) {
+        float d = 0.f;
+        for (int b = 0; b < nsb; ++b) d = __builtin_fmaf(Wt[(size_t)b * K + r], s_v[b][lane], d);
+        dmax = fmaxf(dmax, d);
+    }
+    // float sums of nsb nonnegative products: relative rounding < nsb 2^-24
+    double dm = (double)dmax * (1.0 + (nsb + 2) * 0x1p-24);
+    if (mode == ksm::kBoundSqrt) dm = sqrt(dm);
+    if (!live) return;
+    const float inf = __builtin_inff();
+    colA[t] = bad ? inf : (float)a * (1.f + 0x1p-20f);
+    colD[t] = bad ? inf : (float)((dm + a) * up) * (1.f + 0x1p-20f);
+}
+
+}  // namespace
+
+namespace ksm {
+
+int bound_finish(const float *stats, int nsb2, int ldst, const float *U, const float *W,
+                 const int32_t *perm, int C, int T, int K, int P, int nt, int mode, double kappa,
+                 double zc, double acc, const float *abs_terms, void *bound, hipStream_t s) {
+    if (C > kBoundMaxC)
+        return fail(KSM_ERR_INVALID, "ksm_project_bounded: at most %d channels", kBoundMaxC);
+    const BoundLayout L = bound_layout(C, T, K, nt);
+    if (nsb2 < L.nsb) return fail(KSM_ERR_INVALID, "ksm_project_bounded: statistics layout");
+    char *b = static_cast<char *>(bound);
+    float *colD = reinterpret_cast<float *>(b + L.o_colD);
+    float *colA = reinterpret_cast<float *>(b + L.o_colA);
+    float *S = reinterpret_cast<float *>(b + L.o_S);
+    float *Wt = reinterpret_cast<float *>(b + L.o_Wt);
+    int *srng = reinterpret_cast<int *>(b + L.o_srng);
+    float *Ttab = reinterpret_cast<float *>(b + L.o_T);
+    bound_header_kernel<<<1, 1, 0, s>>>(reinterpret_cast<int *>(b + L.o_hdr), mode, L.nsb);
+    KSM_LAUNCH_CHECK("bound_header_kernel");
+    bound_weights_kernel<<<K, 256, 0, s>>>(U, W, perm, C, K, P, nt, L.nsb, mode, kappa, zc, Wt,
+                                           srng, Ttab);
+    KSM_LAUNCH_CHECK("bound_weights_kernel");
+    bound_columns_kernel<<<(T + 63) / 64, 64, 0, s>>>(stats, nsb2, ldst, T, K, L.nsb, mode,
+                                                         acc, abs_terms, Wt, S, colA, colD);
+    KSM_LAUNCH_CHECK("bound_columns_kernel");
+    return KSM_OK;
+}
+
+}  // namespace ksm
+
+extern "C" size_t ksm_bound_bytes(int C, int T, int K, int P, int nt) {
+    (void)P;
+    if (C <= 0 || T <= 0 || K <= 0 || nt <= 0) return 0;
+    return ksm::bound_layout(C, T, K, nt).bytes;
+}

@@ -21,6 +21,37 @@ constexpr size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 constexpr int kWave = 64;
 
+// Sum over the 64 lanes of a wave (every lane gets the result).
+__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Sum over each 32-lane half of a wave (lanes of a half get its result).
+__device__ __forceinline__ float half_sum_f(float v) {
+    for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Maximum over each 32-lane half of a wave; a NaN on a lane makes its half's
+// result NaN.
+__device__ __forceinline__ float half_max_nan(float v) {
+    for (int o = 16; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = (v != v || w != w) ? __builtin_nanf("") : fmaxf(v, w);
+    }
+    return v;
+}
+
+// Maximum over the 64 lanes of a wave; a NaN on any lane makes the result NaN.
+__device__ __forceinline__ float wave_max_nan(float v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = (v != v || w != w) ? __builtin_nanf("") : fmaxf(v, w);
+    }
+    return v;
+}
+
 // Workspace carving: successive 256-byte aligned sub-buffers.
 struct Carver {
     char *base;

@@ -24,7 +24,8 @@
 //
 // One iteration is two launches on the caller's stream:
 //   peel_select  window max, threshold and ordered compaction into slots
-//                (plus the FP32 re-scoring of reduced-precision scores);
+//                (for reduced-precision scores peel_select_cert: decisions
+//                on their error bound, open ones on FP32 re-scores);
 //   peel_update  a work list over the accepted spikes, spread over every
 //                wave of a grid that fills the chip: per spike, one item per
 //                column of its +-nt window and kXresItems items for its
@@ -39,6 +40,7 @@
 #include <mutex>
 
 #include "ksm_common.h"
+#include "ksm_bound.h"
 #include "ksm_profile.h"
 
 namespace {
@@ -53,12 +55,9 @@ constexpr int kMaxC = 65535;        // channel ranges are packed in 16 bits
 constexpr int kMaxChunks = 4096;    // T <= kChunk * kMaxChunks
 constexpr int kUpdThreads = 256;    // update / first-pass block (4 waves)
 constexpr int kXresItems = 8;       // residual work items per spike
-constexpr int kRsTasks = 32;        // re-scoring tasks (channel blocks) per spike
-constexpr int kRsThreads = 256;     // select block when re-scoring: small blocks
-                                    // co-schedule with the other streams' updates
 constexpr int kXresCh = 16;         // channels per residual step (loads first)
 
-enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_KBASE = 4 };
+enum { ST_DONE = 0, ST_STATUS = 1, ST_ITERS = 2, ST_RESCORED = 3, ST_KBASE = 4 };
 
 struct Slot {
     int x;          // column in the padded batch
@@ -464,26 +463,17 @@ __global__ __launch_bounds__(kUpdThreads) void colmax_all_kernel(
 // (template_matching.py:205-212), and records them in time order.  The window
 // max of the chunk's columns comes from maxima of 2^j consecutive values built
 // by doubling in LDS (max_pool1d semantics: -inf padding, NaN propagates).
-//
-// With `rescore` set (scores from a reduced-precision projection), each
-// accepted spike is then re-scored in FP32 from the residual,
-//   b = sum_c sum_k Xres[c, x + k - nt/2] * (sum_p U[id,p,c] W[p,k]),
-// and b replaces the score entry, the amplitude (b/nm) and th_amp
-// (sqrt(relu(b)^2/nm)), so that what is recorded and subtracted is FP32.
-// Work split: lanes = lags; the spikes' nonzero channel ranges are cut into
-// blocks of 16 (or more) channels, one task per (spike, block), spread over
-// the block's waves; per-task sums meet in LDS and are added in block order.
+// The scores are FP32 (the reduced-precision forms use
+// peel_select_cert_kernel below).
 // NaN-propagating maximum (max_pool1d semantics).
 __device__ __forceinline__ float nan_max(float a, float b) {
     return a != a ? a : (b != b ? b : fmaxf(a, b));
 }
 
-template <int NTH, bool RS>
+template <int NTH>
 __global__ __launch_bounds__(NTH) void peel_select_kernel(
     const float *__restrict__ cfmax, const int *__restrict__ imax, float *__restrict__ Bt,
-    int ldbt, const GramRef *__restrict__ gref, int T, int nt, float th2, int rescore,
-    const float *__restrict__ Xres, int ldx, const float *__restrict__ U,
-    const float *__restrict__ W, const int *__restrict__ crange, int P, int C,
+    int ldbt, const GramRef *__restrict__ gref, int T, int nt, float th2,
     const int32_t *__restrict__ rank, int K, int *__restrict__ state,
     int *__restrict__ chunk_count, Slot *__restrict__ slots) {
     // every load of the prologue is issued together: the done flag, the
@@ -493,10 +483,6 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
     __shared__ float win[kChunk + 2 * kMaxNt];
     __shared__ float win2[kChunk + 2 * kMaxNt];
     __shared__ int s_wc[kChunk / 64];
-    __shared__ int s_sx[kSlotCap], s_sid[kSlotCap];
-    __shared__ int s_tb[kSlotCap + 1], s_clo[kSlotCap], s_cw[kSlotCap];
-    __shared__ float s_part[kSlotCap][kRsTasks];
-    constexpr int NW = NTH / 64;
     const int t0 = blockIdx.x * kChunk;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int x = t0 + tid;
@@ -555,8 +541,6 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
             sl.crg = (unsigned)g.clo | ((unsigned)g.chi << 16);
             sl.off = g.off;
             slots[(size_t)blockIdx.x * kSlotCap + pos] = sl;
-            s_sx[pos] = x;
-            s_sid[pos] = iy;
         }
     }
     if (tid == 0) {
@@ -566,88 +550,614 @@ __global__ __launch_bounds__(NTH) void peel_select_kernel(
         }
         chunk_count[blockIdx.x] = count < kSlotCap ? count : kSlotCap;
     }
-    const int n = count < kSlotCap ? count : kSlotCap;
-    __syncthreads();
-    if constexpr (!RS) return;               // the re-scoring code only in its own form
-    if (!rescore || n == 0) return;          // block-uniform
-    // tasks: (spike q, channel block b); spike q's nonzero channel range is
-    // cut into s_tb[q+1] - s_tb[q] <= kRsTasks blocks of s_cw[q] channels
-    if (tid < n) {
-        const int id = s_sid[tid];
-        const int clo = crange[2 * id], chi = crange[2 * id + 1];
-        int nb = (chi - clo + 15) / 16;
-        nb = nb < kRsTasks ? nb : kRsTasks;
-        const int cw = nb > 0 ? ((chi - clo + nb - 1) / nb + 15) / 16 * 16 : 16;
-        s_clo[tid] = clo;
-        s_cw[tid] = cw;
-        s_tb[tid + 1] = nb > 0 ? (chi - clo + cw - 1) / cw : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Certified selection for reduced-precision scores (BF16, split BF16, FP8).
+//
+// The scores B~ of a reduced-precision projection come with a bound
+// (ksm_project_bounded): per unit template norm, |B~[r,t] - B[r,t]| /
+// sqrt(nm_r) <= D_r(t) + colA[t] <= colD[t], so sqrt(Cf~) = relu(B~)/sqrt(nm)
+// lies within that of sqrt(Cf) for every template (relu is 1-Lipschitz), and
+// a column's best score within colD[t] of its approximate best.  The
+// reference's decision for column x (template_matching.py:197-212: Cmax >
+// Th^2 and Cmax == Cfmax, Cmax the +-nt window maximum) is taken on these
+// intervals where they settle it and on exact values where they do not: a
+// column whose decision is open is re-scored in FP32 from the residual, for
+// every template that can still be its best,
+//   b = sum_c sum_k Xres[c, x + k - nt/2] * T_r[c, k]
+// (T_r the template's waveforms on its significant channels), which makes
+// the column exact (its best score and template known to FP32 accuracy).
+// Rounds repeat until every column of the chunk is decided:
+//   * certainly rejected: its upper bound is <= Th^2, or some column of its
+//     window certainly scores more (a lower bound above its upper bound);
+//   * open and not exact: re-scored;
+//   * exact, with non-exact window columns that may reach its score:
+//     those are re-scored;
+//   * exact, every other window column exact or certainly below: the
+//     reference's test on the exact values.
+// Each round makes at least one window column exact, so the rounds end.
+// Re-scored values are written back into Bt (the scores the update kernel
+// subtracts from) for the chunk's own columns, and a column's exactness is
+// kept in the sign bit of its cfmax entry (cfmax = -Cf): the update kernel
+// writes every column it changes with a non-negative value, which clears
+// it.  Halo columns re-scored by a neighbouring chunk stay in that block's
+// LDS.  Accepted spikes take their FP32 b for amps (b/nm) and th_amps.
+constexpr int kCertThreads = 512;
+constexpr int kCertMaxWin = kChunk + 2 * kMaxNt;
+constexpr float kSqrtSlack = 0x1p-20f;   // sqrt and score rounding, relative
+
+__device__ __forceinline__ bool cf_is_exact(float v) {
+    return (__float_as_uint(v) >> 31) != 0u || v != v;
+}
+
+// The bound D_r(y) of template position r at column y (without colA),
+// rounded up, from the column's sub-block statistics sc[b] = S[b][y] in LDS.
+__device__ __forceinline__ float bound_lds(const ksm::BoundView &bv, const float *sc, int K,
+                                           int r) {
+    float a = 0.f;
+    // 16 weights per step, loads issued together
+    for (int b0 = 0; b0 < bv.nsb; b0 += 16) {
+        float w[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) w[u] = b0 + u < bv.nsb ? bv.Wt[(size_t)(b0 + u) * K + r] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (b0 + u < bv.nsb) a = __builtin_fmaf(w[u], sc[b0 + u], a);
     }
-    __syncthreads();
-    if (tid == 0) {
-        s_tb[0] = 0;
-        for (int q = 0; q < n; ++q) s_tb[q + 1] += s_tb[q];
-    }
-    __syncthreads();
+    a *= 1.f + 0x1p-16f;
+    if (bv.mode == ksm::kBoundSqrt) a = sqrtf(a) * (1.f + kSqrtSlack);
+    return a;
+}
+
+// Partial FP32 score of template position r at column y over its significant
+// channels [c0, c1) (at most kTaskCh): lanes take lags, each lane sums its
+// channels in order, then a fixed butterfly over the lanes (wave-uniform).
+// A column's score b is the sum of its chunks' partials in chunk order, the
+// same whichever wave computes which chunk.
+constexpr int kTaskCh = 16;
+__device__ float rescore_chunk(int r, int y, int c0, int c1, const float *__restrict__ Xres,
+                               int ldx, const ksm::BoundView &bv, int C, int nt) {
+    const int lane = threadIdx.x & 63;
     const int h = nt / 2;
-    const int ntask = s_tb[n];
-    for (int j = uni(wave); j < ntask; j += NW) {
-        int q = 0;
-        while (s_tb[q + 1] <= j) ++q;
-        q = uni(q);
-        const int bi = j - s_tb[q];
-        const int id = uni(s_sid[q]), sx = uni(s_sx[q]);
-        const int chi = uni(crange[2 * id + 1]);
-        const int cw = uni(s_cw[q]);
-        const int ca = uni(s_clo[q]) + bi * cw, cz = min(chi, ca + cw);
-        const float *uj = U + (size_t)id * P * C;
-        float part = 0.f;
-        for (int cs = ca; cs < cz; cs += 16) {
-            const int nch = min(cz - cs, 16);
-            // lane l < nch holds U[id, :, cs + l], broadcast with readlane
-            float ul[8];
+    const int lo = uni(bv.srng[2 * r]);
+    const float *tr = bv.T + (size_t)r * C * nt;
+    float part = 0.f;
+    for (int kb0 = 0; kb0 < nt; kb0 += 64) {
+        const int k = kb0 + lane;
+        const bool kin = k < nt;
+        const int kk = kin ? k : 0;
+        const float *xb = Xres + (size_t)(y - h + kk);
+        const float *tb = tr + kk;
+        float xv[kTaskCh], tv[kTaskCh];
 #pragma unroll
-            for (int p = 0; p < 8; ++p)
-                ul[p] = (p < P && lane < nch) ? uj[(size_t)p * C + cs + lane] : 0.f;
-            for (int kb0 = 0; kb0 < nt; kb0 += 64) {
-                const int k = kb0 + lane;
-                const bool kin = k < nt;
-                float wk[8];
+        for (int u = 0; u < kTaskCh; ++u) {
+            const int c = c0 + u < c1 ? c0 + u : c1 - 1;
+            xv[u] = xb[(size_t)c * ldx];
+            tv[u] = tb[(size_t)(c - lo) * nt];
+        }
 #pragma unroll
-                for (int p = 0; p < 8; ++p) wk[p] = (p < P && kin) ? W[p * nt + k] : 0.f;
-                const float *xb = Xres + sx - h + (kin ? k : 0) + (size_t)cs * ldx;
-                float xv[16];
+        for (int u = 0; u < kTaskCh; ++u)
+            part = __builtin_fmaf((kin && c0 + u < c1) ? xv[u] : 0.f, tv[u], part);
+    }
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
+    return uni(part);
+}
+
+// Partials of two consecutive chunks [c0, c1) and [c1, c2) (each at most
+// kTaskCh channels) with their loads issued together; each equals
+// rescore_chunk of its chunk.
+__device__ void rescore_chunk2(int r, int y, int c0, int c1, int c2,
+                               const float *__restrict__ Xres, int ldx, const ksm::BoundView &bv,
+                               int C, int nt, float &p0, float &p1) {
+    const int lane = threadIdx.x & 63;
+    const int h = nt / 2;
+    const int lo = uni(bv.srng[2 * r]);
+    const float *tr = bv.T + (size_t)r * C * nt;
+    float a0 = 0.f, a1 = 0.f;
+    for (int kb0 = 0; kb0 < nt; kb0 += 64) {
+        const int k = kb0 + lane;
+        const bool kin = k < nt;
+        const int kk = kin ? k : 0;
+        const float *xb = Xres + (size_t)(y - h + kk);
+        const float *tb = tr + kk;
+        float xv[2 * kTaskCh], tv[2 * kTaskCh];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const int c = u < nch ? u : nch - 1;
-                    xv[u] = xb[(size_t)c * ldx];
+        for (int u = 0; u < 2 * kTaskCh; ++u) {
+            const int cs = u < kTaskCh ? c0 + u : c1 + (u - kTaskCh);
+            const int ce = u < kTaskCh ? c1 : c2;
+            const int c = cs < ce ? cs : ce - 1;
+            xv[u] = xb[(size_t)c * ldx];
+            tv[u] = tb[(size_t)(c - lo) * nt];
+        }
+#pragma unroll
+        for (int u = 0; u < kTaskCh; ++u)
+            a0 = __builtin_fmaf((kin && c0 + u < c1) ? xv[u] : 0.f, tv[u], a0);
+#pragma unroll
+        for (int u = 0; u < kTaskCh; ++u)
+            a1 = __builtin_fmaf((kin && c1 + u < c2) ? xv[kTaskCh + u] : 0.f, tv[kTaskCh + u], a1);
+    }
+    for (int o = 32; o > 0; o >>= 1) a0 += __shfl_xor(a0, o, 64);
+    for (int o = 32; o > 0; o >>= 1) a1 += __shfl_xor(a1, o, 64);
+    p0 = uni(a0);
+    p1 = uni(a1);
+}
+
+// The full FP32 score of position r at column y: its chunks' partials in
+// order (two chunks' loads at a time).
+__device__ float rescore_full(int r, int y, const float *__restrict__ Xres, int ldx,
+                              const ksm::BoundView &bv, int C, int nt) {
+    const int lo = uni(bv.srng[2 * r]), hi = uni(bv.srng[2 * r + 1]);
+    float b = 0.f;
+    int c0 = lo;
+    for (; c0 + kTaskCh < hi; c0 += 2 * kTaskCh) {
+        float p0, p1;
+        rescore_chunk2(r, y, c0, c0 + kTaskCh, min(hi, c0 + 2 * kTaskCh), Xres, ldx, bv, C, nt, p0,
+                       p1);
+        b += p0;
+        b += p1;
+    }
+    if (c0 < hi) b += rescore_chunk(r, y, c0, hi, Xres, ldx, bv, C, nt);
+    return b;
+}
+
+constexpr int kMaxPairs = 256;           // (column, template) re-scores per round, shared
+constexpr int kMaxTasks = 1024;          // their channel chunks
+constexpr int kWavePl = 128;             // plausible positions a wave collects per column
+constexpr int kDirectPl = 2;             // at most this many: the scanning wave scores them
+
+__global__ __launch_bounds__(kCertThreads) void peel_select_cert_kernel(
+    float *__restrict__ cfmax, int *__restrict__ imax, ksm::BoundView bv,
+    const int *__restrict__ bhdr, const u64 *__restrict__ gkey, int G, float *__restrict__ Bt,
+    int ldbt, const GramRef *__restrict__ gref, const float *__restrict__ pnm,
+    const int *__restrict__ ppid, int T, int nt, float th2, const float *__restrict__ Xres,
+    int ldx, int C, int K, int *__restrict__ state, int *__restrict__ chunk_count,
+    Slot *__restrict__ slots) {
+    const int done = state[ST_DONE];
+    bv.mode = bhdr[0];
+    __shared__ float s_v[kCertMaxWin];       // Cf of the column's best (approximate or exact)
+    __shared__ float s_d[kCertMaxWin];       // its bound in sqrt(Cf) units; 0: exact
+    __shared__ int s_id[kCertMaxWin];        // its template
+    __shared__ float s_b[kCertMaxWin];       // FP32 score re-scored here (NaN: not re-scored)
+    __shared__ float s_lo0[kCertMaxWin];     // lower bounds
+    __shared__ float s_lo[2][kCertMaxWin];   // their window maxima by doubling
+    __shared__ float s_hn[kCertMaxWin];      // upper bounds of non-exact columns
+    __shared__ float s_ex[kCertMaxWin];      // exact values
+    __shared__ unsigned char s_req[kCertMaxWin];
+    __shared__ unsigned char s_dec[kChunk];  // 0 open, 1 rejected, 2 accepted
+    __shared__ int s_list[kCertMaxWin];      // requested columns (window index)
+    __shared__ int s_cbase[kCertMaxWin], s_ccnt[kCertMaxWin];   // their pairs (-1: done)
+    __shared__ int s_pe[kMaxPairs], s_pr[kMaxPairs], s_pt[kMaxPairs + 1];
+    __shared__ float s_part[kMaxTasks];
+    __shared__ float s_scol[kCertThreads / 64][ksm::kBoundMaxC / ksm::kBoundSub];
+    __shared__ int s_pl[kCertThreads / 64][kWavePl];
+    __shared__ int s_n, s_np, s_nu, s_wc[kChunk / 64];
+    __shared__ int s_ulist[kChunk];          // open columns a wave decides
+    const int t0 = blockIdx.x * kChunk;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = kCertThreads / 64;
+    const int WL = kChunk + 2 * nt;
+    for (int e = tid; e < WL; e += kCertThreads) {
+        const int xe = t0 - nt + e;
+        float v = -INFINITY, d = 0.f;
+        int id = 0;
+        if (xe >= nt && xe < T - nt) {
+            const float c = cfmax[xe], dc = bv.colD[xe];     // loads issued together
+            id = imax[xe];
+            if (cf_is_exact(c)) {
+                v = c != c ? c : -c;
+            } else {
+                v = c;
+                d = dc;
+            }
+        } else if (xe >= 0 && xe < T) {
+            v = 0.f;                          // the zeroed edge columns
+        }
+        s_v[e] = v;
+        s_d[e] = d;
+        s_id[e] = id;
+        s_b[e] = __builtin_nanf("");
+        s_req[e] = 0;
+    }
+    if (tid < kChunk) s_dec[tid] = 0;
+    __syncthreads();
+    if (done) return;                        // block-uniform
+    const float sth = sqrtf(th2) * (1.f - kSqrtSlack);
+    const int L = 2 * nt + 1;
+    const int kdbl = 31 - __clz(L);
+    int rescored = 0;
+    for (int round = 0; round <= 2 * WL; ++round) {
+        // interval ends (sqrt units) of every window column: its lower
+        // bound, the upper bound of a non-exact column (-inf: exact) and the
+        // exact value (-inf: not exact)
+        for (int e = tid; e < WL; e += kCertThreads) {
+            const float v = s_v[e], d = s_d[e];
+            const float sv = v == -INFINITY ? -INFINITY : sqrtf(v);
+            const bool ex = d == 0.f;
+            const float lo = ex ? sv * (1.f - kSqrtSlack) : fmaxf(0.f, sv * (1.f - kSqrtSlack) - d);
+            s_lo0[e] = lo;
+            s_lo[0][e] = lo;
+            s_hn[e] = ex ? -INFINITY : sv * (1.f + kSqrtSlack) + d;
+            s_ex[e] = ex ? v : -INFINITY;
+        }
+        if (tid == 0) {
+            s_n = 0;
+            s_np = 0;
+            s_nu = 0;
+        }
+        __syncthreads();
+        // first round: window maxima of the lower bounds of every column by
+        // doubling, which settles most columns
+        int cur = 0;
+        if (round == 0) {
+            for (int j = 0; j < kdbl; ++j) {
+                const int half = 1 << j;
+                for (int e = tid; e < WL; e += kCertThreads) {
+                    const bool in = e + half < WL;
+                    s_lo[cur ^ 1][e] = in ? nan_max(s_lo[cur][e], s_lo[cur][e + half])
+                                          : s_lo[cur][e];
                 }
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const int c = u < nch ? u : nch - 1;
-                    float tm = 0.f;
-#pragma unroll
-                    for (int p = 0; p < 8; ++p)
-                        if (p < P)
-                            tm = __builtin_fmaf(__int_as_float(__builtin_amdgcn_readlane(
-                                                    __float_as_int(ul[p]), c)),
-                                                wk[p], tm);
-                    part = __builtin_fmaf((kin && u < nch) ? xv[u] : 0.f, tm, part);
+                __syncthreads();
+                cur ^= 1;
+            }
+        }
+        const int x = t0 + tid;
+        if (tid < kChunk && s_dec[tid] == 0) {
+            const int e = tid + nt;
+            if (x < nt || x >= T - nt) {
+                s_dec[tid] = 1;
+            } else if (round == 0 && s_d[e] != 0.f) {
+                const int o = L - (1 << kdbl);
+                const float wlo = nan_max(s_lo[cur][tid], s_lo[cur][tid + o]);
+                const float sv = sqrtf(s_v[e]);
+                const float hi = sv * (1.f + kSqrtSlack) + s_d[e];
+                if (!(hi >= sth) || wlo > hi) s_dec[tid] = 1;   // certainly not accepted
+                else s_req[e] = 1;                              // open: re-score it
+            } else {
+                s_ulist[atomicAdd(&s_nu, 1)] = tid;             // decided by a wave below
+            }
+        }
+        __syncthreads();
+        // the other open columns, one wave each: the window's maxima by a
+        // lane-parallel read and a butterfly
+        const int nu = s_nu;
+        for (int jj = wave; jj < nu; jj += NW) {
+            const int tx = uni(s_ulist[jj]);
+            const int e = tx + nt;
+            const float v = s_v[e], d = s_d[e];
+            const float sv = sqrtf(v);
+            const float hi = sv * (1.f + kSqrtSlack) + d;
+            float wlo = -INFINITY, whn = -INFINITY, wex = -INFINITY;
+            for (int q = tx + lane; q < tx + L; q += 64) {
+                wlo = nan_max(wlo, s_lo0[q]);
+                whn = fmaxf(whn, s_hn[q]);
+                wex = nan_max(wex, s_ex[q]);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                wlo = nan_max(wlo, __shfl_xor(wlo, o, 64));
+                whn = fmaxf(whn, __shfl_xor(whn, o, 64));
+                wex = nan_max(wex, __shfl_xor(wex, o, 64));
+            }
+            if (!(hi >= sth) || wlo > hi) {
+                if (lane == 0) s_dec[tx] = 1;                   // certainly not accepted
+            } else if (d != 0.f) {
+                if (lane == 0) s_req[e] = 1;                    // open: re-score it
+            } else {
+                // exact: re-score the window's non-exact columns that may
+                // reach it; with none, the reference's test on the exact
+                // values (the others are certainly below)
+                const float lim = sv * (1.f - kSqrtSlack);
+                if (whn >= lim) {
+                    for (int q = tx + lane; q < tx + L; q += 64)
+                        if (s_hn[q] >= lim) s_req[q] = 1;
+                } else if (lane == 0) {
+                    const bool acc = (wex > th2) && (fabsf(wex - v) < 1e-9f);
+                    s_dec[tx] = acc ? 2 : 1;
                 }
             }
         }
-        for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o, 64);
-        if (lane == 0) s_part[q][bi] = part;
+        __syncthreads();
+        for (int e = tid; e < WL; e += kCertThreads)
+            if (s_req[e]) {
+                s_req[e] = 0;
+                s_list[atomicAdd(&s_n, 1)] = e;
+            }
+        __syncthreads();
+        const int nreq = s_n;
+        if (nreq == 0) break;                        // block-uniform: all decided
+        // scan: one wave per requested column finds the templates that can
+        // still be its best (groups of 32 positions by their best key first);
+        // their (column, template) pairs go to the shared list, or, where it
+        // is full, the wave re-scores them itself
+        for (int j = wave; j < nreq; j += NW) {
+            const int e = uni(s_list[j]);
+            const int y = t0 - nt + e;
+            // the column's loads issued together: sub-block statistics, its
+            // bounds, the group keys (the first 64 groups)
+            float *sc = s_scol[wave];
+            const u64 *gk = gkey + (size_t)y * G;
+            const float scv = lane < bv.nsb ? bv.S[(size_t)lane * T + y] : 0.f;
+            const u64 gk0 = lane < G ? gk[lane] : 0ull;
+            const float ay = bv.colA[y], dcol = bv.colD[y];
+            if (lane < bv.nsb) sc[lane] = scv;
+            for (int b = 64 + lane; b < bv.nsb; b += 64) sc[b] = bv.S[(size_t)b * T + y];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // the true best is at least the column's approximate best less colD
+            const float lo_col = fmaxf(0.f, sqrtf(s_v[e]) * (1.f - kSqrtSlack) - dcol);
+            // a group's key is within colD of the current scores of its
+            // positions (its values are the approximate ones of the last
+            // reduction, and re-scored values lie within the bound of them)
+            int *pl = s_pl[wave];
+            int npl = 0;
+            bool over = false;
+            for (int g0 = 0; g0 < G; g0 += 64) {
+                const int g = g0 + lane;
+                bool gp = false;
+                if (g < G) {
+                    const float kf = key_score(g0 == 0 ? gk0 : gk[g]);
+                    gp = kf != kf || sqrtf(kf) * (1.f + kSqrtSlack) + 2.f * dcol >= lo_col;
+                }
+                unsigned long long gm = __ballot(gp);
+                while (gm) {
+                    const int gg = g0 + __ffsll((long long)gm) - 1;
+                    gm &= gm - 1;
+                    // lanes 0-31 take the group's positions; the template
+                    // weights are loaded with the scores
+                    const int r = gg * kGroup + (lane & 31);
+                    bool p = false;
+                    if (lane < 32 && r < K) {
+                        const float bval = Bt[(size_t)y * ldbt + r], nmr = pnm[r];
+                        const float dr = bound_lds(bv, sc, K, r);
+                        const float cf = score(bval, nmr);
+                        const float up = sqrtf(cf) * (1.f + kSqrtSlack);
+                        p = cf != cf || up + dr + ay >= lo_col;
+                    }
+                    const unsigned long long m = __ballot(p);
+                    const int cnt = __popcll(m);
+                    if (npl + cnt > kWavePl) {
+                        over = true;
+                    } else if (p) {
+                        pl[npl + __popcll(m & lanemask_lt(lane))] = r;
+                    }
+                    npl = over ? npl : npl + cnt;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            int base = -1;
+            if (!over && npl <= kDirectPl) {
+                // few templates: the scanning wave scores them itself
+                const bool own = e >= nt && e < nt + kChunk;
+                u64 best = 0ull;
+                float bbest = 0.f;
+                for (int q = 0; q < npl; ++q) {
+                    const int rr = uni(pl[q]);
+                    const float b = rescore_full(rr, y, Xres, ldx, bv, C, nt);
+                    const u64 key = make_key(score(b, pnm[rr]), uni(ppid[rr]));
+                    if (key > best) {
+                        best = key;
+                        bbest = b;
+                    }
+                    if (own && lane == 0) Bt[(size_t)y * ldbt + rr] = b;
+                }
+                rescored += npl;
+                if (best == 0ull) {
+                    best = make_key(s_v[e], s_id[e]);
+                    bbest = __builtin_nanf("");
+                }
+                if (lane == 0) {
+                    const float cf = key_score(best);
+                    s_v[e] = cf;
+                    s_id[e] = key_id(best);
+                    s_b[e] = bbest;
+                    s_d[e] = 0.f;
+                    s_ccnt[j] = -1;
+                    if (own) {
+                        cfmax[y] = cf != cf ? cf : -cf;  // sign bit: exact
+                        imax[y] = key_id(best);
+                    }
+                }
+                continue;
+            }
+            if (!over && lane == 0) {
+                // reserve npl entries only where they fit, so that the list
+                // [0, s_np) holds no unwritten entries
+                int old = s_np;
+                for (;;) {
+                    if (old + npl > kMaxPairs) break;
+                    const int prev = atomicCAS(&s_np, old, old + npl);
+                    if (prev == old) {
+                        base = old;
+                        break;
+                    }
+                    old = prev;
+                }
+            }
+            base = uni(__shfl(base, 0, 64));
+            const bool own = e >= nt && e < nt + kChunk;
+            if (base >= 0) {
+                for (int q = lane; q < npl; q += 64) {
+                    s_pe[base + q] = e;
+                    s_pr[base + q] = pl[q];
+                }
+                if (lane == 0) {
+                    s_cbase[j] = base;
+                    s_ccnt[j] = npl;
+                }
+                continue;
+            }
+            // the wave re-scores the column itself: every position of every
+            // group that can hold its best (the full scan when the collected
+            // list overflowed)
+            u64 best = 0ull;
+            float bbest = 0.f;
+            for (int g = 0; g < G; ++g) {
+                const float kf = key_score(gk[g]);
+                if (!(kf != kf || sqrtf(kf) * (1.f + kSqrtSlack) + 2.f * dcol >= lo_col)) continue;
+                const int r = g * kGroup + (lane & 31);
+                bool p = false;
+                if (lane < 32 && r < K) {
+                    const float cf = score(Bt[(size_t)y * ldbt + r], pnm[r]);
+                    const float up = sqrtf(cf) * (1.f + kSqrtSlack);
+                    p = cf != cf || (up + dcol >= lo_col &&
+                                     up + bound_lds(bv, sc, K, r) + ay >= lo_col);
+                }
+                unsigned long long m = __ballot(p);
+                while (m) {
+                    const int rr = g * kGroup + __ffsll((long long)m) - 1;
+                    m &= m - 1;
+                    const float b = rescore_full(rr, y, Xres, ldx, bv, C, nt);
+                    const u64 key = make_key(score(b, pnm[rr]), uni(ppid[rr]));
+                    if (key > best) {
+                        best = key;
+                        bbest = b;
+                    }
+                    ++rescored;
+                    if (own && lane == 0) Bt[(size_t)y * ldbt + rr] = b;
+                }
+            }
+            if (best == 0ull) {                // no template scored (cannot occur for
+                best = make_key(s_v[e], s_id[e]);  // finite bounds): keep the column's own
+                bbest = __builtin_nanf("");
+            }
+            if (lane == 0) {
+                const float cf = key_score(best);
+                s_v[e] = cf;
+                s_id[e] = key_id(best);
+                s_b[e] = bbest;
+                s_d[e] = 0.f;
+                s_ccnt[j] = -1;
+                if (own) {
+                    cfmax[y] = cf != cf ? cf : -cf;  // sign bit: exact
+                    imax[y] = key_id(best);
+                }
+            }
+        }
+        __syncthreads();
+        // channel chunks of the listed pairs, then one chunk per wave
+        const int np = min(s_np, kMaxPairs);
+        if (wave == 0) {
+            int run = 0;
+            for (int q0 = 0; q0 < np; q0 += 64) {
+                const int q = q0 + lane;
+                int nch = 0;
+                if (q < np) {
+                    const int r = s_pr[q];
+                    nch = (bv.srng[2 * r + 1] - bv.srng[2 * r] + kTaskCh - 1) / kTaskCh;
+                }
+                int incl = nch;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int t = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += t;
+                }
+                if (q < np) s_pt[q] = run + incl - nch;
+                run += __shfl(incl, 63, 64);
+            }
+            if (lane == 0) s_pt[np] = run;
+        }
+        __syncthreads();
+        const int ntask = min(s_pt[np], kMaxTasks);
+        for (int j = wave; j < ntask; j += NW) {
+            int lo_q = 0, hi_q = np - 1;       // the pair q with s_pt[q] <= j < s_pt[q+1]
+            while (lo_q < hi_q) {
+                const int mid = (lo_q + hi_q + 1) >> 1;
+                if (s_pt[mid] <= j) lo_q = mid;
+                else hi_q = mid - 1;
+            }
+            const int q = uni(lo_q);
+            const int r = uni(s_pr[q]), y = t0 - nt + uni(s_pe[q]);
+            const int c0 = uni(bv.srng[2 * r]) + (j - s_pt[q]) * kTaskCh;
+            const int c1 = min(uni(bv.srng[2 * r + 1]), c0 + kTaskCh);
+            const float part = rescore_chunk(r, y, c0, c1, Xres, ldx, bv, C, nt);
+            if (lane == 0) s_part[j] = part;
+        }
+        __syncthreads();
+        // per listed column: its pairs' scores (chunks in order) and the best
+        // of them; pairs whose chunks did not fit are scored here in full
+        for (int j = wave; j < nreq; j += NW) {
+            const int cnt = s_ccnt[j];
+            if (cnt < 0) continue;             // done by its scanning wave
+            const int e = uni(s_list[j]);
+            const int y = t0 - nt + e;
+            const int base = uni(s_cbase[j]);
+            const bool own = e >= nt && e < nt + kChunk;
+            u64 best = 0ull;
+            float bbest = 0.f;
+            for (int q = base; q < base + cnt; ++q) {
+                const int r = uni(s_pr[q]);
+                float b = 0.f;
+                if (s_pt[q + 1] <= kMaxTasks) {
+                    for (int t = s_pt[q]; t < s_pt[q + 1]; ++t) b += s_part[t];
+                } else {
+                    b = rescore_full(r, y, Xres, ldx, bv, C, nt);
+                }
+                b = uni(b);
+                const u64 key = make_key(score(b, pnm[r]), uni(ppid[r]));
+                if (key > best) {
+                    best = key;
+                    bbest = b;
+                }
+                if (own && lane == 0) Bt[(size_t)y * ldbt + r] = b;
+            }
+            rescored += cnt;
+            if (best == 0ull) {                // no template scored (cannot occur for
+                best = make_key(s_v[e], s_id[e]);  // finite bounds): keep the column's own
+                bbest = __builtin_nanf("");
+            }
+            if (lane == 0) {
+                const float cf = key_score(best);
+                s_v[e] = cf;
+                s_id[e] = key_id(best);
+                s_b[e] = bbest;
+                s_d[e] = 0.f;
+                if (own) {
+                    cfmax[y] = cf != cf ? cf : -cf;  // sign bit: exact
+                    imax[y] = key_id(best);
+                }
+            }
+        }
+        __syncthreads();
     }
+    if (lane == 0 && rescored) atomicAdd(&state[ST_RESCORED], rescored);
+    // ordered compaction of the accepted columns (time order)
+    const bool acc = tid < kChunk && s_dec[tid] == 2;
+    const unsigned long long m = __ballot(acc);
+    if (lane == 0 && wave < kChunk / 64) s_wc[wave] = __popcll(m);
     __syncthreads();
-    if (tid < n) {
-        const int sx = s_sx[tid], id = s_sid[tid];
-        float b = 0.f;
-        for (int bi = 0; bi < s_tb[tid + 1] - s_tb[tid]; ++bi) b += s_part[tid][bi];
-        const GramRef g = gref[id];
-        Slot *sl = slots + (size_t)blockIdx.x * kSlotCap + tid;
-        sl->amp = b / g.nm;
-        sl->tha = sqrtf(score(b, g.nm));
-        Bt[(size_t)sx * ldbt + g.rank] = b;
+    int off = 0, count = 0;
+#pragma unroll
+    for (int w = 0; w < kChunk / 64; ++w) {
+        if (w < wave) off += s_wc[w];
+        count += s_wc[w];
+    }
+    if (acc) {
+        const int pos = off + __popcll(m & lanemask_lt(lane));
+        if (pos < kSlotCap) {
+            const int x = t0 + tid, e = tid + nt;
+            const int iy = s_id[e];
+            const GramRef g = gref[iy];
+            float b = s_b[e];
+            if (b != b) b = Bt[(size_t)x * ldbt + g.rank];   // exact since an earlier launch
+            Slot sl;
+            sl.x = x;
+            sl.iy = iy;
+            sl.amp = b / g.nm;
+            sl.tha = sqrtf(s_v[e]);
+            sl.lohi = (unsigned)g.lo | ((unsigned)g.hi << 16);
+            sl.crg = (unsigned)g.clo | ((unsigned)g.chi << 16);
+            sl.off = g.off;
+            slots[(size_t)blockIdx.x * kSlotCap + pos] = sl;
+        }
+    }
+    if (tid == 0) {
+        if (count > kSlotCap) {
+            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
+            atomicExch(&state[ST_DONE], 1);
+        }
+        chunk_count[blockIdx.x] = count < kSlotCap ? count : kSlotCap;
     }
 }
 
@@ -1404,7 +1914,7 @@ __global__ void peel_finish_kernel(const int *__restrict__ state, int32_t *__res
     info[0] = state[ST_KBASE + iters];
     info[1] = state[ST_STATUS];
     info[2] = iters;
-    info[3] = 0;
+    info[3] = state[ST_RESCORED];
 }
 
 // tF[s][n][p] = sum_k Xres[iCC[n, iU[id]], x + k - nt/2] W[p,k] + amps[s] Ucc[n,id,p]
@@ -1590,8 +2100,10 @@ static bool peel_args_ok(const float *Bt, int ldbt, const float *Xres, int ldx, 
                          const float *W, const float *nm, const int32_t *perm,
                          const int32_t *rank, const float *cc, const int32_t *lohi,
                          const int64_t *off, int K, int P, int C, int T, int nt, int max_peels,
-                         int flags, const int64_t *st, const float *amps, const float *th_amps,
-                         int capacity, const int32_t *info) {
+                         int flags, const void *bound, const int64_t *st, const float *amps,
+                         const float *th_amps, int capacity, const int32_t *info) {
+    // re-scoring (reduced-precision scores) needs their bound, and only then is one given
+    if (((flags & KSM_PEEL_RESCORE) != 0) != (bound != nullptr)) return false;
     return Bt && Xres && U && W && nm && perm && rank && cc && lohi && off && st && amps &&
            th_amps && info && K > 0 && K <= kMaxK && ldbt >= K && (ldbt % 4) == 0 && P > 0 &&
            P <= 8 && C > 0 && C <= kMaxC && nt > 0 && nt <= kMaxNt && (nt % 2) == 1 &&
@@ -1606,8 +2118,8 @@ static bool peel_args_ok(const float *Bt, int ldbt, const float *Xres, int ldx, 
 static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
                     const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
                     const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
-                    float Th, int max_peels, int flags, int64_t *st, float *amps,
-                    float *th_amps, int capacity, int32_t *info, void *workspace,
+                    float Th, int max_peels, int flags, const void *bound, int64_t *st,
+                    float *amps, float *th_amps, int capacity, int32_t *info, void *workspace,
                     hipStream_t s, bool may_poll) {
     // the kernels are timed only for the entry point that may allocate (events)
     const unsigned prof_group = may_poll ? ksm::kProfPeel : 0u;
@@ -1626,6 +2138,13 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
 
     // Th^2 as the reference forms it (Python float, compared in float32)
     const float th2 = (float)((double)Th * (double)Th);
+    // the reduced-precision scores' bound (ksm_project_bounded)
+    ksm::BoundView bview = {};
+    const int *bhdr = nullptr;
+    if (bound) {
+        bview = ksm::bound_view(bound, C, T, K, nt, 0);
+        bhdr = static_cast<const int *>(bound);
+    }
     const size_t tab_dyn = (size_t)L.Kg * (sizeof(float) + sizeof(int));
     const size_t upd_dyn = tab_dyn + (size_t)(L.nchunks + 1) * sizeof(int);
     const int grid = update_grid(upd_dyn);
@@ -1661,14 +2180,16 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
         {
             ksm::ProfileScope prof("peel_select", prof_group, s);
             if (flags & KSM_PEEL_RESCORE)
-                prof.launch(peel_select_kernel<kRsThreads, true>, dim3(L.nchunks),
-                            dim3(kRsThreads),
-                            (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, 1, Xres,
-                            ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
+                prof.launch(peel_select_cert_kernel, dim3(L.nchunks), dim3(kCertThreads), 0u,
+                            cfmax, imax, bview, bhdr, static_cast<const u64 *>(gkey), L.G, Bt,
+                            ldbt, static_cast<const GramRef *>(gref),
+                            static_cast<const float *>(pnm),
+                            static_cast<const int *>(ppid), T, nt, th2, Xres, ldx, C, K, state,
+                            chunk_count, slots);
             else
-                prof.launch(peel_select_kernel<kChunk, false>, dim3(L.nchunks), dim3(kChunk),
-                            (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, 0, Xres,
-                            ldx, U, W, crange, P, C, rank, K, state, chunk_count, slots);
+                prof.launch(peel_select_kernel<kChunk>, dim3(L.nchunks), dim3(kChunk),
+                            (unsigned)sel_dyn, cfmax, imax, Bt, ldbt, gref, T, nt, th2, rank, K,
+                            state, chunk_count, slots);
         }
         {
             ksm::ProfileScope prof("peel_update", prof_group, s);
@@ -1687,15 +2208,16 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
 int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
              const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
              const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
-             float Th, int max_peels, int flags, int64_t *st, float *amps, float *th_amps,
-             int capacity, int32_t *info, void *workspace, size_t ws_bytes, void *stream) {
+             float Th, int max_peels, int flags, const void *bound, int64_t *st,
+             float *amps, float *th_amps, int capacity, int32_t *info, void *workspace,
+             size_t ws_bytes, void *stream) {
     if (!peel_args_ok(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt,
-                      max_peels, flags, st, amps, th_amps, capacity, info))
+                      max_peels, flags, bound, st, amps, th_amps, capacity, info))
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
     if (!workspace || ws_bytes < ksm_peel_workspace(K, T, max_peels))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
     return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
-                    max_peels, flags, st, amps, th_amps, capacity, info, workspace,
+                    max_peels, flags, bound, st, amps, th_amps, capacity, info, workspace,
                     ksm::as_stream(stream), true);
 }
 
@@ -1704,10 +2226,10 @@ size_t ksm_peel_static_capacity(void) { return kPeelScratchBytes; }
 int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
                     const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
                     const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
-                    float Th, int max_peels, int flags, int64_t *st, float *amps,
-                    float *th_amps, int capacity, int32_t *info, void *stream) {
+                    float Th, int max_peels, int flags, const void *bound, int64_t *st,
+                    float *amps, float *th_amps, int capacity, int32_t *info, void *stream) {
     if (!peel_args_ok(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt,
-                      max_peels, flags, st, amps, th_amps, capacity, info))
+                      max_peels, flags, bound, st, amps, th_amps, capacity, info))
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel_static: bad arguments");
     const size_t need = ksm_peel_workspace(K, T, max_peels);
     if (need > kPeelScratchBytes)
@@ -1719,7 +2241,7 @@ int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
     const int rc = peel_scratch_address(&ws);
     if (rc != KSM_OK) return rc;
     return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
-                    max_peels, flags, st, amps, th_amps, capacity, info, ws,
+                    max_peels, flags, bound, st, amps, th_amps, capacity, info, ws,
                     ksm::as_stream(stream), false);
 }
 

@@ -326,6 +326,7 @@ ProjectLayout project_layout(int C, int T, int K, int P) {
     return L;
 }
 
+
 }  // namespace
 
 namespace ksm {
@@ -366,12 +367,25 @@ size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) 
 int ksm_project(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
                 int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
                 void *workspace, size_t ws_bytes, void *stream) {
+    return ksm_project_bounded(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision, nullptr,
+                               0, workspace, ws_bytes, stream);
+}
+
+int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
+                        const int32_t *perm, int C, int T, int K, int P, int nt, float *Bt,
+                        int ldbt, int precision, void *bound, size_t bound_bytes,
+                        void *workspace, size_t ws_bytes, void *stream) {
     if (!X || !W || !U || !Bt || C <= 0 || T <= 0 || K <= 0 || P <= 0 || nt <= 0 ||
         (nt % 2) == 0 || ldx < T)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: bad arguments");
     if (precision != KSM_PREC_FP32 && precision != KSM_PREC_BF16 &&
         precision != KSM_PREC_BF16X3 && precision != KSM_PREC_FP8)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: unknown precision %d", precision);
+    if (bound && precision == KSM_PREC_FP32)
+        return ksm::fail(KSM_ERR_INVALID,
+                         "ksm_project_bounded: the FP32 form has no reduced-precision bound");
+    if (bound && bound_bytes < ksm_bound_bytes(C, T, K, P, nt))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_project_bounded: bound buffer too small");
     const ProjectLayout L = project_layout(C, T, K, P);
     if (ldbt < K || (ldbt % 4) != 0)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: ldbt %d must be >= K and a multiple of 4",
@@ -380,9 +394,10 @@ int ksm_project(const float *X, int ldx, const float *W, const float *U, const i
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_project: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
     if (precision == KSM_PREC_FP8)
-        return ksm::project_fp8(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, workspace, s);
+        return ksm::project_fp8(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, bound, workspace,
+                                s);
     if (precision != KSM_PREC_FP32)
-        return ksm::project_bf16(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision,
+        return ksm::project_bf16(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision, bound,
                                  workspace, s);
     ksm::Carver cv(workspace);
     float *Bc = cv.take<float>((size_t)L.Rp * L.ldc);

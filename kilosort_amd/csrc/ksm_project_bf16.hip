@@ -14,9 +14,12 @@
 //                     dropped Ul.Bl term is ~2^-16 relative)
 // The three products of the split form are one GEMM over a contraction three
 // times as deep: [Uh | Uh | Ul] . [Bh | Bl | Bh].
+#include <cmath>
+
 #include "ksm_common.h"
 #include "ksm_profile.h"
 #include "ksm_project_internal.h"
+#include "ksm_bound.h"
 
 namespace {
 
@@ -38,10 +41,16 @@ constexpr int kCvTT = 8;      // time samples per thread
 constexpr int kCvT = 4 * kCvTT;  // 32 time samples per block (4 waves)
 
 // Bh[t][c*P + p] (and Bl) for t < Tp, c < Cp (zero for t >= T or c >= C).
+//
+// stats (may be NULL): per 32-channel sub-block b (two per block) and column
+// t, over the sub-block's (channel, PC) rows, the FP32 correlation's sum of
+// squares at stats[(0 * nsb2 + b) * Tp + t] and its largest square at
+// stats[(1 * nsb2 + b) * Tp + t] (inputs of the error bound, ksm_bound.h).
 template <int P, int NTAP, bool SPLIT>
 __global__ __launch_bounds__(256) void conv_pc_bf16_kernel(
     const float *__restrict__ X, int ldx, const float *__restrict__ W, int C, int T, int Tp,
-    int ldr, bf16_t *__restrict__ Bh, bf16_t *__restrict__ Bl, int *__restrict__ nonfinite) {
+    int ldr, bf16_t *__restrict__ Bh, bf16_t *__restrict__ Bl, int *__restrict__ nonfinite,
+    float *__restrict__ stats) {
     constexpr int H = NTAP / 2;
     constexpr int SPAN = kCvT + NTAP - 1;
     constexpr int LD = SPAN | 1;                 // odd row stride: no bank conflicts
@@ -80,6 +89,30 @@ __global__ __launch_bounds__(256) void conv_pc_bf16_kernel(
             for (int p = 0; p < P; ++p) acc[u][p] = __builtin_fmaf(win[u + k], w[p], acc[u][p]);
     }
     const int c = c0 + cl;
+    if (stats) {
+        // per 32-channel half of the block (lanes 0-31, 32-63): sum and largest
+        // square of the correlation values of each column
+        const int nsb2 = 2 * gridDim.y, sb = 2 * blockIdx.y + (cl >> 5);
+#pragma unroll
+        for (int u = 0; u < kCvTT; ++u) {
+            const int t = t0 + tg * kCvTT + u;
+            const bool live = t < T && c < C;
+            float sq = 0.f, mx = 0.f;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float a = live ? acc[u][p] : 0.f;
+                sq = __builtin_fmaf(a, a, sq);
+                mx = fmaxf(mx, a * a);
+                if (a != a) mx = a;             // NaN reaches the bound
+            }
+            sq = ksm::half_sum_f(sq);
+            mx = ksm::half_max_nan(mx);
+            if ((cl & 31) == 0 && t < Tp) {
+                stats[((size_t)0 * nsb2 + sb) * Tp + t] = sq;
+                stats[((size_t)1 * nsb2 + sb) * Tp + t] = mx;
+            }
+        }
+    }
     if (c * P >= ldr) return;
     static_assert(P % 2 == 0, "pairs of bf16 are stored as 32-bit words");
     // the P values of (t, c) are 2P consecutive bytes at a 4-byte aligned
@@ -340,13 +373,15 @@ size_t project_bf16_workspace(int C, int T, int K, int P, int precision) {
     const HLayout L = h_layout(C, T, K, P);
     const int terms = precision == KSM_PREC_BF16X3 ? 2 : 1;
     const size_t nTm = L.Kp / HBM_, nk = L.ldr / HBK;
+    const size_t nblk = (L.ldr / P + kCvCh - 1) / kCvCh;
     return terms * (align_up(L.bc_elems * 2, 256) + align_up(L.u_elems * 2, 256)) +
-           align_up(nTm * nk * sizeof(int), 256) + align_up(nTm * sizeof(int), 256) + 256 + 512;
+           align_up(nTm * nk * sizeof(int), 256) + align_up(nTm * sizeof(int), 256) +
+           align_up(2 * 2 * nblk * L.Tp * sizeof(float), 256) + 256 + 512;
 }
 
 int project_bf16(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
                  int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
-                 void *workspace, hipStream_t s) {
+                 void *bound, void *workspace, hipStream_t s) {
     if (P != 6 || nt != 61)
         return fail(KSM_ERR_INVALID, "ksm_project: BF16 forms are built for P=6, nt=61");
     const HLayout L = h_layout(C, T, K, P);
@@ -360,19 +395,36 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     int *steps = cv.take<int>((size_t)nTm * nk);
     int *counts = cv.take<int>(nTm);
     int *nonfinite = cv.take<int>(1);
+    const int nblk = (L.ldr / P + kCvCh - 1) / kCvCh;
+    float *stats = cv.take<float>((size_t)2 * 2 * nblk * L.Tp);
     set_flag_h_kernel<<<1, 1, 0, s>>>(nonfinite);
     KSM_LAUNCH_CHECK("set_flag_h_kernel");
     {
         ProfileScope prof("conv_pc", kProfProject, s);
-        dim3 grid(L.Tp / kCvT, (L.ldr / P + kCvCh - 1) / kCvCh);
+        dim3 grid(L.Tp / kCvT, nblk);
+        float *st_out = bound ? stats : nullptr;
         if (split)
             prof.launch(conv_pc_bf16_kernel<6, 61, true>, grid, dim3(256), 0u, X, ldx, W, C, T,
-                        L.Tp, L.ldr, Bh, Bl, nonfinite);
+                        L.Tp, L.ldr, Bh, Bl, nonfinite, st_out);
         else
             prof.launch(conv_pc_bf16_kernel<6, 61, false>, grid, dim3(256), 0u, X, ldx, W, C, T,
-                        L.Tp, L.ldr, Bh, Bl, nonfinite);
+                        L.Tp, L.ldr, Bh, Bl, nonfinite, st_out);
     }
     KSM_LAUNCH_CHECK("conv_pc_bf16_kernel");
+    if (bound) {
+        // |B~ - B| <= kappa sum_r |U_r| |Bc_r| with kappa = 2u + u^2 + gamma_n for
+        // one BF16 product (u = 2^-8 per rounded operand), or the dropped
+        // lo * lo term and the split remainders (3 u^2 (1 + u)^3) for the split
+        // form, plus FP32 accumulation over the n products (gamma_n)
+        const double e = std::ldexp(1.0, -8);
+        const int depth = (split ? 3 : 1) * L.ldr;
+        const double g = depth * std::ldexp(1.0, -24) / (1.0 - depth * std::ldexp(1.0, -24));
+        const double kappa = split ? 3.0 * e * e * (1 + e) * (1 + e) * (1 + e) + g * (1 + e) * (1 + e)
+                                   : (2 * e + e * e) + g * (1 + e) * (1 + e);
+        const int rc = bound_finish(stats, 2 * nblk, L.Tp, U, W, perm, C, T, K, P, nt,
+                                    kBoundLinear, kappa, 0.0, 0.0, nullptr, bound, s);
+        if (rc != KSM_OK) return rc;
+    }
     u_bf16_kernel<<<512, 256, 0, s>>>(U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
     KSM_LAUNCH_CHECK("u_bf16_kernel");
     tile_steps_bf16_kernel<<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite, steps,

@@ -13,9 +13,12 @@
 // Both operands are contiguous along r, as in the BF16 forms.  A template
 // tile's reduction steps where Uq is all zero are skipped (exact zeros only;
 // every step is kept when an input is not finite).
+#include <cmath>
+
 #include "ksm_common.h"
 #include "ksm_profile.h"
 #include "ksm_project_internal.h"
+#include "ksm_bound.h"
 
 namespace {
 
@@ -23,6 +26,8 @@ typedef unsigned char fp8_t;
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr float kE4M3Max = 448.f;
+// standard deviations of the modelled rounding error in the FP8 decision margin
+constexpr double kFp8Sigmas = 6.0;
 
 // Finite values beyond e4m3's range are clamped to +-448 (the conversion
 // does not saturate); NaN and +-inf pass and convert to NaN, so a non-finite
@@ -82,10 +87,16 @@ constexpr int kCvT = 4 * kCvTT;
 
 // Bq[t][c P + p] = e4m3(Bc[c, p, t] / s[c P + p]) for t < Tp (zero for t >= T
 // or c >= C).
+//
+// stats (may be NULL): per 32-channel sub-block b (two per block) and
+// column t, over the sub-block's (channel, PC) rows: the sum of squares of
+// the FP32 correlation (j = 0), its largest square (j = 1) and the sum of
+// squares of the scaled values Bc / s (j = 2), at
+// stats[(j * nsb2 + b) * Tp + t] (inputs of the error bound, ksm_bound.h).
 template <int P, int NTAP>
 __global__ __launch_bounds__(256) void conv_pc_fp8_kernel(
     const float *__restrict__ X, int ldx, const float *__restrict__ W, int C, int T, int Tp,
-    int ldr, const float *__restrict__ srow, fp8_t *__restrict__ Bq) {
+    int ldr, const float *__restrict__ srow, fp8_t *__restrict__ Bq, float *__restrict__ stats) {
     constexpr int H = NTAP / 2;
     constexpr int SPAN = kCvT + NTAP - 1;
     constexpr int LD = SPAN | 1;
@@ -119,11 +130,38 @@ __global__ __launch_bounds__(256) void conv_pc_fp8_kernel(
             for (int p = 0; p < P; ++p) acc[u][p] = __builtin_fmaf(win[u + k], w[p], acc[u][p]);
     }
     const int c = c0 + cl;
-    if (c * P >= ldr) return;
-    static_assert(P % 2 == 0, "pairs of e4m3 bytes are stored as 16-bit words");
     float inv[P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) inv[p] = 1.f / srow[c * P + p];
+    for (int p = 0; p < P; ++p) inv[p] = c * P < ldr ? 1.f / srow[c * P + p] : 1.f;
+    if (stats) {
+        // per 32-channel half of the block (lanes 0-31, 32-63)
+        const int nsb2 = 2 * gridDim.y, sb = 2 * blockIdx.y + (cl >> 5);
+#pragma unroll
+        for (int u = 0; u < kCvTT; ++u) {
+            const int t = t0 + tg * kCvTT + u;
+            const bool live = t < T && c < C;
+            float sq = 0.f, mx = 0.f, sq2 = 0.f;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float a = live ? acc[u][p] : 0.f;
+                const float q = a * inv[p];
+                sq = __builtin_fmaf(a, a, sq);
+                sq2 = __builtin_fmaf(q, q, sq2);
+                mx = fmaxf(mx, a * a);
+                if (a != a) mx = a;             // NaN reaches the bound
+            }
+            sq = ksm::half_sum_f(sq);
+            sq2 = ksm::half_sum_f(sq2);
+            mx = ksm::half_max_nan(mx);
+            if ((cl & 31) == 0 && t < Tp) {
+                stats[((size_t)0 * nsb2 + sb) * Tp + t] = sq;
+                stats[((size_t)1 * nsb2 + sb) * Tp + t] = mx;
+                stats[((size_t)2 * nsb2 + sb) * Tp + t] = sq2;
+            }
+        }
+    }
+    if (c * P >= ldr) return;
+    static_assert(P % 2 == 0, "pairs of e4m3 bytes are stored as 16-bit words");
 #pragma unroll
     for (int u = 0; u < kCvTT; ++u) {
         const int t = t0 + tg * kCvTT + u;
@@ -345,6 +383,44 @@ QLayout q_layout(int C, int T, int K, int P) {
     return L;
 }
 
+// Absolute (subnormal) error terms of the FP8 product per unit template
+// norm: each scaled operand is e4m3(x) = x (1 + d) + h with |h| <= 2^-10, so
+// beyond the relative part a term of r = (c, p) errs by at most
+// 2^-10 (1 + 2^-4) (|U_r| s_r + u |Bc_r| / s_r) + 2^-20 u; with
+// u <= ||U|| max_r s_r / 448 and Cauchy-Schwarz over the n = ldr terms:
+//   a0 = 2^-10 (1 + 2^-4) ||s|| + 2^-20 n smax / 448,
+//   a1 = 2^-10 (1 + 2^-4) sqrt(n) smax / 448   (times ||Bc / s|| per column).
+__global__ __launch_bounds__(256) void fp8_abs_terms_kernel(const float *__restrict__ srow,
+                                                            int n, float *__restrict__ out) {
+    __shared__ double s_sq[4], s_mx[4];
+    double sq = 0.0, mx = 0.0;
+    for (int r = threadIdx.x; r < n; r += 256) {
+        const double v = srow[r];
+        sq += v * v;
+        mx = v > mx ? v : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        sq += __shfl_xor(sq, o, 64);
+        const double w = __shfl_xor(mx, o, 64);
+        mx = w > mx ? w : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_sq[threadIdx.x >> 6] = sq;
+        s_mx[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = 0.0, m = 0.0;
+        for (int w = 0; w < 4; ++w) {
+            q += s_sq[w];
+            m = s_mx[w] > m ? s_mx[w] : m;
+        }
+        const double h = 0x1p-10 * (1.0 + 0x1p-4) * (1.0 + 0x1p-16);
+        out[0] = (float)((h * sqrt(q) + 0x1p-20 * n * m / 448.0) * (1.0 + 0x1p-16));
+        out[1] = (float)(h * sqrt((double)n) * m / 448.0 * (1.0 + 0x1p-16));
+    }
+}
+
 }  // namespace
 
 namespace ksm {
@@ -354,12 +430,14 @@ size_t project_fp8_workspace(int C, int T, int K, int P) {
     return align_up((size_t)L.Tp * L.ldr, 256) + align_up((size_t)L.Kp * L.ldr, 256) +
            align_up((size_t)C * 4, 256) + align_up((size_t)L.ldr * 4, 256) +
            align_up((size_t)L.Kp * 4, 256) + align_up((size_t)L.nTm * L.nk * 4, 256) +
-           align_up((size_t)L.nTm * 4, 256) + 256 + 512;
+           align_up((size_t)L.nTm * 4, 256) +
+           align_up(3 * 2 * (size_t)((L.ldr / P + kCvCh - 1) / kCvCh) * L.Tp * 4, 256) + 256 + 256 +
+           512;
 }
 
 int project_fp8(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
-                int C, int T, int K, int P, int nt, float *Bt, int ldbt, void *workspace,
-                hipStream_t s) {
+                int C, int T, int K, int P, int nt, float *Bt, int ldbt, void *bound,
+                void *workspace, hipStream_t s) {
     if (P != 6 || nt != 61)
         return fail(KSM_ERR_INVALID, "ksm_project: the FP8 form is built for P=6, nt=61");
     const QLayout L = q_layout(C, T, K, P);
@@ -372,6 +450,9 @@ int project_fp8(const float *X, int ldx, const float *W, const float *U, const i
     int *steps = cv.take<int>((size_t)L.nTm * L.nk);
     int *counts = cv.take<int>(L.nTm);
     int *nonfinite = cv.take<int>(1);
+    const int nblk = (L.ldr / P + kCvCh - 1) / kCvCh;
+    float *stats = cv.take<float>((size_t)3 * 2 * nblk * L.Tp);
+    float *abs_terms = cv.take<float>(2);
     set_flag_q_kernel<<<1, 1, 0, s>>>(nonfinite);
     KSM_LAUNCH_CHECK("set_flag_q_kernel");
     chan_absmax_kernel<<<C, 256, 0, s>>>(X, ldx, T, amax, nonfinite);
@@ -380,11 +461,30 @@ int project_fp8(const float *X, int ldx, const float *W, const float *U, const i
     KSM_LAUNCH_CHECK("row_scales_kernel");
     {
         ProfileScope prof("conv_pc", kProfProject, s);
-        dim3 grid(L.Tp / kCvT, (L.ldr / P + kCvCh - 1) / kCvCh);
+        dim3 grid(L.Tp / kCvT, nblk);
         prof.launch(conv_pc_fp8_kernel<6, 61>, grid, dim3(256), 0u, X, ldx, W, C, T, L.Tp, L.ldr,
-                    static_cast<const float *>(srow), Bq);
+                    static_cast<const float *>(srow), Bq, bound ? stats : nullptr);
     }
     KSM_LAUNCH_CHECK("conv_pc_fp8_kernel");
+    if (bound) {
+        // Not a worst-case bound: e4m3 keeps 4 significant bits, and the
+        // worst case 2^-3 sum |U||Bc| would leave nearly every column
+        // undecided.  The relative rounding errors d (|d| <= 2^-4, variance
+        // <= 2^-8 / 3 each, independent, zero mean) give a score error of
+        // variance <= (2/3) 2^-8 sum_r U_r^2 Bc_r^2 <= (2/3) 2^-8 sum_b E_b
+        // max_b Bc^2 (ksm_bound.h); the margin is kFp8Sigmas standard
+        // deviations of that, plus the worst-case accumulation and subnormal
+        // terms.
+        fp8_abs_terms_kernel<<<1, 256, 0, s>>>(srow, L.ldr, abs_terms);
+        KSM_LAUNCH_CHECK("fp8_abs_terms_kernel");
+        const double e = 0x1p-4;
+        const int depth = L.ldr;
+        const double g = depth * 0x1p-24 / (1.0 - depth * 0x1p-24);
+        const double zc = kFp8Sigmas * e * std::sqrt(2.0 / 3.0) * (1.0 + 0x1p-10);
+        const int rc = bound_finish(stats, 2 * nblk, L.Tp, U, W, perm, C, T, K, P, nt, kBoundSqrt,
+                                    0.0, zc, g * (1 + e) * (1 + e), abs_terms, bound, s);
+        if (rc != KSM_OK) return rc;
+    }
     u_fp8_kernel<<<L.Kp, 256, 0, s>>>(U, perm, K, P, C, L.ldr, srow, Uq, uscale);
     KSM_LAUNCH_CHECK("u_fp8_kernel");
     tile_steps_fp8_kernel<<<L.nTm, 256, 0, s>>>(Uq, L.ldr, L.nk, W, P * nt, nonfinite, steps,

@@ -29,10 +29,12 @@ Optional key ``ops['matching_precision']``:
   correlation scaled per (channel, PC) by a bound on its magnitude and U per
   template.
 
-In the reduced-precision forms every accepted spike is re-scored in FP32
-from the residual before it is recorded and subtracted, so amplitudes,
-th_amps and the subtraction use FP32 scores; the accept/reject decisions use
-the reduced-precision scores.
+In the reduced-precision forms the projection also returns a per-column
+bound on its scores' error (proven for the BF16 forms, a 6-sigma rounding
+model for FP8), and the peel takes every accept/reject decision that bound
+leaves open on scores re-scored in FP32 from the residual, so decisions
+match FP32 arithmetic; accepted spikes' amplitudes, th_amps and the
+subtraction use their FP32 scores.
 
 Optional key ``ops['matching_streams']`` (default ``MATCHING_STREAMS``):
 device streams over which ``extract`` spreads its batches, one host thread
@@ -290,9 +292,21 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     Bt = _ws.get(device, "Bt", rows_t * ldbt * 4)
     pbytes = lib.ksm_project_workspace(C, T, K, P, nt, prec)
     pws = _ws.get(device, "project", pbytes)
-    _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), g.perm.data_ptr(),
-                               C, T, K, P, nt, Bt.data_ptr(), ldbt, prec, pws.data_ptr(),
-                               pbytes, stream), "ksm_project")
+    if prec == _lib.PREC_FP32:
+        bound = None
+        _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), g.perm.data_ptr(),
+                                   C, T, K, P, nt, Bt.data_ptr(), ldbt, prec, pws.data_ptr(),
+                                   pbytes, stream), "ksm_project")
+    else:
+        # reduced-precision scores and their per-column error bound, with
+        # which the peel takes every decision the bound leaves open on FP32
+        # re-scored values
+        bbytes = lib.ksm_bound_bytes(C, T, K, P, nt)
+        bound = _ws.get(device, "bound", bbytes)
+        _lib.check(lib.ksm_project_bounded(X.data_ptr(), T, W.data_ptr(), U.data_ptr(),
+                                           g.perm.data_ptr(), C, T, K, P, nt, Bt.data_ptr(),
+                                           ldbt, prec, bound.data_ptr(), bbytes, pws.data_ptr(),
+                                           pbytes, stream), "ksm_project_bounded")
 
     Xres = X.clone()
     st = torch.empty((ST_CAPACITY, 2), dtype=torch.int64, device=device)
@@ -305,12 +319,13 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
                             nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),
                             g.cc.data_ptr(), g.lohi.data_ptr(), g.off.data_ptr(), K, P, C, T,
                             nt, Th, max_peels,
-                            0 if prec == _lib.PREC_FP32 else _lib.PEEL_RESCORE,
+                            0 if bound is None else _lib.PEEL_RESCORE,
+                            None if bound is None else bound.data_ptr(),
                             st.data_ptr(), amps.data_ptr(),
                             th_amps.data_ptr(), ST_CAPACITY, info.data_ptr(), kws.data_ptr(),
                             kbytes, stream),
                "ksm_peel")
-    k, status, iters, _ = (int(v) for v in info.cpu())
+    k, status, iters, rescored = (int(v) for v in info.cpu())
     if status == _lib.KSM_ERR_CAPACITY:
         raise RuntimeError(f"more than {ST_CAPACITY} spikes in one batch "
                            "(the reference's per-batch buffer overflows too)")
@@ -318,7 +333,9 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     global last_peel_iterations
     last_peel_iterations = iters
     _iters.n = iters
-    logger.debug("run_matching: %d spikes in %d peel iterations", k, iters)
+    _iters.rescored = rescored
+    logger.debug("run_matching: %d spikes in %d peel iterations (%d FP32 re-scores)", k, iters,
+                 rescored)
     return st[:k], amps[:k], th_amps[:k], Xres
 
 

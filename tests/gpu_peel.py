@@ -39,7 +39,7 @@ def gpu_peel(ops, X, U, B, capacity=20000):
     _lib.check(lib.ksm_peel(Bt.data_ptr(), ldbt, Xres.data_ptr(), T, Ud.data_ptr(),
                             Wd.data_ptr(), nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),
                             g.cc.data_ptr(), g.lohi.data_ptr(), g.off.data_ptr(), K, P, C, T,
-                            nt, float(ops["Th_learned"]), ops["max_peels"], 0, st.data_ptr(),
+                            nt, float(ops["Th_learned"]), ops["max_peels"], 0, None, st.data_ptr(),
                             amps.data_ptr(), th.data_ptr(), capacity, info.data_ptr(),
                             ws.data_ptr(), nb, stream), "ksm_peel")
     k, status, iters, _ = (int(v) for v in info.cpu())

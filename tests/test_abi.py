@@ -67,7 +67,7 @@ def test_invalid_arguments_are_rejected_without_gpu_work():
     assert st == _lib.KSM_ERR_INVALID
     assert b"ksm_template_norms" in lib.ksm_last_error()
     st = lib.ksm_peel(None, 0, None, 0, None, None, None, None, None, None, None, None, 1, 6, 4,
-                      10, 61, 8.0, 100, 0, None, None, None, 10, None, None, 0, None)
+                      10, 61, 8.0, 100, 0, None, None, None, None, 10, None, None, 0, None)
     assert st == _lib.KSM_ERR_INVALID
     with pytest.raises(ValueError):
         _lib.check(st, "ksm_peel")

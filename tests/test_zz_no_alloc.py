@@ -52,10 +52,20 @@ def _inputs(seed, C, K, NT, precision=_lib.PREC_FP32):
     Bt = torch.zeros(((T + 127) // 128 * 128, ldbt), dtype=torch.float32, device=DEV)
     nb = lib.ksm_project_workspace(C, T, K, 6, 61, precision)
     pws = torch.empty((nb,), dtype=torch.uint8, device=DEV)
-    _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), g.perm.data_ptr(), C,
-                               T, K, 6, 61, Bt.data_ptr(), ldbt, precision, pws.data_ptr(), nb,
-                               stream), "ksm_project")
-    return dict(X=X, W=W, U=U, g=g, nm=nm, Bt=Bt, ldbt=ldbt, C=C, K=K, T=T, ctc=ctc)
+    bound = None
+    if precision == _lib.PREC_FP32:
+        _lib.check(lib.ksm_project(X.data_ptr(), T, W.data_ptr(), U.data_ptr(), g.perm.data_ptr(),
+                                   C, T, K, 6, 61, Bt.data_ptr(), ldbt, precision, pws.data_ptr(),
+                                   nb, stream), "ksm_project")
+    else:
+        nbb = lib.ksm_bound_bytes(C, T, K, 6, 61)
+        bound = torch.empty((nbb,), dtype=torch.uint8, device=DEV)
+        _lib.check(lib.ksm_project_bounded(X.data_ptr(), T, W.data_ptr(), U.data_ptr(),
+                                           g.perm.data_ptr(), C, T, K, 6, 61, Bt.data_ptr(), ldbt,
+                                           precision, bound.data_ptr(), nbb, pws.data_ptr(), nb,
+                                           stream), "ksm_project_bounded")
+    return dict(X=X, W=W, U=U, g=g, nm=nm, Bt=Bt, ldbt=ldbt, C=C, K=K, T=T, ctc=ctc,
+                bound=bound)
 
 
 def _peel(inp, static, flags=0, capacity=40000):
@@ -71,8 +81,9 @@ def _peel(inp, static, flags=0, capacity=40000):
     args = [Bt.data_ptr(), inp["ldbt"], Xres.data_ptr(), inp["T"], inp["U"].data_ptr(),
             inp["W"].data_ptr(), inp["nm"].data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),
             g.cc.data_ptr(), g.lohi.data_ptr(), g.off.data_ptr(), inp["K"], 6, inp["C"],
-            inp["T"], 61, 8.0, 100, flags, st.data_ptr(), amps.data_ptr(), th.data_ptr(),
-            capacity, info.data_ptr()]
+            inp["T"], 61, 8.0, 100, flags,
+            None if inp["bound"] is None else inp["bound"].data_ptr(),
+            st.data_ptr(), amps.data_ptr(), th.data_ptr(), capacity, info.data_ptr()]
     if static:
         _lib.check(lib.ksm_peel_static(*args, stream), "ksm_peel_static")
     else:
@@ -121,7 +132,7 @@ def test_static_peel_rejects_sizes_beyond_its_scratch():
     dummy = torch.zeros(16, device=DEV)
     p = dummy.data_ptr()
     st = lib.ksm_peel_static(p, 4096, p, 60122, p, p, p, p, p, p, p, p, 4096, 6, 383, 60122, 61,
-                             8.0, 100, 0, p, p, p, 10, p, _lib.stream_of(DEV))
+                             8.0, 100, 0, None, p, p, p, 10, p, _lib.stream_of(DEV))
     assert st == _lib.KSM_ERR_WORKSPACE
     assert b"ksm_peel_static" in lib.ksm_last_error()
     assert b"scratch" in lib.ksm_last_error()
