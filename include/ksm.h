@@ -36,7 +36,8 @@ typedef enum ksm_status {
     KSM_ERR_CAPACITY = 2,       /* more spikes than the st capacity         */
     KSM_ERR_HIP = 3,            /* a HIP runtime call or launch failed      */
     KSM_ERR_WORKSPACE = 4,      /* workspace smaller than the query result  */
-    KSM_ERR_TIES = 5            /* exact score ties beyond slot capacity    */
+    KSM_ERR_TIES = 5            /* reserved: no longer returned (ties of any
+                                   size are peeled)                          */
 } ksm_status;
 
 typedef enum ksm_dtype {

@@ -46,7 +46,8 @@
 namespace {
 
 constexpr int kChunk = 256;         // columns per select block (one per thread)
-constexpr int kSlotCap = 16;        // accepted spikes per chunk (>= 256/62 + ties)
+constexpr int kSlotCap = 256;       // accepted spikes per chunk: every column of it
+static_assert(kSlotCap >= kChunk, "a chunk can accept each of its columns (exact ties)");
 constexpr int kGroup = 32;          // positions per group (8 lanes x 4)
 constexpr int kPassGroups = 8;      // groups a wave reduces per pass
 constexpr int kMaxK = 8192;
@@ -1229,67 +1230,34 @@ __device__ __forceinline__ ColSpike col_spike(const SpikeG &g, int t, int nt) {
 
 // Subtractions of a column.  The reference subtracts each phase
 // (even-ranked spikes, then odd-ranked) with one indexed write, so where
-// windows of one phase overlap -- only exact score ties place accepted spikes
-// within 2nt+1 columns of each other -- the last spike's write is the one that
-// remains (torch's non-accumulating index_put on the CPU).  A column covered
-// by spikes q .. q+m therefore takes at most two subtractions: the last
-// even-ranked and the last odd-ranked of them, even first.  Up to kMaxCover
-// spikes may cover one column.
-constexpr int kMaxCover = 4;
+// windows of one phase overlap -- exact score ties place accepted spikes
+// within nt columns of each other, and any two within 2nt+1 share columns of
+// their +-nt windows -- the last spike's write is the one that remains
+// (torch's non-accumulating index_put on the CPU).  The spikes covering a
+// column are consecutive in time order, so of alternating rank parity: the
+// column takes exactly two subtractions, the last even-ranked and the last
+// odd-ranked of them -- the last covering spike and the one before it -- even
+// first, however many spikes cover it.  A column is therefore owned by the
+// last spike whose window covers it.
 
-// Spike q with the later spikes whose windows reach its window.
+// Spike q with its neighbours: q owns the columns of its window that the
+// window of spike q+1 does not reach, and spike q-1 covers those it reaches.
 struct SpikeCtx {
-    SpikeG g0, g1;          // spikes q, q+1 (g1 when ncov > 1)
-    int ncov;               // spikes q .. q+ncov-1 reach q's window
-    int x2, x3;             // columns of spikes q+2, q+3; their Gram refs in s_far
-    bool has_p, over;       // over: a (kMaxCover+1)-th spike reaches q's window
-    int x_p, x_over;
+    SpikeG g0;              // spike q
+    SpikeG gp;              // spike q-1 (has_p)
+    bool has_p, has_n;
+    int x_n;                // column of spike q+1 (has_n)
 };
 
-__device__ __forceinline__ void spike_ctx(SpikeCtx &c, int q, int S, int nt, const int *pre,
-                                          int nchunks, const Slot *__restrict__ slots,
-                                          SpikeG *s_far) {
-    // lanes 0 .. kMaxCover + 1: spikes q - 1 .. q + kMaxCover
-    const Slot w = spike_lanes(q - 1, kMaxCover + 2, S, pre, nchunks, slots);
+__device__ __forceinline__ void spike_ctx(SpikeCtx &c, int q, int S, const int *pre, int nchunks,
+                                          const Slot *__restrict__ slots) {
+    // lanes 0, 1, 2: spikes q - 1, q, q + 1
+    const Slot w = spike_lanes(q - 1, 3, S, pre, nchunks, slots);
     c.g0 = spike_gram(w, 1);
     c.has_p = q > 0;
-    c.x_p = c.has_p ? rl(w.x, 0) : 0;
-    const int reach = c.g0.x + 2 * nt;
-    c.ncov = 1;
-    c.over = false;
-    c.x2 = c.x3 = c.x_over = 0;
-    for (int j = 1; j <= kMaxCover && q + j < S; ++j) {
-        const int xj = rl(w.x, j + 1);
-        if (xj > reach) break;
-        if (j == kMaxCover) {
-            c.over = true;
-            c.x_over = xj;
-            break;
-        }
-        const SpikeG gj = spike_gram(w, j + 1);
-        if (j == 1) {
-            c.g1 = gj;
-        } else {
-            if ((threadIdx.x & 63) == 0) s_far[j - 2] = gj;
-            __builtin_amdgcn_wave_barrier();
-            if (j == 2) c.x2 = gj.x;
-            else c.x3 = gj.x;
-        }
-        c.ncov = j + 1;
-    }
-}
-
-__device__ __forceinline__ SpikeG cover_at(const SpikeCtx &c, int j, const SpikeG *s_far) {
-    if (j == 0) return c.g0;
-    if (j == 1) return c.g1;
-    const SpikeG f = s_far[j - 2];
-    SpikeG r;
-    r.x = uni(f.x);
-    r.lo = uni(f.lo);
-    r.hi = uni(f.hi);
-    r.amp = uni(f.amp);
-    r.off = uni(f.off);
-    return r;
+    c.gp = spike_gram(w, 0);
+    c.has_n = q + 1 < S;
+    c.x_n = rl(w.x, 2);
 }
 
 // Per-lane constants of one pass (kPassGroups groups) over the columns of a
@@ -1532,41 +1500,34 @@ struct SegColumnsGeneral {
 // (live, not covered by spike q-1), cut into segments of fixed covering
 // spikes.
 __device__ __forceinline__ void spike_columns(int q, int d_lo, int d_hi, const SpikeCtx &c,
-                                              const SpikeG *s_far, int nt, int T,
-                                              float *__restrict__ Bt, int ldbt, int K, int G,
-                                              const float *__restrict__ cc, const float *s_nm,
-                                              const int *s_pid, u64 *__restrict__ gkey,
-                                              float *__restrict__ cfmax,
-                                              int *__restrict__ imax, bool &ties) {
+                                              int nt, int T, float *__restrict__ Bt, int ldbt,
+                                              int K, int G, const float *__restrict__ cc,
+                                              const float *s_nm, const int *s_pid,
+                                              u64 *__restrict__ gkey, float *__restrict__ cfmax,
+                                              int *__restrict__ imax) {
     int tl = c.g0.x - nt + d_lo, th = c.g0.x - nt + d_hi;
     tl = max(tl, nt);
     th = min(th, T - nt - 1);
-    if (c.has_p) tl = max(tl, c.x_p + nt + 1);
-    if (c.over && th >= c.x_over - nt) {
-        ties = true;
-        th = c.x_over - nt - 1;
-    }
-    const int bnd[4] = {tl, c.ncov > 1 ? c.g1.x - nt : th + 1, c.ncov > 2 ? c.x2 - nt : th + 1,
-                        c.ncov > 3 ? c.x3 - nt : th + 1};
-    for (int m = 0; m < kMaxCover; ++m) {
-        const int s0 = max(tl, bnd[m]);
-        const int s1 = min(th, m + 1 < kMaxCover ? bnd[m + 1] - 1 : th);
+    if (c.has_n) th = min(th, c.x_n - nt - 1);       // spike q+1 owns the later ones
+    const int pe = c.has_p ? c.gp.x + nt : tl - 1;  // spike q-1 covers columns <= pe
+    const bool q_even = (q & 1) == 0;
+    // segment 0: covered by q-1 and q; segment 1: by q alone
+    for (int m = 0; m < 2; ++m) {
+        const int s0 = m == 0 ? tl : max(tl, pe + 1);
+        const int s1 = m == 0 ? min(th, pe) : th;
         if (s0 > s1) continue;
         const ColSpike none = {0, 0, 0, 0.f};
-        const SpikeG gl = cover_at(c, m, s_far);
-        const ColSpike sl = col_spike(gl, s0, nt);
+        const ColSpike sq = col_spike(c.g0, s0, nt);
         ColSpike sp = none;
         int wp = 0;
-        if (m > 0) {
-            const SpikeG gp = cover_at(c, m - 1, s_far);
-            sp = col_spike(gp, s0, nt);
-            wp = gp.hi - gp.lo;
+        if (m == 0) {
+            sp = col_spike(c.gp, s0, nt);
+            wp = c.gp.hi - c.gp.lo;
         }
-        const int wl = gl.hi - gl.lo;
-        // the last even-ranked spike is subtracted first
-        const bool last_even = ((q + m) & 1) == 0;
-        const ColSpike a = last_even ? sl : sp, b = last_even ? sp : sl;
-        const int wa = last_even ? wl : wp, wb = last_even ? wp : wl;
+        const int wq = c.g0.hi - c.g0.lo;
+        // the even-ranked spike is subtracted first
+        const ColSpike a = q_even ? sq : sp, b = q_even ? sp : sq;
+        const int wa = q_even ? wq : wp, wb = q_even ? wp : wq;
         const GroupList L = group_list(a, b, G, false);
         if (G <= 64 && L.count() <= 2 * kPassGroups) {
             seg_columns(s0, s1, a, b, wa, wb, L, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
@@ -1581,84 +1542,61 @@ __device__ __forceinline__ void spike_columns(int q, int d_lo, int d_hi, const S
     }
 }
 
-// Residual subtraction for a cluster of spikes whose windows overlap (gaps of
-// at most 2h = nt - 1 samples; only exact score ties produce them): spikes
-// q .. q+m-1.  As for B, each sample takes the last even-ranked and then the
-// last odd-ranked of the spikes covering it.  Work item k of the cluster's
-// first spike takes its share of the union of the templates' channel ranges;
-// lanes = samples, with per-lane template coefficients.
-constexpr int kMaxCluster = 8;
-struct ClusterMember {
-    int x, id;
-    float amp;
-};
-
-__device__ void xres_cluster(int q, int k, int S, const int *pre, int nchunks,
-                             const Slot *__restrict__ slots, float *__restrict__ Xres, int ldx,
-                             const float *__restrict__ U, const float *__restrict__ W,
-                             const int *__restrict__ crange, int P, int C, int nt,
-                             int *__restrict__ state, ClusterMember *s_cl) {
+// Residual subtraction of spike q's item k where its residual window overlaps
+// a neighbour's (spikes closer than nt samples: exact score ties).  As for B,
+// a sample takes the subtractions of its last covering spike and of the one
+// before it, even-ranked first; spike q takes the samples of its window that
+// spike q+1's window does not reach, on its share of the union of the two
+// templates' channel ranges.  Lanes = samples; the template coefficients are
+// read per channel (wave-uniform).
+__device__ void xres_pair(int q, int k, const Slot &rw, bool has_p, bool has_n,
+                          float *__restrict__ Xres, int ldx, const float *__restrict__ U,
+                          const float *__restrict__ W, const int *__restrict__ crange, int P,
+                          int C, int nt) {
     const int lane = threadIdx.x & 63;
     const int h = nt / 2;
-    int m = 0, clo = C, chi = 0, prev_x = 0;
-    for (int j = 0; j < kMaxCluster && q + j < S; ++j) {
-        const Slot sj = spike_at(q + j, pre, nchunks, slots);
-        const int x = uni(sj.x), id = uni(sj.iy);
-        if (j > 0 && x - prev_x > 2 * h) break;
-        if (lane == 0) s_cl[j] = ClusterMember{x, id, uni(sj.amp)};
-        clo = min(clo, uni(crange[2 * id]));
-        chi = max(chi, uni(crange[2 * id + 1]));
-        prev_x = x;
-        m = j + 1;
-    }
-    if (m == kMaxCluster && q + m < S &&
-        uni(spike_at(q + m, pre, nchunks, slots).x) - prev_x <= 2 * h) {
-        if (lane == 0) {
-            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
-            atomicExch(&state[ST_DONE], 1);
+    const int x = rl(rw.x, 1), id = rl(rw.iy, 1);
+    const float amp = rl(rw.amp, 1);
+    const int xp = rl(rw.x, 0), idp = rl(rw.iy, 0);
+    const float ampp = rl(rw.amp, 0);
+    const int p_lo = x - h;
+    const int p_hi = has_n ? min(x + h, rl(rw.x, 2) - h - 1) : x + h;
+    const int pe = has_p ? xp + h : p_lo - 1;       // spike q-1 covers samples <= pe
+    int clo = uni(crange[2 * id]), chi = uni(crange[2 * id + 1]);
+    if (has_p && pe >= p_lo) {
+        const int plo = uni(crange[2 * idp]), phi = uni(crange[2 * idp + 1]);
+        if (phi > plo) {
+            clo = chi > clo ? min(clo, plo) : plo;
+            chi = max(chi, phi);
         }
-        return;
     }
-    __builtin_amdgcn_wave_barrier();
-    if (chi <= clo) return;
+    if (chi <= clo || p_hi < p_lo) return;
     const int cw = (chi - clo + kXresItems - 1) / kXresItems;
     const int cl0 = clo + k * cw, cl1 = min(chi, cl0 + cw);
-    const int x0 = s_cl[0].x, x1 = s_cl[m - 1].x;
-    for (int s0 = x0 - h; s0 <= x1 + h; s0 += 64) {
+    const bool q_even = (q & 1) == 0;
+    for (int s0 = p_lo; s0 <= p_hi; s0 += 64) {
         const int sm = s0 + lane;
-        // covering members form a run; the last two of it are the last
-        // even-ranked and the last odd-ranked
-        int jl = -1, jf = m;
-        for (int j = 0; j < m; ++j) {
-            const int dx = sm - s_cl[j].x;
-            if (dx >= -h && dx <= h) {
-                jl = j;
-                jf = min(jf, j);
-            }
-        }
-        if (sm > x1 + h) jl = -1;
-        const int jp = (jl > jf) ? jl - 1 : -1;
-        const bool l_even = ((q + jl) & 1) == 0;
-        const int je = jl < 0 ? -1 : (l_even ? jl : jp);
-        const int jo = jl < 0 ? -1 : (l_even ? jp : jl);
-        const ClusterMember me = s_cl[je < 0 ? 0 : je], mo = s_cl[jo < 0 ? 0 : jo];
-        const int ke = sm - me.x + h, ko = sm - mo.x + h;
+        const bool in = sm <= p_hi;
+        const bool cov_p = in && has_p && sm <= pe;
+        const int kq = in ? sm - x + h : 0;
+        const int kp = cov_p ? sm - xp + h : 0;
         for (int c = cl0; c < cl1; ++c) {
-            if (jl < 0) continue;
+            if (!in) continue;
             float xv = Xres[(size_t)c * ldx + sm];
-            if (je >= 0) {
-                float tm = 0.f;
-                for (int p = 0; p < P; ++p)
-                    tm = __builtin_fmaf(U[((size_t)me.id * P + p) * C + c], W[p * nt + ke], tm);
-                const float prod = me.amp * tm;
-                xv = xv - prod;
+            float tq = 0.f, tp = 0.f;
+            for (int p = 0; p < P; ++p) {
+                tq = __builtin_fmaf(U[((size_t)id * P + p) * C + c], W[p * nt + kq], tq);
+                if (cov_p)
+                    tp = __builtin_fmaf(U[((size_t)idp * P + p) * C + c], W[p * nt + kp], tp);
             }
-            if (jo >= 0) {
-                float tm = 0.f;
-                for (int p = 0; p < P; ++p)
-                    tm = __builtin_fmaf(U[((size_t)mo.id * P + p) * C + c], W[p * nt + ko], tm);
-                const float prod = mo.amp * tm;
-                xv = xv - prod;
+            const float prq = amp * tq;
+            const float prp = ampp * tp;
+            if (q_even) {
+                xv = xv - prq;
+                if (cov_p) xv = xv - prp;
+            } else {
+                if (cov_p) xv = xv - prp;
+                xv = xv - prq;
             }
             Xres[(size_t)c * ldx + sm] = xv;
         }
@@ -1668,14 +1606,15 @@ __device__ void xres_cluster(int q, int k, int S, const int *pre, int nchunks,
 // One iteration's subtraction.  Work items per spike q (time order): 2nt+1
 // columns of its window, which the waves take in contiguous ranges, and
 // kXresItems residual items, dealt round-robin over all waves so that no
-// wave gets more than about one.  A column is owned by the earliest spike whose
-// window covers it; its owner applies the subtractions next_column yields
+// wave gets more than about one.  A column is owned by the last spike whose
+// window covers it; its owner applies its own subtraction and that of the
+// spike before it where that one covers the column too, even-ranked first
 // (the reference's two interleaved phases).  A wave runs its columns as a
 // pipeline (the next column's loads are issued before the current column's
 // stores), then its residual items: each subtracts amp * (U[id]^T W) from
 // its share of the template's nonzero channels in Xres[:, x-nt/2 .. x+nt/2]
 // (windows of one iteration's spikes are disjoint unless exact ties put
-// spikes closer than nt: those go through xres_cluster).  The block also
+// spikes closer than nt: those go through xres_pair).  The block also
 // records the spikes into st / amps / th_amps.
 __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int it, float *__restrict__ Bt, int ldbt, float *__restrict__ Xres, int ldx,
@@ -1691,8 +1630,6 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
     int *s_pid = reinterpret_cast<int *>(s_tab + Kg);
     int *s_pre = s_pid + Kg;
     __shared__ int s_wsum[kUpdThreads / 64];
-    __shared__ ClusterMember s_cl[kUpdThreads / 64][kMaxCluster];
-    __shared__ SpikeG s_far[kUpdThreads / 64][kMaxCover - 2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
 
@@ -1775,19 +1712,14 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
 
     // columns: the wave's items, spike by spike
     if (i0 < i1) {
-        bool ties = false;
         const int q0 = i0 / ncol, q1 = (i1 - 1) / ncol;
         for (int q = q0; q <= q1; ++q) {
             const int d_lo = q == q0 ? i0 % ncol : 0;
             const int d_hi = q == q1 ? (i1 - 1) % ncol : ncol - 1;
             SpikeCtx c;
-            spike_ctx(c, q, S, nt, s_pre, nchunks, slots, s_far[wv]);
-            spike_columns(q, d_lo, d_hi, c, s_far[wv], nt, T, Bt, ldbt, K, G, cc, s_nm, s_pid,
-                          gkey, cfmax, imax, ties);
-        }
-        if (ties && lane == 0) {
-            atomicExch(&state[ST_STATUS], KSM_ERR_TIES);
-            atomicExch(&state[ST_DONE], 1);
+            spike_ctx(c, q, S, s_pre, nchunks, slots);
+            spike_columns(q, d_lo, d_hi, c, nt, T, Bt, ldbt, K, G, cc, s_nm, s_pid, gkey, cfmax,
+                          imax);
         }
     }
 
@@ -1801,13 +1733,12 @@ __global__ __launch_bounds__(kUpdThreads) void peel_update_kernel(
         // lanes 0, 1, 2: spikes q - 1, q, q + 1
         const Slot rw = spike_lanes(q - 1, 3, S, s_pre, nchunks, slots);
         const int id = rl(rw.iy, 1), x = rl(rw.x, 1);
-        // windows closer than nt samples (exact ties): the cluster's first
-        // spike handles the whole cluster
-        if (q > 0 && x - rl(rw.x, 0) < nt) continue;
-        if (q + 1 < S && rl(rw.x, 2) - x < nt) {
+        // windows closer than nt samples (exact ties) overlap a neighbour's
+        const bool has_p = q > 0, has_n = q + 1 < S;
+        if ((has_p && x - rl(rw.x, 0) < nt) || (has_n && rl(rw.x, 2) - x < nt)) {
             for (int k = k0; k < k1; ++k)
-                xres_cluster(q, k, S, s_pre, nchunks, slots, Xres, ldx, U, W, crange, P, C, nt,
-                             state, s_cl[wv]);
+                xres_pair(q, k, rw, has_p, has_n && rl(rw.x, 2) - x < nt, Xres, ldx, U, W,
+                          crange, P, C, nt);
             continue;
         }
         const float amp = rl(rw.amp, 1);

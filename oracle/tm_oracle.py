@@ -85,8 +85,8 @@ def peel(ops, B, X, U, ctc, nm, capacity=100000, record=None):
     amps = B/nm and th_amps = sqrt(Cmax); subtract the accepted spikes from the
     residual and their ctc columns from B in two interleaved phases.  Where
     the windows of one phase overlap (only exact score ties place accepted
-    spikes that close) the indexed subtraction keeps the last spike's write,
-    as torch's non-accumulating index_put does on the CPU.
+    spikes that close) the indexed subtraction keeps the last spike's write
+    (``_sub_last_write``).
 
     ``record`` (a list) receives per-iteration decision data for the margin
     report when given.
@@ -130,11 +130,34 @@ def peel(ops, B, X, U, ctc, nm, capacity=100000, record=None):
         k += nsp
         for j in range(2):
             sel = iY[j::2, 0]
-            Xres[:, iX[j::2] + lag_x] -= amp[j::2] * torch.einsum(
-                "ijk, jl -> kil", U[sel], W)
-            B[:, iX[j::2] + lag_b] -= amp[j::2] * ctc[:, sel, :]
+            _sub_last_write(Xres, iX[j::2, 0], lag_x,
+                            amp[j::2] * torch.einsum("ijk, jl -> kil", U[sel], W))
+            _sub_last_write(B, iX[j::2, 0], lag_b, amp[j::2] * ctc[:, sel, :])
 
     return st[:k], amps[:k], th_amps[:k], Xres
+
+
+def _sub_last_write(A, centers, lags, vals):
+    """``A[:, centers[:, None] + lags] -= vals`` (vals [R, m, L]) as the
+    reference's phase update (template_matching.py:236-238) writes it when the
+    windows of two spikes overlap: the subtraction is one non-accumulating
+    indexed write of (A gathered before the write) - vals, so a column covered
+    by several windows keeps the last spike's value.  That is the order of
+    torch's single-threaded index_put; with duplicate indices torch leaves the
+    order undefined (its multi-threaded CPU and CUDA index_put do not keep it),
+    so the rule is written out here: for every column, the last window
+    covering it."""
+    m, L = len(centers), len(lags)
+    if m == 0:
+        return
+    cols = (centers[:, None] + lags[None, :]).reshape(-1)
+    spike = torch.arange(m).repeat_interleave(L)
+    last = torch.full((A.shape[1],), -1, dtype=torch.int64)
+    last.scatter_reduce_(0, cols, spike, reduce="amax")
+    win = cols[last[cols] == spike]                  # each column once, from its last spike
+    who = last[win]
+    lag = win - centers[who] - lags[0]
+    A[:, win] = A[:, win] - vals[:, who, lag]
 
 
 def features(ops, Xres, st, amps, iCC, iU, Ucc):

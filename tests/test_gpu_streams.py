@@ -97,9 +97,10 @@ def test_extract_many_matches_extract():
     say("extract_many: 3 recordings (C, K, batches) = (64, 40, 3), (32, 12, 1), (96, 70, 2)")
     jobs, refs = [], []
     for seed, C, K, nb in ((83, 64, 40, 3), (84, 32, 12, 1), (85, 96, 70, 2)):
-        # whole batches: a partial last batch is padded by repeating its last
-        # sample, and unfiltered data then gives long runs of tied scores
-        rec = synth.make_recording(seed, C, K, nb * NT, rate_range=(5.0, 30.0))
+        # the second recording ends within a batch: its last batch is padded
+        # by repeating its last sample (tied scores where they pass threshold)
+        n = nb * NT + (1700 if nb == 1 else 0)
+        rec = synth.make_recording(seed, C, K, n, rate_range=(5.0, 30.0))
         ops = dict(nt=61, nt0min=rec.nt0min, Th_learned=8, max_peels=100, batch_size=NT,
                    wPCA=torch.from_numpy(rec.wPCA).to(DEV), xc=rec.xc, yc=rec.yc,
                    settings=dict(nearest_chans=10, position_limit=100.0, n_pcs=6))

@@ -230,3 +230,31 @@ def test_planted_lag_flip_is_caught():
     bad = O.run_matching(ops, X, U, torch.flip(ctc, [2]))
     assert not (len(bad[0]) == len(bt) and np.array_equal(bad[0][:, 0].numpy(), bt)
                 and np.allclose(bad[1][:, 0].numpy(), good[1][:, 0].numpy()))
+
+
+def test_phase_update_keeps_the_last_write():
+    """The oracle's phase update (O._sub_last_write) against its definition
+    written as a sequential loop -- each spike's window written with (value
+    before the phase) - (its subtraction), later spikes overwriting earlier
+    ones -- on plateaus of consecutive and of mixed spacing, and against
+    torch's single-threaded index_put, whose order it states."""
+    torch.manual_seed(3)
+    for centers in (torch.arange(50, 350), torch.tensor([40, 41, 90, 95, 200, 201, 202, 330])):
+        R, T = 5, 500
+        lags = torch.arange(-30, 31)
+        A = torch.randn(R, T)
+        vals = torch.randn(R, len(centers), len(lags))
+        got = A.clone()
+        O._sub_last_write(got, centers, lags, vals)
+        want = A.clone()
+        for s in range(len(centers)):
+            want[:, centers[s] + lags] = A[:, centers[s] + lags] - vals[:, s]
+        assert torch.equal(got, want)
+        n0 = torch.get_num_threads()
+        torch.set_num_threads(1)
+        try:
+            ref = A.clone()
+            ref[:, centers[:, None] + lags] -= vals
+        finally:
+            torch.set_num_threads(n0)
+        assert torch.equal(got, ref)
