@@ -179,10 +179,12 @@ size_t ksm_bound_bytes(int C, int T, int K, int P, int nt);
  * least 2^-70 of its largest channel's) and its waveforms there, Ttab[r][c -
  * lo][k] = sum_p U[perm[r],p,c] W[p,k], from which the peel re-scores in FP32.
  * Same workspace as ksm_project.  KSM_PREC_FP32 is rejected; C <= 4096.
+ * tile_m: template rows per tile of the BF16 forms' GEMM, 64 or 128 (0: 128),
+ * an autotuning choice (same results: only exact-zero steps are skipped).
  */
 int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
                         const int32_t *perm, int C, int T, int K, int P, int nt, float *Bt,
-                        int ldbt, int precision, void *bound, size_t bound_bytes,
+                        int ldbt, int precision, int tile_m, void *bound, size_t bound_bytes,
                         void *workspace, size_t ws_bytes, void *stream);
 
 /* Workspace bytes of ksm_peel. */
@@ -243,6 +245,25 @@ int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
 /* Bytes of the static peel state of ksm_peel_static. */
 size_t ksm_peel_static_capacity(void);
 
+/*
+ * Iterations [it_begin, it_end) of the peel of ksm_peel (no reference
+ * counterpart: a split of the same loop), for callers that capture the peel
+ * in a graph or run it in parts.  it_begin == 0 starts a peel (state,
+ * tables, first pass over Bt); a later range continues from the state the
+ * previous range left in `workspace` (same workspace, Bt, Xres, outputs and
+ * sizes).  Nothing is polled or timed: every iteration of the range is
+ * enqueued and those after the end return at once.  info as ksm_peel's after
+ * the range; the loop has ended when info[2] < it_end (an iteration accepted
+ * nothing) or it_end == max_peels.
+ */
+int ksm_peel_range(float *Bt, int ldbt, float *Xres, int ldx, const float *U,
+                   const float *W, const float *nm, const int32_t *perm,
+                   const int32_t *rank, const float *cc, const int32_t *lohi,
+                   const int64_t *off, int K, int P, int C, int T, int nt, float Th,
+                   int max_peels, int flags, const void *bound, int64_t *st, float *amps,
+                   float *th_amps, int capacity, int32_t *info, void *workspace,
+                   size_t ws_bytes, int it_begin, int it_end, void *stream);
+
 /* ksm_peel flags. */
 #define KSM_PEEL_RESCORE 1  /* the scores come from a reduced-precision
                                projection (ksm_project_bounded, same perm) and
@@ -265,6 +286,17 @@ int ksm_features(const float *Xres, int ldx, const float *W,
                  const int64_t *st, const float *amps, int nsp,
                  const int64_t *iCC, const int64_t *iU, const float *Ucc,
                  int nC, int K, int P, int C, int nt, float *tF, void *stream);
+
+/*
+ * ksm_features for a spike count on the device (a peel's info[0]; no
+ * reference counterpart): features of spikes s < min(info[0], capacity),
+ * from a fixed grid, so that the call can be captured in a graph with the
+ * peel before the count is known on the host.
+ */
+int ksm_features_counted(const float *Xres, int ldx, const float *W, const int64_t *st,
+                         const float *amps, const int32_t *info, int capacity,
+                         const int64_t *iCC, const int64_t *iU, const float *Ucc, int nC, int K,
+                         int P, int C, int nt, float *tF, void *stream);
 
 /*
  * Per-kernel timing (benchmark and test use; no reference counterpart).

@@ -67,14 +67,18 @@ SIGNATURES = {
     "ksm_project_workspace": (_sz, [_i, _i, _i, _i, _i, _i]),
     "ksm_project": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz, _p]),
     "ksm_bound_bytes": (_sz, [_i, _i, _i, _i, _i]),
-    "ksm_project_bounded": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _p, _sz,
-                                 _p, _sz, _p]),
+    "ksm_project_bounded": (_i, [_p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _i, _i, _i, _p,
+                                 _sz, _p, _sz, _p]),
     "ksm_peel_workspace": (_sz, [_i, _i, _i]),
     "ksm_peel": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _f,
                       _i, _i, _p, _p, _p, _p, _i, _p, _p, _sz, _p]),
     "ksm_peel_static": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i,
                              _f, _i, _i, _p, _p, _p, _p, _i, _p, _p]),
     "ksm_peel_static_capacity": (_sz, []),
+    "ksm_peel_range": (_i, [_p, _i, _p, _i, _p, _p, _p, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i,
+                            _f, _i, _i, _p, _p, _p, _p, _i, _p, _p, _sz, _i, _i, _p]),
+    "ksm_features_counted": (_i, [_p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p,
+                                  _p]),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
     "ksm_residual_workspace": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
     "ksm_residual": (_i, [_p, _i, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _i, _p, _i,
@@ -166,6 +170,24 @@ def require_gpu(device) -> torch.device:
 
 def stream_of(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
+
+
+def own_stream(device) -> torch.cuda.ExternalStream:
+    """A HIP stream of its own (non-blocking, default priority) on `device`,
+    made through the runtime libksm links: torch.cuda.Stream() hands out its
+    pool's streams in turn, so after enough calls another component's stream
+    can be one already in use, which a graph capture must not share.  The
+    stream lives until the process ends."""
+    lib = load()
+    create = lib.hipStreamCreateWithFlags       # resolved through libksm's dependencies
+    create.restype = ctypes.c_int
+    create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    s = ctypes.c_void_p()
+    with torch.cuda.device(device):
+        rc = create(ctypes.byref(s), 1)          # hipStreamNonBlocking
+    if rc != 0 or not s.value:
+        raise RuntimeError(f"hipStreamCreateWithFlags failed ({rc})")
+    return torch.cuda.ExternalStream(s.value, device=device)
 
 
 def ptr(t) -> int:

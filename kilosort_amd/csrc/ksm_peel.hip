@@ -1858,35 +1858,39 @@ __global__ __launch_bounds__(kFeatWaves * 64) void features_kernel(
     const float *__restrict__ Xres, int ldx, const float *__restrict__ W,
     const int64_t *__restrict__ st, const float *__restrict__ amps, int nsp,
     const int64_t *__restrict__ iCC, const int64_t *__restrict__ iU, const float *__restrict__ Ucc,
-    int nC, int K, int P, int C, int nt, float *__restrict__ tF) {
+    int nC, int K, int P, int C, int nt, float *__restrict__ tF, const int32_t *__restrict__ count) {
     extern __shared__ float f_smem[];
     float *s_w = f_smem;                                  // [P][nt]
     float *s_x = f_smem + P * nt + (threadIdx.x >> 6) * nC * nt;   // this wave: [nC][nt]
     for (int e = threadIdx.x; e < P * nt; e += blockDim.x) s_w[e] = W[e];
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int s = blockIdx.x * kFeatWaves + (threadIdx.x >> 6);
-    if (s >= nsp) return;
+    // nsp spikes, or the device count (a peel's info[0]) when given, at most nsp
+    const int n = count ? min(uni((int)count[0]), nsp) : nsp;
+    for (int s = blockIdx.x * kFeatWaves + (threadIdx.x >> 6); s < n;
+         s += gridDim.x * kFeatWaves) {
     const int x = (int)st[2 * s];
     const int id = (int)st[2 * s + 1];
     const int iu = (int)iU[id];
-    for (int n = 0; n < nC; ++n) {
-        const int ch = (int)iCC[(size_t)n * C + iu];
+    for (int r = 0; r < nC; ++r) {
+        const int ch = (int)iCC[(size_t)r * C + iu];
         const float *xr = Xres + (size_t)ch * ldx + x - nt / 2;
-        for (int k = lane; k < nt; k += 64) s_x[n * nt + k] = xr[k];
+        for (int k = lane; k < nt; k += 64) s_x[r * nt + k] = xr[k];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float a = amps[s];
     for (int j = lane; j < nC * P; j += 64) {
-        const int n = j / P, p = j % P;
-        const float *xs = s_x + n * nt;
+        const int r = j / P, p = j % P;
+        const float *xs = s_x + r * nt;
         const float *ws = s_w + p * nt;
         float f = 0.f;
         for (int k = 0; k < nt; ++k) f = __builtin_fmaf(xs[k], ws[k], f);
-        const float own = a * Ucc[((size_t)n * K + id) * P + p];
-        tF[((size_t)s * nC + n) * P + p] = f + own;
+        const float own = a * Ucc[((size_t)r * K + id) * P + p];
+        tF[((size_t)s * nC + r) * P + p] = f + own;
+    }
+    __builtin_amdgcn_wave_barrier();             // s_x is rewritten for the next spike
     }
 }
 
@@ -2051,7 +2055,7 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
                     const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
                     float Th, int max_peels, int flags, const void *bound, int64_t *st,
                     float *amps, float *th_amps, int capacity, int32_t *info, void *workspace,
-                    hipStream_t s, bool may_poll) {
+                    hipStream_t s, bool may_poll, int it_begin, int it_end) {
     // the kernels are timed only for the entry point that may allocate (events)
     const unsigned prof_group = may_poll ? ksm::kProfPeel : 0u;
     const PeelLayout L = peel_layout(K, T, max_peels);
@@ -2081,20 +2085,22 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
     const int grid = update_grid(upd_dyn);
     const size_t sel_dyn = (size_t)K * sizeof(int);
 
-    peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
-    KSM_LAUNCH_CHECK("peel_init_kernel");
-    channel_range_kernel<<<(K + 3) / 4, 256, 0, s>>>(U, K, P, C, crange);
-    KSM_LAUNCH_CHECK("channel_range_kernel");
-    position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, ppid);
-    KSM_LAUNCH_CHECK("position_tables_kernel");
-    gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, crange, K, gref);
-    KSM_LAUNCH_CHECK("gram_refs_kernel");
-    {
+    if (it_begin == 0) {
+        // a new peel: state, tables and the first pass over the scores (a
+        // later range continues from the state these leave in the workspace)
+        peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
+        KSM_LAUNCH_CHECK("peel_init_kernel");
+        channel_range_kernel<<<(K + 3) / 4, 256, 0, s>>>(U, K, P, C, crange);
+        KSM_LAUNCH_CHECK("channel_range_kernel");
+        position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, ppid);
+        KSM_LAUNCH_CHECK("position_tables_kernel");
+        gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, crange, K, gref);
+        KSM_LAUNCH_CHECK("gram_refs_kernel");
         ksm::ProfileScope prof("colmax_all", prof_group, s);
         prof.launch(colmax_all_kernel, dim3((T + kFirstCols - 1) / kFirstCols), dim3(kUpdThreads),
                     (unsigned)tab_dyn, Bt, ldbt, pnm, ppid, K, L.G, L.Kg, T, nt, cfmax, imax, gkey);
+        KSM_LAUNCH_CHECK("colmax_all_kernel");
     }
-    KSM_LAUNCH_CHECK("colmax_all_kernel");
     // stop enqueuing once the device reports the end (not under stream capture)
     HostFlags none_flags;
     HostFlags &hf = may_poll ? host_flags() : none_flags;
@@ -2106,7 +2112,7 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
     if (poll && ++hf.seq == 0u) ++hf.seq;      // 0 is never a live sequence number
     const unsigned seq = poll ? hf.seq : 0u;
     bool gave_up = false;
-    for (int it = 0; it < max_peels; ++it) {
+    for (int it = it_begin; it < it_end; ++it) {
         if (poll && !gave_up && it >= kLag && wait_progress(hf, seq, it - kLag, &gave_up)) break;
         {
             ksm::ProfileScope prof("peel_select", prof_group, s);
@@ -2149,7 +2155,24 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
     return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
                     max_peels, flags, bound, st, amps, th_amps, capacity, info, workspace,
-                    ksm::as_stream(stream), true);
+                    ksm::as_stream(stream), true, 0, max_peels);
+}
+
+int ksm_peel_range(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const float *W,
+                   const float *nm, const int32_t *perm, const int32_t *rank, const float *cc,
+                   const int32_t *lohi, const int64_t *off, int K, int P, int C, int T, int nt,
+                   float Th, int max_peels, int flags, const void *bound, int64_t *st,
+                   float *amps, float *th_amps, int capacity, int32_t *info, void *workspace,
+                   size_t ws_bytes, int it_begin, int it_end, void *stream) {
+    if (!peel_args_ok(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt,
+                      max_peels, flags, bound, st, amps, th_amps, capacity, info) ||
+        it_begin < 0 || it_end < it_begin || it_end > max_peels)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_peel_range: bad arguments");
+    if (!workspace || ws_bytes < ksm_peel_workspace(K, T, max_peels))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel_range: workspace too small");
+    return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
+                    max_peels, flags, bound, st, amps, th_amps, capacity, info, workspace,
+                    ksm::as_stream(stream), false, it_begin, it_end);
 }
 
 size_t ksm_peel_static_capacity(void) { return kPeelScratchBytes; }
@@ -2173,7 +2196,7 @@ int ksm_peel_static(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
     if (rc != KSM_OK) return rc;
     return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
                     max_peels, flags, bound, st, amps, th_amps, capacity, info, ws,
-                    ksm::as_stream(stream), false);
+                    ksm::as_stream(stream), false, 0, max_peels);
 }
 
 int ksm_features(const float *Xres, int ldx, const float *W, const int64_t *st,
@@ -2188,7 +2211,28 @@ int ksm_features(const float *Xres, int ldx, const float *W, const int64_t *st,
     const size_t smem = (size_t)(P * nt + kFeatWaves * nC * nt) * sizeof(float);
     ksm::ProfileScope prof("features", ksm::kProfPeel, ksm::as_stream(stream));
     prof.launch(features_kernel, dim3((nsp + kFeatWaves - 1) / kFeatWaves), dim3(kFeatWaves * 64),
-                (unsigned)smem, Xres, ldx, W, st, amps, nsp, iCC, iU, Ucc, nC, K, P, C, nt, tF);
+                (unsigned)smem, Xres, ldx, W, st, amps, nsp, iCC, iU, Ucc, nC, K, P, C, nt, tF,
+                static_cast<const int32_t *>(nullptr));
+    KSM_LAUNCH_CHECK("features_kernel");
+    return KSM_OK;
+}
+
+int ksm_features_counted(const float *Xres, int ldx, const float *W, const int64_t *st,
+                         const float *amps, const int32_t *info, int capacity,
+                         const int64_t *iCC, const int64_t *iU, const float *Ucc, int nC, int K,
+                         int P, int C, int nt, float *tF, void *stream) {
+    if (capacity < 0 || nC <= 0 || K <= 0 || P <= 0 || C <= 0 || nt <= 0 || nC > C ||
+        nC > kFeatMaxNC || nt > kMaxNt)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_features_counted: bad arguments");
+    if (capacity == 0) return KSM_OK;
+    if (!Xres || !W || !st || !amps || !info || !iCC || !iU || !Ucc || !tF)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_features_counted: null pointer");
+    const size_t smem = (size_t)(P * nt + kFeatWaves * nC * nt) * sizeof(float);
+    // a fixed grid whose waves step over the spikes the device reports
+    const int blocks = (min(capacity, 8192) + kFeatWaves - 1) / kFeatWaves;
+    ksm::ProfileScope prof("features", ksm::kProfPeel, ksm::as_stream(stream));
+    prof.launch(features_kernel, dim3(blocks), dim3(kFeatWaves * 64), (unsigned)smem, Xres, ldx, W,
+                st, amps, capacity, iCC, iU, Ucc, nC, K, P, C, nt, tF, info);
     KSM_LAUNCH_CHECK("features_kernel");
     return KSM_OK;
 }

@@ -367,13 +367,13 @@ size_t ksm_project_workspace(int C, int T, int K, int P, int nt, int precision) 
 int ksm_project(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
                 int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
                 void *workspace, size_t ws_bytes, void *stream) {
-    return ksm_project_bounded(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision, nullptr,
-                               0, workspace, ws_bytes, stream);
+    return ksm_project_bounded(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision, 0,
+                               nullptr, 0, workspace, ws_bytes, stream);
 }
 
 int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
                         const int32_t *perm, int C, int T, int K, int P, int nt, float *Bt,
-                        int ldbt, int precision, void *bound, size_t bound_bytes,
+                        int ldbt, int precision, int tile_m, void *bound, size_t bound_bytes,
                         void *workspace, size_t ws_bytes, void *stream) {
     if (!X || !W || !U || !Bt || C <= 0 || T <= 0 || K <= 0 || P <= 0 || nt <= 0 ||
         (nt % 2) == 0 || ldx < T)
@@ -381,6 +381,8 @@ int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
     if (precision != KSM_PREC_FP32 && precision != KSM_PREC_BF16 &&
         precision != KSM_PREC_BF16X3 && precision != KSM_PREC_FP8)
         return ksm::fail(KSM_ERR_INVALID, "ksm_project: unknown precision %d", precision);
+    if (tile_m != 0 && tile_m != 64 && tile_m != 128)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_project_bounded: tile_m %d (0, 64 or 128)", tile_m);
     if (bound && precision == KSM_PREC_FP32)
         return ksm::fail(KSM_ERR_INVALID,
                          "ksm_project_bounded: the FP32 form has no reduced-precision bound");
@@ -397,8 +399,8 @@ int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
         return ksm::project_fp8(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, bound, workspace,
                                 s);
     if (precision != KSM_PREC_FP32)
-        return ksm::project_bf16(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision, bound,
-                                 workspace, s);
+        return ksm::project_bf16(X, ldx, W, U, perm, C, T, K, P, nt, Bt, ldbt, precision, tile_m,
+                                 bound, workspace, s);
     ksm::Carver cv(workspace);
     float *Bc = cv.take<float>((size_t)L.Rp * L.ldc);
     float *At = cv.take<float>((size_t)L.Rp * L.Kp);

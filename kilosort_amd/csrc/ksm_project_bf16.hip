@@ -187,12 +187,13 @@ struct Segs {
 
 __global__ void set_flag_h_kernel(int *__restrict__ p) { *p = 0; }
 
-// The depth steps (HBK columns of r = c*P + p) where a tile of HBM_ template
+// The depth steps (HBK columns of r = c*P + p) where a tile of MT template
 // rows of Uh is not all zero, in increasing order: steps[tm][0 .. counts[tm]).
 // A skipped step would add only exact zeros (templates are zero outside their
 // channel footprint); with a non-finite value in X, W or U every step is kept
 // (0 * inf must give NaN as in the dense product).  Ul is zero wherever Uh is.
 constexpr int kHStepsMax = 4096;
+template <int MT>
 __global__ __launch_bounds__(256) void tile_steps_bf16_kernel(
     const bf16_t *__restrict__ Uh, int ldr, int nk, const float *__restrict__ W, int PW,
     const int *__restrict__ nonfinite, int *__restrict__ steps, int *__restrict__ counts) {
@@ -211,9 +212,9 @@ __global__ __launch_bounds__(256) void tile_steps_bf16_kernel(
         const int rsub = lane >> 3, chunk = lane & 7;
         for (int st = wave; st < nk; st += 4) {
             bool nz = false;
-            for (int i = 0; i < HBM_ / 8; ++i) {
+            for (int i = 0; i < MT / 8; ++i) {
                 const uint4 v = *reinterpret_cast<const uint4 *>(
-                    Uh + (size_t)(tm * HBM_ + rsub + 8 * i) * ldr + st * HBK + chunk * 8);
+                    Uh + (size_t)(tm * MT + rsub + 8 * i) * ldr + st * HBK + chunk * 8);
                 // bf16 +0 and -0 are zero; any other pattern (NaN included) is not
                 const unsigned m = 0x7fff7fffu;
                 nz |= ((v.x & m) | (v.y & m) | (v.z & m) | (v.w & m)) != 0u;
@@ -243,24 +244,31 @@ __device__ __forceinline__ int xcd_remap_h(int bid, int nb) {
     return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
 }
 
+// MT template rows per block tile (128: 4 waves of 64 x 64; 64: 4 waves of
+// 32 x 64), 128 time columns.
+template <int MT>
 __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int ldr,
                                                       const int *__restrict__ steps,
                                                       const int *__restrict__ counts,
                                                       float *__restrict__ Bout, int ldb, int K,
                                                       int nTm, int nTn) {
-    __shared__ bf16_t As[2][HBM_ * HLD];
+    constexpr int MI = MT / 64;                  // 32-row MFMA tiles per wave
+    constexpr int AE = MT * HBK / 256;           // A elements each thread loads per step
+    __shared__ bf16_t As[2][MT * HLD];
     __shared__ bf16_t Bs[2][HBN * HLD];
     const int logical = xcd_remap_h(blockIdx.x, nTm * nTn);
     const int tm = logical % nTm, tn = logical / nTm;
-    const int m0 = tm * HBM_, n0 = tn * HBN;
+    const int m0 = tm * MT, n0 = tn * HBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    // global -> LDS: thread owns row tid/2, 32 consecutive elements (4 x 16 B)
+    // global -> LDS: B: thread owns row tid/2, 32 consecutive elements (4 x 16
+    // B); A: row tid / (HBK / AE), AE consecutive elements
     const int grow = tid >> 1, gcol = (tid & 1) * 32;
+    const int arow = tid / (HBK / AE), acol = (tid % (HBK / AE)) * AE;
 
-    f32x16 acc[2][2];
+    f32x16 acc[MI][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
@@ -272,16 +280,18 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
     const int nk = nk_seg * nseg;
     // k step kt reads segment kt / nk_seg at depth lst[kt % nk_seg] * HBK
     const int k00 = nk_seg > 0 ? lst[0] * HBK : 0;
-    const bf16_t *ap = sg.a[0] + (size_t)(m0 + grow) * ldr + k00 + gcol;
+    const bf16_t *ap = sg.a[0] + (size_t)(m0 + arow) * ldr + k00 + acol;
     const bf16_t *bp = sg.b[0] + (size_t)(n0 + grow) * ldr + k00 + gcol;
-    uint4 ra0 = reinterpret_cast<const uint4 *>(ap)[0], ra1 = reinterpret_cast<const uint4 *>(ap)[1];
-    uint4 ra2 = reinterpret_cast<const uint4 *>(ap)[2], ra3 = reinterpret_cast<const uint4 *>(ap)[3];
+    uint4 ra[AE / 8];
+#pragma unroll
+    for (int u = 0; u < AE / 8; ++u) ra[u] = reinterpret_cast<const uint4 *>(ap)[u];
     uint4 rb0 = reinterpret_cast<const uint4 *>(bp)[0], rb1 = reinterpret_cast<const uint4 *>(bp)[1];
     uint4 rb2 = reinterpret_cast<const uint4 *>(bp)[2], rb3 = reinterpret_cast<const uint4 *>(bp)[3];
     {
-        uint4 *la = reinterpret_cast<uint4 *>(&As[0][grow * HLD + gcol]);
+        uint4 *la = reinterpret_cast<uint4 *>(&As[0][arow * HLD + acol]);
         uint4 *lb = reinterpret_cast<uint4 *>(&Bs[0][grow * HLD + gcol]);
-        la[0] = ra0; la[1] = ra1; la[2] = ra2; la[3] = ra3;
+#pragma unroll
+        for (int u = 0; u < AE / 8; ++u) la[u] = ra[u];
         lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
     }
     __syncthreads();
@@ -293,45 +303,47 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
             const int s = (kt + 1) / nk_seg, k0 = lst[(kt + 1) % nk_seg] * HBK;
             const bf16_t *sa = s == 0 ? sg.a[0] : (s == 1 ? sg.a[1] : sg.a[2]);
             const bf16_t *sb = s == 0 ? sg.b[0] : (s == 1 ? sg.b[1] : sg.b[2]);
-            const uint4 *pa = reinterpret_cast<const uint4 *>(sa + (size_t)(m0 + grow) * ldr + k0 + gcol);
+            const uint4 *pa = reinterpret_cast<const uint4 *>(sa + (size_t)(m0 + arow) * ldr + k0 + acol);
             const uint4 *pb = reinterpret_cast<const uint4 *>(sb + (size_t)(n0 + grow) * ldr + k0 + gcol);
-            ra0 = pa[0]; ra1 = pa[1]; ra2 = pa[2]; ra3 = pa[3];
+#pragma unroll
+            for (int u = 0; u < AE / 8; ++u) ra[u] = pa[u];
             rb0 = pb[0]; rb1 = pb[1]; rb2 = pb[2]; rb3 = pb[3];
         }
 #pragma unroll
         for (int kk = 0; kk < HBK; kk += 16) {
-            bf16x8 a[2], b[2];
+            bf16x8 a[MI], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < MI; ++i)
                 a[i] = *reinterpret_cast<const bf16x8 *>(
-                    &As[buf][(wm * 64 + i * 32 + kc) * HLD + kk + kr]);
+                    &As[buf][(wm * (MT / 2) + i * 32 + kc) * HLD + kk + kr]);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
                 b[j] = *reinterpret_cast<const bf16x8 *>(
                     &Bs[buf][(wn * 64 + j * 32 + kc) * HLD + kk + kr]);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < MI; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0,
                                                                         0, 0);
         }
         if (more) {
-            uint4 *la = reinterpret_cast<uint4 *>(&As[buf ^ 1][grow * HLD + gcol]);
+            uint4 *la = reinterpret_cast<uint4 *>(&As[buf ^ 1][arow * HLD + acol]);
             uint4 *lb = reinterpret_cast<uint4 *>(&Bs[buf ^ 1][grow * HLD + gcol]);
-            la[0] = ra0; la[1] = ra1; la[2] = ra2; la[3] = ra3;
+#pragma unroll
+            for (int u = 0; u < AE / 8; ++u) la[u] = ra[u];
             lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
         }
         __syncthreads();
     }
     // transposed store Bout[n][m]: registers 4g..4g+3 are 4 consecutive m
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < MI; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
-                const int m = m0 + wm * 64 + i * 32 + 8 * g + 4 * (lane >> 5);
+                const int m = m0 + wm * (MT / 2) + i * 32 + 8 * g + 4 * (lane >> 5);
                 const int n = n0 + wn * 64 + j * 32 + (lane & 31);
                 float *dst = Bout + (size_t)n * ldb + m;
                 if (m + 3 < K) {
@@ -372,7 +384,7 @@ namespace ksm {
 size_t project_bf16_workspace(int C, int T, int K, int P, int precision) {
     const HLayout L = h_layout(C, T, K, P);
     const int terms = precision == KSM_PREC_BF16X3 ? 2 : 1;
-    const size_t nTm = L.Kp / HBM_, nk = L.ldr / HBK;
+    const size_t nTm = L.Kp / 64, nk = L.ldr / HBK;     // the finer (64-row) tiling
     const size_t nblk = (L.ldr / P + kCvCh - 1) / kCvCh;
     return terms * (align_up(L.bc_elems * 2, 256) + align_up(L.u_elems * 2, 256)) +
            align_up(nTm * nk * sizeof(int), 256) + align_up(nTm * sizeof(int), 256) +
@@ -381,7 +393,7 @@ size_t project_bf16_workspace(int C, int T, int K, int P, int precision) {
 
 int project_bf16(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
                  int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
-                 void *bound, void *workspace, hipStream_t s) {
+                 int tile_m, void *bound, void *workspace, hipStream_t s) {
     if (P != 6 || nt != 61)
         return fail(KSM_ERR_INVALID, "ksm_project: BF16 forms are built for P=6, nt=61");
     const HLayout L = h_layout(C, T, K, P);
@@ -391,7 +403,9 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     bf16_t *Uh = cv.take<bf16_t>(L.u_elems);
     bf16_t *Bl = split ? cv.take<bf16_t>(L.bc_elems) : nullptr;
     bf16_t *Ul = split ? cv.take<bf16_t>(L.u_elems) : nullptr;
-    const int nTm = L.Kp / HBM_, nTn = L.Tp / HBN, nk = L.ldr / HBK;
+    // template rows per GEMM tile: 64 or 128 (Kp is a multiple of both)
+    const int MT = tile_m == 64 ? 64 : 128;
+    const int nTm = (int)align_up(K, MT) / MT, nTn = L.Tp / HBN, nk = L.ldr / HBK;
     int *steps = cv.take<int>((size_t)nTm * nk);
     int *counts = cv.take<int>(nTm);
     int *nonfinite = cv.take<int>(1);
@@ -427,8 +441,12 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     }
     u_bf16_kernel<<<512, 256, 0, s>>>(U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
     KSM_LAUNCH_CHECK("u_bf16_kernel");
-    tile_steps_bf16_kernel<<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite, steps,
-                                               counts);
+    if (MT == 64)
+        tile_steps_bf16_kernel<64><<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite,
+                                                       steps, counts);
+    else
+        tile_steps_bf16_kernel<128><<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite,
+                                                        steps, counts);
     KSM_LAUNCH_CHECK("tile_steps_bf16_kernel");
     Segs sg;
     sg.a[0] = Uh;
@@ -439,8 +457,14 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     sg.b[2] = Bh;
     {
         ProfileScope prof("gemm_bf16", kProfProject, s);
-        prof.launch(gemm_bf16_mfma, dim3(nTm * nTn), dim3(256), 0u, sg, split ? 3 : 1, L.ldr,
-                    steps, counts, Bt, ldbt, K, nTm, nTn);
+        if (MT == 64)
+            prof.launch(gemm_bf16_mfma<64>, dim3(nTm * nTn), dim3(256), 0u, sg, split ? 3 : 1,
+                        L.ldr, static_cast<const int *>(steps), static_cast<const int *>(counts),
+                        Bt, ldbt, K, nTm, nTn);
+        else
+            prof.launch(gemm_bf16_mfma<128>, dim3(nTm * nTn), dim3(256), 0u, sg, split ? 3 : 1,
+                        L.ldr, static_cast<const int *>(steps), static_cast<const int *>(counts),
+                        Bt, ldbt, K, nTm, nTn);
     }
     KSM_LAUNCH_CHECK("gemm_bf16_mfma");
     return KSM_OK;

@@ -17,7 +17,7 @@ size_t project_bf16_workspace(int C, int T, int K, int P, int precision);
 // BF16 / split-BF16 projection into Bt (see ksm_project_bf16.hip).
 int project_bf16(const float *X, int ldx, const float *W, const float *U, const int32_t *perm,
                  int C, int T, int K, int P, int nt, float *Bt, int ldbt, int precision,
-                 void *bound, void *workspace, hipStream_t s);
+                 int tile_m, void *bound, void *workspace, hipStream_t s);
 
 // Workspace bytes of the FP8 projection form.
 size_t project_fp8_workspace(int C, int T, int K, int P);

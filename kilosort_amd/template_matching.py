@@ -259,6 +259,13 @@ def _precision(ops):
     return _PRECISIONS[name]
 
 
+def _tile_m(ops, K):
+    """Template rows per tile of the BF16 forms' projection GEMM (64 or 128;
+    0: the library's default), an autotuning choice: ``ops['matching_tile_m']``."""
+    t = ops.get('matching_tile_m')
+    return 0 if t is None else int(t)
+
+
 def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
     """Matching pursuit over one padded batch (reference ``template_matching.py:170-244``).
 
@@ -305,7 +312,7 @@ def run_matching(ops, X, U, ctc, device=torch.device('cuda')):
         bound = _ws.get(device, "bound", bbytes)
         _lib.check(lib.ksm_project_bounded(X.data_ptr(), T, W.data_ptr(), U.data_ptr(),
                                            g.perm.data_ptr(), C, T, K, P, nt, Bt.data_ptr(),
-                                           ldbt, prec, bound.data_ptr(), bbytes, pws.data_ptr(),
+                                           ldbt, prec, _tile_m(ops, K), bound.data_ptr(), bbytes, pws.data_ptr(),
                                            pbytes, stream), "ksm_project_bounded")
 
     Xres = X.clone()
@@ -389,6 +396,183 @@ def _lane_give(lane):
     lane.jobs.put(None)
 
 
+_CAPTURE_LOCK = threading.Lock()
+
+
+class _GraphStep:
+    """One recording's batches on one lane as replays of captured graphs.
+
+    The batch's work -- residual copy, projection (BF16 forms: with its error
+    bound), the first ``peels`` iterations of the peel (``ksm_peel_range``),
+    the features of the spikes the device counted (``ksm_features_counted``)
+    and the copy of the peel's info word to pinned host memory -- is captured
+    once into a HIP graph over static buffers and replayed per batch, one
+    launch instead of a hundred.  Where a batch needs more iterations than
+    the graph holds, tail graphs of ``_TAIL`` iterations (plus features)
+    continue the same peel.  The first batch runs the same kernels eagerly
+    (the peel with its host-polled end) and its iteration count picks
+    ``peels`` unless ``ops['matching_graph_peels']`` gives it.  Results are
+    those of run_matching + features (same kernels, same order).
+    """
+
+    _TAIL = 8
+
+    def __init__(self, ctx, C, T, device, cap_stream):
+        ops, U = ctx.ops, _check_U(ctx.U, device)
+        self.ctx, self.device = ctx, device
+        self.lib = lib = _lib.load()
+        self.U = U
+        K, P, _ = U.shape
+        self.K, self.P, self.C, self.T = K, P, C, T
+        self.nt = nt = int(ops['nt'])
+        self.Th = float(ops['Th_learned'])
+        self.max_peels = int(ops['max_peels'])
+        self.W = _f32(ops['wPCA'], device, "wPCA")
+        self.prec = _precision(ops)
+        self.tile = _tile_m(ops, K)
+        self.g = _grams.get(ctx.ctc, U, nt, device)
+        stream = _lib.stream_of(device)
+        f32 = dict(dtype=torch.float32, device=device)
+        self.nm = torch.empty((K,), **f32)
+        _lib.check(lib.ksm_template_norms(U.data_ptr(), K, P, C, self.nm.data_ptr(), stream),
+                   "ksm_template_norms")
+        self.ldbt = (K + 63) // 64 * 64
+        rows_t = (T + 127) // 128 * 128
+        self.X = torch.empty((C, T), **f32)
+        self.Xres = torch.empty((C, T), **f32)
+        self.Bt = torch.empty((rows_t, self.ldbt), **f32)
+        self.pbytes = lib.ksm_project_workspace(C, T, K, P, nt, self.prec)
+        self.pws = torch.empty((max(self.pbytes, 1),), dtype=torch.uint8, device=device)
+        self.bbytes = 0 if self.prec == _lib.PREC_FP32 else lib.ksm_bound_bytes(C, T, K, P, nt)
+        self.bound = (torch.empty((self.bbytes,), dtype=torch.uint8, device=device)
+                      if self.bbytes else None)
+        self.kbytes = lib.ksm_peel_workspace(K, T, self.max_peels)
+        self.kws = torch.empty((self.kbytes,), dtype=torch.uint8, device=device)
+        self.st = torch.empty((ST_CAPACITY, 2), dtype=torch.int64, device=device)
+        self.amps = torch.empty((ST_CAPACITY, 1), **f32)
+        self.th = torch.empty((ST_CAPACITY, 1), **f32)
+        self.info = torch.zeros((4,), dtype=torch.int32, device=device)
+        self.info_h = torch.zeros((4,), dtype=torch.int32, pin_memory=True)
+        nC = ctx.Ucc.shape[0]
+        self.nC = nC
+        self.tF = torch.empty((ST_CAPACITY, nC, P), **f32)
+        self.graph = None
+        self.cap_stream = cap_stream
+        self.peels = ops.get('matching_graph_peels')
+        self.tails = {}
+
+    # -- the work of one batch, enqueued on the current stream ---------------
+    def _project(self):
+        lib, g, stream = self.lib, self.g, _lib.stream_of(self.device)
+        self.Xres.copy_(self.X)
+        a = (self.X.data_ptr(), self.T, self.W.data_ptr(), self.U.data_ptr(), g.perm.data_ptr(),
+             self.C, self.T, self.K, self.P, self.nt, self.Bt.data_ptr(), self.ldbt, self.prec)
+        if self.bound is None:
+            _lib.check(lib.ksm_project(*a, self.pws.data_ptr(), self.pbytes, stream),
+                       "ksm_project")
+        else:
+            _lib.check(lib.ksm_project_bounded(*a, self.tile, self.bound.data_ptr(), self.bbytes,
+                                               self.pws.data_ptr(), self.pbytes, stream),
+                       "ksm_project_bounded")
+
+    def _peel_args(self):
+        g = self.g
+        flags = 0 if self.bound is None else _lib.PEEL_RESCORE
+        return (self.Bt.data_ptr(), self.ldbt, self.Xres.data_ptr(), self.T, self.U.data_ptr(),
+                self.W.data_ptr(), self.nm.data_ptr(), g.perm.data_ptr(), g.rank.data_ptr(),
+                g.cc.data_ptr(), g.lohi.data_ptr(), g.off.data_ptr(), self.K, self.P, self.C,
+                self.T, self.nt, self.Th, self.max_peels, flags,
+                None if self.bound is None else self.bound.data_ptr(), self.st.data_ptr(),
+                self.amps.data_ptr(), self.th.data_ptr(), ST_CAPACITY, self.info.data_ptr(),
+                self.kws.data_ptr(), self.kbytes)
+
+    def _features(self):
+        c, stream = self.ctx, _lib.stream_of(self.device)
+        _lib.check(self.lib.ksm_features_counted(
+            self.Xres.data_ptr(), self.T, self.W.data_ptr(), self.st.data_ptr(),
+            self.amps.data_ptr(), self.info.data_ptr(), ST_CAPACITY, c.iCC.data_ptr(),
+            c.iU.data_ptr(), c.Ucc.data_ptr(), self.nC, self.K, self.P, self.C, self.nt,
+            self.tF.data_ptr(), stream), "ksm_features_counted")
+        self.info_h.copy_(self.info, non_blocking=True)
+
+    def _range(self, it0, it1):
+        _lib.check(self.lib.ksm_peel_range(*self._peel_args(), it0, it1,
+                                           _lib.stream_of(self.device)), "ksm_peel_range")
+
+    def _capture(self, fn):
+        # One capture at a time in the process (lanes are threads), on the
+        # lane's own capture stream.  Capturing enqueues nothing, and replays
+        # run on the lane's stream after the work that filled the static
+        # buffers, so no device-wide synchronisation precedes the capture.
+        g = torch.cuda.CUDAGraph()
+        with _CAPTURE_LOCK, torch.cuda.stream(self.cap_stream):
+            g.capture_begin(capture_error_mode="thread_local")
+            try:
+                fn()
+            finally:
+                g.capture_end()
+        return g
+
+    def release(self):
+        """Destroys the captured graphs.  On ROCm a graph's destruction
+        synchronises the device, which fails while another thread captures,
+        so it happens under the capture lock."""
+        with _CAPTURE_LOCK:
+            self.graph = None
+            self.tails = {}
+
+    def _wait(self):
+        ev = torch.cuda.Event()
+        ev.record()
+        ev.synchronize()
+        k, status, iters, rescored = (int(v) for v in self.info_h)
+        if status == _lib.KSM_ERR_CAPACITY:
+            raise RuntimeError(f"more than {ST_CAPACITY} spikes in one batch "
+                               "(the reference's per-batch buffer overflows too)")
+        _lib.check(status, "ksm_peel (device)")
+        return k, iters, rescored
+
+    def run(self, X):
+        """Matching + features of batch X (a device tensor [C, T] ready on the
+        current stream).  Returns (k, iterations, st, th_amps, tF) with st,
+        th_amps and tF views of the static buffers (valid until the next run)."""
+        self.X.copy_(X)
+        if self.graph is None:
+            # first batch: eagerly, with the host-polled end of the peel
+            self._project()
+            _lib.check(self.lib.ksm_peel(*self._peel_args(), _lib.stream_of(self.device)),
+                       "ksm_peel")
+            self._features()
+            k, iters, _ = self._wait()
+            if self.peels is None:
+                self.peels = min(self.max_peels, (iters + 8 + 3) // 4 * 4)
+            n = int(self.peels)
+
+            def head():
+                self._project()
+                self._range(0, n)
+                self._features()
+            self.graph = self._capture(head)
+            return k, iters, self.st[:k], self.th[:k], self.tF[:k]
+        self.graph.replay()
+        k, iters, _ = self._wait()
+        it = int(self.peels)
+        # more iterations than the graph holds: continue the same peel
+        while iters >= it and it < self.max_peels:
+            it1 = min(self.max_peels, it + self._TAIL)
+            tail = self.tails.get(it)
+            if tail is None:
+                def fn(a=it, b=it1):
+                    self._range(a, b)
+                    self._features()
+                tail = self._capture(fn)
+                self.tails[it] = tail
+            tail.replay()
+            k, iters, _ = self._wait()
+            it = it1
+        return k, iters, self.st[:k], self.th[:k], self.tF[:k]
+
+
 _LANE_SLOTS = 3                 # pinned result buffer sets per lane
 
 
@@ -401,11 +585,19 @@ class _Lane:
     def __init__(self, device):
         self.device = device
         self.stream = torch.cuda.Stream(device=device)
+        self.cap_stream = _lib.own_stream(device)      # graph captures only
         self.jobs = queue.Queue()
         self.results = queue.Queue()
         self.bufs = [None] * _LANE_SLOTS
+        self.graph_steps = {}
         self.thread = threading.Thread(target=self._run, daemon=True)
         self.thread.start()
+
+    def drop_graphs(self):
+        """Releases this lane's captured graphs and their static buffers."""
+        steps, self.graph_steps = self.graph_steps, {}
+        for step in steps.values():
+            step.release()
 
     def _host(self, slot, k):
         b = self.bufs[slot]
@@ -423,13 +615,24 @@ class _Lane:
                 job = self.jobs.get()
                 if job is None:
                     return
-                slot, X, tag, ready, c = job
+                slot, X, tag, ready, c, graphed = job
                 try:
                     self.stream.wait_event(ready)
-                    stt, amps, th, Xres = run_matching(c.ops, X, c.U, c.ctc, device=dev)
-                    iters = _iters.n
-                    xfeat = _features(c.ops, Xres, stt, amps, c.iCC, c.iU, c.Ucc, dev)
-                    k = stt.shape[0]
+                    if graphed:
+                        step = self.graph_steps.get(id(c))
+                        if step is None:
+                            # one recording's graph steps at a time per lane
+                            self.drop_graphs()
+                            step = _GraphStep(c, X.shape[0], X.shape[1], dev,
+                                              self.cap_stream)
+                            self.graph_steps = {id(c): step}
+                        k, iters, stt, th, xf = step.run(X)
+                        xfeat = xf.clone()
+                    else:
+                        stt, amps, th, Xres = run_matching(c.ops, X, c.U, c.ctc, device=dev)
+                        iters = _iters.n
+                        xfeat = _features(c.ops, Xres, stt, amps, c.iCC, c.iU, c.Ucc, dev)
+                        k = stt.shape[0]
                     hb = self._host(slot, k)
                     if k:
                         hb[0][:k].copy_(stt, non_blocking=True)
@@ -437,7 +640,7 @@ class _Lane:
                     done = torch.cuda.Event()
                     done.record(self.stream)
                     done.synchronize()
-                    del X, stt, amps, th, Xres
+                    del X, stt, th
                     self.results.put((tag, hb[0][:k], hb[1][:k], xfeat, iters))
                 except BaseException as e:      # handed to the submitting thread
                     self.results.put(e)
@@ -491,7 +694,10 @@ class _BatchPipeline:
         self.peel_iterations += iters
         return tag, st, th, tf
 
-    def submit(self, X, tag=None, ctx=None):
+    def submit(self, X, tag=None, ctx=None, graphed=False):
+        """graphed: run the batch as a replay of the lane's captured graphs of
+        this recording (_GraphStep), for recordings with several batches per
+        lane."""
         ctx = self.ctx if ctx is None else ctx
         n = len(self.lanes)
         i = self.n_sub
@@ -504,7 +710,7 @@ class _BatchPipeline:
             out.append(self._take())
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.device))
-        self.lanes[i % n].jobs.put(((i // n) % _LANE_SLOTS, X, tag, ready, ctx))
+        self.lanes[i % n].jobs.put(((i // n) % _LANE_SLOTS, X, tag, ready, ctx, bool(graphed)))
         self.n_sub += 1
         return out
 
@@ -522,6 +728,7 @@ class _BatchPipeline:
             lane.results.get()
             self.n_done += 1
         for lane in self.lanes:
+            lane.drop_graphs()           # the lane is idle: its graphs and buffers go
             _lane_give(lane)
         self.lanes = []
 
@@ -654,7 +861,7 @@ def _streams(ops):
     return MATCHING_STREAMS if n is None else int(n)
 
 
-def extract_many(jobs, device=torch.device('cuda'), streams=None):
+def extract_many(jobs, device=torch.device('cuda'), streams=None, graphs=None):
     """``extract`` over several recordings of any sizes at once.
 
     ``jobs`` is a list of ``(ops, bfile, U)``; returns a list of ``(st, tF,
@@ -664,12 +871,20 @@ def extract_many(jobs, device=torch.device('cuda'), streams=None):
     leave most of the GPU idle -- share the device streams with their
     neighbours instead of running one after another.  ``streams`` defaults to
     ``MATCHING_STREAMS``.
+
+    ``graphs``: run the batches of a recording as replays of captured HIP
+    graphs (``_GraphStep``) where the recording has at least two batches per
+    stream (True: always; False: never; None: that rule).  Per-recording
+    choices of an autotuner (``autotune.tune_many``) come in each ``ops``:
+    ``matching_tile_m`` (GEMM tile of the BF16 forms),
+    ``matching_graph_peels`` (peel iterations in the captured graph),
+    ``matching_graphs``.
     """
     device = _lib.require_gpu(device)
     cols = [_Collector(ops, bfile, ops['settings']['nearest_chans'], ops['settings']['n_pcs'])
             for ops, bfile, _ in jobs]
-    pipe = _BatchPipeline(None, None, None, None, None, None, device,
-                          lanes=MATCHING_STREAMS if streams is None else streams)
+    lanes = MATCHING_STREAMS if streams is None else int(streams)
+    pipe = _BatchPipeline(None, None, None, None, None, None, device, lanes=lanes)
     r, ibatch = -1, -1
     try:
         for r, (ops, bfile, U) in enumerate(jobs):
@@ -677,9 +892,12 @@ def extract_many(jobs, device=torch.device('cuda'), streams=None):
             # their Gram on the device
             c = _prepare_recording(ops, U, device)
             pipe.prepare(c)
-            for ibatch in range(cols[r].n_batches):
+            nb = cols[r].n_batches
+            g = ops.get('matching_graphs', graphs)
+            graphed = (nb >= 2 * lanes) if g is None else bool(g)
+            for ibatch in range(nb):
                 X = bfile.padded_batch_to_torch(ibatch, ops)
-                for tag, *res in pipe.submit(X, tag=(r, ibatch), ctx=c):
+                for tag, *res in pipe.submit(X, tag=(r, ibatch), ctx=c, graphed=graphed):
                     cols[tag[0]].add(tag[1], *res)
                 X = None
         for tag, *res in pipe.flush():

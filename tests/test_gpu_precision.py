@@ -178,7 +178,7 @@ def test_bound_holds_per_template(batch1, name, prec):
     nb = lib.ksm_project_workspace(C, T, K, P, 61, prec)
     ws = torch.empty(nb, dtype=torch.uint8, device=DEV)
     _lib.check(lib.ksm_project_bounded(Xd.data_ptr(), T, Wd.data_ptr(), Ud.data_ptr(), None, C, T,
-                                       K, P, 61, Bt.data_ptr(), ldbt, prec, bb.data_ptr(), nbb,
+                                       K, P, 61, Bt.data_ptr(), ldbt, prec, 0, bb.data_ptr(), nbb,
                                        ws.data_ptr(), nb, _lib.stream_of(DEV)),
                "ksm_project_bounded")
     raw = bb.cpu().numpy()
@@ -220,7 +220,7 @@ def test_bound_holds_per_template(batch1, name, prec):
         np.testing.assert_allclose(Tt[r, :hi - lo], want, rtol=1e-5, atol=1e-6 * np.abs(want).max())
     with pytest.raises(ValueError):
         _lib.check(lib.ksm_project_bounded(Xd.data_ptr(), T, Wd.data_ptr(), Ud.data_ptr(), None,
-                                           C, T, K, P, 61, Bt.data_ptr(), ldbt, _lib.PREC_FP32,
+                                           C, T, K, P, 61, Bt.data_ptr(), ldbt, _lib.PREC_FP32, 0,
                                            bb.data_ptr(), nbb, ws.data_ptr(), nb,
                                            _lib.stream_of(DEV)), "ksm_project_bounded")
 

@@ -62,7 +62,7 @@ def _inputs(seed, C, K, NT, precision=_lib.PREC_FP32):
         bound = torch.empty((nbb,), dtype=torch.uint8, device=DEV)
         _lib.check(lib.ksm_project_bounded(X.data_ptr(), T, W.data_ptr(), U.data_ptr(),
                                            g.perm.data_ptr(), C, T, K, 6, 61, Bt.data_ptr(), ldbt,
-                                           precision, bound.data_ptr(), nbb, pws.data_ptr(), nb,
+                                           precision, 0, bound.data_ptr(), nbb, pws.data_ptr(), nb,
                                            stream), "ksm_project_bounded")
     return dict(X=X, W=W, U=U, g=g, nm=nm, Bt=Bt, ldbt=ldbt, C=C, K=K, T=T, ctc=ctc,
                 bound=bound)
