@@ -566,18 +566,21 @@ def executed_gemm_flops(U, perm, precision):
 
 def ragged_workload(n_rec, max_batches, seed, device):
     """Recordings of uneven size: channels from {64, 128, 192, 256, 384},
-    templates uniform in [50, 1000], 1 .. max_batches batches of NT, the same
-    generator as the main workload with a seed per recording."""
+    templates uniform in [50, 1000], 1 .. max_batches batches of NT, from the
+    64-channel workload's generator (Poisson 2-20 Hz, amplitudes U(10, 30))
+    with a seed per recording; the data is synthesised on the device."""
     from kilosort_amd import synth
     from kilosort_amd.batches import ArrayBatches
     rng = np.random.default_rng(seed)
     jobs = []
     for r in range(n_rec):
+        if r % 8 == 0:
+            log(f"ragged: synthesising recordings {r}..{min(r + 8, n_rec) - 1}")
         Cr = int(rng.choice([64, 128, 192, 256, 384]))
         Kr = int(rng.integers(50, 1001))
         nb = int(rng.integers(1, max_batches + 1))
-        rec = synth.make_recording(seed * 1000 + r, Cr, Kr, nb * NT, rate_range=(1.0, 20.0),
-                                   amp_range=(8.0, 40.0), overlap_frac=0.2, with_data=False)
+        rec = synth.make_recording(seed * 1000 + r, Cr, Kr, nb * NT, rate_range=(2.0, 20.0),
+                                   amp_range=(10.0, 30.0), with_data=False)
         data = synth.synthesize_torch(rec.n_samples, Cr, rec.times, rec.ids, rec.amps, rec.U,
                                       rec.wPCA, rec.nt0min, device, seed=seed * 1000 + r)
         bf = ArrayBatches(data, NT=NT, nt=NTAP, device=device)
@@ -588,41 +591,75 @@ def ragged_workload(n_rec, max_batches, seed, device):
     return jobs
 
 
-def time_ragged(tm, jobs, device):
-    """The recordings one after another (extract, one stream each: the
-    per-recording sequential loop) against all of them through extract_many
-    with the stream count the autotuner picks; both include each recording's
-    preparation (prepare_extract + prepare_matching)."""
+def _spike_pairs(st):
+    return set(map(tuple, np.round(st[:, :2]).astype(np.int64).tolist()))
+
+
+def time_ragged(tm, jobs, device, precision="bf16", runs=2):
+    """Many uneven recordings.  Sequential: extract() of each recording in
+    turn in FP32 (the library's default streams within a recording).
+    Batched: all of them through one extract_many in `precision` with
+    certified decisions, after the autotuner has picked per recording the
+    projection tile, the peel schedule of the captured graphs and whether
+    batches replay graphs, and the stream count.  Both include each
+    recording's preparation (prepare_extract + prepare_matching) and return
+    st/tF on the host; times are the best of `runs` after one untimed run.
+    Spike (time, template) agreement of the batched form with the sequential
+    FP32 form is reported per recording."""
     from kilosort_amd import autotune
     work = sum(int(U.shape[2]) * NT * int(bf.n_batches) for _, bf, U in jobs)
 
-    def seq():
+    seq_st = []
+
+    def seq(keep=False):
         for ops, bf, U in jobs:
-            tm.extract(dict(ops, matching_streams=1), bf, U, device=device)
+            st, _, _ = tm.extract(dict(ops), bf, U, device=device)
+            if keep:
+                seq_st.append(st)
 
-    def many():
-        tm.extract_many([(dict(o), b, U) for o, b, U in jobs], device=device, streams=best)
+    jobs_b = [(dict(o, matching_precision=precision), b, U) for o, b, U in jobs]
+    got = []
 
-    def best_of(f, n=3):
+    def many(keep=False):
+        out = tm.extract_many([(dict(o), b, U) for o, b, U in jobs_b], device=device,
+                              streams=best)
+        if keep:
+            got.extend(st for st, _, _ in out)
+
+    def timed(f):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         f()
         torch.cuda.synchronize()
-        ts = []
-        for _ in range(n):
-            t0 = time.perf_counter()
-            f()
-            torch.cuda.synchronize()
-            ts.append(time.perf_counter() - t0)
-        return min(ts)
+        return time.perf_counter() - t0
 
-    t_seq = best_of(seq)
-    best, tried = autotune.tune_streams(jobs, device=device, reps=2)
-    t_many = best_of(many)
-    return {"recordings": len(jobs), "timing": "best of 3 runs after one untimed run",
-            "shapes": [[int(U.shape[2]), int(U.shape[0]), int(bf.n_batches)] for _, bf, U in jobs],
+    seq(keep=True)
+    t_seq = min(timed(seq) for _ in range(runs))
+    log(f"ragged: sequential fp32 {t_seq:.2f} s; autotuning")
+    t0 = time.perf_counter()
+    best, plans, tried = autotune.tune_many(jobs_b, device=device)
+    t_tune = time.perf_counter() - t0
+    many(keep=True)
+    t_many = min(timed(many) for _ in range(runs))
+    agree = []
+    for a_st, b_st in zip(seq_st, got):
+        a, b = _spike_pairs(a_st), _spike_pairs(b_st)
+        agree.append(len(a & b) / max(len(a), len(b), 1))
+    nb = [int(bf.n_batches) for _, bf, _ in jobs]
+    return {"recordings": len(jobs), "batches": int(sum(nb)),
+            "timing": f"best of {runs} runs after one untimed run",
+            "shapes": [[int(U.shape[2]), int(U.shape[0]), n] for (_, _, U), n in zip(jobs, nb)],
             "sequential_value": work / t_seq, "sequential_s": t_seq,
-            "batched_value": work / t_many, "batched_s": t_many, "batched_streams": best,
-            "autotune_s": {str(k): v for k, v in tried.items()},
-            "unit": "channel·samples/s", "precision": "fp32"}
+            "sequential_precision": "fp32",
+            "batched_value": work / t_many, "batched_s": t_many, "batched_precision": precision,
+            "batched_streams": best, "speedup": t_seq / t_many,
+            "autotune_s": t_tune, "batched_value_with_autotune": work / (t_many + t_tune),
+            "autotune_streams_s": {str(k): v for k, v in tried.items()},
+            "plans": plans,
+            "graphed_recordings": int(sum(bool(p["graphs"]) for p in plans)),
+            "spikes": int(sum(len(s) for s in seq_st)),
+            "agreement_vs_fp32": {"min": float(min(agree)), "mean": float(np.mean(agree))},
+            "unit": "channel·samples/s"}
 
 
 PROFILED = ["conv_pc", "gemm_f32", "gemm_bf16", "gemm_fp8", "colmax_all", "peel_select",
@@ -698,8 +735,10 @@ def main():
     ap.add_argument("--forms", default=",".join(FORMS),
                     help="comma list of forms to time beside `value` ('' for none)")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--ragged", type=int, default=16,
+    ap.add_argument("--ragged", type=int, default=64,
                     help="recordings of the ragged (many uneven recordings) run; 0 skips it")
+    ap.add_argument("--ragged-batches", type=int, default=40,
+                    help="most batches of one recording in the ragged run")
     ap.add_argument("--streams", type=int, default=None,
                     help="device streams of the batch pipeline (default: the library's)")
     ap.add_argument("--no-cpu-baseline", action="store_true",
@@ -715,6 +754,8 @@ def main():
                          "run a kernel profile of the matching path uses")
     ap.add_argument("--templates-only", action="store_true",
                     help="time only template similarity and alignment and print their JSON object")
+    ap.add_argument("--ragged-only", action="store_true",
+                    help="run only the many-recordings comparison and print its JSON object")
     ap.add_argument("--refine-only", action="store_true",
                     help="time only the template refinement and print its JSON object")
     args = ap.parse_args()
@@ -736,6 +777,10 @@ def main():
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
     lib = _lib.load()
+    if args.ragged_only:
+        jobs = ragged_workload(args.ragged, args.ragged_batches, args.seed + 100, device)
+        print(json.dumps(time_ragged(tm, jobs, device)), flush=True)
+        return
     n_distinct = min(args.steps + args.warmup, 12)
     log(f"generating {n_distinct} synthetic batches")
     rec, ops, batches = make_workload(n_distinct, args.seed, device)
@@ -861,13 +906,16 @@ def main():
 
     ragged = None
     if args.ragged > 0:
-        log(f"ragged: {args.ragged} recordings, sequential extract vs extract_many")
-        jobs = ragged_workload(args.ragged, 3, args.seed + 100, device)
+        log(f"ragged: {args.ragged} recordings of 1-{args.ragged_batches} batches, sequential "
+            "fp32 extract vs autotuned bf16 extract_many")
+        jobs = ragged_workload(args.ragged, args.ragged_batches, args.seed + 100, device)
         ragged = time_ragged(tm, jobs, device)
         del jobs
+        torch.cuda.empty_cache()
         log(f"ragged: sequential {ragged['sequential_value']:.3e}, batched "
             f"{ragged['batched_value']:.3e} channel·samples/s on {ragged['batched_streams']} "
-            f"streams")
+            f"streams ({ragged['speedup']:.2f}x), agreement min "
+            f"{100 * ragged['agreement_vs_fp32']['min']:.3f} %")
 
     cpu = None
     agree = None

@@ -299,6 +299,34 @@ int ksm_features_counted(const float *Xres, int ldx, const float *W, const int64
                          int P, int C, int nt, float *tF, void *stream);
 
 /*
+ * Batch graphs (no reference counterpart: the reference enqueues each batch's
+ * work op by op, template_matching.py:79-110).  Between ksm_graph_begin(s)
+ * and ksm_graph_end(s, &g), on one host thread, the work that these entry
+ * points would enqueue on stream s is recorded as the nodes of one graph, in
+ * call order, and does not run: ksm_project, ksm_project_bounded,
+ * ksm_peel_range, ksm_peel_static, ksm_features, ksm_features_counted,
+ * ksm_copy_async.  (Other entry points run at once; ksm_peel, whose host
+ * follows the device's progress, returns KSM_ERR_INVALID on s.)  The graph is built node by node, not by stream capture, so
+ * other host threads' runtime calls meanwhile do not interact with it.
+ * ksm_graph_end instantiates it (g = NULL and an error when a recorded call
+ * failed or nothing was recorded, and the recording is discarded either
+ * way); ksm_graph_launch(g, s) runs it on s; ksm_graph_destroy(g) frees it
+ * once no launch of it is pending.  Argument values are taken at recording:
+ * every launch works on the same buffers.
+ */
+int ksm_graph_begin(void *stream);
+int ksm_graph_end(void *stream, void **graph_exec);
+int ksm_graph_launch(void *graph_exec, void *stream);
+int ksm_graph_destroy(void *graph_exec);
+
+/* ksm_copy_async kinds. */
+#define KSM_COPY_DEVICE_TO_DEVICE 0
+#define KSM_COPY_DEVICE_TO_HOST   1   /* dst: pinned host memory */
+
+/* An ordered copy of `bytes` on `stream` (recorded when the stream is). */
+int ksm_copy_async(void *dst, const void *src, size_t bytes, int kind, void *stream);
+
+/*
  * Per-kernel timing (benchmark and test use; no reference counterpart).
  * ksm_profile_enable(mask) turns on HIP event pairs on the launch stream
  * around the kernels of the selected groups: bit 0 = projection

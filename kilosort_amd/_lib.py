@@ -27,6 +27,9 @@ PREC_FP8 = 3
 
 PEEL_RESCORE = 1
 
+COPY_D2D = 0
+COPY_D2H = 1
+
 DTYPE_F32 = 0
 DTYPE_F64 = 1
 
@@ -80,6 +83,11 @@ SIGNATURES = {
     "ksm_features_counted": (_i, [_p, _i, _p, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p,
                                   _p]),
     "ksm_features": (_i, [_p, _i, _p, _p, _p, _i, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p]),
+    "ksm_graph_begin": (_i, [_p]),
+    "ksm_graph_end": (_i, [_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "ksm_graph_launch": (_i, [_p, _p]),
+    "ksm_graph_destroy": (_i, [_p]),
+    "ksm_copy_async": (_i, [_p, _p, _sz, _i, _p]),
     "ksm_residual_workspace": (_sz, [_i, _i, _i, _i, _i, _i, _i]),
     "ksm_residual": (_i, [_p, _i, _p, _p, _p, _p, _p, _i, _i, _i, _i, _i, _i, _p, _i, _p, _i,
                           _p, _sz, _p]),
@@ -170,24 +178,6 @@ def require_gpu(device) -> torch.device:
 
 def stream_of(device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
-
-
-def own_stream(device) -> torch.cuda.ExternalStream:
-    """A HIP stream of its own (non-blocking, default priority) on `device`,
-    made through the runtime libksm links: torch.cuda.Stream() hands out its
-    pool's streams in turn, so after enough calls another component's stream
-    can be one already in use, which a graph capture must not share.  The
-    stream lives until the process ends."""
-    lib = load()
-    create = lib.hipStreamCreateWithFlags       # resolved through libksm's dependencies
-    create.restype = ctypes.c_int
-    create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
-    s = ctypes.c_void_p()
-    with torch.cuda.device(device):
-        rc = create(ctypes.byref(s), 1)          # hipStreamNonBlocking
-    if rc != 0 or not s.value:
-        raise RuntimeError(f"hipStreamCreateWithFlags failed ({rc})")
-    return torch.cuda.ExternalStream(s.value, device=device)
 
 
 def ptr(t) -> int:

@@ -16,6 +16,7 @@
 #include <cmath>
 
 #include "ksm_bound.h"
+#include "ksm_graph.h"
 
 namespace {
 
@@ -178,13 +179,13 @@ int bound_finish(const float *stats, int nsb2, int ldst, const float *U, const f
     float *Wt = reinterpret_cast<float *>(b + L.o_Wt);
     int *srng = reinterpret_cast<int *>(b + L.o_srng);
     float *Ttab = reinterpret_cast<float *>(b + L.o_T);
-    bound_header_kernel<<<1, 1, 0, s>>>(reinterpret_cast<int *>(b + L.o_hdr), mode, L.nsb);
+    ksm::launch(bound_header_kernel, 1, 1, 0, s, reinterpret_cast<int *>(b + L.o_hdr), mode, L.nsb);
     KSM_LAUNCH_CHECK("bound_header_kernel");
-    bound_weights_kernel<<<K, 256, 0, s>>>(U, W, perm, C, K, P, nt, L.nsb, mode, kappa, zc, Wt,
-                                           srng, Ttab);
+    ksm::launch(bound_weights_kernel, K, 256, 0, s, U, W, perm, C, K, P, nt, L.nsb, mode, kappa,
+                zc, Wt, srng, Ttab);
     KSM_LAUNCH_CHECK("bound_weights_kernel");
-    bound_columns_kernel<<<(T + 63) / 64, 64, 0, s>>>(stats, nsb2, ldst, T, K, L.nsb, mode,
-                                                         acc, abs_terms, Wt, S, colA, colD);
+    ksm::launch(bound_columns_kernel, (T + 63) / 64, 64, 0, s, stats, nsb2, ldst, T, K, L.nsb,
+                mode, acc, abs_terms, Wt, S, colA, colD);
     KSM_LAUNCH_CHECK("bound_columns_kernel");
     return KSM_OK;
 }

@@ -42,6 +42,7 @@
 #include "ksm_common.h"
 #include "ksm_bound.h"
 #include "ksm_profile.h"
+#include "ksm_graph.h"
 
 namespace {
 
@@ -2088,13 +2089,15 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
     if (it_begin == 0) {
         // a new peel: state, tables and the first pass over the scores (a
         // later range continues from the state these leave in the workspace)
-        peel_init_kernel<<<1, 256, 0, s>>>(state, L.n_state);
+        ksm::launch(peel_init_kernel, 1, 256, 0, s, state, L.n_state);
         KSM_LAUNCH_CHECK("peel_init_kernel");
-        channel_range_kernel<<<(K + 3) / 4, 256, 0, s>>>(U, K, P, C, crange);
+        ksm::launch(channel_range_kernel, (K + 3) / 4, 256, 0, s, U, K, P, C, crange);
         KSM_LAUNCH_CHECK("channel_range_kernel");
-        position_tables_kernel<<<(L.Kg + 255) / 256, 256, 0, s>>>(nm, perm, K, L.Kg, pnm, ppid);
+        ksm::launch(position_tables_kernel, (L.Kg + 255) / 256, 256, 0, s, nm, perm, K, L.Kg, pnm,
+                    ppid);
         KSM_LAUNCH_CHECK("position_tables_kernel");
-        gram_refs_kernel<<<(K + 255) / 256, 256, 0, s>>>(rank, lohi, off, nm, crange, K, gref);
+        ksm::launch(gram_refs_kernel, (K + 255) / 256, 256, 0, s, rank, lohi, off, nm, crange, K,
+                    gref);
         KSM_LAUNCH_CHECK("gram_refs_kernel");
         ksm::ProfileScope prof("colmax_all", prof_group, s);
         prof.launch(colmax_all_kernel, dim3((T + kFirstCols - 1) / kFirstCols), dim3(kUpdThreads),
@@ -2105,7 +2108,7 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
     HostFlags none_flags;
     HostFlags &hf = may_poll ? host_flags() : none_flags;
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-    const bool capturing =
+    const bool capturing = ksm::recording(s) != nullptr ||
         hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone;
     (void)hipGetLastError();
     const bool poll = hf.host != nullptr && !capturing;
@@ -2137,7 +2140,7 @@ static int peel_run(float *Bt, int ldbt, float *Xres, int ldx, const float *U, c
         }
     }
     KSM_LAUNCH_CHECK("peel loop");
-    peel_finish_kernel<<<1, 1, 0, s>>>(state, info);
+    ksm::launch(peel_finish_kernel, 1, 1, 0, s, state, info);
     KSM_LAUNCH_CHECK("peel_finish_kernel");
     return KSM_OK;
 }
@@ -2153,6 +2156,9 @@ int ksm_peel(float *Bt, int ldbt, float *Xres, int ldx, const float *U, const fl
         return ksm::fail(KSM_ERR_INVALID, "ksm_peel: bad arguments");
     if (!workspace || ws_bytes < ksm_peel_workspace(K, T, max_peels))
         return ksm::fail(KSM_ERR_WORKSPACE, "ksm_peel: workspace too small");
+    if (ksm::recording(ksm::as_stream(stream)))
+        return ksm::fail(KSM_ERR_INVALID,
+                         "ksm_peel: the stream is being recorded (ksm_peel_range records)");
     return peel_run(Bt, ldbt, Xres, ldx, U, W, nm, perm, rank, cc, lohi, off, K, P, C, T, nt, Th,
                     max_peels, flags, bound, st, amps, th_amps, capacity, info, workspace,
                     ksm::as_stream(stream), true, 0, max_peels);

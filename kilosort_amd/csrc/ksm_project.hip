@@ -10,6 +10,7 @@
 #include "ksm_common.h"
 #include "ksm_profile.h"
 #include "ksm_project_internal.h"
+#include "ksm_graph.h"
 
 namespace {
 
@@ -408,20 +409,21 @@ int ksm_project_bounded(const float *X, int ldx, const float *W, const float *U,
     int *counts = cv.take<int>(L.nTm);
     int *nonfinite = cv.take<int>(1);
 
-    set_flag_kernel<<<1, 1, 0, s>>>(nonfinite);
+    ksm::launch(set_flag_kernel, 1, 1, 0, s, nonfinite);
     KSM_LAUNCH_CHECK("set_flag_kernel");
     {
         const int rc = ksm::conv_pc(X, ldx, W, C, T, P, nt, Bc, L.ldc, nonfinite, s);
         if (rc != KSM_OK) return rc;
     }
     if (L.Rp > L.R) {
-        zero_rows_kernel<<<64, 256, 0, s>>>(Bc + (size_t)L.R * L.ldc,
-                                            (size_t)(L.Rp - L.R) * L.ldc);
+        ksm::launch(zero_rows_kernel, 64, 256, 0, s, Bc + (size_t)L.R * L.ldc,
+                    (size_t)(L.Rp - L.R) * L.ldc);
         KSM_LAUNCH_CHECK("zero_rows_kernel");
     }
-    transpose_pad_kernel<<<512, 256, 0, s>>>(U, perm, K, L.R, L.Kp, L.Rp, At);
+    ksm::launch(transpose_pad_kernel, 512, 256, 0, s, U, perm, K, L.R, L.Kp, L.Rp, At);
     KSM_LAUNCH_CHECK("transpose_pad_kernel");
-    tile_steps_kernel<<<L.nTm, 256, 0, s>>>(At, L.Kp, L.nk, W, P * nt, nonfinite, steps, counts);
+    ksm::launch(tile_steps_kernel, L.nTm, 256, 0, s, At, L.Kp, L.nk, W, P * nt, nonfinite, steps,
+                counts);
     KSM_LAUNCH_CHECK("tile_steps_kernel");
     const int nTm = L.nTm, nTn = L.ldc / GBN;
     ksm::ProfileScope prof("gemm_f32", ksm::kProfProject, s);

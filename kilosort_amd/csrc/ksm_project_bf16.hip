@@ -20,6 +20,7 @@
 #include "ksm_profile.h"
 #include "ksm_project_internal.h"
 #include "ksm_bound.h"
+#include "ksm_graph.h"
 
 namespace {
 
@@ -411,7 +412,7 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
     int *nonfinite = cv.take<int>(1);
     const int nblk = (L.ldr / P + kCvCh - 1) / kCvCh;
     float *stats = cv.take<float>((size_t)2 * 2 * nblk * L.Tp);
-    set_flag_h_kernel<<<1, 1, 0, s>>>(nonfinite);
+    ksm::launch(set_flag_h_kernel, 1, 1, 0, s, nonfinite);
     KSM_LAUNCH_CHECK("set_flag_h_kernel");
     {
         ProfileScope prof("conv_pc", kProfProject, s);
@@ -439,14 +440,14 @@ int project_bf16(const float *X, int ldx, const float *W, const float *U, const 
                                     kBoundLinear, kappa, 0.0, 0.0, nullptr, bound, s);
         if (rc != KSM_OK) return rc;
     }
-    u_bf16_kernel<<<512, 256, 0, s>>>(U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
+    ksm::launch(u_bf16_kernel, 512, 256, 0, s, U, perm, K, P, C, L.Kp, L.ldr, Uh, Ul);
     KSM_LAUNCH_CHECK("u_bf16_kernel");
     if (MT == 64)
-        tile_steps_bf16_kernel<64><<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite,
-                                                       steps, counts);
+        ksm::launch(tile_steps_bf16_kernel<64>, nTm, 256, 0, s, Uh, L.ldr, nk, W, P * nt,
+                    nonfinite, steps, counts);
     else
-        tile_steps_bf16_kernel<128><<<nTm, 256, 0, s>>>(Uh, L.ldr, nk, W, P * nt, nonfinite,
-                                                        steps, counts);
+        ksm::launch(tile_steps_bf16_kernel<128>, nTm, 256, 0, s, Uh, L.ldr, nk, W, P * nt,
+                    nonfinite, steps, counts);
     KSM_LAUNCH_CHECK("tile_steps_bf16_kernel");
     Segs sg;
     sg.a[0] = Uh;

@@ -19,6 +19,7 @@
 #include "ksm_profile.h"
 #include "ksm_project_internal.h"
 #include "ksm_bound.h"
+#include "ksm_graph.h"
 
 namespace {
 
@@ -453,11 +454,11 @@ int project_fp8(const float *X, int ldx, const float *W, const float *U, const i
     const int nblk = (L.ldr / P + kCvCh - 1) / kCvCh;
     float *stats = cv.take<float>((size_t)3 * 2 * nblk * L.Tp);
     float *abs_terms = cv.take<float>(2);
-    set_flag_q_kernel<<<1, 1, 0, s>>>(nonfinite);
+    ksm::launch(set_flag_q_kernel, 1, 1, 0, s, nonfinite);
     KSM_LAUNCH_CHECK("set_flag_q_kernel");
-    chan_absmax_kernel<<<C, 256, 0, s>>>(X, ldx, T, amax, nonfinite);
+    ksm::launch(chan_absmax_kernel, C, 256, 0, s, X, ldx, T, amax, nonfinite);
     KSM_LAUNCH_CHECK("chan_absmax_kernel");
-    row_scales_kernel<<<(L.ldr + 255) / 256, 256, 0, s>>>(amax, W, C, P, nt, L.ldr, srow);
+    ksm::launch(row_scales_kernel, (L.ldr + 255) / 256, 256, 0, s, amax, W, C, P, nt, L.ldr, srow);
     KSM_LAUNCH_CHECK("row_scales_kernel");
     {
         ProfileScope prof("conv_pc", kProfProject, s);
@@ -475,7 +476,7 @@ int project_fp8(const float *X, int ldx, const float *W, const float *U, const i
         // max_b Bc^2 (ksm_bound.h); the margin is kFp8Sigmas standard
         // deviations of that, plus the worst-case accumulation and subnormal
         // terms.
-        fp8_abs_terms_kernel<<<1, 256, 0, s>>>(srow, L.ldr, abs_terms);
+        ksm::launch(fp8_abs_terms_kernel, 1, 256, 0, s, srow, L.ldr, abs_terms);
         KSM_LAUNCH_CHECK("fp8_abs_terms_kernel");
         const double e = 0x1p-4;
         const int depth = L.ldr;
@@ -485,10 +486,10 @@ int project_fp8(const float *X, int ldx, const float *W, const float *U, const i
                                     0.0, zc, g * (1 + e) * (1 + e), abs_terms, bound, s);
         if (rc != KSM_OK) return rc;
     }
-    u_fp8_kernel<<<L.Kp, 256, 0, s>>>(U, perm, K, P, C, L.ldr, srow, Uq, uscale);
+    ksm::launch(u_fp8_kernel, L.Kp, 256, 0, s, U, perm, K, P, C, L.ldr, srow, Uq, uscale);
     KSM_LAUNCH_CHECK("u_fp8_kernel");
-    tile_steps_fp8_kernel<<<L.nTm, 256, 0, s>>>(Uq, L.ldr, L.nk, W, P * nt, nonfinite, steps,
-                                                counts);
+    ksm::launch(tile_steps_fp8_kernel, L.nTm, 256, 0, s, Uq, L.ldr, L.nk, W, P * nt, nonfinite,
+                steps, counts);
     KSM_LAUNCH_CHECK("tile_steps_fp8_kernel");
     {
         ProfileScope prof("gemm_fp8", kProfProject, s);

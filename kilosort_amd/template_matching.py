@@ -42,6 +42,7 @@ each; results are identical for any value and come back in batch order.
 """
 from __future__ import annotations
 
+import ctypes
 import logging
 import queue
 import threading
@@ -396,20 +397,42 @@ def _lane_give(lane):
     lane.jobs.put(None)
 
 
-_CAPTURE_LOCK = threading.Lock()
+class _Graph:
+    """An instantiated graph of libksm's recorder."""
+
+    __slots__ = ("lib", "h")
+
+    def __init__(self, lib, h):
+        self.lib, self.h = lib, h
+
+    def launch(self, device):
+        _lib.check(self.lib.ksm_graph_launch(self.h, _lib.stream_of(device)), "ksm_graph_launch")
+
+    def destroy(self):
+        if self.h:
+            h, self.h = self.h, None
+            _lib.check(self.lib.ksm_graph_destroy(h), "ksm_graph_destroy")
+
+    def __del__(self):
+        # a launch still in flight is freed by the runtime when it completes
+        if self.h:
+            self.lib.ksm_graph_destroy(self.h)
+            self.h = None
 
 
 class _GraphStep:
-    """One recording's batches on one lane as replays of captured graphs.
+    """One recording's batches on one lane as launches of recorded graphs.
 
     The batch's work -- residual copy, projection (BF16 forms: with its error
     bound), the first ``peels`` iterations of the peel (``ksm_peel_range``),
     the features of the spikes the device counted (``ksm_features_counted``)
-    and the copy of the peel's info word to pinned host memory -- is captured
-    once into a HIP graph over static buffers and replayed per batch, one
-    launch instead of a hundred.  Where a batch needs more iterations than
-    the graph holds, tail graphs of ``_TAIL`` iterations (plus features)
-    continue the same peel.  The first batch runs the same kernels eagerly
+    and the copy of the peel's info words to pinned host memory -- is
+    recorded once by the library into a HIP graph over static buffers
+    (``ksm_graph_begin``/``ksm_graph_end``: nodes are added directly, no
+    stream capture, so the other lanes' threads are unaffected) and launched
+    per batch, one launch instead of a hundred.  Where a batch needs more
+    iterations than the graph holds, tail graphs of ``_TAIL`` iterations
+    (plus features) continue the same peel.  The first batch runs the same kernels eagerly
     (the peel with its host-polled end) and its iteration count picks
     ``peels`` unless ``ops['matching_graph_peels']`` gives it.  Results are
     those of run_matching + features (same kernels, same order).
@@ -417,7 +440,7 @@ class _GraphStep:
 
     _TAIL = 8
 
-    def __init__(self, ctx, C, T, device, cap_stream):
+    def __init__(self, ctx, C, T, device):
         ops, U = ctx.ops, _check_U(ctx.U, device)
         self.ctx, self.device = ctx, device
         self.lib = lib = _lib.load()
@@ -457,14 +480,14 @@ class _GraphStep:
         self.nC = nC
         self.tF = torch.empty((ST_CAPACITY, nC, P), **f32)
         self.graph = None
-        self.cap_stream = cap_stream
         self.peels = ops.get('matching_graph_peels')
         self.tails = {}
 
     # -- the work of one batch, enqueued on the current stream ---------------
     def _project(self):
         lib, g, stream = self.lib, self.g, _lib.stream_of(self.device)
-        self.Xres.copy_(self.X)
+        _lib.check(lib.ksm_copy_async(self.Xres.data_ptr(), self.X.data_ptr(),
+                                      self.X.numel() * 4, _lib.COPY_D2D, stream), "ksm_copy_async")
         a = (self.X.data_ptr(), self.T, self.W.data_ptr(), self.U.data_ptr(), g.perm.data_ptr(),
              self.C, self.T, self.K, self.P, self.nt, self.Bt.data_ptr(), self.ldbt, self.prec)
         if self.bound is None:
@@ -493,33 +516,33 @@ class _GraphStep:
             self.amps.data_ptr(), self.info.data_ptr(), ST_CAPACITY, c.iCC.data_ptr(),
             c.iU.data_ptr(), c.Ucc.data_ptr(), self.nC, self.K, self.P, self.C, self.nt,
             self.tF.data_ptr(), stream), "ksm_features_counted")
-        self.info_h.copy_(self.info, non_blocking=True)
+        _lib.check(self.lib.ksm_copy_async(self.info_h.data_ptr(), self.info.data_ptr(), 16,
+                                           _lib.COPY_D2H, stream), "ksm_copy_async")
 
     def _range(self, it0, it1):
         _lib.check(self.lib.ksm_peel_range(*self._peel_args(), it0, it1,
                                            _lib.stream_of(self.device)), "ksm_peel_range")
 
     def _capture(self, fn):
-        # One capture at a time in the process (lanes are threads), on the
-        # lane's own capture stream.  Capturing enqueues nothing, and replays
-        # run on the lane's stream after the work that filled the static
-        # buffers, so no device-wide synchronisation precedes the capture.
-        g = torch.cuda.CUDAGraph()
-        with _CAPTURE_LOCK, torch.cuda.stream(self.cap_stream):
-            g.capture_begin(capture_error_mode="thread_local")
-            try:
-                fn()
-            finally:
-                g.capture_end()
-        return g
+        """Records fn's library calls on the current stream into a graph."""
+        lib, stream = self.lib, _lib.stream_of(self.device)
+        _lib.check(lib.ksm_graph_begin(stream), "ksm_graph_begin")
+        h = ctypes.c_void_p()
+        try:
+            fn()
+        except BaseException:
+            lib.ksm_graph_end(stream, ctypes.byref(h))      # discards the recording
+            raise
+        _lib.check(lib.ksm_graph_end(stream, ctypes.byref(h)), "ksm_graph_end")
+        return _Graph(lib, h.value)
 
     def release(self):
-        """Destroys the captured graphs.  On ROCm a graph's destruction
-        synchronises the device, which fails while another thread captures,
-        so it happens under the capture lock."""
-        with _CAPTURE_LOCK:
-            self.graph = None
-            self.tails = {}
+        """Frees the recorded graphs (no launch of them may be pending)."""
+        for g in [self.graph, *self.tails.values()]:
+            if g is not None:
+                g.destroy()
+        self.graph = None
+        self.tails = {}
 
     def _wait(self):
         ev = torch.cuda.Event()
@@ -554,7 +577,7 @@ class _GraphStep:
                 self._features()
             self.graph = self._capture(head)
             return k, iters, self.st[:k], self.th[:k], self.tF[:k]
-        self.graph.replay()
+        self.graph.launch(self.device)
         k, iters, _ = self._wait()
         it = int(self.peels)
         # more iterations than the graph holds: continue the same peel
@@ -567,7 +590,7 @@ class _GraphStep:
                     self._features()
                 tail = self._capture(fn)
                 self.tails[it] = tail
-            tail.replay()
+            tail.launch(self.device)
             k, iters, _ = self._wait()
             it = it1
         return k, iters, self.st[:k], self.th[:k], self.tF[:k]
@@ -585,7 +608,6 @@ class _Lane:
     def __init__(self, device):
         self.device = device
         self.stream = torch.cuda.Stream(device=device)
-        self.cap_stream = _lib.own_stream(device)      # graph captures only
         self.jobs = queue.Queue()
         self.results = queue.Queue()
         self.bufs = [None] * _LANE_SLOTS
@@ -623,8 +645,7 @@ class _Lane:
                         if step is None:
                             # one recording's graph steps at a time per lane
                             self.drop_graphs()
-                            step = _GraphStep(c, X.shape[0], X.shape[1], dev,
-                                              self.cap_stream)
+                            step = _GraphStep(c, X.shape[0], X.shape[1], dev)
                             self.graph_steps = {id(c): step}
                         k, iters, stt, th, xf = step.run(X)
                         xfeat = xf.clone()

@@ -111,3 +111,26 @@ def test_graph_replays_equal_eager_batches():
         np.testing.assert_array_equal(got[idx][1].numpy(), ref[1].numpy())
     ref_o = tm.extract(dict(other[1]), other[2], other[3], device=DEV)
     np.testing.assert_array_equal(got[1][0], ref_o[0])
+
+
+def test_recorder_rejects_misuse():
+    """The graph recorder's error paths: ending without a recording, a second
+    recording on one thread, an empty recording, and ksm_peel (whose host
+    follows the device) on a recorded stream."""
+    import ctypes
+    lib = _lib.load()
+    s = _lib.stream_of(DEV)
+    h = ctypes.c_void_p()
+    assert lib.ksm_graph_end(s, ctypes.byref(h)) == _lib.KSM_ERR_INVALID and not h.value
+    assert lib.ksm_graph_begin(s) == _lib.KSM_OK
+    try:
+        assert lib.ksm_graph_begin(s) == _lib.KSM_ERR_INVALID
+        p = torch.zeros(16, device=DEV).data_ptr()
+        st = lib.ksm_peel(p, 64, p, 6000, p, p, p, p, p, p, p, p, 16, 6, 32, 6000, 61, 8.0,
+                          10, 0, None, p, p, p, 10, p, p, 1 << 30, s)
+        assert st == _lib.KSM_ERR_INVALID
+        assert b"recorded" in lib.ksm_last_error()
+    finally:
+        assert lib.ksm_graph_end(s, ctypes.byref(h)) == _lib.KSM_ERR_INVALID   # nothing recorded
+    assert not h.value
+    torch.cuda.synchronize()
