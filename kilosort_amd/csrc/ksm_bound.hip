@@ -13,6 +13,7 @@
 // The buffer also carries each template's significant channels and its
 // waveforms on them, T_r[c][k] = sum_p U[r,p,c] W[p,k], from which the peel
 // re-scores columns in FP32.
+#include <algorithm>
 #include <cmath>
 
 #include "ksm_bound.h"
@@ -111,52 +112,125 @@ __global__ __launch_bounds__(256) void bound_weights_kernel(
     }
 }
 
-// Per column t: S[b][t], colA[t] and colD[t] = max_r D_r(t) + colA[t].  One
-// wave of 64 columns per block: the column statistics sit in LDS, the
-// template loop is uniform (weights through scalar loads).
-__global__ __launch_bounds__(64) void bound_columns_kernel(
+// Per column t: S[b][t], colA[t] and colD[t] = max_r D_r(t) + colA[t].  A
+// block takes 64 columns: wave 0 reads the column statistics and forms
+// colA; then the four waves split the templates, each lane (column) forming
+// D_r = sum_b Wt[b][r] S[b][t] (summed in b order) with the column's
+// statistics in registers and the template's weights read from the block's
+// LDS copy of Wt (template-major rows padded to 4 floats, read 16 bytes at a
+// time, the same address on every lane), and the waves' maxima are combined
+// at the end.
+constexpr int kColThreads = 256;
+constexpr int kWtTile = 8192;           // floats of Wt staged per tile
+constexpr int kRegSub = 16;             // sub-blocks held in registers
+
+__global__ __launch_bounds__(kColThreads) void bound_columns_kernel(
     const float *__restrict__ stats, int nsb2, int ldst, int T, int K, int nsb, int mode,
     double acc, const float *__restrict__ abs_terms, const float *__restrict__ Wt,
     float *__restrict__ S, float *__restrict__ colA, float *__restrict__ colD) {
-    constexpr int kMaxSub = ksm::kBoundMaxC / kBoundSub;
-    __shared__ float s_v[kMaxSub][64];
-    const int lane = threadIdx.x;
+    extern __shared__ float4 b_smem4[];
+    const int nsbp = (nsb + 3) & ~3;          // padded row of one template's weights
+    float *s_v = reinterpret_cast<float *>(b_smem4);            // [nsb][64]
+    float *s_w = reinterpret_cast<float *>(b_smem4) + nsbp * 64; // [tile][nsbp]
+    __shared__ float s_dm[kColThreads / 64][64];
+    __shared__ int s_bad[64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t = blockIdx.x * 64 + lane;
     const bool live = t < T;
     const int tt = live ? t : T - 1;
-    double s0 = 0.0, s2 = 0.0;
-    bool bad = false;
-    for (int b = 0; b < nsb; ++b) {
-        const float q = stats[(size_t)b * ldst + tt];
-        const float m = stats[((size_t)nsb2 + b) * ldst + tt];
-        bad |= !__builtin_isfinite(q) || !__builtin_isfinite(m);
-        s0 += q;
-        if (mode == ksm::kBoundSqrt) {
-            const float q2 = stats[((size_t)2 * nsb2 + b) * ldst + tt];
-            bad |= !__builtin_isfinite(q2);
-            s2 += q2;
+    double a = 0.0;
+    if (wave == 0) {
+        double s0 = 0.0, s2 = 0.0;
+        bool bad = false;
+        auto sub = [&](int b, float q, float m, float q2) {
+            bad |= !__builtin_isfinite(q) || !__builtin_isfinite(m);
+            s0 += q;
+            if (mode == ksm::kBoundSqrt) {
+                bad |= !__builtin_isfinite(q2);
+                s2 += q2;
+            }
+            // sqrt rounded up by one part in 2^20 (q carries its own float rounding)
+            const float v = mode == ksm::kBoundLinear ? sqrtf(q) * (1.f + 0x1p-20f) : m;
+            s_v[b * 64 + lane] = v;
+            if (live) S[(size_t)b * T + t] = bad ? __builtin_inff() : v;
+        };
+        auto ld = [&](int b, int k) { return stats[((size_t)k * nsb2 + b) * ldst + tt]; };
+        const bool sq = mode == ksm::kBoundSqrt;
+        // the first kRegSub sub-blocks' loads issued together, then the rest
+        float qv[kRegSub], mv[kRegSub], q2v[kRegSub];
+#pragma unroll
+        for (int b = 0; b < kRegSub; ++b) {
+            qv[b] = b < nsb ? ld(b, 0) : 0.f;
+            mv[b] = b < nsb ? ld(b, 1) : 0.f;
+            q2v[b] = b < nsb && sq ? ld(b, 2) : 0.f;
         }
-        // sqrt rounded up by one part in 2^20 (q carries its own float rounding)
-        const float v = mode == ksm::kBoundLinear ? sqrtf(q) * (1.f + 0x1p-20f) : m;
-        s_v[b][lane] = v;
-        if (live) S[(size_t)b * T + t] = bad ? __builtin_inff() : v;
+#pragma unroll
+        for (int b = 0; b < kRegSub; ++b)
+            if (b < nsb) sub(b, qv[b], mv[b], q2v[b]);
+        for (int b = kRegSub; b < nsb; ++b) sub(b, ld(b, 0), ld(b, 1), sq ? ld(b, 2) : 0.f);
+        const double up = 1.0 + 0x1p-18;
+        a = (0x1p-14 + acc) * sqrt(s0 * up);
+        if (mode == ksm::kBoundSqrt && abs_terms)
+            a += (double)abs_terms[0] + (double)abs_terms[1] * sqrt(s2 * up);
+        a *= up;
+        s_bad[lane] = bad ? 1 : 0;
     }
-    const double up = 1.0 + 0x1p-18;
-    double a = (0x1p-14 + acc) * sqrt(s0 * up);
-    if (mode == ksm::kBoundSqrt && abs_terms)
-        a += (double)abs_terms[0] + (double)abs_terms[1] * sqrt(s2 * up);
-    a *= up;
+    __syncthreads();
+    float sv[kRegSub];
+#pragma unroll
+    for (int b = 0; b < kRegSub; ++b) sv[b] = b < nsb ? s_v[b * 64 + lane] : 0.f;
+    const int tile = max(1, min(K, kWtTile / nsbp));
     float dmax = 0.f;
-    for (int r = 0; r < K; ++r) {
-        float d = 0.f;
-        for (int b = 0; b < nsb; ++b) d = __builtin_fmaf(Wt[(size_t)b * K + r], s_v[b][lane], d);
-        dmax = fmaxf(dmax, d);
+    for (int r0 = 0; r0 < K; r0 += tile) {
+        const int n = min(tile, K - r0);
+        __syncthreads();                      // the previous tile is consumed
+        // 16 loads per thread in flight, then their stores (pads stay 0)
+        for (int e0 = tid; e0 < n * nsbp; e0 += 16 * kColThreads) {
+            float v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + u * kColThreads;
+                const int bb = e / n, rr = e % n;   // coalesced over templates
+                v[u] = e < n * nsbp && bb < nsb ? Wt[(size_t)bb * K + r0 + rr] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + u * kColThreads;
+                if (e < n * nsbp) s_w[(e % n) * nsbp + e / n] = v[u];
+            }
+        }
+        __syncthreads();
+        for (int rr = wave; rr < n; rr += kColThreads / 64) {
+            const float4 *w4 = reinterpret_cast<const float4 *>(s_w + rr * nsbp);
+            float d = 0.f;
+            if (nsb <= kRegSub) {
+#pragma unroll
+                for (int b4 = 0; b4 < kRegSub / 4; ++b4) {
+                    if (4 * b4 >= nsb) break;
+                    const float4 w = w4[b4];
+                    const float wv[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (4 * b4 + u < nsb) d = __builtin_fmaf(wv[u], sv[4 * b4 + u], d);
+                }
+            } else {
+                for (int b = 0; b < nsb; ++b)
+                    d = __builtin_fmaf(s_w[rr * nsbp + b], s_v[b * 64 + lane], d);
+            }
+            dmax = fmaxf(dmax, d);
+        }
     }
+    s_dm[wave][lane] = dmax;
+    __syncthreads();
+    if (wave != 0 || !live) return;
+#pragma unroll
+    for (int w = 1; w < kColThreads / 64; ++w) dmax = fmaxf(dmax, s_dm[w][lane]);
+    const double up = 1.0 + 0x1p-18;
     // float sums of nsb nonnegative products: relative rounding < nsb 2^-24
     double dm = (double)dmax * (1.0 + (nsb + 2) * 0x1p-24);
     if (mode == ksm::kBoundSqrt) dm = sqrt(dm);
-    if (!live) return;
     const float inf = __builtin_inff();
+    const bool bad = s_bad[lane] != 0;
     colA[t] = bad ? inf : (float)a * (1.f + 0x1p-20f);
     colD[t] = bad ? inf : (float)((dm + a) * up) * (1.f + 0x1p-20f);
 }
@@ -184,8 +258,11 @@ int bound_finish(const float *stats, int nsb2, int ldst, const float *U, const f
     ksm::launch(bound_weights_kernel, K, 256, 0, s, U, W, perm, C, K, P, nt, L.nsb, mode, kappa,
                 zc, Wt, srng, Ttab);
     KSM_LAUNCH_CHECK("bound_weights_kernel");
-    ksm::launch(bound_columns_kernel, (T + 63) / 64, 64, 0, s, stats, nsb2, ldst, T, K, L.nsb,
-                mode, acc, abs_terms, Wt, S, colA, colD);
+    const int nsbp = (L.nsb + 3) & ~3;
+    const int tile = std::max(1, std::min(K, kWtTile / nsbp));
+    const unsigned col_lds = (unsigned)((nsbp * 64 + tile * nsbp) * sizeof(float));
+    ksm::launch(bound_columns_kernel, (T + 63) / 64, kColThreads, col_lds, s, stats, nsb2, ldst,
+                T, K, L.nsb, mode, acc, abs_terms, Wt, S, colA, colD);
     KSM_LAUNCH_CHECK("bound_columns_kernel");
     return KSM_OK;
 }
