@@ -18,8 +18,9 @@ and reads each spike's lag window from it, so no per-spike windows are kept.
 * ``residual_loss(X, U, W, amps, st, nt)`` -- differentiable loss of one
   padded batch (``torch.autograd.Function``).
 * ``residual(X, U, W, amps, st, nt)`` -- the residual itself.
-* ``refine_step`` / ``averaging_update`` -- one preconditioned gradient step
-  on U, and the feature-averaging update it is compared against.
+* ``refine_step`` / ``refine`` / ``averaging_update`` -- one preconditioned
+  gradient step on U, a sequence of them with backtracking, and the
+  feature-averaging update they are compared against.
 """
 from __future__ import annotations
 
@@ -138,13 +139,8 @@ def residual_loss(X, U, W, amps, st, nt):
     return _ResidualLoss.apply(X, U, W, amps, st, nt)
 
 
-def refine_step(batches, U, W, nt):
-    """One gradient step on U over ``batches`` (a list of (X, st, amps)),
-    preconditioned per template by 1 / (2 sum_s amp_s^2) -- the inverse of
-    the loss's curvature in U[i] when the spikes of template i do not overlap
-    and the rows of W are orthonormal, so that the first step from a template
-    set reproduces the amplitude-weighted average of the spikes' windows.
-    Returns the new U and the loss before the step."""
+def _gradient(batches, U, W, nt):
+    """dloss/dU over ``batches``, the loss, and sum_s amp_s^2 per template."""
     K = U.shape[0]
     U = U.detach().requires_grad_(True)
     total = 0.0
@@ -155,10 +151,53 @@ def refine_step(batches, U, W, nt):
         total += float(loss)
         ids = st[:, 1].to(U.device)
         a2.index_add_(0, ids, amps.reshape(-1).to(U.dtype) ** 2)
-    step = torch.where(a2 > 0, 1.0 / (2.0 * a2), torch.zeros_like(a2))
-    with torch.no_grad():
-        Un = U - step[:, None, None] * U.grad
-    return Un.detach(), total
+    return U.grad, total, a2
+
+
+def _loss(batches, U, W, nt):
+    return sum(float(residual(X, U, W, amps, st, nt)[1]) for X, st, amps in batches)
+
+
+def refine_step(batches, U, W, nt, scale=1.0):
+    """One gradient step on U over ``batches`` (a list of (X, st, amps)),
+    preconditioned per template by scale / (2 sum_s amp_s^2) -- at scale 1
+    the inverse of the loss's curvature in U[i] when the spikes of template i
+    overlap no other spike and the rows of W are orthonormal, so that the
+    first step from a template set reproduces the amplitude-weighted average
+    of the spikes' windows.  Returns the new U and the loss before the step."""
+    G, total, a2 = _gradient(batches, U, W, nt)
+    step = torch.where(a2 > 0, scale / (2.0 * a2), torch.zeros_like(a2))
+    return (U.detach() - step[:, None, None] * G).detach(), total
+
+
+def refine(batches, U, W, nt, steps=20, max_halvings=40):
+    """``steps`` preconditioned gradient steps on U, each at the largest scale
+    2^-j (from the last accepted scale, which is doubled again up to 1 after
+    each step) that does not increase the loss.  Where spikes of different
+    templates overlap, the per-template preconditioner ignores their coupling
+    and the unscaled step can overshoot; the backtracking keeps the loss
+    non-increasing.  Returns U and the losses [before, after step 1, ...]."""
+    U = U.detach()
+    scale = 1.0
+    losses = []
+    for _ in range(steps):
+        G, total, a2 = _gradient(batches, U, W, nt)
+        if not losses:
+            losses.append(total)
+        base = torch.where(a2 > 0, 1.0 / (2.0 * a2), torch.zeros_like(a2))[:, None, None] * G
+        for _ in range(max_halvings):
+            Un = U - scale * base
+            ln = _loss(batches, Un, W, nt)
+            if ln <= total:
+                break
+            scale *= 0.5
+        else:
+            losses.append(total)          # no decrease found: U is kept
+            break
+        U = Un.detach()
+        losses.append(ln)
+        scale = min(1.0, 2.0 * scale)
+    return U, losses
 
 
 def averaging_update(batches, U, W, nt):

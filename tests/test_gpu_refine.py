@@ -134,9 +134,10 @@ def averaging_cpu(batches, U, W, nt):
 
 def test_gradient_refinement_vs_averaging_update():
     """From perturbed templates: the averaging update (device) matches its
-    explicit CPU form, and 20 preconditioned gradient steps reach a residual
-    no larger than it (the first step is that update wherever no window is
-    clipped by the batch edges)."""
+    explicit CPU form, and 20 preconditioned gradient steps (backtracking
+    where a step would raise the loss) reach a residual no larger than it
+    (the first step is that update wherever no window is clipped by the batch
+    edges), the loss never rising from step to step."""
     say("refinement: 2 batches C=64 K=32 NT=20000, 20 steps")
     rec = synth.make_recording(95, 64, 32, 2 * 20000 + 122, rate_range=(10.0, 30.0),
                                overlap_frac=0.2)
@@ -161,10 +162,88 @@ def test_gradient_refinement_vs_averaging_update():
     loss = lambda U: sum(float(refine.residual_loss(X, U, W, am, st, 61))
                          for X, st, am in batches)
     l_avg = loss(Uavg)
-    U = U0
-    for _ in range(20):
-        U, _ = refine.refine_step(batches, U, W, 61)
+    U, hist = refine.refine(batches, U0, W, 61, steps=20)
+    assert all(b <= a for a, b in zip(hist, hist[1:]))
     l_gd = loss(U)
     say(f"loss: initial {loss0:.6g}, averaging {l_avg:.6g}, 20 gradient steps {l_gd:.6g}")
     assert l_avg < loss0
+    assert l_gd <= l_avg * (1 + 1e-6)
+
+
+def test_full_size_gradients_and_refinement():
+    """The differentiation workload at its full size: 8 batches of NT=60000
+    at 383 channels, 500 templates from the 64-channel workload's generator,
+    spike times and ids fixed by a forward FP32 matching pass.  Gradients of
+    the float32 kernels match the float64 kernels over every batch (2e-5 of
+    each output's largest entry); the float64 kernels match the float64
+    autograd oracle on a 6000-column slice of batch 0 (1e-10); and from
+    perturbed templates, 20 gradient steps reach a residual no larger than
+    the averaging update's.  At this density spikes of different templates
+    overlap often, so the unscaled per-template step overshoots (the loss
+    diverges without backtracking); refine halves it where it would raise
+    the loss."""
+    from kilosort_amd import template_matching as tm
+    NT, C, K, NB = 60000, 383, 500, 8
+    say(f"differentiation workload: {NB} batches x NT={NT}, C={C}, K={K}")
+    rec = synth.make_recording(96, C, K, NB * NT + 122, rate_range=(2.0, 20.0),
+                               amp_range=(10.0, 30.0), with_data=False)
+    data = synth.synthesize_torch(rec.n_samples, C, rec.times, rec.ids, rec.amps, rec.U,
+                                  rec.wPCA, rec.nt0min, DEV, seed=96)
+    bf = ArrayBatches(data, NT=NT, nt=61, device=DEV)
+    W = torch.from_numpy(rec.wPCA).to(DEV)
+    Ut = torch.from_numpy(rec.U).to(DEV)
+    ops = dict(nt=61, Th_learned=8, max_peels=100, wPCA=W)
+    ctc = tm.prepare_matching(ops, Ut)
+    batches = []
+    for b in range(NB):
+        X = bf.padded_batch_to_torch(b).contiguous()
+        st, amps, _, _ = tm.run_matching(ops, X, Ut, ctc, device=DEV)
+        batches.append((X, st, amps[:, 0].contiguous()))
+    nsp = sum(len(s) for _, s, _ in batches)
+    say(f"forward FP32 pass: {nsp} spikes")
+    assert nsp > 50000
+    # float32 kernels against float64 kernels, every batch
+    g = torch.Generator().manual_seed(4)
+    U0 = torch.from_numpy(rec.U)
+    U0 = (U0 + 0.3 * U0.abs().max() * torch.randn(U0.shape, generator=g) * (U0 != 0)).to(DEV)
+    worst = 0.0
+    for X, st, am in batches:
+        grads = []
+        for dt in (torch.float32, torch.float64):
+            a = [t.detach().to(dt).clone().requires_grad_(True) for t in (U0, W, am)]
+            refine.residual_loss(X.to(dt), *a, st, 61).backward()
+            grads.append([t.grad.cpu() for t in a])
+        for g32, g64 in zip(*grads):
+            worst = max(worst, rel_err(g32, g64))
+    say(f"float32 vs float64 kernels, worst relative error over batches: {worst:.2e}")
+    assert worst <= 2e-5
+    # float64 kernels against the oracle on a slice of batch 0
+    X, st, am = batches[0]
+    lo, n = 20000, 6000
+    keep = ((st[:, 0] >= lo) & (st[:, 0] < lo + n)).cpu()
+    sts = st.cpu()[keep] - torch.tensor([lo, 0])
+    Xs = X[:, lo:lo + n].double().cpu()
+    args = [t.clone().requires_grad_(True) for t in (Xs, U0.double().cpu(), W.double().cpu(),
+                                                      am.cpu()[keep].double())]
+    _, L64 = RO.residual(*args, sts, 61)
+    L64.backward()
+    dargs = [t.to(DEV).requires_grad_(True) for t in (Xs, U0.double().cpu(), W.double().cpu(),
+                                                       am.cpu()[keep].double())]
+    refine.residual_loss(*dargs, sts.to(DEV), 61).backward()
+    say(f"oracle slice: {int(keep.sum())} spikes")
+    for name, a, o in zip(("dX", "dU", "dW", "damp"), dargs, args):
+        e = rel_err(a.grad.cpu(), o.grad)
+        say(f"  {name}: rel err {e:.2e}")
+        assert e <= 1e-10, name
+    # 20 gradient steps against the averaging update
+    loss = lambda U: sum(float(refine.residual_loss(X, U, W, am, st, 61))
+                         for X, st, am in batches)
+    Uavg = refine.averaging_update(batches, U0, W, 61)
+    l0, l_avg = loss(U0), loss(Uavg)
+    U, hist = refine.refine(batches, U0, W, 61, steps=20)
+    say(f"losses of the 20 steps: {hist[1]:.6g} .. {hist[-1]:.6g}")
+    assert all(b <= a for a, b in zip(hist, hist[1:]))
+    l_gd = loss(U)
+    say(f"residual: initial {l0:.6g}, averaging {l_avg:.6g}, 20 gradient steps {l_gd:.6g}")
+    assert l_avg < l0
     assert l_gd <= l_avg * (1 + 1e-6)
