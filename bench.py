@@ -330,6 +330,7 @@ def time_extract_int16(tm, n_batches, seed, device, U, ops, runs=5):
     rounded to int16, with whitening matrix I / 20.  One untimed call, then the
     median of `runs` calls."""
     from kilosort_amd import synth, preprocess as PPr
+    log(f"extract int16: synthesising {n_batches} batches")
     rec = synth.make_recording(seed, C, K, n_batches * NT,
                                rate_range=(1.0, 20.0), amp_range=(8.0, 40.0),
                                overlap_frac=0.2, with_data=False)
@@ -341,6 +342,7 @@ def time_extract_int16(tm, n_batches, seed, device, U, ops, runs=5):
                              hp_filter=PPr.get_highpass_filter(device=device),
                              whiten_mat=np.eye(C, dtype=np.float32) / 20.0)
     nb = int(fb.n_batches)
+    log(f"extract int16: {nb} batches, one untimed call")
     tm.extract(dict(ops), fb, U, device=device)
     torch.cuda.synchronize()
     times = []
@@ -348,6 +350,7 @@ def time_extract_int16(tm, n_batches, seed, device, U, ops, runs=5):
         t0 = time.perf_counter()
         st, tF, _ = tm.extract(dict(ops), fb, U, device=device)
         times.append(time.perf_counter() - t0)
+        log(f"extract int16: {times[-1]:.2f} s")
     med = float(np.median(times))
     return dict(value=C * NT * nb / med, unit="channel·samples/s", batches=nb,
                 ms_per_batch=med * 1e3 / nb, median_s=med, runs=runs, spikes=int(len(st)),
@@ -754,6 +757,10 @@ def main():
                          "run a kernel profile of the matching path uses")
     ap.add_argument("--templates-only", action="store_true",
                     help="time only template similarity and alignment and print their JSON object")
+    ap.add_argument("--extract-batches", type=int, default=300,
+                    help="batches of the int16 recording extract() streams (10 min at 30 kHz)")
+    ap.add_argument("--extract-only", action="store_true",
+                    help="run only the int16 extract() leg and print its JSON object")
     ap.add_argument("--ragged-only", action="store_true",
                     help="run only the many-recordings comparison and print its JSON object")
     ap.add_argument("--refine-only", action="store_true",
@@ -777,6 +784,12 @@ def main():
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
     lib = _lib.load()
+    if args.extract_only:
+        rec, ops, _ = make_workload(1, args.seed, device)
+        U = torch.from_numpy(rec.U).to(device)
+        print(json.dumps(time_extract_int16(tm, args.extract_batches, args.seed, device, U, ops,
+                                            runs=2)), flush=True)
+        return
     if args.ragged_only:
         jobs = ragged_workload(args.ragged, args.ragged_batches, args.seed + 100, device)
         print(json.dumps(time_ragged(tm, jobs, device)), flush=True)
@@ -883,8 +896,9 @@ def main():
     ext = time_extract(tm, 12, args.seed, device, U, ops)
     log(f"extract: {ext['value']:.3e} channel·samples/s, {ext['ms_per_batch']:.3f} ms/batch "
         f"({ext['batches']} batches, median of {ext['runs']})")
-    log("extract() over a raw int16 recording through the device preprocessing")
-    ext16 = time_extract_int16(tm, 12, args.seed, device, U, ops)
+    log(f"extract() over a raw int16 recording of {args.extract_batches} batches through the "
+        "device preprocessing")
+    ext16 = time_extract_int16(tm, args.extract_batches, args.seed, device, U, ops, runs=2)
     ext["int16_preprocessed"] = ext16
     log(f"extract int16: {ext16['value']:.3e} channel·samples/s, {ext16['ms_per_batch']:.3f} "
         f"ms/batch ({ext16['spikes']} spikes)")
