@@ -5,9 +5,12 @@ batches of NT=60000 (+2*61) samples, Th=8, max_peels=100, synthetic whitened
 data (noise + Poisson spikes with 20 % near-coincident overlaps), resident in
 HBM before the timed region.
 
-A step = one batch through the hot path exactly as ``extract`` runs it:
-run_matching (temporal-PC correlation, projection, peel) + PC features + the
-device-to-host copies of st/th_amps/tF.  Prints one JSON line.
+A step = one batch through the hot path as ``extract`` runs it once the batch
+is in HBM: run_matching (temporal-PC correlation, projection, peel) + PC
+features + the device-to-host copies of st/th_amps; the features stay on the
+device (``extract`` copies them to the host when it assembles ``tF``; the
+``extract_wall`` legs time whole ``extract`` calls, that copy included).
+Prints one JSON line.
 
 Every form (fp32: the parity form; bf16x3, bf16: projection on BF16 MFMA;
 fp8: projection on FP8 e4m3 MFMA) is timed the same way; `value` is the
@@ -125,9 +128,9 @@ def agreement(st, amps, ref):
 def time_form(tm, lib, ops, batches, U, ctc, iCC, iU, Ucc, device, precision, steps,
               warmup, lanes=None):
     """Untimed warmup, then `steps` batches end to end -- matching, features
-    and the host copies of st / th_amps / tF -- through the library's batch
-    pipeline (batches spread over its device streams, one host thread each),
-    with no events in the loop."""
+    (left on the device) and the host copies of st / th_amps -- through the
+    library's batch pipeline (batches spread over its device streams, one
+    host thread each), with no events in the loop."""
     o = dict(ops, matching_precision=precision)
     nb = len(batches)
     pipe = tm._BatchPipeline(o, U, ctc, iCC, iU, Ucc, device, lanes=lanes)

@@ -12,11 +12,19 @@ Modules
                same op order, for float32 (bitwise reference rounding on this
                torch build) and float64 (the FP64 oracle), plus a decision-margin
                (near-tie) report.
-``brute``      brute-force matching pursuit in float64 for tiny inputs: B is
+``brute``      brute-force matching pursuit in float64 for tiny inputs (B is
                recomputed from the residual by a direct sum every iteration
-               instead of being updated through the ctc Gram.
-``closed``     closed forms used to pin the oracle (ctc by direct
-               cross-correlation of full templates, nearest channels, times).
+               instead of being updated through the ctc Gram) and the closed
+               forms that pin the oracle (ctc by direct cross-correlation of
+               full templates).
+``filter_oracle``, ``detect_oracle``, ``templates_oracle``, ``phy_oracle``,
+``positions_oracle``, ``refine_oracle``
+               restatements of the neighbouring steps (preprocessing,
+               universal-template detection, template similarity and merges,
+               the Phy export, spike positions, the refinement loss).
 
-Pinning status of each function is listed in ``oracle/PINNING.txt``.
+Each module's header names what pins it: the golden vectors the reference's
+own code wrote (``tests/golden``), closed forms, brute force on tiny inputs,
+finite differences; the tests in ``tests/test_*oracle*.py`` and
+``tests/test_oracle.py`` run those checks.
 """
