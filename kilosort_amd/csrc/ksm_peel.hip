@@ -1358,31 +1358,26 @@ __device__ __forceinline__ void seg_store(const float4 &v, const SegPass &sp, u6
     acc = key > acc ? key : acc;
 }
 
-// Column t's results from the loads in `in`: the new scores (out0, out1)
-// and the column's best key, stored with the kept keys of the groups the
-// spikes do not reach.
-__device__ __forceinline__ void seg_finish(const ColLoads &in, int t, bool two,
-                                           const SegPass &s0, const SegPass &s1, float amp_a,
-                                           float amp_b, int K, const float *s_nm,
-                                           const int *s_pid, float4 &out0, float4 &out1,
-                                           u64 &k0, u64 &k1, u64 &acc) {
-    out0 = in.v0;
-    k0 = seg_compute(out0, in.va0, in.vb0, s0, amp_a, amp_b, K, s_nm, s_pid);
-    k1 = 0ull;
-    if (two) {
-        out1 = in.v1;
-        k1 = seg_compute(out1, in.va1, in.vb1, s1, amp_a, amp_b, K, s_nm, s_pid);
-    }
-    acc = in.kept;
-}
-
-__device__ __forceinline__ void seg_write(const float4 &out0, const float4 &out1, u64 k0, u64 k1,
-                                          u64 acc, int t, bool two, const SegPass &s0,
-                                          const SegPass &s1, float *__restrict__ Bt, int ldbt,
-                                          int G, u64 *__restrict__ gkey,
-                                          float *__restrict__ cfmax, int *__restrict__ imax) {
-    seg_store(out0, s0, k0, t, Bt, ldbt, G, gkey, acc);
-    if (two) seg_store(out1, s1, k1, t, Bt, ldbt, G, gkey, acc);
+// Column t of a segment, whose loads are in `cur`: compute, issue the next
+// column's loads into `nxt`, store, and write the column's best key.
+__device__ __forceinline__ void seg_column(ColLoads &cur, ColLoads &nxt, int t, int t1,
+                                           long long &ba, long long &bb, int wa, int wb,
+                                           bool two, const SegPass &s0, const SegPass &s1,
+                                           bool keep, const GroupList &L, float amp_a,
+                                           float amp_b, float *__restrict__ Bt, int ldbt, int K,
+                                           int G,
+                                           const float *__restrict__ cc, const float *s_nm,
+                                           const int *s_pid, u64 *__restrict__ gkey,
+                                           float *__restrict__ cfmax, int *__restrict__ imax) {
+    const u64 k0 = seg_compute(cur.v0, cur.va0, cur.vb0, s0, amp_a, amp_b, K, s_nm, s_pid);
+    u64 k1 = 0ull;
+    if (two) k1 = seg_compute(cur.v1, cur.va1, cur.vb1, s1, amp_a, amp_b, K, s_nm, s_pid);
+    ba += wa;
+    bb += wb;
+    if (t < t1) seg_load(nxt, t + 1, ba, bb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
+    u64 acc = cur.kept;
+    seg_store(cur.v0, s0, k0, t, Bt, ldbt, G, gkey, acc);
+    if (two) seg_store(cur.v1, s1, k1, t, Bt, ldbt, G, gkey, acc);
     const u64 best = wave_max_u64(acc);
     if ((threadIdx.x & 63) == 0) {
         cfmax[t] = key_score(best);
@@ -1393,13 +1388,9 @@ __device__ __forceinline__ void seg_write(const float4 &out0, const float4 &out1
 // Columns t0 .. t1 of one segment: fixed covering spikes a (first) and b
 // (either may be absent: empty range), whose Gram rows for column t start at
 // a.base + (t - t0) wa and b.base + (t - t0) wb.  At most 2 passes.  The
-// columns alternate between two load buffers, so that a wave has the next
-// column's loads in flight while it computes one column, and issues the
-// loads of the column after next before that column's stores (a store
-// issued ahead of a load would hold that load's wait on gfx9, whose counter
-// covers both).  The buffers are fixed registers (the loop body takes two
-// columns), so no copy of a buffer waits for its loads.  Columns of one
-// segment are distinct rows of Bt and gkey: no load reads a pending store.
+// columns run as a pipeline: the next column's loads are issued before the
+// current column's stores (a store issued ahead of a load would hold that
+// load's wait on gfx9, whose counter covers both).
 __device__ __forceinline__ void seg_columns(int t0, int t1, const ColSpike &a, const ColSpike &b,
                                             int wa, int wb, const GroupList &L,
                                             float *__restrict__ Bt, int ldbt, int K, int G,
@@ -1413,35 +1404,14 @@ __device__ __forceinline__ void seg_columns(int t0, int t1, const ColSpike &a, c
     if (two) seg_pass(s1, L, kPassGroups, a, b, ldbt);
     else s1 = s0;
     const bool keep = lane < G && !L.has(lane);
-    long long la = a.base, lb = b.base;          // Gram rows of the next column to load
-    ColLoads A, B;
-    seg_load(A, t0, la, lb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
-    la += wa;
-    lb += wb;
-    if (t0 + 1 <= t1) {
-        seg_load(B, t0 + 1, la, lb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
-        la += wa;
-        lb += wb;
-    }
-    float4 o0, o1;
-    u64 k0, k1, acc;
-    for (int t = t0;; t += 2) {
-        seg_finish(A, t, two, s0, s1, a.amp, b.amp, K, s_nm, s_pid, o0, o1, k0, k1, acc);
-        if (t + 2 <= t1) {
-            seg_load(A, t + 2, la, lb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
-            la += wa;
-            lb += wb;
-        }
-        seg_write(o0, o1, k0, k1, acc, t, two, s0, s1, Bt, ldbt, G, gkey, cfmax, imax);
-        if (t + 1 > t1) break;
-        seg_finish(B, t + 1, two, s0, s1, a.amp, b.amp, K, s_nm, s_pid, o0, o1, k0, k1, acc);
-        if (t + 3 <= t1) {
-            seg_load(B, t + 3, la, lb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
-            la += wa;
-            lb += wb;
-        }
-        seg_write(o0, o1, k0, k1, acc, t + 1, two, s0, s1, Bt, ldbt, G, gkey, cfmax, imax);
-        if (t + 2 > t1) break;
+    long long ba = a.base, bb = b.base;
+    ColLoads cur, nxt;
+    seg_load(cur, t0, ba, bb, two, s0, s1, keep, L, Bt, ldbt, cc, G, gkey);
+    for (int t = t0;; ++t) {
+        seg_column(cur, nxt, t, t1, ba, bb, wa, wb, two, s0, s1, keep, L, a.amp, b.amp, Bt, ldbt, K,
+                   G, cc, s_nm, s_pid, gkey, cfmax, imax);
+        if (t >= t1) break;
+        cur = nxt;
     }
 }
 

@@ -126,6 +126,7 @@ class _GramCache:
     channel) and the compact lag-major Gram over that order."""
 
     MAX_ENTRIES = 16          # recordings interleaved by extract_many stay cached
+    MAX_BYTES = 8 << 30       # device memory the cache may hold
 
     def __init__(self):
         self.entries = {}         # key -> _Gram, in insertion (use) order
@@ -185,9 +186,27 @@ class _GramCache:
         g.src = (ctc, U)
         with self.lock:
             self.entries[key] = g
-            while len(self.entries) > self.MAX_ENTRIES:
+            # least recently used first out: at most MAX_ENTRIES entries and
+            # MAX_BYTES of device memory (the newest entry always stays)
+            while len(self.entries) > 1 and (len(self.entries) > self.MAX_ENTRIES or
+                                             self._bytes() > self.MAX_BYTES):
                 self.entries.pop(next(iter(self.entries)))
         return g
+
+    def _bytes(self):
+        """Device bytes the cached entries hold (their copies and tables; a
+        caller's own ctc and U are counted too, as the entry keeps them)."""
+        n = 0
+        for g in self.entries.values():
+            st = {t.untyped_storage().data_ptr(): t.untyped_storage().nbytes()
+                  for t in (g.base, g.rows, g.nrows, g.perm, g.rank, g.lohi, g.off, g.cc, *g.src)
+                  if torch.is_tensor(t)}
+            n += sum(st.values())
+        return n
+
+    def clear(self):
+        with self.lock:
+            self.entries.clear()
 
 
 _grams = _GramCache()
