@@ -283,16 +283,33 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
     const int k00 = nk_seg > 0 ? lst[0] * HBK : 0;
     const bf16_t *ap = sg.a[0] + (size_t)(m0 + arow) * ldr + k00 + acol;
     const bf16_t *bp = sg.b[0] + (size_t)(n0 + grow) * ldr + k00 + gcol;
-    uint4 ra[AE / 8];
-#pragma unroll
-    for (int u = 0; u < AE / 8; ++u) ra[u] = reinterpret_cast<const uint4 *>(ap)[u];
+    // the A registers as named variables (an array here was left in scratch
+    // memory by the compiler): AE / 8 = 2 or 4 of them
+    static_assert(AE / 8 == 2 || AE / 8 == 4, "A loads per thread");
+    uint4 ra0, ra1, ra2, ra3;
+    auto load_a = [&](const uint4 *p) {
+        ra0 = p[0];
+        ra1 = p[1];
+        if constexpr (AE / 8 == 4) {
+            ra2 = p[2];
+            ra3 = p[3];
+        }
+    };
+    auto store_a = [&](uint4 *la) {
+        la[0] = ra0;
+        la[1] = ra1;
+        if constexpr (AE / 8 == 4) {
+            la[2] = ra2;
+            la[3] = ra3;
+        }
+    };
+    load_a(reinterpret_cast<const uint4 *>(ap));
     uint4 rb0 = reinterpret_cast<const uint4 *>(bp)[0], rb1 = reinterpret_cast<const uint4 *>(bp)[1];
     uint4 rb2 = reinterpret_cast<const uint4 *>(bp)[2], rb3 = reinterpret_cast<const uint4 *>(bp)[3];
     {
         uint4 *la = reinterpret_cast<uint4 *>(&As[0][arow * HLD + acol]);
         uint4 *lb = reinterpret_cast<uint4 *>(&Bs[0][grow * HLD + gcol]);
-#pragma unroll
-        for (int u = 0; u < AE / 8; ++u) la[u] = ra[u];
+        store_a(la);
         lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
     }
     __syncthreads();
@@ -306,8 +323,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
             const bf16_t *sb = s == 0 ? sg.b[0] : (s == 1 ? sg.b[1] : sg.b[2]);
             const uint4 *pa = reinterpret_cast<const uint4 *>(sa + (size_t)(m0 + arow) * ldr + k0 + acol);
             const uint4 *pb = reinterpret_cast<const uint4 *>(sb + (size_t)(n0 + grow) * ldr + k0 + gcol);
-#pragma unroll
-            for (int u = 0; u < AE / 8; ++u) ra[u] = pa[u];
+            load_a(pa);
             rb0 = pb[0]; rb1 = pb[1]; rb2 = pb[2]; rb3 = pb[3];
         }
 #pragma unroll
@@ -331,8 +347,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_mfma(Segs sg, int nseg, int 
         if (more) {
             uint4 *la = reinterpret_cast<uint4 *>(&As[buf ^ 1][arow * HLD + acol]);
             uint4 *lb = reinterpret_cast<uint4 *>(&Bs[buf ^ 1][grow * HLD + gcol]);
-#pragma unroll
-            for (int u = 0; u < AE / 8; ++u) la[u] = ra[u];
+            store_a(la);
             lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
         }
         __syncthreads();
