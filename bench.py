@@ -883,8 +883,31 @@ def main():
                            "traffic": measured_traffic(f"{gname}_mfma"),
                            "avg_launch_ms": g["avg_launch_ms"]}
 
+    # per form: the projection GEMM against its MFMA peak and the peel update
+    # against HBM (the update's bytes do not depend on the form)
+    form_rooflines = {}
+    perm = tm._grams.get(ctc, U, NTAP, device).perm
+    for f in forms:
+        kf = prof[f][0]
+        gk = kf.get(GEMM_KERNEL[f])
+        fl, kept_f = executed_gemm_flops(U, perm, f)
+        r = {}
+        if gk:
+            ach = fl / (gk["avg_launch_ms"] / 1e3)
+            r["projection"] = {"kernel": f"{GEMM_KERNEL[f]}_mfma", "bound": "mfma",
+                               "achieved": ach / 1e12, "peak": PEAK_MFMA[f] / 1e12,
+                               "unit": "TFLOP/s", "frac": ach / PEAK_MFMA[f],
+                               "flops_per_launch": fl, "steps_kept": kept_f,
+                               "avg_launch_ms": gk["avg_launch_ms"]}
+        uk = kf.get("peel_update")
+        if uk:
+            ub = peel_update_bytes(prof[f][1], nnzK, nnzC, nprof) / uk["ms_per_batch"] * 1e3
+            r["peel_update"] = {"bound": "hbm", "achieved": ub / 1e9, "peak": PEAK_HBM / 1e9,
+                                "unit": "GB/s", "frac": ub / PEAK_HBM}
+        form_rooflines[f] = r
+
     if args.matching_only:
-        print(json.dumps({"metric": METRIC, "forms": {f: {k: v for k, v in r.items()}
+        print(json.dumps({"metric": METRIC, "forms": {f: dict(r, roofline=form_rooflines[f])
                                                       for f, r in results.items()},
                           "roofline": roofline, "roofline_projection": roofline_projection,
                           "kernels_per_batch": kern}), flush=True)
@@ -981,7 +1004,8 @@ def main():
         "peel_iterations_per_batch": main_res["peel_iterations_per_batch"],
         "forms": {f: dict(value=results[f]["value"], ms_per_step=results[f]["ms_per_step"],
                           kernel_ms_per_batch={k: v["ms_per_batch"]
-                                               for k, v in prof[f][0].items()})
+                                               for k, v in prof[f][0].items()},
+                          roofline=form_rooflines[f])
                   for f in forms},
         "agreement_batch0": agree,
         "refinement": refine_res,
