@@ -26,6 +26,7 @@ products ``Kyx @ iKxx`` and ``M @ Wrot`` run on the device (``ksm_gemm_f32``).
 """
 from __future__ import annotations
 
+import os
 import threading
 
 import numpy as np
@@ -109,6 +110,29 @@ def drift_kernel(ops, dshift):
     return np.exp(-ds / (2 * sig ** 2)).astype(xp.dtype)
 
 
+_copy_pool = None
+_COPY_CHUNK = 4 << 20            # bytes per host copy task
+
+
+def _host_copy(dst, src):
+    """dst[:] = src (1-D uint8 arrays) in chunks over a small thread pool:
+    numpy releases the GIL for the copies, and one host thread copies a
+    batch into pinned memory at a fraction of the memory bandwidth."""
+    global _copy_pool
+    n = src.nbytes
+    if n <= 2 * _COPY_CHUNK:
+        dst[:] = src
+        return
+    if _copy_pool is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _copy_pool = ThreadPoolExecutor(max_workers=max(1, min(8, (os.cpu_count() or 2) // 2)))
+    step = max(_COPY_CHUNK, -(-n // 32))
+    futs = [_copy_pool.submit(np.copyto, dst[a:a + step], src[a:a + step])
+            for a in range(0, n, step)]
+    for f in futs:
+        f.result()
+
+
 class _Staging:
     """Two pinned host buffers used in turn; a buffer is rewritten only after
     its previous host-to-device copy has completed."""
@@ -124,7 +148,7 @@ class _Staging:
         if self.events[k] is not None:
             self.events[k].synchronize()
         src = self.bufs[k][:arr.nbytes]
-        src.numpy()[:] = np.ascontiguousarray(arr).view(np.uint8).reshape(-1)
+        _host_copy(src.numpy(), np.ascontiguousarray(arr).view(np.uint8).reshape(-1))
         with torch.cuda.stream(stream):
             dev_buf[:arr.nbytes].copy_(src, non_blocking=True)
             ev = torch.cuda.Event()
