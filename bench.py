@@ -610,8 +610,9 @@ def time_ragged(tm, jobs, device, precision="bf16", runs=2):
     turn in FP32 (the library's default streams within a recording).
     Batched: all of them through one extract_many in `precision` with
     certified decisions, after the autotuner has picked per recording the
-    projection tile, the peel schedule of the captured graphs and whether
-    batches replay graphs, and the stream count.  Both include each
+    projection tile, the peel schedule of the recorded graphs and whether
+    batches launch graphs, and the stream count; the same pipeline is also
+    timed in FP32 (batched_fp32).  All include each
     recording's preparation (prepare_extract + prepare_matching) and return
     st/tF on the host; times are the best of `runs` after one untimed run.
     Spike (time, template) agreement of the batched form with the sequential
@@ -651,6 +652,15 @@ def time_ragged(tm, jobs, device, precision="bf16", runs=2):
     t_tune = time.perf_counter() - t0
     many(keep=True)
     t_many = min(timed(many) for _ in range(runs))
+    # the same pipeline in FP32 (the autotuner's streams and graph schedules),
+    # which separates the pipeline's effect from the form's cost per batch
+    jobs_f = [(dict(o, matching_precision="fp32"), b, U) for o, b, U in jobs]
+    for (o32, _, _), (ob, _, _) in zip(jobs_f, jobs_b):
+        o32["matching_graph_peels"] = ob.get("matching_graph_peels")
+        o32["matching_graphs"] = ob.get("matching_graphs")
+    t_many32 = min(timed(lambda: tm.extract_many([(dict(o), b, U) for o, b, U in jobs_f],
+                                                 device=device, streams=best))
+                   for _ in range(runs))
     agree = []
     for a_st, b_st in zip(seq_st, got):
         a, b = _spike_pairs(a_st), _spike_pairs(b_st)
@@ -663,6 +673,8 @@ def time_ragged(tm, jobs, device, precision="bf16", runs=2):
             "sequential_precision": "fp32",
             "batched_value": work / t_many, "batched_s": t_many, "batched_precision": precision,
             "batched_streams": best, "speedup": t_seq / t_many,
+            "batched_fp32_value": work / t_many32, "batched_fp32_s": t_many32,
+            "speedup_fp32": t_seq / t_many32,
             "autotune_s": t_tune, "batched_value_with_autotune": work / (t_many + t_tune),
             "autotune_streams_s": {str(k): v for k, v in tried.items()},
             "plans": plans,
@@ -958,7 +970,8 @@ def main():
         torch.cuda.empty_cache()
         log(f"ragged: sequential {ragged['sequential_value']:.3e}, batched "
             f"{ragged['batched_value']:.3e} channel·samples/s on {ragged['batched_streams']} "
-            f"streams ({ragged['speedup']:.2f}x), agreement min "
+            f"streams ({ragged['speedup']:.2f}x; batched fp32 {ragged['batched_fp32_value']:.3e}, "
+            f"{ragged['speedup_fp32']:.2f}x), agreement min "
             f"{100 * ragged['agreement_vs_fp32']['min']:.3f} %")
 
     cpu = None
