@@ -1870,28 +1870,28 @@ __global__ __launch_bounds__(kFeatWaves * 64) void features_kernel(
     const int n = count ? min(uni((int)count[0]), nsp) : nsp;
     for (int s = blockIdx.x * kFeatWaves + (threadIdx.x >> 6); s < n;
          s += gridDim.x * kFeatWaves) {
-    const int x = (int)st[2 * s];
-    const int id = (int)st[2 * s + 1];
-    const int iu = (int)iU[id];
-    for (int r = 0; r < nC; ++r) {
-        const int ch = (int)iCC[(size_t)r * C + iu];
-        const float *xr = Xres + (size_t)ch * ldx + x - nt / 2;
-        for (int k = lane; k < nt; k += 64) s_x[r * nt + k] = xr[k];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const float a = amps[s];
-    for (int j = lane; j < nC * P; j += 64) {
-        const int r = j / P, p = j % P;
-        const float *xs = s_x + r * nt;
-        const float *ws = s_w + p * nt;
-        float f = 0.f;
-        for (int k = 0; k < nt; ++k) f = __builtin_fmaf(xs[k], ws[k], f);
-        const float own = a * Ucc[((size_t)r * K + id) * P + p];
-        tF[((size_t)s * nC + r) * P + p] = f + own;
-    }
-    __builtin_amdgcn_wave_barrier();             // s_x is rewritten for the next spike
+        const int x = (int)st[2 * s];
+        const int id = (int)st[2 * s + 1];
+        const int iu = (int)iU[id];
+        for (int r = 0; r < nC; ++r) {
+            const int ch = (int)iCC[(size_t)r * C + iu];
+            const float *xr = Xres + (size_t)ch * ldx + x - nt / 2;
+            for (int k = lane; k < nt; k += 64) s_x[r * nt + k] = xr[k];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const float a = amps[s];
+        for (int j = lane; j < nC * P; j += 64) {
+            const int r = j / P, p = j % P;
+            const float *xs = s_x + r * nt;
+            const float *ws = s_w + p * nt;
+            float f = 0.f;
+            for (int k = 0; k < nt; ++k) f = __builtin_fmaf(xs[k], ws[k], f);
+            const float own = a * Ucc[((size_t)r * K + id) * P + p];
+            tF[((size_t)s * nC + r) * P + p] = f + own;
+        }
+        __builtin_amdgcn_wave_barrier();             // s_x is rewritten for the next spike
     }
 }
 
