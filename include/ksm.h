@@ -333,11 +333,15 @@ int ksm_copy_async(void *dst, const void *src, size_t bytes, int kind, void *str
  * ("conv_pc", "gemm_f32"), bit 1 = peel ("colmax_all", "peel_select",
  * "peel_update"); 0 turns timing off.  ksm_profile_query synchronises the
  * recorded events and returns the summed milliseconds and launch count of
- * one kernel name.
+ * one kernel name.  ksm_profile_launch returns launch i of every recorded
+ * launch in launch order: its kernel name, its start relative to the first
+ * recorded launch's start and its duration, in milliseconds (the start
+ * offsets are comparable between launches of one stream).
  */
 int ksm_profile_enable(int mask);
 int ksm_profile_reset(void);
 int ksm_profile_query(const char *name, double *total_ms, int *launches);
+int ksm_profile_launch(int i, char *name, int name_cap, double *start_ms, double *dur_ms);
 
 /*
  * Template refinement: the residual of a fixed spike set and its gradients.

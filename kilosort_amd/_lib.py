@@ -124,6 +124,8 @@ SIGNATURES = {
     "ksm_profile_reset": (_i, []),
     "ksm_profile_query": (_i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
                                ctypes.POINTER(ctypes.c_int)]),
+    "ksm_profile_launch": (_i, [_i, ctypes.c_char_p, _i, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)]),
 }
 
 _lib = None
@@ -192,3 +194,21 @@ def profile_query(name: str):
     check(lib.ksm_profile_query(name.encode(), ctypes.byref(ms), ctypes.byref(n)),
           "ksm_profile_query")
     return ms.value, n.value
+
+
+def profile_timeline():
+    """Every recorded launch in launch order: (kernel name, start ms relative
+    to the first launch, duration ms)."""
+    lib = load()
+    out = []
+    name = ctypes.create_string_buffer(64)
+    t0 = ctypes.c_double(0.0)
+    d = ctypes.c_double(0.0)
+    i = 0
+    while True:
+        rc = lib.ksm_profile_launch(i, name, 64, ctypes.byref(t0), ctypes.byref(d))
+        if rc != 0:
+            break
+        out.append((name.value.decode(), t0.value, d.value))
+        i += 1
+    return out

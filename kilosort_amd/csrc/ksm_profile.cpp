@@ -100,4 +100,25 @@ int ksm_profile_query(const char *name, double *total_ms, int *launches) {
     return KSM_OK;
 }
 
+int ksm_profile_launch(int i, char *name, int name_cap, double *start_ms, double *dur_ms) {
+    if (!name || name_cap <= 0 || !start_ms || !dur_ms)
+        return ksm::fail(KSM_ERR_INVALID, "ksm_profile_launch: null argument");
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (i < 0 || (size_t)i >= g_records.size())
+        return ksm::fail(KSM_ERR_INVALID, "ksm_profile_launch: launch %d of %zu", i,
+                         g_records.size());
+    const Record &r = g_records[(size_t)i];
+    hipError_t e = hipEventSynchronize(r.stop);
+    if (e != hipSuccess) return ksm::hip_fail(e, "ksm_profile_launch");
+    float t0 = 0.f, d = 0.f;
+    e = hipEventElapsedTime(&t0, g_records[0].start, r.start);
+    if (e == hipSuccess) e = hipEventElapsedTime(&d, r.start, r.stop);
+    if (e != hipSuccess) return ksm::hip_fail(e, "ksm_profile_launch");
+    std::strncpy(name, r.name.c_str(), (size_t)name_cap - 1);
+    name[name_cap - 1] = '\0';
+    *start_ms = t0;
+    *dur_ms = d;
+    return KSM_OK;
+}
+
 }  // extern "C"
