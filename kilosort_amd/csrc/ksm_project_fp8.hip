@@ -1,5 +1,5 @@
-// FP8 (OCP e4m3) form of the projection B = U * (W (*) X) on
-// v_mfma_f32_32x32x16_fp8_fp8 with FP32 accumulation.
+// FP8 (OCP e4m3) form of the projection B = U * (W (*) X) on gfx950's
+// v_mfma_f32_32x32x64_f8f6f4 (64-deep e4m3 products) with FP32 accumulation.
 //
 // Reference: kilosort/template_matching.py:180-181 (computed there in FP32).
 //
@@ -25,6 +25,7 @@ namespace {
 
 typedef unsigned char fp8_t;
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
 
 constexpr float kE4M3Max = 448.f;
 // standard deviations of the modelled rounding error in the FP8 decision margin
@@ -267,7 +268,7 @@ __device__ __forceinline__ int xcd_remap_q(int bid, int nb) {
 }
 
 // TN GEMM Bout[n][m] = u[m] * sum_r Uq[m][r] Bq[n][r] over the tile's steps.
-// Block tile 128 x 128, 4 waves each 64 x 64 as 2 x 2 tiles of 32x32x16.
+// Block tile 128 x 128, 4 waves each 64 x 64 as 2 x 2 tiles of 32x32x64.
 __global__ __launch_bounds__(256, 2) void gemm_fp8_mfma(
     const fp8_t *__restrict__ Uq, const fp8_t *__restrict__ Bq, int ldr,
     const float *__restrict__ uscale, const int *__restrict__ steps,
@@ -304,7 +305,9 @@ __global__ __launch_bounds__(256, 2) void gemm_fp8_mfma(
         lb[0] = rb0; lb[1] = rb1; lb[2] = rb2; lb[3] = rb3;
     }
     __syncthreads();
-    const int kr = (lane >> 5) * 8, kc = lane & 31;
+    // v_mfma_f32_32x32x64_f8f6f4 operands: lane l holds row l % 32 and the 32
+    // consecutive depth bytes 32 (l / 32) .. + 31 of the 64-byte step
+    const int kr = (lane >> 5) * 32, kc = lane & 31;
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         const bool more = kt + 1 < nk;
@@ -316,22 +319,23 @@ __global__ __launch_bounds__(256, 2) void gemm_fp8_mfma(
             rb0 = pb[0]; rb1 = pb[1]; rb2 = pb[2]; rb3 = pb[3];
         }
 #pragma unroll
-        for (int kk = 0; kk < QBK; kk += 16) {
-            long a[2], b[2];
+        for (int kk = 0; kk < QBK; kk += 64) {
+            i32x8 a[2], b[2];
 #pragma unroll
             for (int i = 0; i < 2; ++i)
-                a[i] = *reinterpret_cast<const long *>(
+                a[i] = *reinterpret_cast<const i32x8 *>(
                     &As[buf][(wm * 64 + i * 32 + kc) * QLD + kk + kr]);
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                b[j] = *reinterpret_cast<const long *>(
+                b[j] = *reinterpret_cast<const i32x8 *>(
                     &Bs[buf][(wn * 64 + j * 32 + kc) * QLD + kk + kr]);
+            // formats 0, 0: both operands e4m3; scales 0: the unscaled form
 #pragma unroll
             for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a[i], b[j], acc[i][j],
-                                                                           0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
+                        a[i], b[j], acc[i][j], 0, 0, 0, 0, 0, 0);
         }
         if (more) {
             uint4 *la = reinterpret_cast<uint4 *>(&As[buf ^ 1][grow * QLD + gcol]);
