@@ -257,4 +257,38 @@ def test_extract_reads_filtered_batches():
     agree = len(a & b) / max(len(a), len(b))
     say(f"{len(st)} spikes vs {len(st_o)}: agreement {100 * agree:.3f} %")
     assert len(st_o) > 500
-    assert agree >= 0.995
+    assert agree >= 0.999
+
+
+def test_extract_reads_filtered_batches_383_channels():
+    """The streamed form at the benchmark's shape (383 channels, 700
+    templates, two batches of NT=60000 from a raw int16 recording): spikes
+    found through FilteredBatches against those found from the oracle's
+    filtered batches.  The filtered inputs differ by float32 rounding
+    (<= 1e-5 relative, test_full_size_383_channels), so a decision can only
+    differ where two scores are that close."""
+    from kilosort_amd import template_matching as tm
+    C, K, NT = 383, 700, 60000
+    rec = synth.make_recording(21, C, K, 2 * NT, rate_range=(1.0, 20.0),
+                               amp_range=(8.0, 40.0), overlap_frac=0.2)
+    rng = np.random.default_rng(21)
+    raw = 20.0 * rec.data + rng.uniform(-300, 300, C)[None, :] \
+        + 30.0 * rng.standard_normal((2 * NT, 1))
+    raw = np.clip(np.round(raw), -32768, 32767).astype(np.int16)
+    W = (np.eye(C) / 20.0).astype(np.float32)
+    fb = PP.FilteredBatches(n_chan_bin=C, NT=NT, file_object=raw, device=DEV, whiten_mat=W,
+                            hp_filter=PP.get_highpass_filter(device=DEV))
+    ops = dict(nt=61, nt0min=rec.nt0min, Th_learned=8, max_peels=100, batch_size=NT,
+               wPCA=torch.from_numpy(rec.wPCA).to(DEV), xc=rec.xc, yc=rec.yc,
+               settings=dict(nearest_chans=10, position_limit=100.0, n_pcs=6))
+    U = torch.from_numpy(rec.U).to(DEV)
+    say("383 ch x 700 templates: extract from FilteredBatches vs oracle-filtered batches")
+    st, tF, _ = tm.extract(dict(ops), fb, U, device=DEV)
+    st_o, tF_o, _ = tm.extract(dict(ops), _OracleFiltered(raw, fb, W), U, device=DEV)
+    a = set(map(tuple, st[:, :2].astype(np.int64).tolist()))
+    b = set(map(tuple, st_o[:, :2].astype(np.int64).tolist()))
+    agree = len(a & b) / max(len(a), len(b))
+    say(f"383ch: {len(st)} spikes vs {len(st_o)}: agreement {100 * agree:.4f} %")
+    assert len(st_o) > 10000
+    assert agree >= 0.999
+
