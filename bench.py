@@ -28,6 +28,18 @@ import os
 import sys
 import time
 
+# HIP hardware queues of this process, read when HIP starts (before torch
+# touches the GPU): at least eight, so that the library's four matching
+# streams and the default stream each have a queue of their own (with HIP's
+# default of four the library runs three streams).  A runtime setting of the
+# caller's process, as extract() users would set it; it changes no result.
+try:
+    _hwq = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+except ValueError:
+    _hwq = 4
+if _hwq < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 import numpy as np
 import torch
 
@@ -1009,7 +1021,8 @@ def main():
                                "Th=8, max_peels=100, Poisson 1-20 Hz, 20% overlaps",
                    "channels": C, "templates": K, "batch_samples": NT,
                    "precision": args.precision, "distinct_batches": n_distinct,
-                   "streams": main_res["streams"]},
+                   "streams": main_res["streams"],
+                   "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))},
         "roofline": roofline,
         "roofline_projection": roofline_projection,
         "kernels_per_batch": kern,

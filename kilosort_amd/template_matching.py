@@ -44,6 +44,7 @@ from __future__ import annotations
 
 import ctypes
 import logging
+import os
 import queue
 import threading
 
@@ -61,8 +62,22 @@ ST_CAPACITY = 100000          # per-batch spike buffer of run_matching (ref :186
 last_peel_iterations = 0
 _iters = threading.local()
 
-# device streams (one host thread each) over which extract runs its batches
-MATCHING_STREAMS = 3
+def _default_streams() -> int:
+    """Device streams (one host thread each) over which extract runs its
+    batches: one per HIP hardware queue the process has beside the default
+    stream's, up to 4.  HIP gives a process GPU_MAX_HW_QUEUES hardware queues
+    (4 unless the environment sets it before HIP starts); streams beyond them
+    share queues and serialise.  Measured at 383 channels x 700 templates:
+    3 streams on 4 queues 2.42 ms per batch, 4 streams on 8 queues 2.32 ms,
+    4 streams on 4 queues 2.64 ms."""
+    try:
+        queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "4"))
+    except ValueError:
+        queues = 4
+    return max(1, min(4, queues - 1))
+
+
+MATCHING_STREAMS = _default_streams()
 
 
 def _lags(nt: int) -> int:
