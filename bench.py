@@ -515,23 +515,42 @@ def time_detect(ops, batches, device, reps=10, cpu_baseline=True):
     return out
 
 
-def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
-    """Template refinement on the workload: spikes and amplitudes from the FP32
-    matching of `nbatch` batches, then the residual loss and its gradients for
-    U, wPCA and the amplitudes (float32 kernels), timed with events."""
-    from kilosort_amd import refine
-    W = ops["wPCA"].to(device)
-    data = []
+def time_refine(tm, device, nbatch=8, reps=3, steps=20, seed=96):
+    """The differentiation workload: `nbatch` batches of NT=60000 at 383
+    channels, 500 templates from the 64-channel workload's generator (Poisson
+    2-20 Hz, amplitudes U(10, 30)), spike times, ids and amplitudes fixed by a
+    forward FP32 matching pass.  Times the residual and the loss with its
+    gradients for U, wPCA and the amplitudes (float32 kernels, events), and
+    `steps` gradient-refinement steps on U from perturbed templates (wall
+    time, line-search evaluations included), with the residual loss they
+    reach beside the averaging update's from the same templates."""
+    from kilosort_amd import refine, synth
+    from kilosort_amd.batches import ArrayBatches
+    Kr = 500
+    rec = synth.make_recording(seed, C, Kr, nbatch * NT + 2 * NTAP, rate_range=(2.0, 20.0),
+                               amp_range=(10.0, 30.0), with_data=False)
+    data = synth.synthesize_torch(rec.n_samples, C, rec.times, rec.ids, rec.amps, rec.U,
+                                  rec.wPCA, rec.nt0min, device, seed=seed)
+    bf = ArrayBatches(data, NT=NT, nt=NTAP, device=device)
+    W = torch.from_numpy(rec.wPCA).to(device)
+    Ut = torch.from_numpy(rec.U).to(device)
+    ops = dict(nt=NTAP, Th_learned=8, max_peels=100, wPCA=W, matching_precision="fp32")
+    ctc = tm.prepare_matching(ops, Ut)
+    data_b = []
     for i in range(nbatch):
-        st, amps, _, _ = tm.run_matching(dict(ops, matching_precision="fp32"),
-                                         batches[i % len(batches)], U, ctc, device=device)
-        data.append((batches[i % len(batches)], st, amps[:, 0].contiguous()))
-    nsp = sum(len(d[1]) for d in data)
+        X = bf.padded_batch_to_torch(i).contiguous()
+        st, amps, _, _ = tm.run_matching(ops, X, Ut, ctc, device=device)
+        data_b.append((X, st, amps[:, 0].contiguous()))
+    del data
+    nsp = sum(len(d[1]) for d in data_b)
+    g = torch.Generator().manual_seed(4)
+    U0 = torch.from_numpy(rec.U)
+    U0 = (U0 + 0.3 * U0.abs().max() * torch.randn(U0.shape, generator=g) * (U0 != 0)).to(device)
 
     def one_pass():
-        Ur = U.detach().requires_grad_(True)
+        Ur = U0.detach().requires_grad_(True)
         Wr = W.detach().requires_grad_(True)
-        for X, st, a in data:
+        for X, st, a in data_b:
             ar = a.detach().requires_grad_(True)
             refine.residual_loss(X, Ur, Wr, ar, st, NTAP).backward()
 
@@ -540,8 +559,8 @@ def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
     e0.record()
     for _ in range(reps):
-        for X, st, a in data:
-            refine.residual(X, U, W, a, st, NTAP)
+        for X, st, a in data_b:
+            refine.residual(X, U0, W, a, st, NTAP)
     e1.record()
     for _ in range(reps):
         one_pass()
@@ -549,11 +568,28 @@ def time_refine(tm, ops, batches, U, ctc, device, nbatch=4, reps=5):
     torch.cuda.synchronize()
     fwd = e0.elapsed_time(e1) / (reps * nbatch)
     both = e1.elapsed_time(e2) / (reps * nbatch)
-    return {"batches": nbatch, "spikes_per_batch": nsp / nbatch, "templates": K,
+
+    def loss(Uc):
+        return sum(float(refine.residual_loss(X, Uc, W, a, st, NTAP)) for X, st, a in data_b)
+
+    loss0 = loss(U0)
+    l_avg = loss(refine.averaging_update(data_b, U0, W, NTAP))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    U_gd, hist = refine.refine(data_b, U0, W, NTAP, steps=steps)
+    torch.cuda.synchronize()
+    t_steps = time.perf_counter() - t0
+    l_gd = loss(U_gd)
+    return {"batches": nbatch, "channels": C, "templates": Kr,
+            "spikes_per_batch": nsp / nbatch,
             "residual_ms_per_batch": fwd, "loss_and_gradients_ms_per_batch": both,
-            "note": "float32; loss = sum of squared residual over a padded batch, gradients "
-                    "for U, wPCA and amplitudes with spike times fixed (backward recomputes "
-                    "the residual)"}
+            "refinement_steps": len(hist) - 1, "refinement_s": t_steps,
+            "loss_initial": loss0, "loss_averaging_update": l_avg, "loss_after_steps": l_gd,
+            "note": "float32; loss = sum over the batches of the squared residual of a padded "
+                    "batch, gradients for U, wPCA and amplitudes with spike times fixed "
+                    "(backward recomputes the residual); refinement from templates perturbed "
+                    "by 0.3 max|U| Gaussian noise on their footprint, each step a "
+                    "preconditioned gradient step with backtracking"}
 
 
 def executed_gemm_flops(U, perm, precision):
@@ -845,7 +881,7 @@ def main():
               flush=True)
         return
     if args.refine_only:
-        print(json.dumps(time_refine(tm, ops, batches, U, ctc, device)), flush=True)
+        print(json.dumps(time_refine(tm, device)), flush=True)
         return
     nnzK = (ctc != 0).any(dim=2).sum(dim=0).cpu().numpy().astype(np.int64)
     nnzC = (U != 0).any(dim=1).sum(dim=1).cpu().numpy().astype(np.int64)
@@ -938,9 +974,13 @@ def main():
         return
 
     log("template refinement (residual loss and gradients)")
-    refine_res = time_refine(tm, ops, batches, U, ctc, device)
-    log(f"refinement: residual {refine_res['residual_ms_per_batch']:.3f} ms, "
-        f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch")
+    refine_res = time_refine(tm, device)
+    log(f"refinement ({refine_res['batches']} x {refine_res['channels']} ch x "
+        f"{refine_res['templates']} templates): residual {refine_res['residual_ms_per_batch']:.3f} ms, "
+        f"loss+gradients {refine_res['loss_and_gradients_ms_per_batch']:.3f} ms per batch, "
+        f"{refine_res['refinement_steps']} steps {refine_res['refinement_s']:.2f} s, loss "
+        f"{refine_res['loss_initial']:.4g} -> {refine_res['loss_after_steps']:.4g} "
+        f"(averaging update {refine_res['loss_averaging_update']:.4g})")
 
     log("template similarity and alignment (700 templates)")
     tmpl = time_templates(ops, U, device, cpu_baseline=not args.no_cpu_baseline)

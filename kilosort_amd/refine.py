@@ -148,7 +148,7 @@ def _gradient(batches, U, W, nt):
     for X, st, amps in batches:
         loss = residual_loss(X, U, W, amps, st, nt)
         loss.backward()
-        total += float(loss)
+        total += float(loss.detach())
         ids = st[:, 1].to(U.device)
         a2.index_add_(0, ids, amps.reshape(-1).to(U.dtype) ** 2)
     return U.grad, total, a2
