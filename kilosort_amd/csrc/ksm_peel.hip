@@ -35,9 +35,12 @@
 //                only the position groups those spikes touch, and folds the
 //                group keys into the column's best score.
 // Once an iteration accepts nothing the done flag makes every later launch
-// return at once, so the host never waits inside the loop.
+// return at once, so results never depend on the host: it follows the
+// device's progress words only to stay at most kLag iterations ahead and to
+// stop enqueuing once the loop has ended.
 #include <chrono>
 #include <mutex>
+#include <thread>
 
 #include "ksm_common.h"
 #include "ksm_bound.h"
@@ -1969,18 +1972,28 @@ HostFlags &host_flags() {
 }
 
 // Waits until the device has started iteration >= want or finished the loop
-// (true: finished).  Gives up after a bounded wait (sets *gave_up).
+// (true: finished).  The host runs kLag iterations ahead, tens of
+// microseconds each, so the wait spins with a pause hint for its first
+// 100 us and then yields the core between polls.  Gives up after a bounded
+// wait (sets *gave_up; the caller then enqueues the remaining iterations,
+// which return at once past the end).
 bool wait_progress(const HostFlags &f, unsigned seq, int want, bool *gave_up) {
     const auto t0 = std::chrono::steady_clock::now();
+    bool yielding = false;
     for (unsigned spin = 0;; ++spin) {
         const u64 done = __atomic_load_n(f.host + 1, __ATOMIC_ACQUIRE);
         if ((unsigned)done == seq) return true;
         const u64 pr = __atomic_load_n(f.host, __ATOMIC_ACQUIRE);
         if ((unsigned)(pr >> 32) == seq && (int)(unsigned)pr >= want) return false;
-        if ((spin & 1023) == 1023 &&
-            std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
-            *gave_up = true;
-            return false;
+        if (yielding) std::this_thread::yield();
+        else __builtin_ia32_pause();
+        if ((spin & 255) == 255) {
+            const auto waited = std::chrono::steady_clock::now() - t0;
+            if (waited > std::chrono::seconds(2)) {
+                *gave_up = true;
+                return false;
+            }
+            yielding = waited > std::chrono::microseconds(100);
         }
     }
 }
