@@ -597,11 +597,18 @@ int ksm_pc_map(float *X, const int64_t *rows, long long nrows, int R, int P, con
  * (float64) = the sum of tF[s][j][p] over its spikes and counts[t] their
  * number; presence (may be NULL) [n_groups][K] = 1 where label g holds a
  * spike of template t.  tF float32 [n][nC][P]; template ids int64 in
- * [0, K) and labels in [0, n_groups), kept in range by the caller.
+ * [0, K) and labels in [0, n_groups), kept in range by the caller;
+ * n < 2^31, K <= 16384.  workspace: ksm_template_sums_workspace(n, K) bytes of device
+ * memory (a counting sort of the spikes by template: each feature row is
+ * read once and summed by its template's block, no floating-point atomics).
  */
 int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *labels, long long n,
                       int nC, int P, int K, int n_groups, double *sums, uint64_t *counts,
-                      uint8_t *presence, void *stream);
+                      uint8_t *presence, void *workspace, size_t ws_bytes, void *stream);
+
+/* Device scratch bytes ksm_template_sums needs for n spikes of K templates
+ * (its counting sort of the spikes by template). */
+size_t ksm_template_sums_workspace(long long n, int K);
 
 /*
  * The per-cluster channel ranking of make_pc_features (postprocessing.py:

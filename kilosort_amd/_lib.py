@@ -114,7 +114,9 @@ SIGNATURES = {
     "ksm_ccg": (_i, [_p, ctypes.c_longlong, _p, ctypes.c_longlong, _i, ctypes.c_double, _p,
                      _p]),
     "ksm_pc_map": (_i, [_p, _p, ctypes.c_longlong, _i, _i, _p, _p]),
-    "ksm_template_sums": (_i, [_p, _p, _p, ctypes.c_longlong, _i, _i, _i, _i, _p, _p, _p, _p]),
+    "ksm_template_sums": (_i, [_p, _p, _p, ctypes.c_longlong, _i, _i, _i, _i, _p, _p, _p, _p,
+                               _sz, _p]),
+    "ksm_template_sums_workspace": (_sz, [ctypes.c_longlong, _i]),
     "ksm_pc_channel_rank": (_i, [_p, _p, _p, _p, _i, _i, _i, _i, _i, _p, _p, _p, _p]),
     "ksm_gather_features": (_i, [_p, _p, ctypes.c_longlong, _i, _i, _p, _p, _i, _p]),
     "ksm_detect_workspace": (_sz, [_i, _i, _i, _i]),

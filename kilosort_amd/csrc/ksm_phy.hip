@@ -312,21 +312,146 @@ __global__ void pc_map_kernel(float *__restrict__ X, const int64_t *__restrict__
     }
 }
 
-__global__ void template_sums_kernel(const float *__restrict__ tF, const int64_t *__restrict__ tmpl,
-                                     const int32_t *__restrict__ lab, long long n, int R, int K,
-                                     double *__restrict__ sums,
-                                     unsigned long long *__restrict__ cnt,
-                                     uint8_t *__restrict__ presence) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n * R) return;
-    const long long s = e / R;
-    const int r = (int)(e - s * R);
-    const long long t = tmpl[s];
-    atomicAdd(&sums[(size_t)t * R + r], (double)tF[e]);
-    if (r == 0) {
-        atomicAdd(&cnt[t], 1ull);
-        if (presence) presence[(size_t)lab[s] * K + t] = 1;
+// Per-template sums of the spikes' feature rows, by a counting sort of the
+// spikes by template and one block per template summing its spikes' rows in
+// float64 -- each feature row is read once, with no floating-point atomics.
+// The sort: each of NB blocks takes a contiguous range of spikes and counts
+// them per template in LDS (H[b][t]); per template, the column prefix over
+// the blocks (O[b][t]) and the total; the exclusive prefix of the totals
+// over templates (off[t]); then each block places its spikes at off[t] +
+// O[b][t] + their rank in the block (an LDS cursor).  float64 sums of
+// float32 values are exact while the values' exponents and the count span
+// fewer than 29 bits, so the order of a template's spikes within a block
+// (set by the LDS cursor) does not change the sums.
+constexpr int kTsThreads = 1024;
+constexpr int kTsMaxK = 16384;              // LDS: one int per template (64 KB)
+
+__global__ __launch_bounds__(kTsThreads) void ts_hist_kernel(const int64_t *__restrict__ tmpl,
+                                                             long long n, long long per, int K,
+                                                             int *__restrict__ H) {
+    extern __shared__ int s_h[];
+    for (int k = threadIdx.x; k < K; k += kTsThreads) s_h[k] = 0;
+    __syncthreads();
+    const long long s0 = (long long)blockIdx.x * per, s1 = min(n, s0 + per);
+    for (long long s = s0 + threadIdx.x; s < s1; s += kTsThreads) atomicAdd(&s_h[tmpl[s]], 1);
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += kTsThreads) H[(size_t)blockIdx.x * K + k] = s_h[k];
+}
+
+// O[b][t] = sum of H[b'][t] over b' < b, tot[t] = the sum over all blocks:
+// a wave per template, lanes = blocks, 64 at a time (a shuffle scan).
+__global__ __launch_bounds__(256) void ts_columns_kernel(const int *__restrict__ H, int NB, int K,
+                                                         int *__restrict__ O,
+                                                         int *__restrict__ tot) {
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= K) return;                        // wave-uniform
+    int run = 0;
+    for (int b0 = 0; b0 < NB; b0 += 64) {
+        const int b = b0 + lane;
+        const int h = b < NB ? H[(size_t)b * K + t] : 0;
+        int incl = h;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        if (b < NB) O[(size_t)b * K + t] = run + incl - h;
+        run += __shfl(incl, 63, 64);
     }
+    if (lane == 0) tot[t] = run;
+}
+
+// off[t] = spikes of templates before t (off[K] = n).  One block of 1024
+// threads, each taking consecutive templates.
+__global__ __launch_bounds__(1024) void ts_scan_kernel(const int *__restrict__ cnt, int K,
+                                                       int *__restrict__ off) {
+    __shared__ int s_w[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int per = (K + 1023) / 1024;
+    const int k0 = tid * per, k1 = min(K, k0 + per);
+    int mine = 0;
+    for (int k = k0; k < k1; ++k) mine += cnt[k];
+    int incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int i = 0; i < w; ++i) base += s_w[i];
+    int run = base + incl - mine;
+    for (int k = k0; k < k1; ++k) {
+        off[k] = run;
+        run += cnt[k];
+    }
+    if (tid == 1023) off[K] = base + incl;
+}
+
+__global__ __launch_bounds__(kTsThreads) void ts_place_kernel(const int64_t *__restrict__ tmpl,
+                                                              long long n, long long per, int K,
+                                                              const int *__restrict__ off,
+                                                              const int *__restrict__ O,
+                                                              int *__restrict__ list) {
+    extern __shared__ int s_c[];
+    for (int k = threadIdx.x; k < K; k += kTsThreads) s_c[k] = 0;
+    __syncthreads();
+    const long long s0 = (long long)blockIdx.x * per, s1 = min(n, s0 + per);
+    const int *ob = O + (size_t)blockIdx.x * K;
+    for (long long s = s0 + threadIdx.x; s < s1; s += kTsThreads) {
+        const int t = (int)tmpl[s];
+        list[off[t] + ob[t] + atomicAdd(&s_c[t], 1)] = (int)s;
+    }
+}
+
+// Block t: sums[t][r] = sum of tF[s][r] over the spikes s of template t
+// (lanes = r, up to 64 at a time; the block's 8 waves take every 8th spike,
+// 8 rows in flight per wave, spike indices staged in LDS), the 8 partial
+// sums added in wave order; counts[t] and presence[labels[s]][t].
+constexpr int kTsWaves = 8;
+constexpr int kTsStage = 2048;
+__global__ __launch_bounds__(kTsWaves * 64) void ts_reduce_kernel(
+    const float *__restrict__ tF, const int *__restrict__ list, const int *__restrict__ off,
+    const int32_t *__restrict__ labels, int R, int K, double *__restrict__ sums,
+    unsigned long long *__restrict__ counts, uint8_t *__restrict__ presence) {
+    __shared__ int s_list[kTsStage];
+    __shared__ double s_part[kTsWaves][64];
+    const int t = blockIdx.x;
+    const int b = off[t], e = off[t + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int r0 = 0; r0 < R; r0 += 64) {
+        const int r = r0 + lane;
+        const bool in = r < R;
+        const int rr = in ? r : 0;
+        double acc = 0.0;
+        for (int c0 = b; c0 < e; c0 += kTsStage) {
+            const int cn = min(kTsStage, e - c0);
+            __syncthreads();                       // s_list of the previous stage is read
+            for (int i = tid; i < cn; i += kTsWaves * 64) s_list[i] = list[c0 + i];
+            __syncthreads();
+            int i = w;
+            for (; i + 7 * kTsWaves < cn; i += 8 * kTsWaves) {
+                float v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = tF[(size_t)s_list[i + u * kTsWaves] * R + rr];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc += (double)v[u];
+            }
+            for (; i < cn; i += kTsWaves) acc += (double)tF[(size_t)s_list[i] * R + rr];
+        }
+        s_part[w][lane] = in ? acc : 0.0;
+        __syncthreads();
+        if (w == 0 && in) {
+            double tot = 0.0;
+            for (int k = 0; k < kTsWaves; ++k) tot += s_part[k][lane];
+            sums[(size_t)t * R + r] = tot;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) counts[t] = (unsigned long long)(e - b);
+    if (presence)
+        for (int i = b + tid; i < e; i += kTsWaves * 64)
+            presence[(size_t)labels[list[i]] * K + t] = 1;
 }
 
 __global__ void gather_features_kernel(const float *__restrict__ tF,
@@ -635,23 +760,56 @@ int ksm_pc_map(float *X, const int64_t *rows, long long nrows, int R, int P, con
     return KSM_OK;
 }
 
+namespace {
+// Blocks of the counting sort for n spikes (each a contiguous range).
+int ts_blocks(long long n) {
+    return (int)std::max<long long>(1, std::min<long long>(256, (n + 8191) / 8192));
+}
+}  // namespace
+
+size_t ksm_template_sums_workspace(long long n, int K) {
+    if (n < 0 || K <= 0) return 0;
+    const size_t NB = (size_t)ts_blocks(n);
+    return ksm::align_up(NB * K * 4, 256) * 2 + ksm::align_up((size_t)K * 4, 256) +
+           ksm::align_up(((size_t)K + 1) * 4, 256) + ksm::align_up((size_t)n * 4, 256) + 256;
+}
+
 int ksm_template_sums(const float *tF, const int64_t *tmpl, const int32_t *labels, long long n,
                       int nC, int P, int K, int n_groups, double *sums, uint64_t *counts,
-                      uint8_t *presence, void *stream) {
-    if (n < 0 || nC <= 0 || P <= 0 || K <= 0 || !sums || !counts ||
-        (presence && (n_groups <= 0 || (double)n_groups * K > (double)(1LL << 30))) ||
+                      uint8_t *presence, void *workspace, size_t ws_bytes, void *stream) {
+    if (n < 0 || n >= (1LL << 31) || nC <= 0 || P <= 0 || K <= 0 || K > kTsMaxK || !sums ||
+        !counts || (presence && (n_groups <= 0 || (double)n_groups * K > (double)(1LL << 30))) ||
         (n > 0 && (!tF || !tmpl || (presence && !labels))))
-        return ksm::fail(KSM_ERR_INVALID, "ksm_template_sums: bad arguments");
+        return ksm::fail(KSM_ERR_INVALID, "ksm_template_sums: bad arguments (K <= %d)", kTsMaxK);
+    if (!workspace || ws_bytes < ksm_template_sums_workspace(n, K))
+        return ksm::fail(KSM_ERR_WORKSPACE, "ksm_template_sums: workspace too small");
     hipStream_t s = ksm::as_stream(stream);
     const int R = nC * P;
-    int rc = zero_async(sums, (size_t)K * R * sizeof(double), s);
-    if (rc == KSM_OK) rc = zero_async(counts, (size_t)K * sizeof(uint64_t), s);
-    if (rc == KSM_OK && presence) rc = zero_async(presence, (size_t)n_groups * K, s);
-    if (rc != KSM_OK || n == 0) return rc;
-    const long long total = n * R;
-    template_sums_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(
-        tF, tmpl, labels, n, R, K, sums, reinterpret_cast<unsigned long long *>(counts), presence);
-    KSM_LAUNCH_CHECK("template_sums_kernel");
+    const int NB = ts_blocks(n);
+    const long long per = (n + NB - 1) / NB;
+    ksm::Carver cv(workspace);
+    int *H = cv.take<int>((size_t)NB * K);
+    int *O = cv.take<int>((size_t)NB * K);
+    int *tot = cv.take<int>(K);
+    int *off = cv.take<int>((size_t)K + 1);
+    int *list = cv.take<int>((size_t)n);
+    if (presence) {
+        const int rc = zero_async(presence, (size_t)n_groups * K, s);
+        if (rc != KSM_OK) return rc;
+    }
+    const size_t lds = (size_t)K * sizeof(int);
+    ts_hist_kernel<<<NB, kTsThreads, lds, s>>>(tmpl, n, per, K, H);
+    KSM_LAUNCH_CHECK("ts_hist_kernel");
+    ts_columns_kernel<<<(K + 3) / 4, 256, 0, s>>>(H, NB, K, O, tot);
+    KSM_LAUNCH_CHECK("ts_columns_kernel");
+    ts_scan_kernel<<<1, 1024, 0, s>>>(tot, K, off);
+    KSM_LAUNCH_CHECK("ts_scan_kernel");
+    ts_place_kernel<<<NB, kTsThreads, lds, s>>>(tmpl, n, per, K, off, O, list);
+    KSM_LAUNCH_CHECK("ts_place_kernel");
+    ts_reduce_kernel<<<(unsigned)K, kTsWaves * 64, 0, s>>>(
+        tF, list, off, labels, R, K, sums, reinterpret_cast<unsigned long long *>(counts),
+        presence);
+    KSM_LAUNCH_CHECK("ts_reduce_kernel");
     return KSM_OK;
 }
 

@@ -154,9 +154,12 @@ def make_pc_features(ops, spike_templates, spike_clusters, tF, device=None):
     counts = torch.empty(K, dtype=torch.int64, device=device)
     presence = torch.empty((G, K), dtype=torch.uint8, device=device)
     st = _lib.stream_of(device)
+    nws = lib.ksm_template_sums_workspace(n, K)
+    ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=device)
     _lib.check(lib.ksm_template_sums(tFd.data_ptr(), tmpl.data_ptr(), lab.data_ptr(), n, nC, P,
                                      K, G, sums.data_ptr(), counts.data_ptr(),
-                                     presence.data_ptr(), st), "ksm_template_sums")
+                                     presence.data_ptr(), ws.data_ptr(), nws, st),
+               "ksm_template_sums")
     sigma = torch.empty((K, nC), dtype=torch.int32, device=device)
     sig_tmp = torch.empty_like(sigma)
     fi = torch.zeros((max(clusters.size, 1), nC), dtype=torch.int32, device=device)
