@@ -7,7 +7,10 @@
 //   B[k][c][t] = sum_m wTEMP[k, m] X[c, t + m - nt/2]          (conv_pc, FP32)
 //   A[i, k, f, t] = sum_j weigh[i, j, f] B[k][iC[j, f]][t]       (score_kernel)
 //   As[f, t] = max_{q = i nk + k} |A|, first q on ties;
-//   Im[f, t] = (1 + q) sign(A[q])
+//   Im[f, t] = (1 + q) sign(A[q]) -- stored by the score pass for other
+//     shapes than the reference's defaults; at the defaults the score pass
+//     keeps only As and the detected spikes' q and sign are recomputed
+//     (emit_one), which spares the pass its index bookkeeping
 //   Amaxs[f, t] = max over the nC2 nearest templates f' = iC2[n, f] and over
 //     |dt| <= nt0 of As[f', t + dt], where columns t + dt outside
 //     [nt, T - nt) count as 0 (the reference zeroes them before its
@@ -117,41 +120,52 @@ __global__ __launch_bounds__(256) void score_kernel(
             for (int j = 0; j < NCH; ++j) cr[j] = s_c[g][j] - cmin;
         }
         if constexpr (NS > 0 && NCH > 0 && NK > 0) {
-            // the NK x NCH correlations of the template's channels in
-            // registers; sizes outer, waveforms inner (q = i NK + k
-            // increasing, so a strictly larger |A| keeps the first maximum);
-            // one size's weights at a time from LDS
-            float b[NK][NCH];
+            // channels outer: for channel j, its NK correlations and the NS
+            // weights, folded into the NS x NK sums (each sum takes the
+            // channels in order, as a chain of FMAs from 0).  Only the
+            // maximum |A| is kept (NaN skipped, as a strictly larger |A|
+            // would skip it); the index and sign of its first occurrence are
+            // recomputed for the detected spikes alone (emit_one)
+            float acc[NS][NK];
 #pragma unroll
-            for (int k = 0; k < NK; ++k) {
-                if (staged) {
+            for (int i = 0; i < NS; ++i)
 #pragma unroll
-                    for (int j = 0; j < NCH; ++j) b[k][j] = s_b[(k * span + cr[j]) * kScoreT + u];
-                } else {
+                for (int k = 0; k < NK; ++k) acc[i][k] = 0.f;
+            auto fold = [&](int j, const float (&bj)[NK]) {
+                float wj[NS];
 #pragma unroll
-                    for (int j = 0; j < NCH; ++j)
-                        b[k][j] = B[((size_t)k * C + cr[j] + cmin) * ldc + t];
+                for (int i = 0; i < NS; ++i) wj[i] = s_w[g][i * NCH + j];
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) acc[i][k] = __builtin_fmaf(wj[i], bj[k], acc[i][k]);
+            };
+            // two channels' loads in flight at a time: unrolling all NCH
+            // hoists every load and costs occupancy
+            if (staged) {
+#pragma unroll 2
+                for (int j = 0; j < NCH; ++j) {
+                    const int cj = s_c[g][j] - cmin;
+                    float bj[NK];
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) bj[k] = s_b[(k * span + cj) * kScoreT + u];
+                    fold(j, bj);
+                }
+            } else {
+#pragma unroll 2
+                for (int j = 0; j < NCH; ++j) {
+                    const int cj = s_c[g][j];
+                    float bj[NK];
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) bj[k] = B[((size_t)k * C + cj) * ldc + t];
+                    fold(j, bj);
                 }
             }
 #pragma unroll
-            for (int i = 0; i < NS; ++i) {
-                float w[NCH];
+            for (int i = 0; i < NS; ++i)
 #pragma unroll
-                for (int j = 0; j < NCH; ++j) w[j] = s_w[g][i * NCH + j];
-#pragma unroll
-                for (int k = 0; k < NK; ++k) {
-                    float a = 0.f;
-#pragma unroll
-                    for (int j = 0; j < NCH; ++j) a = __builtin_fmaf(w[j], b[k][j], a);
-                    const bool take = fabsf(a) > best;
-                    best = take ? fabsf(a) : best;
-                    bq = take ? i * NK + k : bq;
-                    bval = take ? a : bval;
-                }
-            }
-            const int sg = bval > 0.f ? 1 : (bval < 0.f ? -1 : 0);
+                for (int k = 0; k < NK; ++k) best = fmaxf(best, fabsf(acc[i][k]));
             As[(size_t)f * ldc + t] = best;
-            Im[(size_t)f * ldc + t] = (int16_t)((1 + bq) * sg);
             continue;
         }
         for (int k = 0; k < nk; ++k) {
@@ -197,6 +211,182 @@ __global__ __launch_bounds__(256) void score_kernel(
         const int sg = bval > 0.f ? 1 : (bval < 0.f ? -1 : 0);
         As[(size_t)f * ldc + t] = best;
         Im[(size_t)f * ldc + t] = (int16_t)((1 + bq) * sg);
+    }
+}
+
+// The score pass at compile-time sizes (the reference's defaults): as
+// score_kernel, but a block keeps its group of kScoreG positions over a run
+// of `tiles` column tiles of kScoreT.  The group's weights and channel table
+// are set up once; its distinct channels (at most kTileU, found through a
+// channel bitmap, so that a group where the position order wraps across the
+// probe stages too) are staged per tile in LDS, and the next tile's rows are
+// loaded into registers while the current tile is scored.  Each sum takes
+// the channels in order as a chain of FMAs from 0 (channels outer, NS x NK
+// sums); only the maximum |A| is kept (NaN skipped, as a strictly larger |A|
+// would skip it), the index and sign of its first occurrence being
+// recomputed for the detected spikes alone (emit_one).
+// Compute units of the current device (cached per host thread and device).
+int device_cus() {
+    static thread_local int c_dev = -1, c_cus = 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if (dev != c_dev || c_cus <= 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        c_dev = dev;
+        c_cus = cus;
+    }
+    return c_cus;
+}
+
+constexpr int kTileU = 32;        // distinct channels a staged group may use
+constexpr int kTileBits = 4096;   // channel bitmap: C <= kTileBits
+template <int NS, int NCH, int NK>
+__global__ __launch_bounds__(256) void score_tiles_kernel(
+    const float *__restrict__ B, int C, int ldc, const int32_t *__restrict__ iC,
+    const float *__restrict__ weigh, int Nfilt, int T, int tiles, float *__restrict__ As) {
+    constexpr int kPf = (NK * kTileU * kScoreT + 255) / 256;   // prefetch registers
+    __shared__ float s_b[NK * kTileU * kScoreT];
+    __shared__ float s_w[kScoreG][NS * NCH];
+    __shared__ int s_slot[kScoreG][NCH];   // staged: slot of (g, j); else its channel
+    __shared__ int s_ch[kTileU];
+    __shared__ unsigned s_bits[kTileBits / 32];
+    __shared__ int s_pre[kTileBits / 32];
+    __shared__ int s_nu;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int f0 = blockIdx.x * kScoreG;
+    for (int e = tid; e < kTileBits / 32; e += 256) s_bits[e] = 0u;
+    __syncthreads();
+    for (int e = tid; e < kScoreG * NCH; e += 256) {
+        const int g = e / NCH, j = e - g * NCH, f = f0 + g;
+        if (f < Nfilt) {
+            const int c = iC[(size_t)j * Nfilt + f];
+            atomicOr(&s_bits[c >> 5], 1u << (c & 31));
+        }
+    }
+    for (int e = tid; e < kScoreG * NS * NCH; e += 256) {
+        const int g = e / (NS * NCH), r = e - g * (NS * NCH);
+        const int i = r / NCH, j = r - i * NCH, f = f0 + g;
+        s_w[g][r] = f < Nfilt ? weigh[((size_t)i * NCH + j) * Nfilt + f] : 0.f;
+    }
+    __syncthreads();
+    if (tid < 64) {                          // slots: prefix of the words' popcounts
+        int run = 0;
+        for (int w0 = 0; w0 < kTileBits / 32; w0 += 64) {
+            const int pc = __popc(s_bits[w0 + lane]);
+            int incl = pc;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += v;
+            }
+            s_pre[w0 + lane] = run + incl - pc;
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) s_nu = run;
+    }
+    __syncthreads();
+    const int nu = s_nu;
+    const bool staged = nu <= kTileU;        // block-uniform
+    for (int e = tid; e < kScoreG * NCH; e += 256) {
+        const int g = e / NCH, j = e - g * NCH, f = f0 + g;
+        int sl = -1;
+        if (f < Nfilt) {
+            const int c = iC[(size_t)j * Nfilt + f];
+            const int slot = s_pre[c >> 5] + __popc(s_bits[c >> 5] & ((1u << (c & 31)) - 1u));
+            sl = staged ? slot : c;
+            if (staged) s_ch[slot] = c;      // every writer of a slot stores the same c
+        }
+        s_slot[g][j] = sl;
+    }
+    __syncthreads();
+    const int u = tid & (kScoreT - 1), slot_t = tid / kScoreT;
+    const int ntiles = (T + kScoreT - 1) / kScoreT;
+    const int tile0 = blockIdx.y * tiles, tile1 = min(ntiles, tile0 + tiles);
+    // staging map: thread tid holds column tid % 32 of rows r = tid / 32 + 8 m
+    // (m < kPf), row r = k kTileU + slot, so k = m / 4 and slot = 8 (m % 4) +
+    // tid / 32; rows with slot >= nu are not loaded
+    static_assert(kTileU == 32 && kScoreT == 32 && kPf == NK * 4, "staging map");
+    const int uu = tid & 31, r0 = tid >> 5;
+    int chq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) chq[q] = (staged && 8 * q + r0 < nu) ? s_ch[8 * q + r0] : -1;
+    float pf[kPf];
+    auto load = [&](int tile) {
+        const int t = tile * kScoreT + uu;
+#pragma unroll
+        for (int m = 0; m < kPf; ++m) {
+            const int c = chq[m & 3];
+            pf[m] = (c >= 0 && t < ldc) ? B[((size_t)(m >> 2) * C + c) * ldc + t] : 0.f;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int m = 0; m < kPf; ++m)
+            if (chq[m & 3] >= 0) s_b[((m >> 2) * kTileU + 8 * (m & 3) + r0) * kScoreT + uu] = pf[m];
+    };
+    if (staged && tile0 < tile1) {
+        load(tile0);
+        store();
+    }
+    __syncthreads();
+    for (int tile = tile0; tile < tile1; ++tile) {
+        const bool more = staged && tile + 1 < tile1;     // block-uniform
+        if (more) load(tile + 1);
+        const int t = tile * kScoreT + u;
+        if (t < T) {
+#pragma unroll 1
+            for (int g = slot_t; g < kScoreG; g += 256 / kScoreT) {
+                const int f = f0 + g;
+                if (f >= Nfilt) break;
+                float acc[NS][NK];
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) acc[i][k] = 0.f;
+                auto fold = [&](int j, const float (&bj)[NK]) {
+                    float wj[NS];
+#pragma unroll
+                    for (int i = 0; i < NS; ++i) wj[i] = s_w[g][i * NCH + j];
+#pragma unroll
+                    for (int i = 0; i < NS; ++i)
+#pragma unroll
+                        for (int k = 0; k < NK; ++k)
+                            acc[i][k] = __builtin_fmaf(wj[i], bj[k], acc[i][k]);
+                };
+                // two channels' loads in flight at a time: unrolling all NCH
+                // hoists every load and costs occupancy
+                if (staged) {
+#pragma unroll 2
+                    for (int j = 0; j < NCH; ++j) {
+                        const int sl = s_slot[g][j];
+                        float bj[NK];
+#pragma unroll
+                        for (int k = 0; k < NK; ++k) bj[k] = s_b[(k * kTileU + sl) * kScoreT + u];
+                        fold(j, bj);
+                    }
+                } else {
+#pragma unroll 2
+                    for (int j = 0; j < NCH; ++j) {
+                        const int c = s_slot[g][j];
+                        float bj[NK];
+#pragma unroll
+                        for (int k = 0; k < NK; ++k) bj[k] = B[((size_t)k * C + c) * ldc + t];
+                        fold(j, bj);
+                    }
+                }
+                float best = -1.f;
+#pragma unroll
+                for (int i = 0; i < NS; ++i)
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) best = fmaxf(best, fabsf(acc[i][k]));
+                As[(size_t)f * ldc + t] = best;
+            }
+        }
+        __syncthreads();                     // the tile's rows are read
+        if (more) store();
+        __syncthreads();
     }
 }
 
@@ -303,16 +493,51 @@ __global__ __launch_bounds__(1024) void row_scan_kernel(const int *__restrict__ 
 
 // The outputs of spike s = (f, t) except its position, written by the whole
 // wave (lane j < nC: adist and y; lane j * P + p: feature p of channel j).
+// The index q of the first maximum |A| and the sign of that A come from Im,
+// or, where the score pass kept only the maximum (Im null), are recomputed
+// here: lane q = i nk + k forms A[i, k] with the score pass's operands in
+// its order (so bit for bit the same value), NaN never chosen, and the
+// lanes' first maximum (largest |A|, then smallest q) is the sequential
+// scan's "strictly larger |A|" choice.
 __device__ __forceinline__ void emit_one(
     int f, int t, int s, const float *__restrict__ As, const int16_t *__restrict__ Im,
+    const float *__restrict__ weigh, int nsizes,
     const float *__restrict__ B, int C, int ldc, const float *__restrict__ X, int ldx, int T,
     const int32_t *__restrict__ iC, int nC, int Nfilt, int nk, const float *__restrict__ yc,
     const float *__restrict__ wPCA, int P, int nt, int h, int cap, int32_t *__restrict__ imax,
     float *__restrict__ amp, float *__restrict__ adist, float *__restrict__ yct,
     float *__restrict__ tF, int lane) {
-    const int im = Im[(size_t)f * ldc + t];
-    const int q = (im < 0 ? -im : im) - 1;
-    const float sg = im > 0 ? 1.f : (im < 0 ? -1.f : 0.f);
+    int q;
+    float sg;
+    if (Im) {
+        const int im = Im[(size_t)f * ldc + t];
+        q = (im < 0 ? -im : im) - 1;
+        sg = im > 0 ? 1.f : (im < 0 ? -1.f : 0.f);
+    } else {
+        float a = 0.f, key = -2.f;
+        int ql = 64;
+        if (lane < nsizes * nk) {
+            const int i = lane / nk, kk = lane - i * nk;
+            for (int j = 0; j < nC; ++j) {
+                const int c = iC[(size_t)j * Nfilt + f];
+                a = __builtin_fmaf(weigh[((size_t)i * nC + j) * Nfilt + f],
+                                   B[((size_t)kk * C + c) * ldc + t], a);
+            }
+            key = a != a ? -1.f : fabsf(a);
+            ql = lane;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const float ok = __shfl_xor(key, o, 64);
+            const int oq = __shfl_xor(ql, o, 64);
+            if (ok > key || (ok == key && oq < ql)) {
+                key = ok;
+                ql = oq;
+            }
+        }
+        q = ql;
+        const float bv = __shfl(a, q, 64);
+        sg = bv > 0.f ? 1.f : (bv < 0.f ? -1.f : 0.f);
+    }
     const int k = q % nk;
     float ad = 0.f, yj = 0.f;
     if (lane < nC) {
@@ -349,7 +574,7 @@ __device__ __forceinline__ void emit_one(
 // the position xy[s] written by emit_kernel.
 __global__ __launch_bounds__(256) void spike_features_kernel(
     const int *__restrict__ count, const float *__restrict__ As, const int16_t *__restrict__ Im,
-    const float *__restrict__ B, int C, int ldc, const float *__restrict__ X, int ldx, int T,
+    const float *__restrict__ weigh, int nsizes, const float *__restrict__ B, int C, int ldc, const float *__restrict__ X, int ldx, int T,
     const int32_t *__restrict__ iC, int nC, int Nfilt, int nk, const float *__restrict__ yc,
     const float *__restrict__ wPCA, int P, int nt, int cap, const int32_t *__restrict__ xy,
     int32_t *__restrict__ imax, float *__restrict__ amp, float *__restrict__ adist,
@@ -358,8 +583,8 @@ __global__ __launch_bounds__(256) void spike_features_kernel(
     const int n = min(*count, cap);
     for (int s = blockIdx.x * 4 + (threadIdx.x >> 6); s < n; s += gridDim.x * 4) {
         const int f = xy[2 * (size_t)s], t = xy[2 * (size_t)s + 1];
-        emit_one(f, t, s, As, Im, B, C, ldc, X, ldx, T, iC, nC, Nfilt, nk, yc, wPCA, P, nt,
-                 nt / 2, cap, imax, amp, adist, yct, tF, lane);
+        emit_one(f, t, s, As, Im, weigh, nsizes, B, C, ldc, X, ldx, T, iC, nC, Nfilt, nk, yc, wPCA,
+                 P, nt, nt / 2, cap, imax, amp, adist, yct, tF, lane);
     }
 }
 
@@ -577,6 +802,8 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
     int *cnt = cv.take<int>(Nfilt);
     int *nonfinite = cv.take<int>(1);
 
+    // the reference's defaults take the score pass that keeps only max |A|
+    const bool fast = nsizes == 5 && nC == 10 && nk == 6 && C <= kTileBits;
     clear_int_kernel<<<1, 1, 0, s>>>(nonfinite);
     KSM_LAUNCH_CHECK("clear_int_kernel");
     int rc = ksm::conv_pc(X, ldx, wTEMP, C, T, nk, nt, B, L.ldc, nonfinite, s);
@@ -586,11 +813,20 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
         const int max_span = kScoreLds / (nk * kScoreT * (int)sizeof(float));
         const unsigned shm = (unsigned)(nk * max_span * kScoreT * sizeof(float));
         const dim3 grid((Nfilt + kScoreG - 1) / kScoreG, (T + kScoreT - 1) / kScoreT);
-        if (nsizes == 5 && nC == 10 && nk == 6)          // the reference's defaults
-            prof.launch(score_kernel<5, 10, 6>, grid, dim3(256), shm,
-                        static_cast<const float *>(B), C, L.ldc, iC, nC, weigh, nsizes, nk, Nfilt,
-                        T, max_span, As, Im);
-        else
+        if (fast) {                                     // the reference's defaults
+            // as many blocks as are resident at once, each over a run of tiles
+            const int gx = (Nfilt + kScoreG - 1) / kScoreG;
+            const int ntiles = (T + kScoreT - 1) / kScoreT;
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &per_cu, score_tiles_kernel<5, 10, 6>, 256, 0) != hipSuccess || per_cu <= 0)
+                per_cu = 2;
+            const int gy = std::max(1, std::min(ntiles, per_cu * device_cus() / gx));
+            const int tiles = (ntiles + gy - 1) / gy;
+            prof.launch(score_tiles_kernel<5, 10, 6>, dim3(gx, (ntiles + tiles - 1) / tiles),
+                        dim3(256), 0u, static_cast<const float *>(B), C, L.ldc, iC, weigh, Nfilt,
+                        T, tiles, As);
+        } else
             prof.launch(score_kernel<0, 0, 0>, grid, dim3(256), shm,
                         static_cast<const float *>(B), C, L.ldc, iC, nC, weigh, nsizes, nk, Nfilt,
                         T, max_span, As, Im);
@@ -625,7 +861,8 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
         const int nblk = (min(cap, 16384) + 3) / 4;
         prof.launch(spike_features_kernel, dim3(nblk), dim3(256), 0u,
                     static_cast<const int *>(count), static_cast<const float *>(As),
-                    static_cast<const int16_t *>(Im), static_cast<const float *>(B), C, L.ldc, X,
+                    static_cast<const int16_t *>(fast ? nullptr : Im), weigh, nsizes,
+                    static_cast<const float *>(B), C, L.ldc, X,
                     ldx, T, iC, nC, Nfilt, nk, yc, wPCA, P, nt, cap,
                     static_cast<const int32_t *>(xy), imax, amp, adist, yct, tF);
         KSM_LAUNCH_CHECK("spike_features_kernel");
