@@ -45,18 +45,16 @@ constexpr int kMaxNt0 = 256;
 // B[k][cmin..cmax][tile] is staged once in LDS and every template of the
 // group reads its nC channels from there (a group whose range exceeds
 // max_span channels reads B from global memory instead).  Thread: one column, the
-// group's templates slot, slot + 8, ...
-// NS, NCH, NK > 0: nsizes, nC and nk fixed at compile time (the template's
-// NK x NCH correlations live in registers); 0: the sizes passed at run time.
-template <int NS, int NCH, int NK>
+// group's templates slot, slot + 8, ...  Sizes passed at run time; the
+// reference's defaults take score_tiles_kernel below.
 __global__ __launch_bounds__(256) void score_kernel(
     const float *__restrict__ B, int C, int ldc, const int32_t *__restrict__ iC, int nC_,
     const float *__restrict__ weigh, int nsizes_, int nk, int Nfilt, int T, int max_span,
     float *__restrict__ As, int16_t *__restrict__ Im) {
-    const int nC = NCH > 0 ? NCH : nC_;
-    const int nsizes = NS > 0 ? NS : nsizes_;
-    constexpr int MC = NCH > 0 ? NCH : kMaxNC;
-    constexpr int MS = NS > 0 ? NS : kMaxSizes;
+    const int nC = nC_;
+    const int nsizes = nsizes_;
+    constexpr int MC = kMaxNC;
+    constexpr int MS = kMaxSizes;
     extern __shared__ float s_b[];                  // [nk][span][kScoreT], span <= max_span
     __shared__ float s_w[kScoreG][kMaxW];
     __shared__ int s_c[kScoreG][kMaxNC];
@@ -114,60 +112,6 @@ __global__ __launch_bounds__(256) void score_kernel(
         if (f >= Nfilt) break;
         float best = -1.f, bval = 0.f;
         int bq = 0;
-        int cr[NCH > 0 ? NCH : 1];
-        if constexpr (NS > 0 && NCH > 0) {
-#pragma unroll
-            for (int j = 0; j < NCH; ++j) cr[j] = s_c[g][j] - cmin;
-        }
-        if constexpr (NS > 0 && NCH > 0 && NK > 0) {
-            // channels outer: for channel j, its NK correlations and the NS
-            // weights, folded into the NS x NK sums (each sum takes the
-            // channels in order, as a chain of FMAs from 0).  Only the
-            // maximum |A| is kept (NaN skipped, as a strictly larger |A|
-            // would skip it); the index and sign of its first occurrence are
-            // recomputed for the detected spikes alone (emit_one)
-            float acc[NS][NK];
-#pragma unroll
-            for (int i = 0; i < NS; ++i)
-#pragma unroll
-                for (int k = 0; k < NK; ++k) acc[i][k] = 0.f;
-            auto fold = [&](int j, const float (&bj)[NK]) {
-                float wj[NS];
-#pragma unroll
-                for (int i = 0; i < NS; ++i) wj[i] = s_w[g][i * NCH + j];
-#pragma unroll
-                for (int i = 0; i < NS; ++i)
-#pragma unroll
-                    for (int k = 0; k < NK; ++k) acc[i][k] = __builtin_fmaf(wj[i], bj[k], acc[i][k]);
-            };
-            // two channels' loads in flight at a time: unrolling all NCH
-            // hoists every load and costs occupancy
-            if (staged) {
-#pragma unroll 2
-                for (int j = 0; j < NCH; ++j) {
-                    const int cj = s_c[g][j] - cmin;
-                    float bj[NK];
-#pragma unroll
-                    for (int k = 0; k < NK; ++k) bj[k] = s_b[(k * span + cj) * kScoreT + u];
-                    fold(j, bj);
-                }
-            } else {
-#pragma unroll 2
-                for (int j = 0; j < NCH; ++j) {
-                    const int cj = s_c[g][j];
-                    float bj[NK];
-#pragma unroll
-                    for (int k = 0; k < NK; ++k) bj[k] = B[((size_t)k * C + cj) * ldc + t];
-                    fold(j, bj);
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < NS; ++i)
-#pragma unroll
-                for (int k = 0; k < NK; ++k) best = fmaxf(best, fabsf(acc[i][k]));
-            As[(size_t)f * ldc + t] = best;
-            continue;
-        }
         for (int k = 0; k < nk; ++k) {
             float b[MC];
             // the two sources in separate loops: a select between an LDS and a
@@ -176,19 +120,13 @@ __global__ __launch_bounds__(256) void score_kernel(
 #pragma unroll
                 for (int j = 0; j < MC; ++j)
                     if (j < nC) {
-                        int c;
-                        if constexpr (NS > 0 && NCH > 0) c = cr[j];
-                        else c = s_c[g][j] - cmin;
-                        b[j] = s_b[(k * span + c) * kScoreT + u];
+                        b[j] = s_b[(k * span + s_c[g][j] - cmin) * kScoreT + u];
                     }
             } else {
 #pragma unroll
                 for (int j = 0; j < MC; ++j)
                     if (j < nC) {
-                        int c;
-                        if constexpr (NS > 0 && NCH > 0) c = cr[j] + cmin;
-                        else c = s_c[g][j];
-                        b[j] = B[((size_t)k * C + c) * ldc + t];
+                        b[j] = B[((size_t)k * C + s_c[g][j]) * ldc + t];
                     }
             }
 #pragma unroll
@@ -827,7 +765,7 @@ int ksm_detect(const float *X, int ldx, int C, int T, const float *wTEMP, int nk
                         dim3(256), 0u, static_cast<const float *>(B), C, L.ldc, iC, weigh, Nfilt,
                         T, tiles, As);
         } else
-            prof.launch(score_kernel<0, 0, 0>, grid, dim3(256), shm,
+            prof.launch(score_kernel, grid, dim3(256), shm,
                         static_cast<const float *>(B), C, L.ldc, iC, nC, weigh, nsizes, nk, Nfilt,
                         T, max_span, As, Im);
     }

@@ -71,6 +71,20 @@ def test_template_match_64_channels():
     assert n > 100
 
 
+def test_template_match_other_sizes():
+    """Other nearest-channel and template-size counts than the defaults (8
+    channels, 3 sizes): the score pass for sizes given at run time, which
+    stores the first maximum's index and sign for every position."""
+    rec = synth.make_recording(63, 64, 48, 30000 + 122, rate_range=(5.0, 20.0))
+    ops = make_ops(rec, 30000)
+    ops["settings"] = dict(ops["settings"], nearest_chans=8, template_sizes=3)
+    iC, iC2, weigh = DO.setup(dict(ops))
+    assert iC.shape[0] == 8 and weigh.shape[0] == 3
+    X = ArrayBatches(rec.data, NT=30000, nt=61, device=CPU).padded_batch_to_torch(0)
+    n = check_batch(X, ops, iC, iC2, weigh, "C=64, 8 channels, 3 sizes")
+    assert n > 100
+
+
 def test_template_match_383_channels():
     """The bench's geometry: 383 channels, 1149 template positions."""
     rec = synth.make_recording(62, 383, 256, 60000 + 122, rate_range=(2.0, 10.0))
