@@ -178,6 +178,46 @@ def phy_case(seed=11, C=64, K=30, nC=10, n=30000, n_clu=20, fs=30000.0):
     return st, clu.astype(np.int32), tF, Wall, probe, ops
 
 
+@pytest.mark.parametrize("n,K,G", [(200000, 700, 500), (1000, 50, 7), (0, 30, 4)])
+def test_template_sums_exact(n, K, G):
+    """ksm_template_sums against numpy: features that are multiples of 2^-8
+    below 2^7 in magnitude sum exactly in float64 in any order, so the sums
+    must equal numpy's bit for bit; counts and the label x template presence
+    exactly; templates without spikes give zero rows."""
+    lib = _lib.load()
+    rng = np.random.default_rng(n + K)
+    nC, P = 10, 6
+    tF = (rng.integers(-2 ** 15, 2 ** 15, size=(n, nC, P)) / 256.0).astype(np.float32)
+    tmpl = rng.integers(0, K - 3, size=n).astype(np.int64)      # the last 3 templates empty
+    lab = rng.integers(0, G, size=n).astype(np.int32)
+    say(f"template sums n={n} K={K} G={G}")
+    tFd = torch.from_numpy(tF).to(DEV)
+    td = torch.from_numpy(tmpl).to(DEV)
+    ld = torch.from_numpy(lab).to(DEV)
+    sums = torch.full((K, nC, P), float("nan"), dtype=torch.float64, device=DEV)
+    counts = torch.full((K,), -1, dtype=torch.int64, device=DEV)
+    presence = torch.full((G, K), 7, dtype=torch.uint8, device=DEV)
+    nws = lib.ksm_template_sums_workspace(n, K)
+    ws = torch.empty(max(nws, 1), dtype=torch.uint8, device=DEV)
+    _lib.check(lib.ksm_template_sums(tFd.data_ptr(), td.data_ptr(), ld.data_ptr(), n, nC, P, K,
+                                     G, sums.data_ptr(), counts.data_ptr(),
+                                     presence.data_ptr(), ws.data_ptr(), nws,
+                                     _lib.stream_of(DEV)), "ksm_template_sums")
+    ref = np.zeros((K, nC, P), np.float64)
+    np.add.at(ref, tmpl, tF.astype(np.float64))
+    np.testing.assert_array_equal(sums.cpu().numpy(), ref)
+    np.testing.assert_array_equal(counts.cpu().numpy(), np.bincount(tmpl, minlength=K))
+    pres = np.zeros((G, K), np.uint8)
+    pres[lab, tmpl] = 1
+    np.testing.assert_array_equal(presence.cpu().numpy(), pres)
+    # a workspace one byte short is refused
+    if nws > 256:
+        st = lib.ksm_template_sums(tFd.data_ptr(), td.data_ptr(), ld.data_ptr(), n, nC, P, K, G,
+                                   sums.data_ptr(), counts.data_ptr(), presence.data_ptr(),
+                                   ws.data_ptr(), nws - 1, _lib.stream_of(DEV))
+        assert st == _lib.KSM_ERR_WORKSPACE
+
+
 def test_save_to_phy_matches_oracle(tmp_path):
     from kilosort_amd import phy as PHY
     from oracle import positions_oracle as PO
