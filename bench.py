@@ -954,11 +954,21 @@ def main():
         r = {}
         if gk:
             ach = fl / (gk["avg_launch_ms"] / 1e3)
+            # the same launch against HBM: its algorithmic bytes are the
+            # correlation operand read once in the form's storage plus the
+            # FP32 scores written
+            Kf, Pf, Cf = U.shape
+            Tb = int(batches[0].shape[1])
+            op_bytes = {"fp32": 4, "bf16": 2, "bf16x3": 4, "fp8": 1}[f] * Pf * Cf * Tb
+            hb = (op_bytes + 4 * Kf * Tb) / (gk["avg_launch_ms"] / 1e3)
             r["projection"] = {"kernel": f"{GEMM_KERNEL[f]}_mfma", "bound": "mfma",
                                "achieved": ach / 1e12, "peak": PEAK_MFMA[f] / 1e12,
                                "unit": "TFLOP/s", "frac": ach / PEAK_MFMA[f],
                                "flops_per_launch": fl, "steps_kept": kept_f,
-                               "avg_launch_ms": gk["avg_launch_ms"]}
+                               "avg_launch_ms": gk["avg_launch_ms"],
+                               "hbm": {"bytes_per_launch": op_bytes + 4 * Kf * Tb,
+                                       "achieved": hb / 1e9, "peak": PEAK_HBM / 1e9,
+                                       "unit": "GB/s", "frac": hb / PEAK_HBM}}
         uk = kf.get("peel_update")
         if uk:
             ub = peel_update_bytes(prof[f][1], nnzK, nnzC, nprof) / uk["ms_per_batch"] * 1e3
