@@ -126,7 +126,8 @@ def pc_case(seed, n, K, C, nC, n_clu, shared=0.3):
 
 @pytest.mark.parametrize("n,K,C,nC,n_clu,shared", [(3000, 12, 32, 4, 8, 0.3),
                                                    (20000, 40, 64, 10, 30, 0.3),
-                                                   (20000, 40, 64, 10, 30, 0.0)])
+                                                   (20000, 40, 64, 10, 30, 0.0),
+                                                   (60000, 700, 383, 10, 200, 0.1)])
 def test_make_pc_features_matches_oracle(n, K, C, nC, n_clu, shared):
     """Channel slots whose ranked norm is exactly 0 (all-zero rows left by an
     earlier cluster) are ties that the reference orders by torch's unstable
