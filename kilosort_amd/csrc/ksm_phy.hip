@@ -483,11 +483,19 @@ __global__ void gather_features_kernel(const float *__restrict__ tF,
 constexpr int kRankThreads = 1024;
 constexpr int kMaxRankC = 2048;                 // channels a thread's candidates cover
 
-__device__ __forceinline__ void better(double &v, int &c, double v2, int c2) {
-    if (v2 > v || (v2 == v && c2 < c)) {
-        v = v2;
-        c = c2;
-    }
+constexpr int kPer = (kMaxRankC + kRankThreads - 1) / kRankThreads;   // channels per thread
+
+// (v2, c2) comes before (v, c): larger norm, or the same norm and a lower
+// channel
+__device__ __forceinline__ int first(double v2, int c2, double v, int c) {
+    return (v2 > v || (v2 == v && c2 < c)) ? 1 : 0;
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int m) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), m);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), m);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 __global__ __launch_bounds__(kRankThreads) void pc_rank_kernel(
@@ -567,64 +575,55 @@ __global__ __launch_bounds__(kRankThreads) void pc_rank_kernel(
         }
         __syncthreads();
         // top nC channels by (norm descending, channel ascending), a strict
-        // order: each wave takes its own top nC by shuffles, then wave 0 takes
-        // the top nC of those candidates
+        // order (NaN norms rank below every channel, above none): each wave
+        // ranks its channels by counting, over its lanes read by readlane,
+        // those that come first, and keeps ranks < nC in order; then wave 0
+        // ranks those candidates the same way
         {
-            constexpr int kPer = (kMaxRankC + kRankThreads - 1) / kRankThreads;
-            double lv[kPer];
-            int lc[kPer];
+            const int nwr = C > kRankThreads ? kRankThreads / 64 : (C + 63) / 64;
+            if (wv < nwr) {                      // wave-uniform
+                double lv[kPer];
+                int lc[kPer];
 #pragma unroll
-            for (int u = 0; u < kPer; ++u) {
-                const int c = tid + u * kRankThreads;
-                lv[u] = c < C ? s_norm[c] : -3.0;
-                lc[u] = c < C ? c : 0x7fffffff;
-            }
-            for (int k = 0; k < nC; ++k) {
-                double v = -4.0;
-                int bc = 0x7fffffff;
-#pragma unroll
-                for (int u = 0; u < kPer; ++u) better(v, bc, lv[u], lc[u]);
-                for (int o = 32; o > 0; o >>= 1) {
-                    const double v2 = __shfl_xor(v, o, 64);
-                    const int c2 = __shfl_xor(bc, o, 64);
-                    better(v, bc, v2, c2);
+                for (int u = 0; u < kPer; ++u) {
+                    const int c = tid + u * kRankThreads;
+                    const double nv = c < C ? s_norm[c] : -3.0;
+                    lv[u] = nv != nv ? -2.5 : nv;
+                    lc[u] = c;                   // distinct, padding past C
                 }
+                int rk[kPer] = {};
+                const int nu = C > kRankThreads ? kPer : 1;
+                for (int u2 = 0; u2 < nu; ++u2)
+#pragma unroll 4
+                    for (int m = 0; m < 64; ++m) {
+                        const double vm = readlane_f64(lv[u2], m);
+                        const int cm = __builtin_amdgcn_readlane(lc[u2], m);
+#pragma unroll
+                        for (int u = 0; u < kPer; ++u) rk[u] += first(vm, cm, lv[u], lc[u]);
+                    }
 #pragma unroll
                 for (int u = 0; u < kPer; ++u)
-                    if (lc[u] == bc) lv[u] = -5.0;           // taken
-                if (lane == 0) {
-                    s_cv[wv * 32 + k] = v;
-                    s_cc[wv * 32 + k] = bc;
-                }
+                    if (rk[u] < nC && (u == 0 || nu == kPer)) {
+                        s_cv[wv * 32 + rk[u]] = lv[u];
+                        s_cc[wv * 32 + rk[u]] = lc[u];
+                    }
             }
             __syncthreads();
             if (wv == 0) {
-                constexpr int kCand = (kRankThreads / 64) * 32 / 64;
-                double cv[kCand];
-                int cc[kCand];
-#pragma unroll
-                for (int u = 0; u < kCand; ++u) {
-                    const int e = lane + u * 64, w = e / 32, r = e % 32;
-                    const bool ok = r < nC;
-                    cv[u] = ok ? s_cv[w * 32 + r] : -6.0;
-                    cc[u] = ok ? s_cc[w * 32 + r] : 0x7fffffff;
-                }
-                for (int k = 0; k < nC; ++k) {
-                    double v = -7.0;
-                    int bc = 0x7fffffff;
-#pragma unroll
-                    for (int u = 0; u < kCand; ++u) better(v, bc, cv[u], cc[u]);
-                    for (int o = 32; o > 0; o >>= 1) {
-                        const double v2 = __shfl_xor(v, o, 64);
-                        const int c2 = __shfl_xor(bc, o, 64);
-                        better(v, bc, v2, c2);
+                const int M = nwr * nC;          // candidates, wave w's at w * 32 + r
+                for (int e = lane; e - lane < M; e += 64) {
+                    const bool ok = e < M;
+                    const int w = ok ? e / nC : 0, r = ok ? e - w * nC : 0;
+                    const double v = s_cv[w * 32 + r];
+                    const int c = s_cc[w * 32 + r];
+                    int rank = 0;
+                    for (int m = 0; m < M; ++m) {
+                        const int wm = m / nC, idx = wm * 32 + (m - wm * nC);
+                        rank += first(s_cv[idx], s_cc[idx], v, c);
                     }
-#pragma unroll
-                    for (int u = 0; u < kCand; ++u)
-                        if (cc[u] == bc) cv[u] = -8.0;
-                    if (lane == 0) {
-                        s_top[k] = bc;
-                        feature_ind[(size_t)g * nC + k] = (uint32_t)bc;
+                    if (ok && rank < nC) {
+                        s_top[rank] = c;
+                        feature_ind[(size_t)g * nC + rank] = (uint32_t)c;
                     }
                 }
             }
